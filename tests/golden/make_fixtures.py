@@ -1,0 +1,77 @@
+"""Generate the molecular fixtures under tests/golden/ (run in a container
+that has the reference's basis library; the outputs are committed).
+
+For each molecule: RHF in the Dalton basis set, canonical MOs, MO
+integrals, and an exact pivoted-Cholesky factor B with
+(pq|rs) = sum_P B[P,p,q] B[P,r,s] to ~1e-13.  Stored: eps, B packed over
+p<=q (np.triu_indices order), the SCF and nuclear-repulsion energies.
+
+Molecules (geometries from the reference's test inputs):
+* h2o_ccpvdz_test   DALTON/test/newccsdpt_h2o/newccsdpt_h2o.mol (Angstrom)
+* hf_631gss         DALTON/test/cc_energy_triples:34-40 (bohr); the
+                    degenerate pi pairs are rotated into eigenfunctions of
+                    the y -> -y reflection, as Dalton's symmetry-blocked SCF
+                    produces them (Dalton's DIIS inner product runs over
+                    packed ai<=bj amplitudes, which is not invariant under
+                    rotations inside a degenerate set, so the printed
+                    iteration energies depend on this choice)
+* h2o_ccpvdz_base   the water geometry O (0,0,0), H (0,+-0.757,0.586) A
+                    used for the timing/parity configuration (no Dalton
+                    output exists for it)
+
+Usage:  python tests/golden/make_fixtures.py [/path/to/reference]
+"""
+from __future__ import annotations
+
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+import gto  # noqa: E402
+
+REF = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
+A = gto.ANG2BOHR_DALTON
+
+MOLECULES = {
+    "h2o_ccpvdz_test": dict(
+        basis="cc-pVDZ",
+        atoms=[(8, np.array([0.0, -0.07579, 5.0]) * A),
+               (1, np.array([0.86681, 0.60144, 5.0]) * A),
+               (1, np.array([-0.86681, 0.60144, 5.0]) * A)],
+        nocc=5, adapt=False),
+    "hf_631gss": dict(
+        basis="6-31G**",
+        atoms=[(9, np.array([0.0, 0.0, 1.7632])), (1, np.array([0.0, 0.0, 0.0]))],
+        nocc=5, adapt=True),
+    "h2o_ccpvdz_base": dict(
+        basis="cc-pVDZ",
+        atoms=[(8, np.array([0.0, 0.0, 0.0]) * A),
+               (1, np.array([0.0, 0.757, 0.586]) * A),
+               (1, np.array([0.0, -0.757, 0.586]) * A)],
+        nocc=5, adapt=False),
+}
+
+
+def build(name: str, spec: dict) -> dict:
+    bas = gto.Basis(spec["atoms"], os.path.join(REF, "basis", spec["basis"]))
+    S, T, V = gto.one_electron(bas)
+    G = gto.two_electron(bas)
+    enuc = bas.nuclear_repulsion()
+    E, eps, C = gto.rhf(S, T + V, G, spec["nocc"], enuc)
+    if spec["adapt"]:
+        C = gto.adapt_degenerate(C, eps, S, gto.reflection_y(bas))
+    Gmo = np.einsum("pqrs,pa,qb,rc,sd->abcd", G, C, C, C, C, optimize=True)
+    B = gto.mo_cholesky(Gmo)
+    err = np.abs(np.einsum("Ppq,Prs->pqrs", B, B, optimize=True) - Gmo).max()
+    iu = np.triu_indices(bas.nbf)
+    print(f"{name}: nbf={bas.nbf} naux={B.shape[0]} E_nuc={enuc:.12f} E_SCF={E:.12f} chol_err={err:.1e}")
+    return dict(eps=eps, Bpk=B[:, iu[0], iu[1]], escf=E, enuc=enuc, nocc=spec["nocc"])
+
+
+if __name__ == "__main__":
+    for name, spec in MOLECULES.items():
+        d = build(name, spec)
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), **d)
