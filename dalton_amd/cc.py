@@ -1,0 +1,134 @@
+"""Host-side mirror of Dalton's closed-shell CC driver calls, over the C ABI.
+
+CCContext plays the part of one CCSD_ENERGY invocation
+(cc/ccsd_energy.F:20): it owns the device copy of the Hamiltonian and the
+amplitudes, and exposes the calls Dalton's driver makes -- CCRHSN
+(rhs), CCSD_GUESS+CCSD_ECCSD (guess), one loop pass (iterate) and the whole
+loop (ccsd_energy).  Arrays use Dalton's layouts: t1/Omega1 (a,i) with a
+fastest, doubles packed over ai<=bj."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from ._lib import CCGPUError, check, lib, ptr  # noqa: F401
+
+FP64 = 0
+MIXED = 1
+
+
+def npack(o: int, v: int) -> int:
+    n = o * v
+    return n * (n + 1) // 2
+
+
+def pack_t2(T: np.ndarray) -> np.ndarray:
+    """t[a,i,b,j] -> Dalton packed (ai<=bj, ai = a + v*i, bj-major)."""
+    v, o = T.shape[0], T.shape[1]
+    M = T.transpose(1, 0, 3, 2).reshape(o * v, o * v)
+    r, c = np.tril_indices(o * v)  # row-major lower = (bj, ai<=bj) in bj-major order
+    return np.ascontiguousarray(M[r, c])
+
+
+def unpack_t2(p: np.ndarray, o: int, v: int) -> np.ndarray:
+    n = o * v
+    r, c = np.tril_indices(n)
+    M = np.zeros((n, n))
+    M[r, c] = p
+    M[c, r] = p
+    return M.reshape(o, v, o, v).transpose(1, 0, 3, 2).copy()
+
+
+class CCContext:
+    def __init__(self, eps: np.ndarray, B: np.ndarray, nocc: int, mxdiis: int = 8):
+        eps = np.ascontiguousarray(eps, dtype=np.float64)
+        B = np.ascontiguousarray(B, dtype=np.float64)
+        N = eps.shape[0]
+        if B.ndim != 3 or B.shape[1] != N or B.shape[2] != N:
+            raise ValueError("B must have shape (naux, N, N) with N = len(eps)")
+        self.o, self.v, self.naux = nocc, N - nocc, B.shape[0]
+        h = C.c_void_p()
+        check(lib().ccgpu_init(self.o, self.v, self.naux, ptr(eps), ptr(B), mxdiis, C.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            check(lib().ccgpu_finalize(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def nt1(self):
+        return self.o * self.v
+
+    @property
+    def nt2(self):
+        return npack(self.o, self.v)
+
+    def rhs(self, t1: np.ndarray, t2p: np.ndarray, precision: int = FP64):
+        """CCRHSN: Omega1, Omega2 (packed) for amplitudes t1 (Dalton vector) and t2 (packed)."""
+        t1 = np.ascontiguousarray(t1, dtype=np.float64).reshape(-1)
+        t2p = np.ascontiguousarray(t2p, dtype=np.float64).reshape(-1)
+        if t1.size != self.nt1 or t2p.size != self.nt2:
+            raise ValueError("amplitude sizes do not match NT1AMX/NT2AMX")
+        om1 = np.empty(self.nt1)
+        om2 = np.empty(self.nt2)
+        check(lib().ccgpu_rhs(self._h, ptr(t1), ptr(t2p), ptr(om1), ptr(om2), precision))
+        return om1, om2
+
+    def guess(self) -> float:
+        e = C.c_double()
+        check(lib().ccgpu_ccsd_guess(self._h, C.byref(e)))
+        return e.value
+
+    def iterate(self, precision: int = FP64):
+        e, n = C.c_double(), C.c_double()
+        check(lib().ccgpu_ccsd_iterate(self._h, precision, C.byref(e), C.byref(n)))
+        return e.value, n.value
+
+    def ccsd_energy(self, maxit: int = 100, threnr: float = 1e-10, thrvec: float = 1e9,
+                    precision: int = FP64):
+        emp2, e, n = C.c_double(), C.c_double(), C.c_int()
+        hist = np.zeros(maxit)
+        check(lib().ccgpu_ccsd_energy(self._h, maxit, threnr, thrvec, precision, C.byref(emp2),
+                                      C.byref(e), C.byref(n), ptr(hist)))
+        return {"e_mp2": emp2.value, "e_ccsd": e.value, "niter": n.value, "history": hist[: n.value].copy()}
+
+    def amplitudes(self):
+        t1 = np.empty(self.nt1)
+        t2 = np.empty(self.nt2)
+        check(lib().ccgpu_get_amplitudes(self._h, ptr(t1), ptr(t2)))
+        return t1, t2
+
+    def set_amplitudes(self, t1: np.ndarray, t2p: np.ndarray):
+        t1 = np.ascontiguousarray(t1, dtype=np.float64).reshape(-1)
+        t2p = np.ascontiguousarray(t2p, dtype=np.float64).reshape(-1)
+        check(lib().ccgpu_set_amplitudes(self._h, ptr(t1), ptr(t2p)))
+
+
+def dgemm(transa: str, transb: str, alpha: float, A: np.ndarray, B: np.ndarray, beta: float = 0.0,
+          Cm: np.ndarray | None = None) -> np.ndarray:
+    """BLAS DGEMM on the MFMA kernel; arrays are Fortran-ordered matrices."""
+    A = np.asfortranarray(A, dtype=np.float64)
+    B = np.asfortranarray(B, dtype=np.float64)
+    m = A.shape[1] if transa.upper() == "T" else A.shape[0]
+    k = A.shape[0] if transa.upper() == "T" else A.shape[1]
+    n = B.shape[0] if transb.upper() == "T" else B.shape[1]
+    Cm = np.zeros((m, n), order="F") if Cm is None else np.array(Cm, dtype=np.float64, order="F", copy=True)
+    dp = C.POINTER(C.c_double)
+    check(lib().ccgpu_dgemm(transa.encode(), transb.encode(), m, n, k, alpha,
+                            A.ctypes.data_as(dp), max(1, A.shape[0]), B.ctypes.data_as(dp), max(1, B.shape[0]),
+                            beta, Cm.ctypes.data_as(dp), max(1, m)))
+    return Cm

@@ -1,0 +1,152 @@
+// extern "C" entry points of libccgpu (declared in include/ccgpu.h).
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <string>
+
+#include "../../include/ccgpu.h"
+#include "ccsd.h"
+
+struct ccgpu_ctx {
+  std::unique_ptr<ccgpu::CCSD> cc;
+};
+
+namespace {
+thread_local std::string g_err;
+
+template <typename F>
+int guarded(F&& f) {
+  try {
+    f();
+    return 0;
+  } catch (const ccgpu::Error& e) {
+    g_err = e.what();
+    return e.code;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return 2;
+  }
+}
+
+void check_prec(int precision) {
+  CCGPU_REQUIRE(precision == CCGPU_FP64, "precision not available in this build (FP64 only)");
+}
+}  // namespace
+
+extern "C" {
+
+int ccgpu_version(void) { return 1; }
+
+const char* ccgpu_last_error(void) { return g_err.c_str(); }
+
+int ccgpu_init(int nocc, int nvir, int naux, const double* eps, const double* B, int mxdiis,
+               ccgpu_ctx** ctx) {
+  return guarded([&] {
+    static const bool dry = [] {
+      const char* e = std::getenv("CCGPU_DRYRUN");
+      return e != nullptr && std::atoi(e) != 0;
+    }();
+    ccgpu::dryrun() = dry;
+    CCGPU_REQUIRE(ctx != nullptr && (dry || (eps != nullptr && B != nullptr)), "ccgpu_init: null argument");
+    *ctx = nullptr;
+    auto c = std::make_unique<ccgpu_ctx>();
+    c->cc = std::make_unique<ccgpu::CCSD>(nocc, nvir, naux, eps, B, mxdiis);
+    *ctx = c.release();
+  });
+}
+
+int ccgpu_finalize(ccgpu_ctx* ctx) {
+  return guarded([&] { delete ctx; });
+}
+
+int ccgpu_rhs(ccgpu_ctx* ctx, const double* t1am, const double* t2am, double* omega1, double* omega2,
+              int precision) {
+  return guarded([&] {
+    CCGPU_REQUIRE(ctx && t1am && t2am && omega1 && omega2, "ccgpu_rhs: null argument");
+    check_prec(precision);
+    ctx->cc->residual_host(t1am, t2am, omega1, omega2);
+  });
+}
+
+int ccgpu_ccsd_guess(ccgpu_ctx* ctx, double* e_mp2) {
+  return guarded([&] {
+    CCGPU_REQUIRE(ctx, "ccgpu_ccsd_guess: null context");
+    const double e = ctx->cc->mp2_start();
+    if (e_mp2) *e_mp2 = e;
+  });
+}
+
+int ccgpu_ccsd_iterate(ccgpu_ctx* ctx, int precision, double* e_corr, double* omega_norm) {
+  return guarded([&] {
+    CCGPU_REQUIRE(ctx, "ccgpu_ccsd_iterate: null context");
+    check_prec(precision);
+    double nrm = 0.0;
+    const double e = ctx->cc->iterate(&nrm);
+    if (e_corr) *e_corr = e;
+    if (omega_norm) *omega_norm = nrm;
+  });
+}
+
+int ccgpu_ccsd_energy(ccgpu_ctx* ctx, int maxit, double threnr, double thrvec, int precision,
+                      double* e_mp2, double* e_ccsd, int* niter, double* history) {
+  return guarded([&] {
+    CCGPU_REQUIRE(ctx && maxit > 0, "ccgpu_ccsd_energy: bad argument");
+    check_prec(precision);
+    double en2 = ctx->cc->mp2_start();
+    if (e_mp2) *e_mp2 = en2;
+    for (int it = 1;; ++it) {
+      const double en1 = en2;
+      double nrm = 0.0;
+      en2 = ctx->cc->iterate(&nrm);
+      if (history) history[it - 1] = en2;
+      if (niter) *niter = it;
+      if (e_ccsd) *e_ccsd = en2;
+      if (std::fabs(en2 - en1) < threnr && nrm < thrvec) break;
+      if (it + 1 > maxit) throw ccgpu::Error(4, "CC equations not converged.");
+    }
+  });
+}
+
+int ccgpu_get_amplitudes(ccgpu_ctx* ctx, double* t1am, double* t2am) {
+  return guarded([&] {
+    CCGPU_REQUIRE(ctx, "ccgpu_get_amplitudes: null context");
+    ctx->cc->get_amplitudes(t1am, t2am);
+  });
+}
+
+int ccgpu_set_amplitudes(ccgpu_ctx* ctx, const double* t1am, const double* t2am) {
+  return guarded([&] {
+    CCGPU_REQUIRE(ctx && t1am && t2am, "ccgpu_set_amplitudes: null argument");
+    ctx->cc->set_amplitudes(t1am, t2am);
+  });
+}
+
+int ccgpu_dgemm(char transa, char transb, int m, int n, int k, double alpha, const double* a, int lda,
+                const double* b, int ldb, double beta, double* c, int ldc) {
+  return guarded([&] {
+    const bool ta = transa == 'T' || transa == 't' || transa == 'C' || transa == 'c';
+    const bool tb = transb == 'T' || transb == 't' || transb == 'C' || transb == 'c';
+    CCGPU_REQUIRE(m >= 0 && n >= 0 && k >= 0, "ccgpu_dgemm: negative dimension");
+    CCGPU_REQUIRE(lda >= std::max(1, ta ? k : m) && ldb >= std::max(1, tb ? n : k) && ldc >= std::max(1, m),
+                  "ccgpu_dgemm: leading dimension too small");
+    if (m == 0 || n == 0) return;
+    const size_t na = (size_t)lda * (ta ? m : k), nb = (size_t)ldb * (tb ? k : n), nc = (size_t)ldc * n;
+    ccgpu::DBuf<double> A(na), B(nb), C(nc);
+    CCGPU_HIPX(hipMemcpy(A.p, a, na * sizeof(double), hipMemcpyHostToDevice));
+    CCGPU_HIPX(hipMemcpy(B.p, b, nb * sizeof(double), hipMemcpyHostToDevice));
+    CCGPU_HIPX(hipMemcpy(C.p, c, nc * sizeof(double), hipMemcpyHostToDevice));
+    ccgpu::GemmDesc d;
+    d.M = m; d.N = n; d.K = k; d.alpha = alpha; d.beta = beta;
+    d.A = A.p; d.B = B.p; d.C = C.p;
+    if (!ta) { d.a_m = 1; d.a_k = lda; } else { d.a_m = lda; d.a_k = 1; }
+    if (!tb) { d.b_k = 1; d.b_n = ldb; } else { d.b_k = ldb; d.b_n = 1; }
+    d.c_m = 1; d.c_n = ldc;
+    if (k == 0) { d.K = 0; }
+    ccgpu::gemm(d, ccgpu::GemmPrec::F64, nullptr);
+    CCGPU_HIPX(hipDeviceSynchronize());
+    CCGPU_HIPX(hipMemcpy(c, C.p, nc * sizeof(double), hipMemcpyDeviceToHost));
+  });
+}
+
+}  // extern "C"

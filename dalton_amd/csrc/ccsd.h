@@ -1,0 +1,81 @@
+// Closed-shell DF-CCSD on one GPU: device-resident amplitudes, T1-dressed
+// residual as MFMA GEMMs + layout kernels, Jacobi step, DIIS with Dalton's
+// restart rule, energy.
+#pragma once
+#include <vector>
+
+#include "common.h"
+
+namespace ccgpu {
+
+struct Dims {
+  int o = 0, v = 0, naux = 0;
+  long N = 0, nov = 0, nov2 = 0;
+  long npo = 0, npv = 0;  // o(o+1)/2, v(v+1)/2
+  long ntamp = 0;         // nov + nov(nov+1)/2 (Dalton NT1AMX + NT2AMX)
+};
+
+class CCSD {
+ public:
+  CCSD(int o, int v, int naux, const double* eps_host, const double* B_host, int mxdiis);
+  ~CCSD();
+
+  const Dims& dims() const { return dm_; }
+  hipStream_t stream() const { return st_; }
+
+  // CCRHSN mirror on device arrays: t1 [i][a], T [(i,a),(j,b)] -> om1, Om.
+  void residual(const double* t1, const double* T, double* om1, double* Om);
+
+  // Device-resident iteration (CCSD_ENERGY loop body).
+  double mp2_start();  // t1 = 0, t2 = MP2; returns E_MP2 (correlation)
+  // One iteration: residual, Jacobi step, DIIS, energy.  Returns the new
+  // correlation energy; *omnm = 2-norm of the incoming residual.
+  double iterate(double* omnm);
+  int iteration() const { return iter_; }
+
+  // Host <-> device amplitude exchange in Dalton's layouts.
+  void set_amplitudes(const double* t1_host, const double* t2_packed_host);
+  void get_amplitudes(double* t1_host, double* t2_packed_host);
+
+  // residual with host arrays (Dalton layouts), used by ccgpu_rhs
+  void residual_host(const double* t1, const double* t2p, double* om1, double* om2p);
+
+  const double* t1() const { return t1_.p; }
+  const double* T() const { return T_.p; }
+  const double* eps() const { return eps_.p; }
+  const double* B() const { return B_.p; }
+  const double* J() const { return J_.p; }
+
+  // per-term timing (ms) of the last residual, for the bench
+  std::vector<std::pair<const char*, float>> timings;
+  bool time_terms = false;
+
+ private:
+  void dress(const double* t1);
+  void ladder(const double* T, double* Om);
+  void mark(const char* name);
+
+  Dims dm_;
+  int mxdiis_;
+  int iter_ = 0;
+  hipStream_t st_ = nullptr;
+  std::vector<hipEvent_t> ev_;
+  std::vector<const char*> evname_;
+
+  // inputs / constants
+  DBuf<double> eps_, B_, h_, Bov_, J_, K_, L_, Joovv_;
+  // dressing
+  DBuf<double> Bt_, Bvv_, Boo_, BvoT_, F_, ht_, Jd_, G1_, Gam_, Z_, W_, E1_, E2_, hx_;
+  // amplitudes / residual
+  DBuf<double> t1_, T_, om1_, Om_;
+  DBuf<double> bufR_, bufX2_, bufTx_, bufU_, bufC_, bufD_;
+  // ladder
+  DBuf<double> taup_, taum_, Y_, Vp_, Vm_, Sp_, Sm_;
+  std::vector<int> lad_batches_;
+  // DIIS
+  DBuf<double> diisS_, diisE_, dvec_, dots_, coef_;
+  std::vector<double> Amat_;
+  DBuf<double> scal_;
+};
+
+}  // namespace ccgpu

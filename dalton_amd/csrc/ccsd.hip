@@ -1,0 +1,434 @@
+// Closed-shell DF-CCSD residual and iteration on one GPU.
+//
+// Layouts (row-major): ov pair index (i,a) -> i*v + a (Dalton's NAI order).
+//   T, U, Tx, Om, X2, C, D : [(i,a)][(j,b)]  (o*v x o*v, symmetric matrices)
+//   B, Bt : [P][p][q] (p,q over N = o + v, occupied first)
+//   Bov [P][(k,c)], Bvv [P][a][c], Boo [P][k][i], BvoT [P][(i,a)] = Bt[P][a][i]
+// Every large contraction is an MFMA GEMM (gemm.hip); permutations and
+// updates are the bandwidth kernels of kernels.hip.
+#include "ccsd.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "kernels.h"
+
+namespace ccgpu {
+
+namespace {
+
+void g(long M, long N, long K, double alpha, const double* A, long a_m, long a_k, const double* B,
+       long b_k, long b_n, double beta, double* C, long c_m, long c_n, hipStream_t s, int batch = 1,
+       long a_b = 0, long b_b = 0, long c_b = 0, long Kin = 0, long a_ko = 0, long b_ko = 0) {
+  GemmDesc d;
+  d.M = M; d.N = N; d.K = K; d.alpha = alpha; d.beta = beta;
+  d.A = A; d.a_m = a_m; d.a_k = a_k; d.a_b = a_b;
+  d.B = B; d.b_k = b_k; d.b_n = b_n; d.b_b = b_b;
+  d.C = C; d.c_m = c_m; d.c_n = c_n; d.c_b = c_b;
+  d.batch = batch; d.Kin = Kin; d.a_ko = a_ko; d.b_ko = b_ko;
+  gemm(d, GemmPrec::F64, s);
+}
+
+void perm(double* out, const double* in, long n0, long n1, long n2, long n3, long s0, long s1, long s2,
+          long s3, double alpha, double beta, hipStream_t st) {
+  const long n[4] = {n0, n1, n2, n3};
+  const long s[4] = {s0, s1, s2, s3};
+  permute4(out, in, n, s, alpha, beta, st);
+}
+
+// Small dense LU solve with partial pivoting (the DIIS system).
+std::vector<double> lu_solve(std::vector<double> A, std::vector<double> b, int n) {
+  std::vector<int> piv(n);
+  for (int j = 0; j < n; ++j) {
+    int p = j;
+    for (int i = j + 1; i < n; ++i)
+      if (std::fabs(A[i * n + j]) > std::fabs(A[p * n + j])) p = i;
+    if (A[p * n + j] == 0.0) throw Error(3, "DIIS: ZERO PIVOT - MATRIX SINGULAR");
+    if (p != j) {
+      for (int k = 0; k < n; ++k) std::swap(A[j * n + k], A[p * n + k]);
+      std::swap(b[j], b[p]);
+    }
+    for (int i = j + 1; i < n; ++i) {
+      const double f = A[i * n + j] / A[j * n + j];
+      for (int k = j; k < n; ++k) A[i * n + k] -= f * A[j * n + k];
+      b[i] -= f * b[j];
+    }
+  }
+  std::vector<double> x(n);
+  for (int i = n - 1; i >= 0; --i) {
+    double s = b[i];
+    for (int k = i + 1; k < n; ++k) s -= A[i * n + k] * x[k];
+    x[i] = s / A[i * n + i];
+  }
+  return x;
+}
+
+constexpr int kMaxVec = 80;  // Dalton MAXVEC, cc/ccsd_diis.F
+
+}  // namespace
+
+CCSD::CCSD(int o, int v, int naux, const double* eps_host, const double* B_host, int mxdiis)
+    : mxdiis_(mxdiis) {
+  CCGPU_REQUIRE(o > 0 && v > 0 && naux > 0, "ccgpu: bad dimensions");
+  CCGPU_REQUIRE(mxdiis >= 1 && mxdiis <= kMaxVec, "ccgpu: MXDIIS out of range (1..80)");
+  dm_.o = o; dm_.v = v; dm_.naux = naux;
+  dm_.N = o + v;
+  dm_.nov = (long)o * v;
+  dm_.nov2 = dm_.nov * dm_.nov;
+  dm_.npo = (long)o * (o + 1) / 2;
+  dm_.npv = (long)v * (v + 1) / 2;
+  dm_.ntamp = dm_.nov + dm_.nov * (dm_.nov + 1) / 2;
+  CCGPU_REQUIRE(dm_.nov <= 65535 * 8, "ccgpu: o*v too large");
+  CCGPU_HIPX(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
+  const long N = dm_.N, nov = dm_.nov, nov2 = dm_.nov2;
+
+  eps_.alloc(N);
+  B_.alloc((size_t)naux * N * N);
+  CCGPU_HIPX(hipMemcpyAsync(eps_.p, eps_host, N * sizeof(double), hipMemcpyHostToDevice, st_));
+  CCGPU_HIPX(hipMemcpyAsync(B_.p, B_host, B_.bytes(), hipMemcpyHostToDevice, st_));
+
+  // implicit one-electron operator: h = diag(eps) - sum_k [2(pq|kk) - (pk|kq)]
+  h_.alloc(N * N);
+  Jd_.alloc(naux);
+  coulomb_diag(Jd_.p, B_.p, naux, (int)N, o, st_);
+  gemv_mc(N * N, naux, -1.0, B_.p, N * N, Jd_.p, 0.0, h_.p, st_);
+  g(N, N, (long)naux * o, 1.0, B_.p, N, 1, B_.p, N, 1, 1.0, h_.p, N, 1, st_, 1, 0, 0, 0, o, N * N, N * N);
+  if (!dryrun()) {
+    std::vector<double> hh(N * N);
+    CCGPU_HIPX(hipMemcpyAsync(hh.data(), h_.p, N * N * sizeof(double), hipMemcpyDeviceToHost, st_));
+    CCGPU_HIPX(hipStreamSynchronize(st_));
+    for (long p = 0; p < N; ++p) hh[p * N + p] += eps_host[p];
+    CCGPU_HIP(hipMemcpy(h_.p, hh.data(), N * N * sizeof(double), hipMemcpyHostToDevice));
+  }
+
+  // ovov constants: J = (kc|ld), K = X(J) = (kd|lc), L = 2J - K, Joovv[(k,l),(c,d)]
+  Bov_.alloc((size_t)naux * nov);
+  perm(Bov_.p, B_.p + o, 1, naux, o, v, 0, N * N, N, 1, 1.0, 0.0, st_);
+  J_.alloc(nov2);
+  K_.alloc(nov2);
+  L_.alloc(nov2);
+  Joovv_.alloc(nov2);
+  g(nov, nov, naux, 1.0, Bov_.p, 1, nov, Bov_.p, nov, 1, 0.0, J_.p, nov, 1, st_);
+  perm(K_.p, J_.p, o, v, o, v, v, nov, (long)v * nov, 1, 1.0, 0.0, st_);
+  perm(L_.p, J_.p, 1, 1, 1, nov2, 0, 0, 0, 1, 2.0, 0.0, st_);
+  perm(L_.p, K_.p, 1, 1, 1, nov2, 0, 0, 0, 1, -1.0, 1.0, st_);
+  perm(Joovv_.p, J_.p, o, o, v, v, (long)v * nov, v, nov, 1, 1.0, 0.0, st_);
+
+  // per-iteration buffers
+  Bt_.alloc((size_t)naux * N * N);
+  Bvv_.alloc((size_t)naux * v * v);
+  Boo_.alloc((size_t)naux * o * o);
+  BvoT_.alloc((size_t)naux * nov);
+  F_.alloc(N * N);
+  ht_.alloc(N * N);
+  hx_.alloc(nov);
+  G1_.alloc((size_t)o * o * o * o);
+  Gam_.alloc((size_t)o * o * o * o);
+  Z_.alloc((size_t)naux * nov);
+  W_.alloc((size_t)o * o * nov);
+  E1_.alloc((size_t)v * v);
+  E2_.alloc((size_t)o * o);
+  t1_.alloc(nov);
+  T_.alloc(nov2);
+  om1_.alloc(nov);
+  Om_.alloc(nov2);
+  bufR_.alloc(std::max<long>(nov2, (long)v * v * o * o));
+  bufX2_.alloc(std::max<long>(nov2, dm_.ntamp));
+  bufTx_.alloc(nov2);
+  bufU_.alloc(nov2);
+  bufC_.alloc(nov2);
+  bufD_.alloc(nov2);
+  taup_.alloc(dm_.npo * dm_.npv);
+  taum_.alloc(dm_.npo * dm_.npv);
+
+  // particle-ladder batches over a: V+/- rows (a,b>=a) x npv and the
+  // Y[(a,c),(b,d)] block must fit a fixed budget; rows < 65536 (grid.y).
+  const double budget = 10.0e9;  // bytes per V+/- / Y buffer
+  long maxrows = std::max<long>(1, std::min<long>(65535, (long)(budget / (8.0 * dm_.npv))));
+  lad_batches_.clear();
+  lad_batches_.push_back(0);
+  long rows = 0, maxr = 0, maxy = 0;
+  int a0 = 0;
+  for (int a = 0; a < v; ++a) {
+    const long add = v - a;
+    const long ysz = (long)(a + 1 - a0) * v * (long)(v - a0) * v;
+    if (rows > 0 && (rows + add > maxrows || 8.0 * ysz > budget)) {
+      lad_batches_.push_back(a);
+      maxr = std::max(maxr, rows);
+      maxy = std::max(maxy, (long)(a - a0) * v * (long)(v - a0) * v);
+      a0 = a;
+      rows = 0;
+    }
+    rows += add;
+  }
+  lad_batches_.push_back(v);
+  maxr = std::max(maxr, rows);
+  maxy = std::max(maxy, (long)(v - a0) * v * (long)(v - a0) * v);
+  Y_.alloc(maxy);
+  Vp_.alloc(maxr * dm_.npv);
+  Vm_.alloc(maxr * dm_.npv);
+  Sp_.alloc(maxr * dm_.npo);
+  Sm_.alloc(maxr * dm_.npo);
+
+  // DIIS history (Dalton DIIST/DIISS files)
+  diisS_.alloc((size_t)mxdiis * dm_.ntamp);
+  diisE_.alloc((size_t)mxdiis * dm_.ntamp);
+  dots_.alloc(kMaxVec + 1);
+  coef_.alloc(kMaxVec + 1);
+  Amat_.assign((kMaxVec + 1) * (kMaxVec + 1), 0.0);
+  scal_.alloc(4);
+  CCGPU_HIPX(hipStreamSynchronize(st_));
+}
+
+CCSD::~CCSD() {
+  if (dryrun()) return;
+  for (auto e : ev_) (void)hipEventDestroy(e);
+  if (st_) (void)hipStreamDestroy(st_);
+}
+
+void CCSD::mark(const char* name) {
+  if (!time_terms || dryrun()) return;
+  hipEvent_t e;
+  CCGPU_HIP(hipEventCreate(&e));
+  CCGPU_HIP(hipEventRecord(e, st_));
+  ev_.push_back(e);
+  evname_.push_back(name);
+}
+
+// T1 transformation of B and h, then the dressed Fock matrix
+// (LAMMAT/LAMDA1 cc/ccsd_rhs.F:5887,6030; CC_AOFOCK/CC_FCKMO :3388,2724).
+void CCSD::dress(const double* t1) {
+  const int o = dm_.o, v = dm_.v, naux = dm_.naux;
+  const long N = dm_.N, NN = N * N, nov = dm_.nov;
+  CCGPU_HIPX(hipMemcpyAsync(Bt_.p, B_.p, B_.bytes(), hipMemcpyDeviceToDevice, st_));
+  // Bt[P][o+a][q] -= sum_l t(l,a) B[P][l][q]
+  g(v, N, o, -1.0, t1, 1, v, B_.p, N, 1, 1.0, Bt_.p + o * N, N, 1, st_, naux, 0, NN, NN);
+  // Bt[P][p][i] += sum_a Bt[P][p][o+a] t(i,a)
+  g(N, o, v, 1.0, Bt_.p + o, N, 1, t1, 1, v, 1.0, Bt_.p, N, 1, st_, naux, NN, 0, NN);
+  CCGPU_HIPX(hipMemcpyAsync(ht_.p, h_.p, NN * sizeof(double), hipMemcpyDeviceToDevice, st_));
+  g(v, N, o, -1.0, t1, 1, v, h_.p, N, 1, 1.0, ht_.p + o * N, N, 1, st_);
+  g(N, o, v, 1.0, ht_.p + o, N, 1, t1, 1, v, 1.0, ht_.p, N, 1, st_);
+  // F = ht + sum_P Bt[P] * 2 sum_k Bt[P,k,k] - sum_{P,k} Bt[P,p,k] Bt[P,k,q]
+  coulomb_diag(Jd_.p, Bt_.p, naux, (int)N, o, st_);
+  CCGPU_HIPX(hipMemcpyAsync(F_.p, ht_.p, NN * sizeof(double), hipMemcpyDeviceToDevice, st_));
+  gemv_mc(NN, naux, 1.0, Bt_.p, NN, Jd_.p, 1.0, F_.p, st_);
+  g(N, N, (long)naux * o, -1.0, Bt_.p, N, 1, Bt_.p, N, 1, 1.0, F_.p, N, 1, st_, 1, 0, 0, 0, o, NN, NN);
+  // contiguous blocks
+  perm(Bvv_.p, Bt_.p + o * N + o, 1, naux, v, v, 0, NN, N, 1, 1.0, 0.0, st_);
+  perm(Boo_.p, Bt_.p, 1, naux, o, o, 0, NN, N, 1, 1.0, 0.0, st_);
+  perm(BvoT_.p, Bt_.p + o * N, 1, naux, o, v, 0, NN, 1, N, 1.0, 0.0, st_);
+  (void)nov;
+}
+
+// Particle-particle ladder sum_cd t_ij^cd (ac~|bd~) in the symmetric /
+// antisymmetric scheme (CC_BF/CC_BF1/CCRHS_IPM, cc/ccsd_rhs.F:9129,9319,7571):
+// V+/-_{ab,cd} = (ac|bd) +/- (ad|bc) for a<=b, c<=d, formed per a-batch from
+// the dressed DF factor (v^4 never stored), contracted with tau+/- on MFMA.
+void CCSD::ladder(const double* T, double* Om) {
+  (void)T;
+  const int o = dm_.o, v = dm_.v, naux = dm_.naux;
+  const long vv = (long)v * v;
+  for (size_t bi = 0; bi + 1 < lad_batches_.size(); ++bi) {
+    const int a0 = lad_batches_[bi], a1 = lad_batches_[bi + 1];
+    long rows = 0;
+    for (int a = a0; a < a1; ++a) rows += v - a;
+    const long M = (long)(a1 - a0) * v, Nn = (long)(v - a0) * v;
+    g(M, Nn, naux, 1.0, Bvv_.p + (long)a0 * v, 1, vv, Bvv_.p + (long)a0 * v, vv, 1, 0.0, Y_.p, Nn, 1, st_);
+    ladder_vpm(Vp_.p, Vm_.p, Y_.p, Nn, a0, a1, v, st_);
+    g(rows, dm_.npo, dm_.npv, 1.0, Vp_.p, dm_.npv, 1, taup_.p, 1, dm_.npv, 0.0, Sp_.p, dm_.npo, 1, st_);
+    g(rows, dm_.npo, dm_.npv, 1.0, Vm_.p, dm_.npv, 1, taum_.p, 1, dm_.npv, 0.0, Sm_.p, dm_.npo, 1, st_);
+    ladder_scatter(Om, Sp_.p, Sm_.p, a0, a1, o, v, st_);
+  }
+}
+
+// CCSD vector function (CCRHSN, cc/ccsd_rhs.F:20-2278), T1-dressed form.
+void CCSD::residual(const double* t1, const double* T, double* om1, double* Om) {
+  const int o = dm_.o, v = dm_.v, naux = dm_.naux;
+  const long N = dm_.N, nov = dm_.nov, nov2 = dm_.nov2, oo = (long)o * o, vv = (long)v * v;
+  if (!dryrun())
+    for (auto e : ev_) (void)hipEventDestroy(e);
+  ev_.clear();
+  evname_.clear();
+  mark("start");
+  dress(t1);
+  mark("dress");
+
+  // A2 integral part: (ai~|bj~)
+  g(nov, nov, naux, 1.0, BvoT_.p, 1, nov, BvoT_.p, nov, 1, 0.0, Om, nov, 1, st_);
+  mark("A2_int");
+
+  // Too[(i,j),(a,b)] (into bufR), tau+/-, hole-hole ladder (CCRHS_A :2505)
+  double* Too = bufR_.p;
+  perm(Too, T, o, o, v, v, (long)v * nov, v, nov, 1, 1.0, 0.0, st_);
+  ladder_tau(taup_.p, taum_.p, Too, o, v, st_);
+  g(oo, oo, naux, 1.0, Boo_.p, 1, oo, Boo_.p, oo, 1, 0.0, G1_.p, oo, 1, st_);  // [(k,i),(l,j)]
+  perm(Gam_.p, G1_.p, o, o, o, o, oo * o, o, oo, 1, 1.0, 0.0, st_);               // [(k,l),(i,j)]
+  g(oo, oo, vv, 1.0, Joovv_.p, vv, 1, Too, 1, vv, 1.0, Gam_.p, oo, 1, st_);
+  double* LADh = bufX2_.p;
+  g(oo, vv, oo, 1.0, Gam_.p, 1, oo, Too, vv, 1, 0.0, LADh, vv, 1, st_);           // [(i,j),(a,b)]
+  perm(Om, LADh, o, v, o, v, (long)o * vv, v, vv, 1, 1.0, 1.0, st_);
+  mark("B2_hole_ladder");
+
+  ladder(T, Om);
+  mark("A2_ladder");
+
+  // ring terms (CCRHS_C/CIO :4272,7656; CCRHS_D/DIO :3525,8023)
+  double* Tx = bufTx_.p;
+  double* U = bufU_.p;
+  perm(Tx, T, o, v, o, v, v, nov, (long)v * nov, 1, 1.0, 0.0, st_);
+  perm(U, T, 1, 1, 1, nov2, 0, 0, 0, 1, 2.0, 0.0, st_);
+  perm(U, Tx, 1, 1, 1, nov2, 0, 0, 0, 1, -1.0, 1.0, st_);
+  double* R = bufR_.p;
+  g(vv, oo, naux, 1.0, Bvv_.p, 1, vv, Boo_.p, oo, 1, 0.0, R, oo, 1, st_);  // [(a,c),(k,i)] = (ki~|ac~)
+  double* C = bufC_.p;
+  double* D = bufD_.p;
+  perm(C, R, o, v, o, v, 1, (long)v * oo, o, oo, 1.0, 0.0, st_);   // C[(i,a),(k,c)]
+  perm(D, R, o, v, o, v, 1, (long)v * oo, o, oo, -1.0, 0.0, st_);  // D = -(ac~|ki~)
+  mark("CD_int");
+  g(nov, nov, nov, -0.5, Tx, nov, 1, K_.p, nov, 1, 1.0, C, nov, 1, st_);
+  g(nov, nov, naux, 2.0, BvoT_.p, 1, nov, Bov_.p, nov, 1, 1.0, D, nov, 1, st_);
+  g(nov, nov, nov, 0.5, U, nov, 1, L_.p, nov, 1, 1.0, D, nov, 1, st_);
+  mark("CD_intermediates");
+  double* P = bufR_.p;
+  double* X2 = bufX2_.p;
+  g(nov, nov, nov, 1.0, C, nov, 1, Tx, nov, 1, 0.0, P, nov, 1, st_);
+  perm(X2, P, 1, 1, 1, nov2, 0, 0, 0, 1, -0.5, 0.0, st_);
+  perm(X2, P, o, v, o, v, v, nov, (long)v * nov, 1, -1.0, 1.0, st_);
+  g(nov, nov, nov, 0.5, D, nov, 1, U, nov, 1, 1.0, X2, nov, 1, st_);
+  mark("CD_contract");
+
+  // E terms (CCRHS_EI/EFCK/E :3158,8604,2280)
+  perm(E1_.p, F_.p + o * N + o, 1, 1, v, v, 0, 0, N, 1, 1.0, 0.0, st_);
+  g(v, v, (long)o * nov, -1.0, U, nov, 1, J_.p, 1, nov, 1.0, E1_.p, v, 1, st_, 1, 0, 0, 0, nov,
+    (long)v * nov, (long)v * nov);
+  perm(E2_.p, F_.p, 1, 1, o, o, 0, 0, N, 1, 1.0, 0.0, st_);
+  g(o, o, (long)v * nov, 1.0, J_.p, (long)v * nov, 1, U, 1, (long)v * nov, 1.0, E2_.p, o, 1, st_, 1, 0,
+    0, 0, nov, nov, nov);
+  g(oo * v, v, v, 1.0, T, v, 1, E1_.p, 1, v, 1.0, X2, v, 1, st_);
+  g(o, v, o, -1.0, E2_.p, 1, o, T, v, 1, 1.0, X2, v, 1, st_, (int)nov, 0, nov, nov);
+  mark("E");
+
+  // Om += X2 + X2^T  (P_ij^ab)
+  perm(Om, X2, 1, 1, 1, nov2, 0, 0, 0, 1, 1.0, 1.0, st_);
+  perm(Om, X2, 1, 1, nov, nov, 0, 0, 1, nov, 1.0, 1.0, st_);
+  mark("sym");
+
+  // singles (CCRHS_G/H/I/J :2986,2858,2450,2686)
+  g(naux, nov, nov, 1.0, Bov_.p, nov, 1, U, nov, 1, 0.0, Z_.p, nov, 1, st_);
+  g(o, v, (long)naux * v, 1.0, Z_.p, v, 1, Bvv_.p, 1, v, 0.0, om1, v, 1, st_, 1, 0, 0, 0, v, nov, vv);
+  g(oo, nov, naux, 1.0, Boo_.p, 1, oo, Bov_.p, nov, 1, 0.0, W_.p, nov, 1, st_);
+  g(o, v, (long)o * nov, -1.0, W_.p, nov, 1, U, 1, nov, 1.0, om1, v, 1, st_, 1, 0, 0, 0, nov,
+    (long)o * nov, (long)v * nov);
+  perm(hx_.p, F_.p + o, 1, 1, o, v, 0, 0, N, 1, 1.0, 0.0, st_);  // F_kc
+  gemv_mc(nov, nov, 1.0, U, nov, hx_.p, 1.0, om1, st_);
+  perm(om1, F_.p + o * N, 1, 1, o, v, 0, 0, 1, N, 1.0, 1.0, st_);  // F~_ai
+  mark("singles");
+
+  if (time_terms && !dryrun()) {
+    CCGPU_HIPX(hipStreamSynchronize(st_));
+    timings.clear();
+    for (size_t i = 1; i < ev_.size(); ++i) {
+      float ms = 0.f;
+      CCGPU_HIP(hipEventElapsedTime(&ms, ev_[i - 1], ev_[i]));
+      timings.emplace_back(evname_[i], ms);
+    }
+  }
+}
+
+double CCSD::mp2_start() {
+  const long nov = dm_.nov;
+  fill(t1_.p, 0.0, nov, st_);
+  mp2_guess(T_.p, J_.p, eps_.p, dm_.o, dm_.v, st_);
+  ccsd_energy(scal_.p + 1, L_.p, T_.p, t1_.p, nov, st_);
+  double e = 0.0;
+  CCGPU_HIPX(hipMemcpyAsync(&e, scal_.p + 1, sizeof(double), hipMemcpyDeviceToHost, st_));
+  CCGPU_HIPX(hipStreamSynchronize(st_));
+  iter_ = 1;
+  return e;
+}
+
+// One pass of the CCSD_ENERGY loop (cc/ccsd_energy.F:742-982): residual,
+// CCSD_NXTAM, CCSD_DIIS (cc/ccsd_diis.F:251-365), CCSD_ECCSD.
+double CCSD::iterate(double* omnm) {
+  const int o = dm_.o, v = dm_.v;
+  const long nov = dm_.nov, nt = dm_.ntamp;
+  CCGPU_REQUIRE(iter_ >= 1, "ccgpu: call mp2_start (or set amplitudes) before iterating");
+  residual(t1_.p, T_.p, om1_.p, Om_.p);
+  nxtam(om1_.p, Om_.p, t1_.p, T_.p, eps_.p, o, v, scal_.p, st_);
+  const int mx = mxdiis_;
+  const int ivec = iter_ > mx ? iter_ - ((iter_ - 1) / mx) * mx : iter_;
+  const int nloop = ivec;
+  double* svec = diisS_.p + (long)(ivec - 1) * nt;
+  double* evec = diisE_.p + (long)(ivec - 1) * nt;
+  pack_diis(svec, evec, om1_.p, Om_.p, t1_.p, T_.p, nov, st_);
+  multidot(dots_.p, evec, diisE_.p, nt, nloop, nt, st_);
+  std::vector<double> dots(nloop + 1);
+  double nrm2 = 0.0;
+  CCGPU_HIPX(hipMemcpyAsync(dots.data(), dots_.p, nloop * sizeof(double), hipMemcpyDeviceToHost, st_));
+  CCGPU_HIPX(hipMemcpyAsync(&nrm2, scal_.p, sizeof(double), hipMemcpyDeviceToHost, st_));
+  CCGPU_HIPX(hipStreamSynchronize(st_));
+  if (dryrun())
+    for (int i = 0; i < nloop; ++i) dots[i] = (i == ivec - 1) ? 1.0 : 0.0;
+  const int LD = kMaxVec + 1;
+  for (int i = 0; i < nloop; ++i) {
+    Amat_[i * LD + (ivec - 1)] = dots[i];
+    Amat_[(ivec - 1) * LD + i] = dots[i];
+  }
+  const int n = nloop + 1;
+  std::vector<double> M(n * n, 0.0), rhs(n, 0.0);
+  for (int i = 0; i < nloop; ++i) {
+    for (int j = 0; j < nloop; ++j) M[i * n + j] = Amat_[i * LD + j];
+    M[i * n + nloop] = -1.0;
+    M[nloop * n + i] = -1.0;
+  }
+  rhs[nloop] = -1.0;
+  std::vector<double> c = lu_solve(M, rhs, n);
+  CCGPU_HIPX(hipMemcpyAsync(coef_.p, c.data(), nloop * sizeof(double), hipMemcpyHostToDevice, st_));
+  double* newv = bufX2_.p;
+  lincomb(newv, diisS_.p, nt, coef_.p, nloop, nt, st_);
+  CCGPU_HIPX(hipMemcpyAsync(t1_.p, newv, nov * sizeof(double), hipMemcpyDeviceToDevice, st_));
+  unpack_sym(T_.p, newv + nov, nov, st_);
+  ccsd_energy(scal_.p + 1, L_.p, T_.p, t1_.p, nov, st_);
+  double e = 0.0;
+  CCGPU_HIPX(hipMemcpyAsync(&e, scal_.p + 1, sizeof(double), hipMemcpyDeviceToHost, st_));
+  CCGPU_HIPX(hipStreamSynchronize(st_));
+  if (omnm) *omnm = std::sqrt(nrm2);
+  ++iter_;
+  return e;
+}
+
+void CCSD::set_amplitudes(const double* t1_host, const double* t2p_host) {
+  const long nov = dm_.nov;
+  CCGPU_HIPX(hipMemcpyAsync(t1_.p, t1_host, nov * sizeof(double), hipMemcpyHostToDevice, st_));
+  CCGPU_HIPX(hipMemcpyAsync(bufX2_.p, t2p_host, (dm_.ntamp - nov) * sizeof(double),
+                           hipMemcpyHostToDevice, st_));
+  unpack_sym(T_.p, bufX2_.p, nov, st_);
+  CCGPU_HIPX(hipStreamSynchronize(st_));
+  iter_ = 1;
+}
+
+void CCSD::get_amplitudes(double* t1_host, double* t2p_host) {
+  const long nov = dm_.nov;
+  if (t1_host)
+    CCGPU_HIPX(hipMemcpyAsync(t1_host, t1_.p, nov * sizeof(double), hipMemcpyDeviceToHost, st_));
+  if (t2p_host) {
+    pack_sym(bufX2_.p, T_.p, nov, st_);
+    CCGPU_HIPX(hipMemcpyAsync(t2p_host, bufX2_.p, (dm_.ntamp - nov) * sizeof(double),
+                             hipMemcpyDeviceToHost, st_));
+  }
+  CCGPU_HIPX(hipStreamSynchronize(st_));
+}
+
+void CCSD::residual_host(const double* t1, const double* t2p, double* om1, double* om2p) {
+  const long nov = dm_.nov;
+  set_amplitudes(t1, t2p);
+  residual(t1_.p, T_.p, om1_.p, Om_.p);
+  CCGPU_HIPX(hipMemcpyAsync(om1, om1_.p, nov * sizeof(double), hipMemcpyDeviceToHost, st_));
+  pack_sym(bufX2_.p, Om_.p, nov, st_);
+  CCGPU_HIPX(hipMemcpyAsync(om2p, bufX2_.p, (dm_.ntamp - nov) * sizeof(double), hipMemcpyDeviceToHost,
+                           st_));
+  CCGPU_HIPX(hipStreamSynchronize(st_));
+}
+
+}  // namespace ccgpu

@@ -1,0 +1,122 @@
+// Internal helpers shared by the ccgpu translation units (gfx950 only).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <stdexcept>
+#include <string>
+
+namespace ccgpu {
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define CCGPU_HIP(expr)                                                              \
+  do {                                                                               \
+    hipError_t e_ = (expr);                                                          \
+    if (e_ != hipSuccess)                                                            \
+      throw ::ccgpu::Error(2, std::string(#expr) + ": " + hipGetErrorString(e_) +    \
+                                  " (" __FILE__ ":" + std::to_string(__LINE__) + ")"); \
+  } while (0)
+
+#define CCGPU_REQUIRE(cond, msg)                                                     \
+  do {                                                                               \
+    if (!(cond)) throw ::ccgpu::Error(1, std::string(msg));                          \
+  } while (0)
+
+// ---------------------------------------------------------------- dry run
+// With CCGPU_DRYRUN=1 in the environment at ccgpu_init, allocations get
+// fake addresses, no kernel is launched and no copy is made; every GEMM and
+// layout-kernel launch still has its address span checked against the
+// allocation it points into.  This lets the whole host orchestration be
+// bounds-checked on a machine without a GPU, at any problem size.
+bool& dryrun();
+void* dry_alloc(size_t bytes);
+void register_alloc(const void* p, size_t bytes);
+void unregister_alloc(const void* p);
+// Throws unless [p + lo*esz, p + (hi+1)*esz) lies inside one registered allocation.
+void check_span(const void* p, long lo, long hi, size_t esz, const char* what);
+
+#define CCGPU_LAUNCH(kernel, grid, block, smem, stream, ...)                          \
+  do {                                                                               \
+    if (!::ccgpu::dryrun()) hipLaunchKernelGGL(kernel, grid, block, smem, stream, __VA_ARGS__); \
+  } while (0)
+
+// HIP runtime call that is skipped in a dry run.
+#define CCGPU_HIPX(expr)             \
+  do {                               \
+    if (!::ccgpu::dryrun()) CCGPU_HIP(expr); \
+  } while (0)
+
+// Owning device buffer.
+template <typename T>
+struct DBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DBuf() = default;
+  explicit DBuf(size_t count) { alloc(count); }
+  DBuf(const DBuf&) = delete;
+  DBuf& operator=(const DBuf&) = delete;
+  DBuf(DBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+  DBuf& operator=(DBuf&& o) noexcept {
+    if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+    return *this;
+  }
+  ~DBuf() { release(); }
+  void alloc(size_t count) {
+    release();
+    if (count == 0) return;
+    if (dryrun())
+      p = static_cast<T*>(dry_alloc(count * sizeof(T)));
+    else
+      CCGPU_HIP(hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T)));
+    register_alloc(p, count * sizeof(T));
+    n = count;
+  }
+  void ensure(size_t count) {
+    if (count > n) alloc(count);
+  }
+  void release() {
+    if (p) {
+      unregister_alloc(p);
+      if (!dryrun()) (void)hipFree(p);
+    }
+    p = nullptr;
+    n = 0;
+  }
+  size_t bytes() const { return n * sizeof(T); }
+};
+
+inline unsigned cdiv(long a, long b) { return static_cast<unsigned>((a + b - 1) / b); }
+
+// ------------------------------------------------------------------ GEMM API
+// C(m,n) = alpha * sum_k A(m,k) B(k,n) + beta * C(m,n), all operands given by
+// element strides, optionally batched (blockIdx.z) and with a two-level K
+// index k = ko*Kin + ki (A/B offsets ko*a_ko / ko*b_ko + ki*a_k / ki*b_k).
+struct GemmDesc {
+  long M = 0, N = 0, K = 0;
+  int batch = 1;
+  double alpha = 1.0, beta = 0.0;
+  const void* A = nullptr;
+  long a_m = 0, a_k = 0, a_b = 0;
+  const void* B = nullptr;
+  long b_k = 0, b_n = 0, b_b = 0;
+  void* C = nullptr;
+  long c_m = 0, c_n = 0, c_b = 0;
+  long Kin = 0, a_ko = 0, b_ko = 0;
+};
+
+// Arithmetic used by a GEMM call.
+enum class GemmPrec { F64 = 0, F32 = 1, BF16X3 = 2 };
+
+void gemm(const GemmDesc& d, GemmPrec prec, hipStream_t s);
+
+// Convenience: row-major C[M][N] (+)= alpha * op(A) op(B) for contiguous matrices.
+// ta=false: A is [M][lda]; ta=true: A is [K][lda] used transposed.
+void gemm_rm(bool ta, bool tb, long M, long N, long K, double alpha, const void* A, long lda,
+             const void* B, long ldb, double beta, void* C, long ldc, GemmPrec prec, hipStream_t s);
+
+}  // namespace ccgpu

@@ -1,0 +1,601 @@
+// Bandwidth-bound kernels of the CC path (see kernels.h).
+#include <cmath>
+
+#include "kernels.h"
+
+namespace ccgpu {
+
+// ------------------------------------------------------------------ permute4
+struct P4 {
+  long n[4];
+  long s[4];   // input strides per output dim
+  long os[4];  // output strides (row-major)
+  int q, r1, r2;
+  long nt3, ntq;
+};
+
+__global__ void k_perm_direct(double* __restrict__ out, const double* __restrict__ in, P4 p,
+                              double alpha, double beta) {
+  long bx = blockIdx.x;
+  const long t3 = bx % p.nt3;
+  bx /= p.nt3;
+  const long j2 = bx % p.n[2];
+  bx /= p.n[2];
+  const long j1 = bx % p.n[1];
+  const long j0 = bx / p.n[1];
+  const long j3 = t3 * 256 + threadIdx.x;
+  if (j3 >= p.n[3]) return;
+  const double v = in[j0 * p.s[0] + j1 * p.s[1] + j2 * p.s[2] + j3 * p.s[3]];
+  double* o = out + ((j0 * p.n[1] + j1) * p.n[2] + j2) * p.n[3] + j3;
+  double r = alpha * v;
+  if (beta != 0.0) r += beta * *o;
+  *o = r;
+}
+
+// 32x32 LDS transpose over (output-fastest dim 3, input-stride-1 dim q).
+__global__ void k_perm_tiled(double* __restrict__ out, const double* __restrict__ in, P4 p,
+                             double alpha, double beta) {
+  __shared__ double tile[32][33];
+  long bx = blockIdx.x;
+  const long t3 = bx % p.nt3;
+  bx /= p.nt3;
+  const long tq = bx % p.ntq;
+  bx /= p.ntq;
+  const long jr2 = bx % p.n[p.r2];
+  const long jr1 = bx / p.n[p.r2];
+  const long ib = jr1 * p.s[p.r1] + jr2 * p.s[p.r2];
+  const long ob = jr1 * p.os[p.r1] + jr2 * p.os[p.r2];
+  const int tx = threadIdx.x, ty = threadIdx.y;
+  const long j30 = t3 * 32, jq0 = tq * 32;
+  const long nq = p.n[p.q], n3 = p.n[3];
+#pragma unroll
+  for (int k = 0; k < 32; k += 8) {
+    const long jq = jq0 + tx, j3 = j30 + ty + k;
+    if (jq < nq && j3 < n3) tile[ty + k][tx] = in[ib + jq + j3 * p.s[3]];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < 32; k += 8) {
+    const long j3 = j30 + tx, jq = jq0 + ty + k;
+    if (jq < nq && j3 < n3) {
+      double* o = out + ob + jq * p.os[p.q] + j3;
+      double r = alpha * tile[tx][ty + k];
+      if (beta != 0.0) r += beta * *o;
+      *o = r;
+    }
+  }
+}
+
+void permute4(double* out, const double* in, const long n[4], const long s[4], double alpha,
+              double beta, hipStream_t st) {
+  P4 p;
+  long total = 1;
+  for (int i = 0; i < 4; ++i) {
+    p.n[i] = n[i];
+    p.s[i] = s[i];
+    total *= n[i];
+  }
+  if (total == 0) return;
+  {
+    long hi = 0;
+    for (int i = 0; i < 4; ++i) {
+      CCGPU_REQUIRE(s[i] >= 0 || n[i] == 1, "permute4: negative stride");
+      hi += (n[i] - 1) * s[i];
+    }
+    check_span(in, 0, hi, sizeof(double), "permute4 in");
+    check_span(out, 0, total - 1, sizeof(double), "permute4 out");
+  }
+  p.os[3] = 1;
+  p.os[2] = n[3];
+  p.os[1] = n[2] * n[3];
+  p.os[0] = n[1] * n[2] * n[3];
+  int q = -1;
+  if (s[3] != 1 && n[3] > 1)
+    for (int i = 2; i >= 0; --i)
+      if (s[i] == 1 && n[i] > 1) { q = i; break; }
+  if (q < 0) {
+    p.nt3 = (n[3] + 255) / 256;
+    const long nb = p.nt3 * n[0] * n[1] * n[2];
+    CCGPU_REQUIRE(nb < (1L << 31), "permute4: grid too large");
+    CCGPU_LAUNCH(k_perm_direct, dim3((unsigned)nb), dim3(256), 0, st, out, in, p, alpha, beta);
+  } else {
+    p.q = q;
+    int others[2], k = 0;
+    for (int i = 0; i < 3; ++i)
+      if (i != q) others[k++] = i;
+    p.r1 = others[0];
+    p.r2 = others[1];
+    p.nt3 = (n[3] + 31) / 32;
+    p.ntq = (n[q] + 31) / 32;
+    const long nb = p.nt3 * p.ntq * n[p.r1] * n[p.r2];
+    CCGPU_REQUIRE(nb < (1L << 31), "permute4: grid too large");
+    CCGPU_LAUNCH(k_perm_tiled, dim3((unsigned)nb), dim3(32, 8), 0, st, out, in, p, alpha, beta);
+  }
+  CCGPU_HIPX(hipGetLastError());
+}
+
+// ------------------------------------------------------------------ reductions
+// Partial sums are written per block and summed in block order by one
+// block: results are bitwise reproducible run to run.
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) sh[w] = v;
+  __syncthreads();
+  double r = 0.0;
+  if (threadIdx.x == 0)
+    for (int i = 0; i < (int)(blockDim.x >> 6); ++i) r += sh[i];
+  __syncthreads();
+  return r;  // valid in thread 0
+}
+
+__global__ void k_sum_partials(double* out, const double* part, int n, int stride, int nout) {
+  // out[j] = sum_b part[b*stride + j], one thread per j, fixed order
+  const int j = threadIdx.x + blockIdx.x * blockDim.x;
+  if (j >= nout) return;
+  double s = 0.0;
+  for (int b = 0; b < n; ++b) s += part[(long)b * stride + j];
+  out[j] = s;
+}
+
+namespace {
+// partial-sum workspace (stream-ordered reuse); never destroyed
+DBuf<double>& g_part_ref() {
+  static auto* p = new DBuf<double>();
+  return *p;
+}
+#define g_part g_part_ref()
+double* partials(long n) {
+  g_part.ensure(n);
+  return g_part.p;
+}
+}  // namespace
+
+// ------------------------------------------------------------------ gemv
+__global__ void k_gemv_mc(long M, long K, long kchunk, double alpha, const double* A, long lda,
+                          const double* x, double* part) {
+  const long m = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= M) return;
+  const long k0 = (long)blockIdx.y * kchunk, k1 = min(K, k0 + kchunk);
+  double s = 0.0;
+  for (long k = k0; k < k1; ++k) s += A[k * lda + m] * x[k];
+  part[(long)blockIdx.y * M + m] = alpha * s;
+}
+
+__global__ void k_gemv_finish(long M, int nk, const double* part, double beta, double* y) {
+  const long m = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= M) return;
+  double s = 0.0;
+  for (int b = 0; b < nk; ++b) s += part[(long)b * M + m];
+  y[m] = (beta != 0.0 ? beta * y[m] : 0.0) + s;
+}
+
+void gemv_mc(long M, long K, double alpha, const double* A, long lda, const double* x, double beta,
+             double* y, hipStream_t st) {
+  if (M <= 0) return;
+  if (K > 0) {
+    check_span(A, 0, (K - 1) * lda + M - 1, sizeof(double), "gemv A");
+    check_span(x, 0, K - 1, sizeof(double), "gemv x");
+  }
+  check_span(y, 0, M - 1, sizeof(double), "gemv y");
+  const long mb = (M + 255) / 256;
+  long nk = std::max<long>(1, std::min<long>(std::max<long>(1, 2048 / mb), (K + 255) / 256));
+  const long kchunk = (K + nk - 1) / nk;
+  nk = (K + kchunk - 1) / kchunk;
+  if (nk < 1) nk = 1;
+  double* part = partials(nk * M);
+  CCGPU_LAUNCH(k_gemv_mc, dim3((unsigned)mb, (unsigned)nk), dim3(256), 0, st, M, K, kchunk,
+                     alpha, A, lda, x, part);
+  CCGPU_LAUNCH(k_gemv_finish, dim3((unsigned)mb), dim3(256), 0, st, M, (int)nk, part, beta, y);
+  CCGPU_HIPX(hipGetLastError());
+}
+
+// ------------------------------------------------------------------ packing
+__global__ void k_pack_sym(double* __restrict__ packed, const double* __restrict__ F, long n) {
+  const long bj = blockIdx.x;
+  const long base = bj * (bj + 1) / 2;
+  for (long ai = threadIdx.x; ai <= bj; ai += blockDim.x) packed[base + ai] = F[bj * n + ai];
+}
+
+__global__ void k_unpack_lower(double* __restrict__ F, const double* __restrict__ packed, long n) {
+  const long bj = blockIdx.x;
+  const long base = bj * (bj + 1) / 2;
+  for (long ai = threadIdx.x; ai <= bj; ai += blockDim.x) F[bj * n + ai] = packed[base + ai];
+}
+
+// F[c][r] = F[r][c] for c > r (copy lower triangle to upper), 32x32 tiles
+__global__ void k_mirror_lower(double* F, long n) {
+  __shared__ double tile[32][33];
+  const long tr = blockIdx.y, tc = blockIdx.x;  // source tile (row tr, col tc), tr >= tc
+  if (tc > tr) return;
+  const int tx = threadIdx.x, ty = threadIdx.y;
+#pragma unroll
+  for (int k = 0; k < 32; k += 8) {
+    const long r = tr * 32 + ty + k, c = tc * 32 + tx;
+    if (r < n && c < n) tile[ty + k][tx] = F[r * n + c];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < 32; k += 8) {
+    const long r = tc * 32 + ty + k, c = tr * 32 + tx;  // destination (row r, col c) = src (c, r)
+    if (r < n && c < n && c > r) F[r * n + c] = tile[tx][ty + k];
+  }
+}
+
+void pack_sym(double* packed, const double* F, long n, hipStream_t st) {
+  if (n <= 0) return;
+  check_span(packed, 0, n * (n + 1) / 2 - 1, sizeof(double), "pack_sym packed");
+  check_span(F, 0, n * n - 1, sizeof(double), "pack_sym F");
+  CCGPU_LAUNCH(k_pack_sym, dim3((unsigned)n), dim3(256), 0, st, packed, F, n);
+  CCGPU_HIPX(hipGetLastError());
+}
+
+void unpack_sym(double* F, const double* packed, long n, hipStream_t st) {
+  if (n <= 0) return;
+  check_span(packed, 0, n * (n + 1) / 2 - 1, sizeof(double), "unpack_sym packed");
+  check_span(F, 0, n * n - 1, sizeof(double), "unpack_sym F");
+  CCGPU_LAUNCH(k_unpack_lower, dim3((unsigned)n), dim3(256), 0, st, F, packed, n);
+  const unsigned nt = cdiv(n, 32);
+  CCGPU_REQUIRE(nt < 65536, "unpack_sym: n too large");
+  CCGPU_LAUNCH(k_mirror_lower, dim3(nt, nt), dim3(32, 8), 0, st, F, n);
+  CCGPU_HIPX(hipGetLastError());
+}
+
+// ------------------------------------------------------------------ Jacobi step
+__global__ void k_nxtam2(double* __restrict__ om2, const double* __restrict__ t2,
+                         const double* __restrict__ eps, int o, int v, double* part) {
+  __shared__ double sh[16];
+  const long nov = (long)o * v;
+  const long r = blockIdx.x;  // row (i,a)
+  const int i = (int)(r / v), a = (int)(r % v);
+  const double dia = eps[i] - eps[o + a];
+  double s = 0.0;
+  for (long c = threadIdx.x; c < nov; c += blockDim.x) {
+    const int j = (int)(c / v), b = (int)(c % v);
+    const long e = r * nov + c;
+    const double om = om2[e];
+    s += (c == r) ? 2.0 * om * om : om * om;  // packed norm: (full + diag) / 2
+    om2[e] = t2[e] + om / (dia + eps[j] - eps[o + b]);
+  }
+  const double tot = block_sum(s, sh);
+  if (threadIdx.x == 0) part[r] = 0.5 * tot;
+}
+
+__global__ void k_nxtam1(double* __restrict__ om1, const double* __restrict__ t1,
+                         const double* __restrict__ eps, int o, int v, double* part) {
+  __shared__ double sh[16];
+  const long nov = (long)o * v;
+  double s = 0.0;
+  for (long e = threadIdx.x; e < nov; e += blockDim.x) {
+    const int i = (int)(e / v), a = (int)(e % v);
+    const double om = om1[e];
+    s += om * om;
+    om1[e] = t1[e] + om / (eps[i] - eps[o + a]);
+  }
+  const double tot = block_sum(s, sh);
+  if (threadIdx.x == 0) part[0] = tot;
+}
+
+void nxtam(double* om1, double* om2, const double* t1, const double* t2, const double* eps, int o,
+           int v, double* norm2, hipStream_t st) {
+  const long nov = (long)o * v;
+  check_span(om1, 0, nov - 1, sizeof(double), "nxtam om1");
+  check_span(t1, 0, nov - 1, sizeof(double), "nxtam t1");
+  check_span(om2, 0, nov * nov - 1, sizeof(double), "nxtam om2");
+  check_span(t2, 0, nov * nov - 1, sizeof(double), "nxtam t2");
+  check_span(eps, 0, o + v - 1, sizeof(double), "nxtam eps");
+  check_span(norm2, 0, 0, sizeof(double), "nxtam norm2");
+  double* part = partials(nov + 1);
+  CCGPU_LAUNCH(k_nxtam1, dim3(1), dim3(256), 0, st, om1, t1, eps, o, v, part + nov);
+  CCGPU_LAUNCH(k_nxtam2, dim3((unsigned)nov), dim3(256), 0, st, om2, t2, eps, o, v, part);
+  CCGPU_LAUNCH(k_sum_partials, dim3(1), dim3(64), 0, st, norm2, part, (int)(nov + 1), 1, 1);
+  CCGPU_HIPX(hipGetLastError());
+}
+
+// ------------------------------------------------------------------ DIIS algebra
+__global__ void k_pack_diis2(double* __restrict__ svec, double* __restrict__ evec,
+                             const double* __restrict__ S, const double* __restrict__ T, long n) {
+  const long bj = blockIdx.x;
+  const long base = bj * (bj + 1) / 2;
+  for (long ai = threadIdx.x; ai <= bj; ai += blockDim.x) {
+    const double s = S[bj * n + ai];
+    svec[base + ai] = s;
+    evec[base + ai] = s - T[bj * n + ai];
+  }
+}
+
+__global__ void k_diff1(double* svec, double* evec, const double* s, const double* t, long n) {
+  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
+    svec[e] = s[e];
+    evec[e] = s[e] - t[e];
+  }
+}
+
+void pack_diis(double* svec, double* evec, const double* s1, const double* s2, const double* t1,
+               const double* t2, long nov, hipStream_t st) {
+  const long ntot = nov + nov * (nov + 1) / 2;
+  check_span(svec, 0, ntot - 1, sizeof(double), "pack_diis svec");
+  check_span(evec, 0, ntot - 1, sizeof(double), "pack_diis evec");
+  check_span(s1, 0, nov - 1, sizeof(double), "pack_diis s1");
+  check_span(t1, 0, nov - 1, sizeof(double), "pack_diis t1");
+  check_span(s2, 0, nov * nov - 1, sizeof(double), "pack_diis s2");
+  check_span(t2, 0, nov * nov - 1, sizeof(double), "pack_diis t2");
+  CCGPU_LAUNCH(k_diff1, dim3(cdiv(nov, 256)), dim3(256), 0, st, svec, evec, s1, t1, nov);
+  CCGPU_LAUNCH(k_pack_diis2, dim3((unsigned)nov), dim3(256), 0, st, svec + nov, evec + nov, s2,
+                     t2, nov);
+  CCGPU_HIPX(hipGetLastError());
+}
+
+constexpr int kDotGroup = 8;
+
+__global__ void k_multidot(const double* __restrict__ e, const double* __restrict__ E, long ld,
+                           int g0, int ng, long n, double* part) {
+  __shared__ double sh[16];
+  double s[kDotGroup];
+#pragma unroll
+  for (int i = 0; i < kDotGroup; ++i) s[i] = 0.0;
+  for (long x = (long)blockIdx.x * blockDim.x + threadIdx.x; x < n; x += (long)gridDim.x * blockDim.x) {
+    const double ex = e[x];
+#pragma unroll
+    for (int i = 0; i < kDotGroup; ++i)
+      if (i < ng) s[i] += ex * E[(long)(g0 + i) * ld + x];
+  }
+  for (int i = 0; i < ng; ++i) {
+    const double t = block_sum(s[i], sh);
+    if (threadIdx.x == 0) part[(long)blockIdx.x * kDotGroup + i] = t;
+  }
+}
+
+void multidot(double* dots, const double* e, const double* E, long ld, int nvec, long n,
+              hipStream_t st) {
+  check_span(dots, 0, nvec - 1, sizeof(double), "multidot dots");
+  check_span(e, 0, n - 1, sizeof(double), "multidot e");
+  check_span(E, 0, (long)(nvec - 1) * ld + n - 1, sizeof(double), "multidot E");
+  const int nb = (int)std::min<long>(2048, std::max<long>(1, (n + 255) / 256));
+  double* part = partials((long)nb * kDotGroup);
+  for (int g0 = 0; g0 < nvec; g0 += kDotGroup) {
+    const int ng = std::min(kDotGroup, nvec - g0);
+    CCGPU_LAUNCH(k_multidot, dim3(nb), dim3(256), 0, st, e, E, ld, g0, ng, n, part);
+    CCGPU_LAUNCH(k_sum_partials, dim3(1), dim3(64), 0, st, dots + g0, part, nb, kDotGroup, ng);
+  }
+  CCGPU_HIPX(hipGetLastError());
+}
+
+__global__ void k_lincomb(double* __restrict__ out, const double* __restrict__ S, long ld,
+                          const double* __restrict__ c, int nvec, long n) {
+  for (long x = (long)blockIdx.x * blockDim.x + threadIdx.x; x < n; x += (long)gridDim.x * blockDim.x) {
+    double s = 0.0;
+    for (int i = 0; i < nvec; ++i) s += c[i] * S[(long)i * ld + x];
+    out[x] = s;
+  }
+}
+
+void lincomb(double* out, const double* S, long ld, const double* c, int nvec, long n, hipStream_t st) {
+  check_span(out, 0, n - 1, sizeof(double), "lincomb out");
+  check_span(S, 0, (long)(nvec - 1) * ld + n - 1, sizeof(double), "lincomb S");
+  check_span(c, 0, nvec - 1, sizeof(double), "lincomb c");
+  const unsigned nb = (unsigned)std::min<long>(65536, (n + 255) / 256);
+  CCGPU_LAUNCH(k_lincomb, dim3(nb), dim3(256), 0, st, out, S, ld, c, nvec, n);
+  CCGPU_HIPX(hipGetLastError());
+}
+
+// ------------------------------------------------------------------ energy
+__global__ void k_energy(const double* __restrict__ L, const double* __restrict__ T,
+                         const double* __restrict__ t1, long nov, double* part) {
+  __shared__ double sh[16];
+  const long r = blockIdx.x;
+  const double tr = t1[r];
+  double s = 0.0;
+  for (long c = threadIdx.x; c < nov; c += blockDim.x) s += L[r * nov + c] * (T[r * nov + c] + tr * t1[c]);
+  const double tot = block_sum(s, sh);
+  if (threadIdx.x == 0) part[r] = tot;
+}
+
+void ccsd_energy(double* out, const double* L, const double* t2, const double* t1, long nov,
+                 hipStream_t st) {
+  check_span(out, 0, 0, sizeof(double), "energy out");
+  check_span(L, 0, nov * nov - 1, sizeof(double), "energy L");
+  check_span(t2, 0, nov * nov - 1, sizeof(double), "energy t2");
+  check_span(t1, 0, nov - 1, sizeof(double), "energy t1");
+  double* part = partials(nov);
+  CCGPU_LAUNCH(k_energy, dim3((unsigned)nov), dim3(256), 0, st, L, t2, t1, nov, part);
+  CCGPU_LAUNCH(k_sum_partials, dim3(1), dim3(64), 0, st, out, part, (int)nov, 1, 1);
+  CCGPU_HIPX(hipGetLastError());
+}
+
+// ------------------------------------------------------------------ ladder helpers
+__device__ __forceinline__ void tri_decode(long p, long& lo, long& hi) {
+  // p = hi*(hi+1)/2 + lo, 0 <= lo <= hi
+  long h = (long)((sqrt(8.0 * (double)p + 1.0) - 1.0) * 0.5);
+  while (h * (h + 1) / 2 > p) --h;
+  while ((h + 1) * (h + 2) / 2 <= p) ++h;
+  hi = h;
+  lo = p - h * (h + 1) / 2;
+}
+
+__global__ void k_ladder_tau(double* __restrict__ taup, double* __restrict__ taum,
+                             const double* __restrict__ Too, int o, int v) {
+  const long q = blockIdx.y;  // ij, i <= j
+  long i, j;
+  tri_decode(q, i, j);
+  const long npv = (long)v * (v + 1) / 2;
+  const double* t = Too + (i * o + j) * (long)v * v;
+  for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < npv; p += (long)gridDim.x * blockDim.x) {
+    long c, d;
+    tri_decode(p, c, d);
+    const double tcd = t[c * v + d], tdc = t[d * v + c];
+    taup[q * npv + p] = (c == d) ? 0.5 * tcd : 0.5 * (tcd + tdc);
+    taum[q * npv + p] = 0.5 * (tcd - tdc);
+  }
+}
+
+void ladder_tau(double* taup, double* taum, const double* Too, int o, int v, hipStream_t st) {
+  const long npo = (long)o * (o + 1) / 2, npv = (long)v * (v + 1) / 2;
+  CCGPU_REQUIRE(npo < 65536, "ladder_tau: too many occupied pairs");
+  check_span(taup, 0, npo * npv - 1, sizeof(double), "ladder_tau taup");
+  check_span(taum, 0, npo * npv - 1, sizeof(double), "ladder_tau taum");
+  check_span(Too, 0, (long)o * o * v * v - 1, sizeof(double), "ladder_tau Too");
+  const unsigned nbx = (unsigned)std::min<long>(256, (npv + 255) / 256);
+  CCGPU_LAUNCH(k_ladder_tau, dim3(nbx, (unsigned)npo), dim3(256), 0, st, taup, taum, Too, o, v);
+  CCGPU_HIPX(hipGetLastError());
+}
+
+__device__ __forceinline__ void batch_row_decode(long r, int a0, int a1, int v, int& a, int& b) {
+  int aa = a0;
+  long off = 0;
+  while (aa < a1 && off + (v - aa) <= r) {
+    off += v - aa;
+    ++aa;
+  }
+  a = aa;
+  b = aa + (int)(r - off);
+}
+
+__global__ void k_ladder_vpm(double* __restrict__ vp, double* __restrict__ vm,
+                             const double* __restrict__ Y, long ldY, int a0, int a1, int v) {
+  __shared__ double Mcd[32][33];
+  __shared__ double Mdc[32][33];
+  const long r = blockIdx.y;
+  int a, b;
+  batch_row_decode(r, a0, a1, v, a, b);
+  long tc, td;
+  tri_decode(blockIdx.x, tc, td);  // tile pair tc <= td
+  const int tx = threadIdx.x, ty = threadIdx.y;
+  const long c0 = tc * 32, d0 = td * 32;
+  const double* Ya = Y + (long)(a - a0) * v * ldY + (long)(b - a0) * v;  // M[c][d] = Ya[c*ldY + d]
+#pragma unroll
+  for (int k = 0; k < 32; k += 8) {
+    const long c = c0 + ty + k, d = d0 + tx;
+    Mcd[ty + k][tx] = (c < v && d < v) ? Ya[c * ldY + d] : 0.0;      // Mcd[c_l][d_l] = M[c][d]
+    const long dd = d0 + ty + k, cc = c0 + tx;
+    Mdc[ty + k][tx] = (dd < v && cc < v) ? Ya[dd * ldY + cc] : 0.0;  // Mdc[d_l][c_l] = M[d][c]
+  }
+  __syncthreads();
+  const long npv = (long)v * (v + 1) / 2;
+#pragma unroll
+  for (int k = 0; k < 32; k += 8) {
+    const long c = c0 + tx, d = d0 + ty + k;
+    if (c < v && d < v && c <= d) {
+      const double x = Mcd[tx][ty + k], y = Mdc[ty + k][tx];
+      const long p = d * (d + 1) / 2 + c;
+      vp[r * npv + p] = x + y;
+      vm[r * npv + p] = x - y;
+    }
+  }
+}
+
+void ladder_vpm(double* vp, double* vm, const double* Y, long ldY, int a0, int a1, int v,
+                hipStream_t st) {
+  long rows = 0;
+  for (int a = a0; a < a1; ++a) rows += v - a;
+  CCGPU_REQUIRE(rows < 65536, "ladder_vpm: batch too large");
+  const long npv_ = (long)v * (v + 1) / 2;
+  check_span(vp, 0, rows * npv_ - 1, sizeof(double), "ladder_vpm vp");
+  check_span(vm, 0, rows * npv_ - 1, sizeof(double), "ladder_vpm vm");
+  check_span(Y, 0, ((long)(a1 - a0) * v - 1) * ldY + (long)(v - a0) * v - 1, sizeof(double), "ladder_vpm Y");
+  CCGPU_REQUIRE(ldY >= (long)(v - a0) * v, "ladder_vpm: ldY too small");
+  const long nt = (v + 31) / 32;
+  CCGPU_LAUNCH(k_ladder_vpm, dim3((unsigned)(nt * (nt + 1) / 2), (unsigned)rows), dim3(32, 8), 0,
+                     st, vp, vm, Y, ldY, a0, a1, v);
+  CCGPU_HIPX(hipGetLastError());
+}
+
+__global__ void k_ladder_scatter(double* __restrict__ om2, const double* __restrict__ sp,
+                                 const double* __restrict__ sm, int a0, int a1, int o, int v) {
+  const long r = blockIdx.y;
+  int a, b;
+  batch_row_decode(r, a0, a1, v, a, b);
+  const long npo = (long)o * (o + 1) / 2, nov = (long)o * v;
+  for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < npo; q += (long)gridDim.x * blockDim.x) {
+    long i, j;
+    tri_decode(q, i, j);
+    const double P = sp[r * npo + q], M = sm[r * npo + q];
+    om2[(i * v + a) * nov + j * v + b] += P + M;
+    if (i != j) om2[(j * v + a) * nov + i * v + b] += P - M;
+    if (a != b) om2[(i * v + b) * nov + j * v + a] += P - M;
+    if (i != j && a != b) om2[(j * v + b) * nov + i * v + a] += P + M;
+  }
+}
+
+void ladder_scatter(double* om2, const double* sp, const double* sm, int a0, int a1, int o, int v,
+                    hipStream_t st) {
+  long rows = 0;
+  for (int a = a0; a < a1; ++a) rows += v - a;
+  const long npo = (long)o * (o + 1) / 2;
+  const unsigned nbx = (unsigned)std::min<long>(64, (npo + 255) / 256);
+  CCGPU_REQUIRE(rows < 65536, "ladder_scatter: batch too large");
+  check_span(om2, 0, (long)o * v * o * v - 1, sizeof(double), "ladder_scatter om2");
+  check_span(sp, 0, rows * npo - 1, sizeof(double), "ladder_scatter sp");
+  check_span(sm, 0, rows * npo - 1, sizeof(double), "ladder_scatter sm");
+  CCGPU_LAUNCH(k_ladder_scatter, dim3(nbx, (unsigned)rows), dim3(256), 0, st, om2, sp, sm, a0,
+                     a1, o, v);
+  CCGPU_HIPX(hipGetLastError());
+}
+
+// ------------------------------------------------------------------ misc
+__global__ void k_coulomb_diag(double* Jd, const double* Bt, long naux, int N, int o) {
+  const long P = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (P >= naux) return;
+  double s = 0.0;
+  for (int k = 0; k < o; ++k) s += Bt[P * N * N + (long)k * N + k];
+  Jd[P] = 2.0 * s;
+}
+
+void coulomb_diag(double* Jd, const double* Bt, long naux, int N, int o, hipStream_t st) {
+  check_span(Jd, 0, naux - 1, sizeof(double), "coulomb_diag Jd");
+  check_span(Bt, 0, naux * N * N - 1, sizeof(double), "coulomb_diag Bt");
+  CCGPU_LAUNCH(k_coulomb_diag, dim3(cdiv(naux, 256)), dim3(256), 0, st, Jd, Bt, naux, N, o);
+  CCGPU_HIPX(hipGetLastError());
+}
+
+__global__ void k_fill(double* p, double val, long n) {
+  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x)
+    p[e] = val;
+}
+
+__global__ void k_axpy(double* y, double a, const double* x, long n) {
+  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x)
+    y[e] += a * x[e];
+}
+
+void fill(double* p, double val, long n, hipStream_t st) {
+  if (n <= 0) return;
+  check_span(p, 0, n - 1, sizeof(double), "fill");
+  CCGPU_LAUNCH(k_fill, dim3((unsigned)std::min<long>(65536, (n + 255) / 256)), dim3(256), 0, st, p,
+                     val, n);
+  CCGPU_HIPX(hipGetLastError());
+}
+
+void axpy(double* y, double a, const double* x, long n, hipStream_t st) {
+  if (n <= 0) return;
+  check_span(y, 0, n - 1, sizeof(double), "axpy y");
+  check_span(x, 0, n - 1, sizeof(double), "axpy x");
+  CCGPU_LAUNCH(k_axpy, dim3((unsigned)std::min<long>(65536, (n + 255) / 256)), dim3(256), 0, st, y,
+                     a, x, n);
+  CCGPU_HIPX(hipGetLastError());
+}
+
+}  // namespace ccgpu
+
+namespace ccgpu {
+__global__ void k_mp2(double* __restrict__ T, const double* __restrict__ J,
+                      const double* __restrict__ eps, int o, int v) {
+  const long nov = (long)o * v;
+  const long r = blockIdx.x;
+  const int i = (int)(r / v), a = (int)(r % v);
+  const double dia = eps[i] - eps[o + a];
+  for (long c = threadIdx.x; c < nov; c += blockDim.x) {
+    const int j = (int)(c / v), b = (int)(c % v);
+    T[r * nov + c] = J[r * nov + c] / (dia + eps[j] - eps[o + b]);
+  }
+}
+
+void mp2_guess(double* T, const double* J, const double* eps, int o, int v, hipStream_t st) {
+  const long nov = (long)o * v;
+  check_span(T, 0, nov * nov - 1, sizeof(double), "mp2 T");
+  check_span(J, 0, nov * nov - 1, sizeof(double), "mp2 J");
+  check_span(eps, 0, o + v - 1, sizeof(double), "mp2 eps");
+  CCGPU_LAUNCH(k_mp2, dim3((unsigned)nov), dim3(256), 0, st, T, J, eps, o, v);
+  CCGPU_HIPX(hipGetLastError());
+}
+}  // namespace ccgpu

@@ -1,0 +1,88 @@
+/*
+ * ccgpu: MI355X (gfx950) closed-shell coupled-cluster hot path, C ABI.
+ *
+ * Drop-in boundary for Dalton's closed-shell CCSD / (T) path.  All arrays
+ * are caller-owned host arrays in Dalton's own layouts:
+ *   - singles T1AM/OMEGA1: (a,i) with a fastest, NAI = a + NVIR*(i-1)
+ *     (cc/ccsd_energy.F, e.g. :1685-1690);
+ *   - doubles T2AM/OMEGA2: packed over ai <= bj, INDEX(ai,bj) =
+ *     bj*(bj-1)/2 + ai (1-based; statement function at cc/ccsd_energy.F:1666),
+ *     NT2AMX = NT1AMX*(NT1AMX+1)/2 elements (C1 symmetry);
+ *   - orbital energies: occupied first, then virtual (FOCKD in Dalton);
+ *   - two-electron integrals: a density-fitting / Cholesky factor B of shape
+ *     [naux][N][N] (row-major, N = nocc + nvir, symmetric in the last two
+ *     indices) with (pq|rs) = sum_P B[P][p][q] B[P][r][s].  Dalton's CC code
+ *     reads AO integral distributions instead (CCRDAO, cc/ccsd_rhs.F:5652);
+ *     the factor replaces that source.
+ * Every function returns 0 on success and a nonzero code otherwise
+ * (1 = bad argument, 2 = HIP runtime error, 3 = numerical failure such as a
+ * singular DIIS matrix, 4 = not converged); ccgpu_last_error() gives the
+ * message.  A Fortran caller maps nonzero codes to QUIT (gp/gptrygve.F:256),
+ * which is Dalton's only error convention.
+ * Calls on one context are synchronous and must not overlap.
+ */
+#ifndef CCGPU_H
+#define CCGPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ccgpu_ctx ccgpu_ctx;
+
+/* precision selectors */
+#define CCGPU_FP64 0  /* all contractions in FP64 on MFMA */
+#define CCGPU_MIXED 1 /* bulk contractions FP32 / BF16x3 on MFMA, FP64 update/DIIS/energy */
+
+int ccgpu_version(void);
+const char* ccgpu_last_error(void);
+
+/* Registers the Hamiltonian (f = diag(eps), integrals from B) and allocates
+ * device memory for one closed-shell system.  Replaces the setup done in
+ * CCSD_ENERGY before its loop (orbital energies read from SIRIFC,
+ * cc/ccsd_energy.F:504-518; (ia|jb) built by CCSD_IAJB, cc/ccsd_energy.F:2328).
+ * mxdiis: DIIS subspace size (Dalton .MXDIIS, default 8, cc/ccsd_init0.F:179-186). */
+int ccgpu_init(int nocc, int nvir, int naux, const double* eps, const double* B, int mxdiis,
+               ccgpu_ctx** ctx);
+int ccgpu_finalize(ccgpu_ctx* ctx);
+
+/* CCSD vector function: replaces
+ *   SUBROUTINE CCRHSN(OMEGA1,OMEGA2,T1AM,T2AM,WORK,LWORK,APROXR12)
+ * cc/ccsd_rhs.F:20 (plain closed-shell CCSD, APROXR12 blank).  T1AM/T2AM
+ * are read only (Dalton squares T2AM in place and restores it; no caller
+ * workspace is needed here).  Also replaces the context's current amplitudes. */
+int ccgpu_rhs(ccgpu_ctx* ctx, const double* t1am, const double* t2am, double* omega1,
+              double* omega2, int precision);
+
+/* MP2 starting guess and its energy: CCSD_GUESS (cc/ccsd_energy.F:1654) followed
+ * by CCSD_ECCSD (cc/ccsd_energy.F:1964).  e_mp2 is the correlation energy. */
+int ccgpu_ccsd_guess(ccgpu_ctx* ctx, double* e_mp2);
+
+/* One pass of the CCSD_ENERGY iteration (cc/ccsd_energy.F:742-982): CCRHSN,
+ * the 2-norm of the vector function, CCSD_NXTAM (:1814), CCSD_DIIS
+ * (cc/ccsd_diis.F:251, restart every MXDIIS vectors) and CCSD_ECCSD.
+ * e_corr: new correlation energy; omega_norm: norm of the residual that was
+ * used for this step (Dalton's OMNM). */
+int ccgpu_ccsd_iterate(ccgpu_ctx* ctx, int precision, double* e_corr, double* omega_norm);
+
+/* The whole loop: guess, then iterate until |dE| < threnr and norm < thrvec
+ * (cc/ccsd_energy.F:946-949) or maxit (returns 4, "CC equations not
+ * converged", :952-956).  history (may be NULL) receives maxit energies. */
+int ccgpu_ccsd_energy(ccgpu_ctx* ctx, int maxit, double threnr, double thrvec, int precision,
+                      double* e_mp2, double* e_ccsd, int* niter, double* history);
+
+/* Amplitudes in Dalton's layouts (CC_WRRSP/CC_RDRSP 'R0' content, cc/cc_wrrsp.F:20). */
+int ccgpu_get_amplitudes(ccgpu_ctx* ctx, double* t1am, double* t2am);
+int ccgpu_set_amplitudes(ccgpu_ctx* ctx, const double* t1am, const double* t2am);
+
+/* DGEMM with BLAS semantics (column-major, host arrays), the operation
+ * Dalton's hot path spends its time in (pdpack/gp_blas.F DGEMM, e.g. the
+ * ladder DGEMMs at cc/ccsd_rhs.F:9589-9597), run on the FP64 MFMA kernel. */
+int ccgpu_dgemm(char transa, char transb, int m, int n, int k, double alpha, const double* a,
+                int lda, const double* b, int ldb, double beta, double* c, int ldc);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CCGPU_H */

@@ -1,0 +1,52 @@
+"""Host orchestration bounds check without a GPU.
+
+With CCGPU_DRYRUN=1 the library allocates fake device addresses, skips every
+launch and copy, and checks the address span of every GEMM operand and
+layout-kernel argument against the allocation it points into.  Running the
+full residual / iteration sequence this way proves, on the CPU, that no
+kernel of the path indexes outside its buffers at these shapes (including
+the largest single-GPU configuration)."""
+import os
+import subprocess
+import sys
+
+import pytest
+
+from conftest import ROOT
+
+SCRIPT = r"""
+import ctypes as C, sys
+import numpy as np
+sys.path.insert(0, {root!r})
+from dalton_amd import _lib
+L = _lib.lib()
+for (o, v, naux) in {shapes!r}:
+    h = C.c_void_p()
+    rc = L.ccgpu_init(o, v, naux, None, None, 8, C.byref(h))
+    assert rc == 0, L.ccgpu_last_error()
+    e = C.c_double(); n = C.c_double()
+    assert L.ccgpu_ccsd_guess(h, C.byref(e)) == 0, L.ccgpu_last_error()
+    for it in range(10):
+        rc = L.ccgpu_ccsd_iterate(h, 0, C.byref(e), C.byref(n))
+        assert rc == 0, L.ccgpu_last_error()
+    nt1 = o * v; nt2 = nt1 * (nt1 + 1) // 2
+    if nt2 < 5e7:
+        a1 = np.zeros(nt1); a2 = np.zeros(nt2)
+        rc = L.ccgpu_rhs(h, _lib.ptr(a1), _lib.ptr(a2), _lib.ptr(a1), _lib.ptr(a2), 0)
+        assert rc == 0, L.ccgpu_last_error()
+    assert L.ccgpu_finalize(h) == 0
+    print("ok", o, v, naux, flush=True)
+"""
+
+
+@pytest.mark.parametrize("shapes", [
+    [(5, 19, 280), (5, 14, 172), (1, 1, 1), (6, 40, 90), (3, 200, 10)],
+    [(32, 256, 1024)],
+    [(64, 512, 2048)],
+])
+def test_dryrun_bounds(shapes):
+    env = dict(os.environ, CCGPU_DRYRUN="1")
+    r = subprocess.run([sys.executable, "-c", SCRIPT.format(root=ROOT, shapes=shapes)],
+                       env=env, capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert r.stdout.count("ok") == len(shapes)

@@ -1,0 +1,76 @@
+"""HIP CCSD path against the oracle and against Dalton's printed trajectories."""
+import numpy as np
+import pytest
+
+from conftest import load_molecule
+from oracle import ccsd as occ
+
+pytestmark = pytest.mark.gpu
+
+
+def _ctx(eps, B, o):
+    import dalton_amd as d
+
+    return d.CCContext(eps, B, o)
+
+
+@pytest.mark.parametrize("name", ["h2o_ccpvdz_test", "synthetic_o6_v40"])
+def test_residual_matches_oracle(name):
+    if name.startswith("synthetic"):
+        from dalton_amd.synthetic import make_system
+
+        eps, B = make_system(6, 40, 90, seed=7)
+        o = 6
+    else:
+        eps, B, o, _ = load_molecule(name)
+    print(f"rhs parity {name}", flush=True)
+    v = eps.size - o
+    rng = np.random.default_rng(4)
+    t1 = 0.02 * rng.standard_normal((v, o))
+    t2 = 0.02 * rng.standard_normal((v, o, v, o))
+    t2 = t2 + t2.transpose(2, 3, 0, 1)
+    sysm = occ.DFSystem(eps, B, o)
+    r1, r2 = occ.residual(sysm, t1, t2)
+    with _ctx(eps, B, o) as c:
+        om1, om2 = c.rhs(occ.pack_t1(t1), occ.pack_t2(t2))
+    ref1, ref2 = occ.pack_t1(r1), occ.pack_t2(r2)
+    s = max(1.0, np.abs(ref2).max())
+    assert np.abs(om1 - ref1).max() < 1e-12 * s
+    assert np.abs(om2 - ref2).max() < 1e-12 * s
+
+
+@pytest.mark.parametrize("name,threnr", [("h2o_ccpvdz_test", 1e-12), ("hf_631gss", 1e-10)])
+def test_ccsd_trajectory_matches_dalton(name, threnr, golden):
+    eps, B, o, _ = load_molecule(name)
+    print(f"ccsd trajectory {name}", flush=True)
+    g = golden[name]
+    with _ctx(eps, B, o) as c:
+        res = c.ccsd_energy(maxit=60, threnr=threnr)
+    assert abs(res["e_mp2"] + g["e_scf"] - g["e_mp2"]) < 5e-12
+    hist = res["history"] + g["e_scf"]
+    ref = np.array(g["ccsd_iterations"])
+    assert res["niter"] == ref.size
+    assert np.abs(hist - ref).max() < 5e-12
+    assert abs(res["e_ccsd"] + g["e_scf"] - g["e_ccsd"]) < 6e-11
+
+
+def test_synthetic_iterations_match_oracle():
+    from dalton_amd.synthetic import make_system
+
+    o, v, naux = 8, 56, 120
+    eps, B = make_system(o, v, naux, seed=2)
+    print(f"synthetic iterations o={o} v={v} naux={naux}", flush=True)
+    sysm = occ.DFSystem(eps, B, o)
+    hist = []
+    try:
+        occ.solve(sysm, maxiter=10, threnr=1e-30, callback=lambda it, e, n: hist.append((e, n)))
+    except RuntimeError:
+        pass
+    with _ctx(eps, B, o) as c:
+        emp2 = c.guess()
+        got = [c.iterate() for _ in range(10)]
+        t1, t2 = c.amplitudes()
+    assert len(hist) == 10
+    for (e, n), (ge, gn) in zip(hist, got):
+        assert abs(e - ge) < 1e-10 * max(1.0, abs(e))
+        assert abs(n - gn) < 1e-9 * max(1.0, n)
