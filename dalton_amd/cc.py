@@ -118,6 +118,24 @@ class CCContext:
         check(lib().ccgpu_set_amplitudes(self._h, ptr(t1), ptr(t2p)))
 
 
+def profile_start():
+    check(lib().ccgpu_profile_start())
+
+
+def profile_stop() -> dict:
+    """Stop launch/phase timing; returns {"kernels": {...}, "phases": {...}}."""
+    import json
+
+    cap = 1 << 20
+    buf = C.create_string_buffer(cap)
+    n = lib().ccgpu_profile_stop(buf, cap)
+    if n < 0:
+        check(-n)
+    if n > cap:
+        raise RuntimeError("profile report larger than 1 MiB")
+    return json.loads(buf.value.decode())
+
+
 def dgemm(transa: str, transb: str, alpha: float, A: np.ndarray, B: np.ndarray, beta: float = 0.0,
           Cm: np.ndarray | None = None) -> np.ndarray:
     """BLAS DGEMM on the MFMA kernel; arrays are Fortran-ordered matrices."""

@@ -122,6 +122,22 @@ int ccgpu_set_amplitudes(ccgpu_ctx* ctx, const double* t1am, const double* t2am)
   });
 }
 
+int ccgpu_profile_start(void) {
+  return guarded([&] { ccgpu::prof_start(); });
+}
+
+int ccgpu_profile_stop(char* buf, int len) {
+  std::string js;
+  const int rc = guarded([&] { js = ccgpu::prof_stop_json(); });
+  if (rc != 0) return -rc;
+  if (buf && len > 0) {
+    const size_t n = std::min<size_t>(js.size(), (size_t)len - 1);
+    std::memcpy(buf, js.data(), n);
+    buf[n] = '\0';
+  }
+  return static_cast<int>(js.size()) + 1;
+}
+
 int ccgpu_dgemm(char transa, char transb, int m, int n, int k, double alpha, const double* a, int lda,
                 const double* b, int ldb, double beta, double* c, int ldc) {
   return guarded([&] {

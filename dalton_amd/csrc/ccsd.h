@@ -46,10 +46,6 @@ class CCSD {
   const double* B() const { return B_.p; }
   const double* J() const { return J_.p; }
 
-  // per-term timing (ms) of the last residual, for the bench
-  std::vector<std::pair<const char*, float>> timings;
-  bool time_terms = false;
-
  private:
   void dress(const double* t1);
   void ladder(const double* T, double* Om);
@@ -59,8 +55,6 @@ class CCSD {
   int mxdiis_;
   int iter_ = 0;
   hipStream_t st_ = nullptr;
-  std::vector<hipEvent_t> ev_;
-  std::vector<const char*> evname_;
 
   // inputs / constants
   DBuf<double> eps_, B_, h_, Bov_, J_, K_, L_, Joovv_;

@@ -183,18 +183,10 @@ CCSD::CCSD(int o, int v, int naux, const double* eps_host, const double* B_host,
 
 CCSD::~CCSD() {
   if (dryrun()) return;
-  for (auto e : ev_) (void)hipEventDestroy(e);
   if (st_) (void)hipStreamDestroy(st_);
 }
 
-void CCSD::mark(const char* name) {
-  if (!time_terms || dryrun()) return;
-  hipEvent_t e;
-  CCGPU_HIP(hipEventCreate(&e));
-  CCGPU_HIP(hipEventRecord(e, st_));
-  ev_.push_back(e);
-  evname_.push_back(name);
-}
+void CCSD::mark(const char* name) { prof_phase(name, st_); }
 
 // T1 transformation of B and h, then the dressed Fock matrix
 // (LAMMAT/LAMDA1 cc/ccsd_rhs.F:5887,6030; CC_AOFOCK/CC_FCKMO :3388,2724).
@@ -246,10 +238,6 @@ void CCSD::ladder(const double* T, double* Om) {
 void CCSD::residual(const double* t1, const double* T, double* om1, double* Om) {
   const int o = dm_.o, v = dm_.v, naux = dm_.naux;
   const long N = dm_.N, nov = dm_.nov, nov2 = dm_.nov2, oo = (long)o * o, vv = (long)v * v;
-  if (!dryrun())
-    for (auto e : ev_) (void)hipEventDestroy(e);
-  ev_.clear();
-  evname_.clear();
   mark("start");
   dress(t1);
   mark("dress");
@@ -324,16 +312,6 @@ void CCSD::residual(const double* t1, const double* T, double* om1, double* Om) 
   gemv_mc(nov, nov, 1.0, U, nov, hx_.p, 1.0, om1, st_);
   perm(om1, F_.p + o * N, 1, 1, o, v, 0, 0, 1, N, 1.0, 1.0, st_);  // F~_ai
   mark("singles");
-
-  if (time_terms && !dryrun()) {
-    CCGPU_HIPX(hipStreamSynchronize(st_));
-    timings.clear();
-    for (size_t i = 1; i < ev_.size(); ++i) {
-      float ms = 0.f;
-      CCGPU_HIP(hipEventElapsedTime(&ms, ev_[i - 1], ev_[i]));
-      timings.emplace_back(evname_[i], ms);
-    }
-  }
 }
 
 double CCSD::mp2_start() {

@@ -6,6 +6,7 @@
 #include <cstdio>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 namespace ccgpu {
 
@@ -119,4 +120,18 @@ void gemm(const GemmDesc& d, GemmPrec prec, hipStream_t s);
 void gemm_rm(bool ta, bool tb, long M, long N, long K, double alpha, const void* A, long lda,
              const void* B, long ldb, double beta, void* C, long ldc, GemmPrec prec, hipStream_t s);
 
+}  // namespace ccgpu
+
+namespace ccgpu {
+// ---------------------------------------------------------------- profiling
+// Optional HIP-event timing of individual launches (per kernel instance, with
+// the algorithmic flop count of each launch) and of residual phases, on the
+// stream the work is issued to.  Off unless ccgpu_profile_start() was called.
+bool profiling();
+void prof_start();
+// Records a launch interval: call prof_begin before and prof_end after the launch.
+int prof_begin(const char* kernel, double flops, hipStream_t s);
+void prof_end(int id, hipStream_t s);
+void prof_phase(const char* name, hipStream_t s);  // phase boundary marker
+std::string prof_stop_json();                       // synchronises, aggregates, clears
 }  // namespace ccgpu

@@ -27,14 +27,50 @@ __device__ __forceinline__ long koff(long k, long s_k, long s_ko, long Kin) {
   return k * s_k;
 }
 
+// Logical tile for a linear block id.  Workgroups are dispatched round-robin
+// over the 8 XCDs (block b runs on XCD b % 8), each with its own L2: give each
+// XCD a contiguous range of tiles, and walk that range in groups of 8 m-tiles
+// so the workgroups resident on one XCD share A and B panels in its L2.
+__device__ __forceinline__ void tile_of(long b, long tiles_m, long tiles_n, long& tm, long& tn) {
+  const long nt = tiles_m * tiles_n;
+  const long per = nt / 8, full = per * 8;
+  const long lg = (b < full) ? (b % 8) * per + b / 8 : b;
+  constexpr long GM = 8;
+  const long grp = lg / (GM * tiles_n);
+  const long fm = grp * GM;
+  const long gsz = min(tiles_m - fm, GM);
+  const long r = lg - grp * GM * tiles_n;
+  tm = fm + r % gsz;
+  tn = r / gsz;
+}
+
+// LDS staging: an operand loaded k-fastest is stored [row][k] (stride TK+1,
+// conflict-free for both the stores and the 16x4 fragment reads); one loaded
+// row-fastest is stored [k][row] (stride 128+16).
+template <int MODE>
+struct Stage;
+template <>
+struct Stage<KFAST> {
+  double s[128][17];
+  __device__ __forceinline__ double& at(int r, int k) { return s[r][k]; }
+};
+template <>
+struct Stage<RFAST> {
+  double s[16][144];
+  __device__ __forceinline__ double& at(int r, int k) { return s[k][r]; }
+};
+
 template <int AMODE, int BMODE, bool K2>
 __global__ __launch_bounds__(256, 2) void k_dgemm(GemmDesc d, long kchunk, double* ws) {
-  constexpr int TM = 128, TN = 128, TK = 16, PAD = 16;
-  __shared__ double As[TK][TM + PAD];
-  __shared__ double Bs[TK][TN + PAD];
+  constexpr int TM = 128, TN = 128, TK = 16;
+  __shared__ Stage<AMODE> As;
+  __shared__ Stage<BMODE> Bs;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
-  const long m0 = (long)blockIdx.y * TM, n0 = (long)blockIdx.x * TN;
+  const long tiles_m = (d.M + TM - 1) / TM, tiles_n = (d.N + TN - 1) / TN;
+  long tm, tn;
+  tile_of(blockIdx.x, tiles_m, tiles_n, tm, tn);
+  const long m0 = tm * TM, n0 = tn * TN;
   const double* A = static_cast<const double*>(d.A);
   const double* B = static_cast<const double*>(d.B);
   long kbeg = 0, kend = d.K;
@@ -47,22 +83,50 @@ __global__ __launch_bounds__(256, 2) void k_dgemm(GemmDesc d, long kchunk, doubl
   }
 
   double ra[8], rb[8];
+  // k-fastest: thread owns column c = tid % 16 and rows tid/16 + 16 i;
+  // row-fastest: thread owns row tid % 128 and columns tid/128 + 2 i.
   auto load = [&](long k0) {
+    if (AMODE == KFAST) {
+      const long k = k0 + (tid % TK);
+      const bool kok = k < kend;
+      const long ko = kok ? koff<K2>(k, d.a_k, d.a_ko, d.Kin) : 0;
+      const long mb = m0 + tid / TK;
 #pragma unroll
-    for (int i = 0; i < 8; ++i) {
-      const int e = tid + i * 256;
-      int r, c;
-      if (AMODE == KFAST) { r = e / TK; c = e % TK; } else { r = e % TM; c = e / TM; }
-      const long m = m0 + r, k = k0 + c;
-      ra[i] = (m < d.M && k < kend) ? A[m * d.a_m + koff<K2>(k, d.a_k, d.a_ko, d.Kin)] : 0.0;
+      for (int i = 0; i < 8; ++i) {
+        const long m = mb + 16 * i;
+        ra[i] = (kok && m < d.M) ? A[m * d.a_m + ko] : 0.0;
+      }
+    } else {
+      const long m = m0 + (tid % TM);
+      const bool mok = m < d.M;
+      const double* Ar = A + (mok ? m : 0) * d.a_m;
+      const long kb = k0 + tid / TM;
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        const long k = kb + 2 * i;
+        ra[i] = (mok && k < kend) ? Ar[koff<K2>(k, d.a_k, d.a_ko, d.Kin)] : 0.0;
+      }
     }
+    if (BMODE == KFAST) {
+      const long k = k0 + (tid % TK);
+      const bool kok = k < kend;
+      const long ko = kok ? koff<K2>(k, d.b_k, d.b_ko, d.Kin) : 0;
+      const long nb = n0 + tid / TK;
 #pragma unroll
-    for (int i = 0; i < 8; ++i) {
-      const int e = tid + i * 256;
-      int r, c;
-      if (BMODE == KFAST) { r = e / TK; c = e % TK; } else { r = e % TN; c = e / TN; }
-      const long n = n0 + r, k = k0 + c;
-      rb[i] = (n < d.N && k < kend) ? B[n * d.b_n + koff<K2>(k, d.b_k, d.b_ko, d.Kin)] : 0.0;
+      for (int i = 0; i < 8; ++i) {
+        const long n = nb + 16 * i;
+        rb[i] = (kok && n < d.N) ? B[n * d.b_n + ko] : 0.0;
+      }
+    } else {
+      const long n = n0 + (tid % TN);
+      const bool nok = n < d.N;
+      const double* Br = B + (nok ? n : 0) * d.b_n;
+      const long kb = k0 + tid / TN;
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        const long k = kb + 2 * i;
+        rb[i] = (nok && k < kend) ? Br[koff<K2>(k, d.b_k, d.b_ko, d.Kin)] : 0.0;
+      }
     }
   };
   auto store = [&]() {
@@ -71,14 +135,14 @@ __global__ __launch_bounds__(256, 2) void k_dgemm(GemmDesc d, long kchunk, doubl
       const int e = tid + i * 256;
       int r, c;
       if (AMODE == KFAST) { r = e / TK; c = e % TK; } else { r = e % TM; c = e / TM; }
-      As[c][r] = ra[i];
+      As.at(r, c) = ra[i];
     }
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
       const int e = tid + i * 256;
       int r, c;
       if (BMODE == KFAST) { r = e / TK; c = e % TK; } else { r = e % TN; c = e / TN; }
-      Bs[c][r] = rb[i];
+      Bs.at(r, c) = rb[i];
     }
   };
 
@@ -99,9 +163,9 @@ __global__ __launch_bounds__(256, 2) void k_dgemm(GemmDesc d, long kchunk, doubl
     for (int kk = 0; kk < TK / 4; ++kk) {
       double a[4], b[4];
 #pragma unroll
-      for (int t = 0; t < 4; ++t) a[t] = As[kk * 4 + lk][wm * 64 + t * 16 + lr];
+      for (int t = 0; t < 4; ++t) a[t] = As.at(wm * 64 + t * 16 + lr, kk * 4 + lk);
 #pragma unroll
-      for (int t = 0; t < 4; ++t) b[t] = Bs[kk * 4 + lk][wn * 64 + t * 16 + lr];
+      for (int t = 0; t < 4; ++t) b[t] = Bs.at(wn * 64 + t * 16 + lr, kk * 4 + lk);
 #pragma unroll
       for (int i = 0; i < 4; ++i)
 #pragma unroll
@@ -167,13 +231,19 @@ DBuf<double>& g_ws_ref() {
 }
 #define g_ws g_ws_ref()
 
+const char* const kDgemmName[2][2][2] = {
+    {{"k_dgemm<0,0,false>", "k_dgemm<0,0,true>"}, {"k_dgemm<0,1,false>", "k_dgemm<0,1,true>"}},
+    {{"k_dgemm<1,0,false>", "k_dgemm<1,0,true>"}, {"k_dgemm<1,1,false>", "k_dgemm<1,1,true>"}}};
+
 template <int AM, int BM>
 void launch_f64(const GemmDesc& d, dim3 grid, long kchunk, double* ws, hipStream_t s) {
   const bool k2 = d.Kin > 0 && d.Kin < d.K;
+  const int id = prof_begin(kDgemmName[AM][BM][k2 ? 1 : 0], 2.0 * d.M * d.N * d.K * d.batch, s);
   if (k2)
     CCGPU_LAUNCH((k_dgemm<AM, BM, true>), grid, dim3(256), 0, s, d, kchunk, ws);
   else
     CCGPU_LAUNCH((k_dgemm<AM, BM, false>), grid, dim3(256), 0, s, d, kchunk, ws);
+  prof_end(id, s);
 }
 }  // namespace
 
@@ -200,7 +270,6 @@ void check_gemm(const GemmDesc& d, size_t esz) {
 void gemm_f64(const GemmDesc& d, hipStream_t s) {
   if (d.M <= 0 || d.N <= 0 || d.batch <= 0) return;
   check_gemm(d, sizeof(double));
-  CCGPU_REQUIRE(d.M / 128 < 65535, "gemm: M too large for grid.y");
   const int am = (d.a_m == 1 && d.a_k != 1) ? RFAST : KFAST;
   const int bm = (d.b_n == 1 && d.b_k != 1) ? RFAST : KFAST;
   const long tiles = (long)cdiv(d.M, 128) * cdiv(d.N, 128);
@@ -224,7 +293,9 @@ void gemm_f64(const GemmDesc& d, hipStream_t s) {
     }
   }
   CCGPU_REQUIRE(nz <= 65535, "gemm: too many batches");
-  dim3 grid(cdiv(d.N, 128), cdiv(d.M, 128), nz);
+  const long ntiles = (long)cdiv(d.M, 128) * cdiv(d.N, 128);
+  CCGPU_REQUIRE(ntiles < (1L << 31), "gemm: too many tiles");
+  dim3 grid((unsigned)ntiles, 1, nz);
   if (am == KFAST && bm == KFAST) launch_f64<KFAST, KFAST>(d, grid, kchunk, ws, s);
   else if (am == KFAST) launch_f64<KFAST, RFAST>(d, grid, kchunk, ws, s);
   else if (bm == KFAST) launch_f64<RFAST, KFAST>(d, grid, kchunk, ws, s);

@@ -62,3 +62,110 @@ void check_span(const void* p, long lo, long hi, size_t esz, const char* what) {
 }
 
 }  // namespace ccgpu
+
+namespace ccgpu {
+namespace {
+struct LaunchRec {
+  const char* name;
+  double flops;
+  hipEvent_t e0, e1;
+};
+struct PhaseRec {
+  const char* name;
+  hipEvent_t e;
+};
+struct Prof {
+  bool on = false;
+  std::vector<LaunchRec> launches;
+  std::vector<PhaseRec> phases;
+};
+Prof& prof() {
+  static auto* p = new Prof;
+  return *p;
+}
+hipEvent_t new_event(hipStream_t s) {
+  hipEvent_t e;
+  CCGPU_HIP(hipEventCreate(&e));
+  CCGPU_HIP(hipEventRecord(e, s));
+  return e;
+}
+}  // namespace
+
+bool profiling() { return prof().on && !dryrun(); }
+
+void prof_start() {
+  Prof& p = prof();
+  for (auto& l : p.launches) {
+    (void)hipEventDestroy(l.e0);
+    (void)hipEventDestroy(l.e1);
+  }
+  for (auto& f : p.phases) (void)hipEventDestroy(f.e);
+  p.launches.clear();
+  p.phases.clear();
+  p.on = true;
+}
+
+int prof_begin(const char* kernel, double flops, hipStream_t s) {
+  if (!profiling()) return -1;
+  Prof& p = prof();
+  p.launches.push_back({kernel, flops, new_event(s), nullptr});
+  return static_cast<int>(p.launches.size()) - 1;
+}
+
+void prof_end(int id, hipStream_t s) {
+  if (id < 0 || !profiling()) return;
+  prof().launches[id].e1 = new_event(s);
+}
+
+void prof_phase(const char* name, hipStream_t s) {
+  if (!profiling()) return;
+  prof().phases.push_back({name, new_event(s)});
+}
+
+std::string prof_stop_json() {
+  Prof& p = prof();
+  if (dryrun()) return "{}";
+  CCGPU_HIP(hipDeviceSynchronize());
+  struct Agg {
+    long n = 0;
+    double ms = 0, flops = 0;
+  };
+  std::map<std::string, Agg> kern;
+  for (auto& l : p.launches) {
+    float ms = 0.f;
+    if (l.e1) CCGPU_HIP(hipEventElapsedTime(&ms, l.e0, l.e1));
+    Agg& a = kern[l.name];
+    a.n += 1;
+    a.ms += ms;
+    a.flops += l.flops;
+  }
+  std::map<std::string, double> ph;
+  for (size_t i = 1; i < p.phases.size(); ++i) {
+    if (std::string(p.phases[i].name) == "start") continue;
+    float ms = 0.f;
+    CCGPU_HIP(hipEventElapsedTime(&ms, p.phases[i - 1].e, p.phases[i].e));
+    ph[p.phases[i].name] += ms;
+  }
+  std::string js = "{\"kernels\": {";
+  bool first = true;
+  for (auto& kv : kern) {
+    char buf[512];
+    snprintf(buf, sizeof buf, "%s\"%s\": {\"count\": %ld, \"ms\": %.6f, \"flops\": %.6e}", first ? "" : ", ",
+             kv.first.c_str(), kv.second.n, kv.second.ms, kv.second.flops);
+    js += buf;
+    first = false;
+  }
+  js += "}, \"phases\": {";
+  first = true;
+  for (auto& kv : ph) {
+    char buf[256];
+    snprintf(buf, sizeof buf, "%s\"%s\": %.6f", first ? "" : ", ", kv.first.c_str(), kv.second);
+    js += buf;
+    first = false;
+  }
+  js += "}}";
+  prof_start();
+  p.on = false;
+  return js;
+}
+}  // namespace ccgpu

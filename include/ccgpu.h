@@ -75,6 +75,16 @@ int ccgpu_ccsd_energy(ccgpu_ctx* ctx, int maxit, double threnr, double thrvec, i
 int ccgpu_get_amplitudes(ccgpu_ctx* ctx, double* t1am, double* t2am);
 int ccgpu_set_amplitudes(ccgpu_ctx* ctx, const double* t1am, const double* t2am);
 
+/* Per-kernel and per-term timing, the counterpart of CCRHSN's per-term
+ * timers (cc/ccsd_rhs.F:2161-2232, printed at IPRINT > 9).  Between start and
+ * stop every MFMA GEMM launch and every residual phase is bracketed by HIP
+ * events on the stream it runs on.  stop synchronises and writes a JSON
+ * summary {"kernels": {name: {count, ms, flops}}, "phases": {name: ms}} into
+ * buf (NUL-terminated, truncated to len); it returns the full length needed,
+ * or a negative error code. */
+int ccgpu_profile_start(void);
+int ccgpu_profile_stop(char* buf, int len);
+
 /* DGEMM with BLAS semantics (column-major, host arrays), the operation
  * Dalton's hot path spends its time in (pdpack/gp_blas.F DGEMM, e.g. the
  * ladder DGEMMs at cc/ccsd_rhs.F:9589-9597), run on the FP64 MFMA kernel. */

@@ -127,3 +127,19 @@ def test_packing_matches_dalton_index_function():
                     assert p[idx - 1] == T[a, i, b, j]
     assert np.array_equal(ccsd.unpack_t2(p, o, v), T)
     assert p.size == nov * (nov + 1) // 2
+
+
+def test_blas_residual_matches_einsum_oracle():
+    from oracle import ccsd_blas
+    from dalton_amd.synthetic import make_system
+
+    for (o, v, naux, rows) in [(5, 19, 40, 50), (6, 40, 90, 4096), (7, 33, 50, 7)]:
+        eps, B = make_system(o, v, naux, 3)
+        s = ccsd.DFSystem(eps, B, o)
+        rng = np.random.default_rng(1)
+        t1 = 0.02 * rng.standard_normal((v, o))
+        t2 = 0.02 * rng.standard_normal((v, o, v, o))
+        t2 = t2 + t2.transpose(2, 3, 0, 1)
+        a1, a2 = ccsd.residual(s, t1, t2)
+        b1, b2 = ccsd_blas.residual(s, t1, t2, ladder_rows=rows)
+        assert np.abs(a1 - b1).max() < 1e-13 and np.abs(a2 - b2).max() < 1e-13
