@@ -95,6 +95,8 @@ def main():
     ap.add_argument("--naux", type=int, default=2048)
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--pt-triples", type=int, default=24,
+                    help="(T) triples timed, spread evenly over the i>=j>=k list (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; --gpus must be 1 "
@@ -127,6 +129,29 @@ def main():
     sync()
     dt = time.perf_counter() - t0
     prof = profile_stop()
+
+    pt = None
+    if args.pt_triples > 0:
+        _, _, _, ntr = ctx.pt_partial(0, 0, 1)  # build (T) integral layouts (untimed setup)
+        stride = max(1, ntr // args.pt_triples)
+        sync()
+        profile_start()
+        t1 = time.perf_counter()
+        e4, e5, fl, _ = ctx.pt_partial(0, args.pt_triples, stride)
+        sync()
+        dtp = time.perf_counter() - t1
+        pprof = profile_stop()
+        pk = pprof["kernels"]
+        pdom = max(pk, key=lambda k: pk[k]["ms"])
+        pt = {"triples_timed": args.pt_triples, "triples_total": ntr, "stride": stride,
+              "seconds": round(dtp, 4), "tflops": round(fl / dtp / 1e12, 3),
+              "est_full_seconds": round(dtp / args.pt_triples * ntr, 1),
+              "flops_per_triple": fl / args.pt_triples, "e4_sample": e4, "e5_sample": e5,
+              "dominant_kernel": pdom,
+              "kernels": {k: {"count": v_["count"], "ms": round(v_["ms"], 3),
+                              "tflops": round(v_["flops"] / max(v_["ms"], 1e-9) / 1e9, 2)} for k, v_ in pk.items()}}
+        print(f"[bench] (T): {args.pt_triples} triples in {dtp:.3f} s, {fl / dtp / 1e12:.2f} TFLOP/s", file=sys.stderr,
+              flush=True)
     ctx.close()
 
     spi = dt / args.steps
@@ -165,6 +190,7 @@ def main():
         "ccsd_tflops": round(F / spi / 1e12, 3),
         "ccsd_flops_per_iter": F,
         "energies": hist,
+        "pt": pt,
         "roofline": roof,
         "phases_ms_per_iter": {k: round(v_ / args.steps, 2) for k, v_ in prof["phases"].items()},
         "kernels": {k: {"count": v_["count"], "ms": round(v_["ms"], 3),

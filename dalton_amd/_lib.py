@@ -27,6 +27,9 @@ SIGNATURES = {
                                     C.POINTER(C.c_int), _dp]),
     "ccgpu_get_amplitudes": (C.c_int, [C.c_void_p, _dp, _dp]),
     "ccgpu_set_amplitudes": (C.c_int, [C.c_void_p, _dp, _dp]),
+    "ccgpu_pt": (C.c_int, [C.c_void_p, C.c_int, _dp, _dp]),
+    "ccgpu_pt_partial": (C.c_int, [C.c_void_p, C.c_int, C.c_longlong, C.c_longlong, C.c_longlong, _dp, _dp, _dp,
+                                   C.POINTER(C.c_longlong)]),
     "ccgpu_profile_start": (C.c_int, []),
     "ccgpu_profile_stop": (C.c_int, [C.c_char_p, C.c_int]),
     "ccgpu_dgemm": (C.c_int, [C.c_char, C.c_char, C.c_int, C.c_int, C.c_int, C.c_double, _dp, C.c_int,

@@ -106,6 +106,19 @@ class CCContext:
                                       C.byref(e), C.byref(n), ptr(hist)))
         return {"e_mp2": emp2.value, "e_ccsd": e.value, "niter": n.value, "history": hist[: n.value].copy()}
 
+    def pt(self, precision: int = FP64):
+        """(T) from the current amplitudes: returns (E4, E5)."""
+        e4, e5 = C.c_double(), C.c_double()
+        check(lib().ccgpu_pt(self._h, precision, C.byref(e4), C.byref(e5)))
+        return e4.value, e5.value
+
+    def pt_partial(self, first: int, count: int, stride: int = 1, precision: int = FP64):
+        """(T) over part of the restricted triple list: (E4, E5, flops, ntriples)."""
+        e4, e5, fl, nt = C.c_double(), C.c_double(), C.c_double(), C.c_longlong()
+        check(lib().ccgpu_pt_partial(self._h, precision, first, count, stride, C.byref(e4), C.byref(e5),
+                                     C.byref(fl), C.byref(nt)))
+        return e4.value, e5.value, fl.value, nt.value
+
     def amplitudes(self):
         t1 = np.empty(self.nt1)
         t2 = np.empty(self.nt2)

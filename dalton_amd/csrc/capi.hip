@@ -122,6 +122,24 @@ int ccgpu_set_amplitudes(ccgpu_ctx* ctx, const double* t1am, const double* t2am)
   });
 }
 
+int ccgpu_pt(ccgpu_ctx* ctx, int precision, double* e4, double* e5) {
+  return guarded([&] {
+    CCGPU_REQUIRE(ctx, "ccgpu_pt: null context");
+    check_prec(precision);
+    ctx->cc->pt_energy(0, ctx->cc->pt_ntriples(), 1, e4, e5, nullptr);
+  });
+}
+
+int ccgpu_pt_partial(ccgpu_ctx* ctx, int precision, long long first, long long count, long long stride,
+                     double* e4, double* e5, double* flops, long long* ntriples) {
+  return guarded([&] {
+    CCGPU_REQUIRE(ctx, "ccgpu_pt_partial: null context");
+    check_prec(precision);
+    if (ntriples) *ntriples = ctx->cc->pt_ntriples();
+    ctx->cc->pt_energy(first, count, stride, e4, e5, flops);
+  });
+}
+
 int ccgpu_profile_start(void) {
   return guarded([&] { ccgpu::prof_start(); });
 }

@@ -40,6 +40,15 @@ class CCSD {
   // residual with host arrays (Dalton layouts), used by ccgpu_rhs
   void residual_host(const double* t1, const double* t2p, double* om1, double* om2p);
 
+  // (T) energy over the restricted triple list i >= j >= k (i = j = k
+  // excluded), enumerated i-major; triples first, first+stride, ... (count
+  // of them).  Adds E4/E5 of those triples; *flops gets their algorithmic
+  // flop count.  Releases the iteration workspace on first use.
+  long pt_ntriples() const;
+  void pt_energy(long first, long count, long stride, double* e4, double* e5, double* flops);
+  void pt_release();
+  void release_work();
+
   const double* t1() const { return t1_.p; }
   const double* T() const { return T_.p; }
   const double* eps() const { return eps_.p; }
@@ -50,10 +59,16 @@ class CCSD {
   void dress(const double* t1);
   void ladder(const double* T, double* Om);
   void mark(const char* name);
+  void ensure_work();
+  void pt_prepare();
 
   Dims dm_;
   int mxdiis_;
   int iter_ = 0;
+  int diis_off_ = 0;  // DIIS cycle restarts after the history was released
+  bool work_ = false;
+  bool pt_ready_ = false;
+  std::vector<int> tri_i_, tri_j_, tri_k_;
   hipStream_t st_ = nullptr;
 
   // inputs / constants
@@ -70,6 +85,8 @@ class CCSD {
   DBuf<double> diisS_, diisE_, dvec_, dots_, coef_;
   std::vector<double> Amat_;
   DBuf<double> scal_;
+  // (T)
+  DBuf<double> G_, Gp_, Tvvo_, Q_, Wt_, ptpart_, etrip_;
 };
 
 }  // namespace ccgpu

@@ -13,29 +13,11 @@
 #include <cstring>
 
 #include "kernels.h"
+#include "ops.h"
 
 namespace ccgpu {
 
 namespace {
-
-void g(long M, long N, long K, double alpha, const double* A, long a_m, long a_k, const double* B,
-       long b_k, long b_n, double beta, double* C, long c_m, long c_n, hipStream_t s, int batch = 1,
-       long a_b = 0, long b_b = 0, long c_b = 0, long Kin = 0, long a_ko = 0, long b_ko = 0) {
-  GemmDesc d;
-  d.M = M; d.N = N; d.K = K; d.alpha = alpha; d.beta = beta;
-  d.A = A; d.a_m = a_m; d.a_k = a_k; d.a_b = a_b;
-  d.B = B; d.b_k = b_k; d.b_n = b_n; d.b_b = b_b;
-  d.C = C; d.c_m = c_m; d.c_n = c_n; d.c_b = c_b;
-  d.batch = batch; d.Kin = Kin; d.a_ko = a_ko; d.b_ko = b_ko;
-  gemm(d, GemmPrec::F64, s);
-}
-
-void perm(double* out, const double* in, long n0, long n1, long n2, long n3, long s0, long s1, long s2,
-          long s3, double alpha, double beta, hipStream_t st) {
-  const long n[4] = {n0, n1, n2, n3};
-  const long s[4] = {s0, s1, s2, s3};
-  permute4(out, in, n, s, alpha, beta, st);
-}
 
 // Small dense LU solve with partial pivoting (the DIIS system).
 std::vector<double> lu_solve(std::vector<double> A, std::vector<double> b, int n) {
@@ -115,6 +97,24 @@ CCSD::CCSD(int o, int v, int naux, const double* eps_host, const double* B_host,
   perm(L_.p, K_.p, 1, 1, 1, nov2, 0, 0, 0, 1, -1.0, 1.0, st_);
   perm(Joovv_.p, J_.p, o, o, v, v, (long)v * nov, v, nov, 1, 1.0, 0.0, st_);
 
+  t1_.alloc(nov);
+  T_.alloc(nov2);
+  om1_.alloc(nov);
+  Om_.alloc(nov2);
+  dots_.alloc(kMaxVec + 1);
+  coef_.alloc(kMaxVec + 1);
+  Amat_.assign((kMaxVec + 1) * (kMaxVec + 1), 0.0);
+  scal_.alloc(4);
+  CCGPU_HIPX(hipStreamSynchronize(st_));
+}
+
+void CCSD::ensure_work() {
+  if (work_) return;
+  pt_release();
+  const int o = dm_.o, v = dm_.v, naux = dm_.naux;
+  const int mxdiis = mxdiis_;
+  const long N = dm_.N, nov = dm_.nov, nov2 = dm_.nov2;
+  (void)N;
   // per-iteration buffers
   Bt_.alloc((size_t)naux * N * N);
   Bvv_.alloc((size_t)naux * v * v);
@@ -129,10 +129,6 @@ CCSD::CCSD(int o, int v, int naux, const double* eps_host, const double* B_host,
   W_.alloc((size_t)o * o * nov);
   E1_.alloc((size_t)v * v);
   E2_.alloc((size_t)o * o);
-  t1_.alloc(nov);
-  T_.alloc(nov2);
-  om1_.alloc(nov);
-  Om_.alloc(nov2);
   bufR_.alloc(std::max<long>(nov2, (long)v * v * o * o));
   bufX2_.alloc(std::max<long>(nov2, dm_.ntamp));
   bufTx_.alloc(nov2);
@@ -174,11 +170,17 @@ CCSD::CCSD(int o, int v, int naux, const double* eps_host, const double* B_host,
   // DIIS history (Dalton DIIST/DIISS files)
   diisS_.alloc((size_t)mxdiis * dm_.ntamp);
   diisE_.alloc((size_t)mxdiis * dm_.ntamp);
-  dots_.alloc(kMaxVec + 1);
-  coef_.alloc(kMaxVec + 1);
-  Amat_.assign((kMaxVec + 1) * (kMaxVec + 1), 0.0);
-  scal_.alloc(4);
-  CCGPU_HIPX(hipStreamSynchronize(st_));
+  work_ = true;
+}
+
+void CCSD::release_work() {
+  if (!work_) return;
+  for (DBuf<double>* b : {&Bt_, &Bvv_, &Boo_, &BvoT_, &F_, &ht_, &hx_, &G1_, &Gam_, &Z_, &W_, &E1_, &E2_,
+                          &bufR_, &bufX2_, &bufTx_, &bufU_, &bufC_, &bufD_, &taup_, &taum_, &Y_, &Vp_,
+                          &Vm_, &Sp_, &Sm_, &diisS_, &diisE_})
+    b->release();
+  work_ = false;
+  diis_off_ = iter_ > 0 ? iter_ - 1 : 0;  // the DIIS history is gone: restart the cycle
 }
 
 CCSD::~CCSD() {
@@ -236,6 +238,7 @@ void CCSD::ladder(const double* T, double* Om) {
 
 // CCSD vector function (CCRHSN, cc/ccsd_rhs.F:20-2278), T1-dressed form.
 void CCSD::residual(const double* t1, const double* T, double* om1, double* Om) {
+  ensure_work();
   const int o = dm_.o, v = dm_.v, naux = dm_.naux;
   const long N = dm_.N, nov = dm_.nov, nov2 = dm_.nov2, oo = (long)o * o, vv = (long)v * v;
   mark("start");
@@ -323,6 +326,7 @@ double CCSD::mp2_start() {
   CCGPU_HIPX(hipMemcpyAsync(&e, scal_.p + 1, sizeof(double), hipMemcpyDeviceToHost, st_));
   CCGPU_HIPX(hipStreamSynchronize(st_));
   iter_ = 1;
+  diis_off_ = 0;
   return e;
 }
 
@@ -335,7 +339,8 @@ double CCSD::iterate(double* omnm) {
   residual(t1_.p, T_.p, om1_.p, Om_.p);
   nxtam(om1_.p, Om_.p, t1_.p, T_.p, eps_.p, o, v, scal_.p, st_);
   const int mx = mxdiis_;
-  const int ivec = iter_ > mx ? iter_ - ((iter_ - 1) / mx) * mx : iter_;
+  const int it = iter_ - diis_off_;
+  const int ivec = it > mx ? it - ((it - 1) / mx) * mx : it;
   const int nloop = ivec;
   double* svec = diisS_.p + (long)(ivec - 1) * nt;
   double* evec = diisE_.p + (long)(ivec - 1) * nt;
@@ -379,11 +384,12 @@ double CCSD::iterate(double* omnm) {
 void CCSD::set_amplitudes(const double* t1_host, const double* t2p_host) {
   const long nov = dm_.nov;
   CCGPU_HIPX(hipMemcpyAsync(t1_.p, t1_host, nov * sizeof(double), hipMemcpyHostToDevice, st_));
-  CCGPU_HIPX(hipMemcpyAsync(bufX2_.p, t2p_host, (dm_.ntamp - nov) * sizeof(double),
+  CCGPU_HIPX(hipMemcpyAsync(Om_.p, t2p_host, (dm_.ntamp - nov) * sizeof(double),
                            hipMemcpyHostToDevice, st_));
-  unpack_sym(T_.p, bufX2_.p, nov, st_);
+  unpack_sym(T_.p, Om_.p, nov, st_);
   CCGPU_HIPX(hipStreamSynchronize(st_));
   iter_ = 1;
+  diis_off_ = 0;
 }
 
 void CCSD::get_amplitudes(double* t1_host, double* t2p_host) {
@@ -391,8 +397,8 @@ void CCSD::get_amplitudes(double* t1_host, double* t2p_host) {
   if (t1_host)
     CCGPU_HIPX(hipMemcpyAsync(t1_host, t1_.p, nov * sizeof(double), hipMemcpyDeviceToHost, st_));
   if (t2p_host) {
-    pack_sym(bufX2_.p, T_.p, nov, st_);
-    CCGPU_HIPX(hipMemcpyAsync(t2p_host, bufX2_.p, (dm_.ntamp - nov) * sizeof(double),
+    pack_sym(Om_.p, T_.p, nov, st_);
+    CCGPU_HIPX(hipMemcpyAsync(t2p_host, Om_.p, (dm_.ntamp - nov) * sizeof(double),
                              hipMemcpyDeviceToHost, st_));
   }
   CCGPU_HIPX(hipStreamSynchronize(st_));

@@ -1,0 +1,238 @@
+// (T) energy on one GPU, occupied-driven over i >= j >= k
+// (mlcc/mlccsdpt_e_calc.F90:71, loop :326-411).
+//
+// Per triple, W_ijk^abc (v^3, FP64) is formed by 12 MFMA GEMMs into one
+// buffer laid out [a][b][c]: six "virtual" terms sum_d t_pq^{xd} (yd|zr)
+// (K = v) and six "occupied" terms -sum_l t_pl^{xy} (zr|lq) (K = o), one per
+// simultaneous permutation of (ai),(bj),(ck) (w_calc :966).  Each GEMM is
+// oriented (plain, transposed or batched over one virtual index) so that its
+// store into W is coalesced.  The integrals (yd|zr) are kept in two layouts,
+// G[r][d][y][z] and Gp[r][d][z][y], built once from the DF factor; together
+// with t2 re-laid as Tvvo[p][x][y][l] = t_pl^{xy} and (zr|lq) as
+// Q[r][q][l][z] every GEMM reads unit-stride panels.
+//
+// The energy kernel evaluates the permutation-symmetrised functional of
+// w_prod (:1547-1640) per orbit {a,b,c}: each W element is read once.
+// E4 takes V = W, E5 takes the t1 part of V (v_calc :1345, :1483-1527).
+#include <cmath>
+
+#include "ccsd.h"
+#include "kernels.h"
+#include "ops.h"
+
+namespace ccgpu {
+
+namespace {
+
+__device__ __forceinline__ void tri_dec(long p, long& lo, long& hi) {
+  long h = (long)((sqrt(8.0 * (double)p + 1.0) - 1.0) * 0.5);
+  while (h * (h + 1) / 2 > p) --h;
+  while ((h + 1) * (h + 2) / 2 <= p) ++h;
+  hi = h;
+  lo = p - h * (h + 1) / 2;
+}
+
+// One block per virtual pair a <= b, threads over c in [b, v).
+// Writes (E4, E5) block partials, already scaled by mult/3.
+__global__ void k_pt_energy(const double* __restrict__ W, const double* __restrict__ J,
+                            const double* __restrict__ t1, const double* __restrict__ eps, int i, int j,
+                            int k, int o, int v, double scale, double* part) {
+  __shared__ double sh[2][8];
+  long a_, b_;
+  tri_dec(blockIdx.x, a_, b_);
+  const int a = (int)a_, b = (int)b_;
+  const long nov = (long)o * v, vv = (long)v * v;
+  const double eijk = eps[i] + eps[j] + eps[k];
+  double s4 = 0.0, s5 = 0.0;
+  for (int c = b + (int)threadIdx.x; c < v; c += blockDim.x) {
+    // slots: 0 abc, 1 acb, 2 bac, 3 bca, 4 cab, 5 cba; even class {0,3,4}, odd {1,2,5}
+    const int X[6] = {a, a, b, b, c, c}, Y[6] = {b, c, a, c, a, b}, Z[6] = {c, b, c, a, b, a};
+    double w[6], y[6];
+#pragma unroll
+    for (int s = 0; s < 6; ++s) {
+      const int x = X[s], yy = Y[s], z = Z[s];
+      w[s] = W[(long)x * vv + (long)yy * v + z];
+      y[s] = t1[(long)i * v + x] * J[((long)j * v + yy) * nov + (long)k * v + z] +
+             t1[(long)j * v + yy] * J[((long)i * v + x) * nov + (long)k * v + z] +
+             t1[(long)k * v + z] * J[((long)i * v + x) * nov + (long)j * v + yy];
+    }
+    const double wE = w[0] + w[3] + w[4], wO = w[1] + w[2] + w[5];
+    const double yE = y[0] + y[3] + y[4], yO = y[1] + y[2] + y[5];
+    double ww = 0.0, wy = 0.0;
+#pragma unroll
+    for (int s = 0; s < 6; ++s) {
+      ww += w[s] * w[s];
+      wy += w[s] * y[s];
+    }
+    // sum over the 6 slots of W_p (4X_p + X_cyc1 + X_cyc2 - 2 X_swaps)
+    const double f4 = 3.0 * ww + 3.0 * (wE * wE + wO * wO) - 2.0 * (wE + wO) * (wE + wO);
+    const double f5 = 3.0 * wy + 3.0 * (wE * yE + wO * yO) - 2.0 * (wE + wO) * (yE + yO);
+    // each distinct tuple is counted once: slots alias when indices coincide
+    const double wt = (a == b && b == c) ? (1.0 / 6.0) : ((a == b || b == c) ? 0.5 : 1.0);
+    const double D = eijk - eps[o + a] - eps[o + b] - eps[o + c];
+    s4 += wt * f4 / D;
+    s5 += wt * f5 / D;
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    s4 += __shfl_down(s4, off, 64);
+    s5 += __shfl_down(s5, off, 64);
+  }
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0) {
+    sh[0][wv] = s4;
+    sh[1][wv] = s5;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double r4 = 0.0, r5 = 0.0;
+    for (int q = 0; q < (int)(blockDim.x >> 6); ++q) {
+      r4 += sh[0][q];
+      r5 += sh[1][q];
+    }
+    part[2 * (long)blockIdx.x] = scale * r4;
+    part[2 * (long)blockIdx.x + 1] = scale * r5;
+  }
+}
+
+// out[0..1] = sum over n pairs of part (fixed order: deterministic)
+__global__ void k_pt_reduce(const double* __restrict__ part, long n, double* out) {
+  __shared__ double sh[2][8];
+  double s4 = 0.0, s5 = 0.0;
+  for (long q = threadIdx.x; q < n; q += blockDim.x) {
+    s4 += part[2 * q];
+    s5 += part[2 * q + 1];
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    s4 += __shfl_down(s4, off, 64);
+    s5 += __shfl_down(s5, off, 64);
+  }
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0) {
+    sh[0][wv] = s4;
+    sh[1][wv] = s5;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double r4 = 0.0, r5 = 0.0;
+    for (int q = 0; q < (int)(blockDim.x >> 6); ++q) {
+      r4 += sh[0][q];
+      r5 += sh[1][q];
+    }
+    out[0] = r4;
+    out[1] = r5;
+  }
+}
+
+}  // namespace
+
+long CCSD::pt_ntriples() const {
+  const long o = dm_.o;
+  return o * (o + 1) * (o + 2) / 6 - o;  // i >= j >= k without i = j = k
+}
+
+void CCSD::pt_release() {
+  for (DBuf<double>* b : {&G_, &Gp_, &Tvvo_, &Q_, &Wt_, &ptpart_, &etrip_}) b->release();
+  pt_ready_ = false;
+}
+
+// Integral and amplitude layouts for the triple loop (built once per call
+// sequence; the iteration workspace is released first to make room).
+void CCSD::pt_prepare() {
+  if (pt_ready_) return;
+  release_work();
+  const int o = dm_.o, v = dm_.v, naux = dm_.naux;
+  const long N = dm_.N, NN = N * N, nov = dm_.nov, vv = (long)v * v, vvv = vv * v;
+  G_.alloc((size_t)o * vvv);
+  Gp_.alloc((size_t)o * vvv);
+  Tvvo_.alloc((size_t)o * vv * o);
+  Q_.alloc((size_t)o * o * o * v);
+  Wt_.alloc(vvv);
+  ptpart_.alloc(2 * (size_t)(vv + v) / 2 + 2);
+  {
+    // undressed virtual-virtual and occupied-occupied blocks of B
+    DBuf<double> Bvv0((size_t)naux * vv), Boo0((size_t)naux * o * o), R((size_t)nov * o * o);
+    perm(Bvv0.p, B_.p + o * N + o, 1, naux, v, v, 0, NN, N, 1, 1.0, 0.0, st_);
+    perm(Boo0.p, B_.p, 1, naux, o, o, 0, NN, N, 1, 1.0, 0.0, st_);
+    // G[k][d][y][z] = (yd|zk) = sum_P B[P][d][y] B[P][k][z], batched over k
+    g(vv, v, naux, 1.0, Bvv0.p, 1, vv, Bov_.p, nov, 1, 0.0, G_.p, v, 1, st_, o, 0, v, vvv);
+    // Gp[k][d][y][z] = G[k][d][z][y]
+    perm(Gp_.p, G_.p, 1, (long)o * v, v, v, 0, vv, 1, v, 1.0, 0.0, st_);
+    // R[(k,c)][(l,j)] = (ck|lj);  Q[k][j][l][c] = R
+    g(nov, (long)o * o, naux, 1.0, Bov_.p, 1, nov, Boo0.p, (long)o * o, 1, 0.0, R.p, (long)o * o, 1, st_);
+    perm(Q_.p, R.p, o, o, o, v, (long)v * o * o, 1, o, (long)o * o, 1.0, 0.0, st_);
+  }
+  // Tvvo[i][a][b][l] = t_il^ab = T[(i,a),(l,b)]
+  perm(Tvvo_.p, T_.p, o, v, v, o, (long)v * nov, nov, 1, v, 1.0, 0.0, st_);
+  tri_i_.clear();
+  tri_j_.clear();
+  tri_k_.clear();
+  for (int i = 0; i < o; ++i)
+    for (int j = 0; j <= i; ++j)
+      for (int k = 0; k <= j; ++k) {
+        if (i == j && j == k) continue;
+        tri_i_.push_back(i);
+        tri_j_.push_back(j);
+        tri_k_.push_back(k);
+      }
+  pt_ready_ = true;
+}
+
+void CCSD::pt_energy(long first, long count, long stride, double* e4, double* e5, double* flops) {
+  pt_prepare();
+  const int o = dm_.o, v = dm_.v;
+  const long nov = dm_.nov, vv = (long)v * v, vvv = vv * v, vo = (long)v * o;
+  const long ntr = pt_ntriples();
+  CCGPU_REQUIRE(first >= 0 && stride >= 1 && count >= 0, "pt: bad triple range");
+  long n = 0;
+  for (long t = first; t < ntr && n < count; t += stride) ++n;
+  etrip_.ensure(2 * (size_t)std::max<long>(n, 1));
+  const double* T = T_.p;
+  double* W = Wt_.p;
+  const double* G = G_.p;
+  const double* Gp = Gp_.p;
+  const double* Tv = Tvvo_.p;
+  const double* Qp = Q_.p;
+  auto Tpq = [&](int p, int q) { return T + (long)p * v * nov + (long)q * v; };  // [x][d], row stride nov
+  auto Qrq = [&](int r, int q) { return Qp + ((long)r * o + q) * o * v; };       // [l][z]
+  long idx = 0;
+  const long npv = (vv + v) / 2;
+  for (long t = first; t < ntr && idx < n; t += stride, ++idx) {
+    const int i = tri_i_[t], j = tri_j_[t], k = tri_k_[t];
+    // virtual terms
+    g(v, vv, v, 1.0, Tpq(i, j), nov, 1, G + (long)k * vvv, vv, 1, 0.0, W, vv, 1, st_);
+    g(v, vv, v, 1.0, Tpq(i, k), nov, 1, Gp + (long)j * vvv, vv, 1, 1.0, W, vv, 1, st_);
+    g(v, v, v, 1.0, Tpq(j, i), nov, 1, G + (long)k * vvv, vv, 1, 1.0, W, v, 1, st_, v, 0, v, vv);
+    g(v, v, v, 1.0, Tpq(j, k), nov, 1, Gp + (long)i * vvv, vv, 1, 1.0, W, v, 1, st_, v, 0, v, vv);
+    g(vv, v, v, 1.0, G + (long)j * vvv, 1, vv, Tpq(k, i), 1, nov, 1.0, W, v, 1, st_);
+    g(vv, v, v, 1.0, Gp + (long)i * vvv, 1, vv, Tpq(k, j), 1, nov, 1.0, W, v, 1, st_);
+    // occupied terms
+    g(vv, v, o, -1.0, Tv + (long)i * vv * o, o, 1, Qrq(k, j), v, 1, 1.0, W, v, 1, st_);
+    g(v, v, o, -1.0, Qrq(j, k), 1, v, Tv + (long)i * vv * o, 1, o, 1.0, W, v, 1, st_, v, 0, vo, vv);
+    g(v, v, o, -1.0, Tv + (long)j * vv * o, o, 1, Qrq(k, i), v, 1, 1.0, W, vv, 1, st_, v, vo, 0, v);
+    g(v, v, o, -1.0, Qrq(i, k), 1, v, Tv + (long)j * vv * o, 1, o, 1.0, W, vv, 1, st_, v, 0, vo, v);
+    g(v, v, o, -1.0, Qrq(j, i), 1, v, Tv + (long)k * vv * o, 1, vo, 1.0, W, v, 1, st_, v, 0, o, vv);
+    g(v, v, o, -1.0, Qrq(i, j), 1, v, Tv + (long)k * vv * o, 1, vo, 1.0, W, vv, 1, st_, v, 0, o, v);
+    // energy
+    const double mult = (i == j || j == k) ? 3.0 : 6.0;
+    check_span(W, 0, vvv - 1, sizeof(double), "pt W");
+    check_span(ptpart_.p, 0, 2 * npv - 1, sizeof(double), "pt part");
+    check_span(J_.p, 0, dm_.nov2 - 1, sizeof(double), "pt J");
+    CCGPU_LAUNCH(k_pt_energy, dim3((unsigned)npv), dim3(256), 0, st_, W, J_.p, t1_.p, eps_.p, i, j, k, o, v,
+                 mult / 3.0, ptpart_.p);
+    CCGPU_LAUNCH(k_pt_reduce, dim3(1), dim3(256), 0, st_, ptpart_.p, npv, etrip_.p + 2 * idx);
+    CCGPU_HIPX(hipGetLastError());
+  }
+  std::vector<double> et(2 * std::max<long>(n, 1), 0.0);
+  CCGPU_HIPX(hipMemcpyAsync(et.data(), etrip_.p, 2 * n * sizeof(double), hipMemcpyDeviceToHost, st_));
+  CCGPU_HIPX(hipStreamSynchronize(st_));
+  double s4 = 0.0, s5 = 0.0;
+  for (long q = 0; q < n; ++q) {
+    s4 += et[2 * q];
+    s5 += et[2 * q + 1];
+  }
+  if (e4) *e4 = s4;
+  if (e5) *e5 = s5;
+  if (flops) *flops = (double)n * (12.0 * (double)vv * vv + 12.0 * (double)o * vvv);
+}
+
+}  // namespace ccgpu

@@ -75,6 +75,21 @@ int ccgpu_ccsd_energy(ccgpu_ctx* ctx, int maxit, double threnr, double thrvec, i
 int ccgpu_get_amplitudes(ccgpu_ctx* ctx, double* t1am, double* t2am);
 int ccgpu_set_amplitudes(ccgpu_ctx* ctx, const double* t1am, const double* t2am);
 
+/* (T) energy correction from the context's current amplitudes, returned as
+ * Dalton's two parts: E4 ("doubles and triples") and E5 ("singles and
+ * triples"), E_(T) = E4 + E5 (cc/ccsd_energy.F:1239-1243, printed :1468-1499).
+ * The triples are evaluated occupied-driven over i >= j >= k as in
+ * mlccsdpt_e_calc (mlcc/mlccsdpt_e_calc.F90:71, loop :326-411).  Releases
+ * the iteration workspace (a later iterate restarts the DIIS cycle). */
+int ccgpu_pt(ccgpu_ctx* ctx, int precision, double* e4, double* e5);
+
+/* Part of the (T) sum: the restricted triples (i >= j >= k, i = j = k
+ * excluded, enumerated i-major then j then k) with list positions first,
+ * first+stride, ... (count of them).  flops receives their algorithmic flop
+ * count (12 v^4 + 12 o v^3 per triple), ntriples the length of the full list. */
+int ccgpu_pt_partial(ccgpu_ctx* ctx, int precision, long long first, long long count, long long stride,
+                     double* e4, double* e5, double* flops, long long* ntriples);
+
 /* Per-kernel and per-term timing, the counterpart of CCRHSN's per-term
  * timers (cc/ccsd_rhs.F:2161-2232, printed at IPRINT > 9).  Between start and
  * stop every MFMA GEMM launch and every residual phase is bracketed by HIP

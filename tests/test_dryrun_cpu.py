@@ -34,13 +34,20 @@ for (o, v, naux) in {shapes!r}:
         a1 = np.zeros(nt1); a2 = np.zeros(nt2)
         rc = L.ccgpu_rhs(h, _lib.ptr(a1), _lib.ptr(a2), _lib.ptr(a1), _lib.ptr(a2), 0)
         assert rc == 0, L.ccgpu_last_error()
+    fl = C.c_double(); nt = C.c_longlong()
+    rc = L.ccgpu_pt_partial(h, 0, 0, 3, 1, C.byref(e), C.byref(n), C.byref(fl), C.byref(nt))
+    assert rc == 0, L.ccgpu_last_error()
+    rc = L.ccgpu_pt_partial(h, 0, max(0, nt.value - 2), 2, 1, C.byref(e), C.byref(n), C.byref(fl), C.byref(nt))
+    assert rc == 0, L.ccgpu_last_error()
+    rc = L.ccgpu_ccsd_iterate(h, 0, C.byref(e), C.byref(n))  # workspace comes back after (T)
+    assert rc == 0, L.ccgpu_last_error()
     assert L.ccgpu_finalize(h) == 0
     print("ok", o, v, naux, flush=True)
 """
 
 
 @pytest.mark.parametrize("shapes", [
-    [(5, 19, 280), (5, 14, 172), (1, 1, 1), (6, 40, 90), (3, 200, 10)],
+    [(5, 19, 280), (5, 14, 172), (1, 1, 1), (2, 1, 3), (6, 40, 90), (3, 200, 10)],
     [(32, 256, 1024)],
     [(64, 512, 2048)],
 ])

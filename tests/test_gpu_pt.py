@@ -1,0 +1,51 @@
+"""HIP (T) energy against the reference's printed results and the oracle."""
+import numpy as np
+import pytest
+
+from conftest import load_molecule
+from oracle import ccsd as occ
+from oracle import triples
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("name,threnr", [("h2o_ccpvdz_test", 1e-12), ("hf_631gss", 1e-10)])
+def test_pt_matches_dalton(name, threnr, golden):
+    import dalton_amd as d
+
+    eps, B, o, _ = load_molecule(name)
+    print(f"(T) vs Dalton {name}", flush=True)
+    g = golden[name]
+    with d.CCContext(eps, B, o) as c:
+        c.ccsd_energy(maxit=60, threnr=threnr)
+        e4, e5 = c.pt()
+    if "e_t" in g:
+        assert abs(e4 + e5 - g["e_t"]) < 6e-11
+    if "e4" in g:
+        assert abs(e4 - g["e4"]) < 6e-11
+        assert abs(e5 - g["e5"]) < 6e-11
+
+
+def test_pt_matches_oracle_random_amplitudes():
+    import dalton_amd as d
+    from dalton_amd.synthetic import make_system
+
+    o, v, naux = 6, 30, 60
+    eps, B = make_system(o, v, naux, seed=5)
+    print(f"(T) vs oracle o={o} v={v}", flush=True)
+    rng = np.random.default_rng(9)
+    t1 = 0.05 * rng.standard_normal((v, o))
+    t2 = 0.05 * rng.standard_normal((v, o, v, o))
+    t2 = t2 + t2.transpose(2, 3, 0, 1)
+    r4, r5 = triples.pt_energy(eps, B, o, t1, t2)
+    with d.CCContext(eps, B, o) as c:
+        c.set_amplitudes(occ.pack_t1(t1), occ.pack_t2(t2))
+        e4, e5 = c.pt()
+        # partial sums over an interleaved split add up to the total
+        a4, a5, fa, nt = c.pt_partial(0, 10**9, 2)
+        b4, b5, fb, _ = c.pt_partial(1, 10**9, 2)
+    assert abs(e4 - r4) < 1e-12 * max(1.0, abs(r4))
+    assert abs(e5 - r5) < 1e-12 * max(1.0, abs(r5))
+    assert nt == o * (o + 1) * (o + 2) // 6 - o
+    assert abs(a4 + b4 - e4) < 1e-12 * max(1.0, abs(e4)) and abs(a5 + b5 - e5) < 1e-12 * max(1.0, abs(e5))
+    assert fa + fb == pytest.approx(nt * (12.0 * v**4 + 12.0 * o * v**3))
