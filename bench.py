@@ -88,7 +88,8 @@ def cpu_baseline(sample=(16, 128, 512), seed=3):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=3, help="timed mixed-precision iterations")
+    ap.add_argument("--fp64-steps", type=int, default=2, help="timed FP64 iterations (same starting point)")
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--o", type=int, default=64)
     ap.add_argument("--v", type=int, default=512)
@@ -107,94 +108,108 @@ def main():
     from dalton_amd.cc import profile_start, profile_stop
     from dalton_amd.synthetic import make_system
 
+    def log(msg):
+        print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
     o, v, naux = args.o, args.v, args.naux
     t0 = time.perf_counter()
     eps, B = make_system(o, v, naux, args.seed)
-    print(f"[bench] system o={o} v={v} naux={naux} generated in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+    log(f"system o={o} v={v} naux={naux} generated in {time.perf_counter() - t0:.1f} s")
     ctx = d.CCContext(eps, B, o)
     del B
     e_mp2 = ctx.guess()
-    print(f"[bench] E_MP2 = {e_mp2:.10f}", file=sys.stderr, flush=True)
-    for w in range(args.warmup):
-        e, n = ctx.iterate()
-        print(f"[bench] warmup {w + 1}: E = {e:.10f} |Omega| = {n:.3e}", file=sys.stderr, flush=True)
-    sync()
-    profile_start()
-    t0 = time.perf_counter()
-    hist = []
-    for s in range(args.steps):
-        e, n = ctx.iterate()
-        hist.append(e)
-        print(f"[bench] step {s + 1}: E = {e:.10f} |Omega| = {n:.3e}", file=sys.stderr, flush=True)
-    sync()
-    dt = time.perf_counter() - t0
-    prof = profile_stop()
+    log(f"E_MP2 = {e_mp2:.10f}")
+    for w in range(args.warmup):  # allocates the iteration workspace
+        e, n = ctx.iterate(d.MIXED)
+        log(f"warmup {w + 1} (mixed): E = {e:.10f} |Omega| = {n:.3e}")
 
-    pt = None
+    def timed_steps(prec, nsteps, label):
+        ctx.guess()  # both precisions start from the MP2 amplitudes with an empty DIIS history
+        sync()
+        profile_start()
+        t0 = time.perf_counter()
+        hist = []
+        for s_ in range(nsteps):
+            e, n = ctx.iterate(prec)
+            hist.append(e)
+            log(f"{label} step {s_ + 1}: E = {e:.10f} |Omega| = {n:.3e}")
+        sync()
+        dt = time.perf_counter() - t0
+        return dt / nsteps, hist, profile_stop()
+
+    spi_m, hist_m, prof_m = timed_steps(d.MIXED, args.steps, "mixed")
+    spi_d, hist_d, prof_d = timed_steps(d.FP64, args.fp64_steps, "fp64")
+
+    def kern_table(prof):
+        return {k: {"count": v_["count"], "ms": round(v_["ms"], 3),
+                    "tflops": round(v_["flops"] / max(v_["ms"], 1e-9) / 1e9, 2)} for k, v_ in prof["kernels"].items()}
+
+    def roofline(prof, dt_total, peak, peak_note):
+        kern = prof["kernels"]
+        dom = max(kern, key=lambda k: kern[k]["ms"])
+        kd = kern[dom]
+        ach = kd["flops"] / (kd["ms"] * 1e-3) / 1e12
+        return {"bound": "mfma", "achieved": round(ach, 3), "peak": peak, "unit": "TFLOP/s",
+                "frac": round(ach / peak, 4), "traffic": None, "kernel": dom, "launches": kd["count"],
+                "avg_launch_ms": round(kd["ms"] / kd["count"], 4),
+                "share_of_step": round(kd["ms"] * 1e-3 / dt_total, 4), "peak_note": peak_note}
+
+    F = f_ccsd(o, v, naux)
+    pt = {}
     if args.pt_triples > 0:
         _, _, _, ntr = ctx.pt_partial(0, 0, 1)  # build (T) integral layouts (untimed setup)
         stride = max(1, ntr // args.pt_triples)
-        sync()
-        profile_start()
-        t1 = time.perf_counter()
-        e4, e5, fl, _ = ctx.pt_partial(0, args.pt_triples, stride)
-        sync()
-        dtp = time.perf_counter() - t1
-        pprof = profile_stop()
-        pk = pprof["kernels"]
-        pdom = max(pk, key=lambda k: pk[k]["ms"])
-        pt = {"triples_timed": args.pt_triples, "triples_total": ntr, "stride": stride,
-              "seconds": round(dtp, 4), "tflops": round(fl / dtp / 1e12, 3),
-              "est_full_seconds": round(dtp / args.pt_triples * ntr, 1),
-              "flops_per_triple": fl / args.pt_triples, "e4_sample": e4, "e5_sample": e5,
-              "dominant_kernel": pdom,
-              "kernels": {k: {"count": v_["count"], "ms": round(v_["ms"], 3),
-                              "tflops": round(v_["flops"] / max(v_["ms"], 1e-9) / 1e9, 2)} for k, v_ in pk.items()}}
-        print(f"[bench] (T): {args.pt_triples} triples in {dtp:.3f} s, {fl / dtp / 1e12:.2f} TFLOP/s", file=sys.stderr,
-              flush=True)
+        for label, prec in (("mixed", d.MIXED), ("fp64", d.FP64)):
+            sync()
+            profile_start()
+            t0 = time.perf_counter()
+            e4, e5, fl, _ = ctx.pt_partial(0, args.pt_triples, stride, precision=prec)
+            sync()
+            dtp = time.perf_counter() - t0
+            pp = profile_stop()
+            pt[label] = {"seconds": round(dtp, 4), "tflops": round(fl / dtp / 1e12, 3),
+                         "est_full_seconds": round(dtp / args.pt_triples * ntr, 1), "e4": e4, "e5": e5,
+                         "kernels": kern_table(pp)}
+            log(f"(T) {label}: {args.pt_triples} triples in {dtp:.3f} s, {fl / dtp / 1e12:.2f} TFLOP/s")
+        pt["triples_timed"] = args.pt_triples
+        pt["triples_total"] = ntr
+        pt["stride"] = stride
+        pt["flops_per_triple"] = 12.0 * v**4 + 12.0 * o * v**3
+        pt["rel_err_e4_mixed_vs_fp64"] = abs(pt["mixed"]["e4"] - pt["fp64"]["e4"]) / abs(pt["fp64"]["e4"])
+        pt["rel_err_e5_mixed_vs_fp64"] = abs(pt["mixed"]["e5"] - pt["fp64"]["e5"]) / abs(pt["fp64"]["e5"])
+        pt["speedup_mixed_vs_fp64"] = round(pt["fp64"]["seconds"] / pt["mixed"]["seconds"], 3)
     ctx.close()
 
-    spi = dt / args.steps
-    F = f_ccsd(o, v, naux)
-    kern = prof["kernels"]
-    dom = max(kern, key=lambda k: kern[k]["ms"])
-    kd = kern[dom]
-    achieved = kd["flops"] / (kd["ms"] * 1e-3) / 1e12
-    roof = {
-        "bound": "mfma",
-        "achieved": round(achieved, 3),
-        "peak": PEAK["fp64"],
-        "unit": "TFLOP/s",
-        "frac": round(achieved / PEAK["fp64"], 4),
-        "traffic": None,
-        "kernel": dom,
-        "launches": kd["count"],
-        "avg_launch_ms": round(kd["ms"] / kd["count"], 4),
-        "share_of_step": round(kd["ms"] * 1e-3 / dt, 4),
-    }
     out = {
         "metric": "CCSD s/iter + (T) TFLOP/s at o64/v512: mixed vs FP64 MFMA, |dE|<=1uEh",
-        "value": round(spi, 4),
+        "value": round(spi_m, 4),
         "unit": "s/iter",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(spi * 1e3, 2),
+        "ms_per_step": round(spi_m * 1e3, 2),
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "bf16x3/f32-accumulate (bulk contractions); f64 update, DIIS, energy",
         "data": "synthetic",
-        "config": {"workload": f"synthetic DF-CCSD iteration, o={o} v={v} naux={naux}, seed {args.seed}, MP2 start",
-                   "o": o, "v": v, "naux": naux, "precision": "fp64"},
-        "ccsd_tflops": round(F / spi / 1e12, 3),
+        "config": {"workload": f"synthetic DF-CCSD iterations from the MP2 start, o={o} v={v} naux={naux}, "
+                               f"seed {args.seed}",
+                   "o": o, "v": v, "naux": naux, "precision": "mixed"},
+        "ccsd_tflops": round(F / spi_m / 1e12, 3),
         "ccsd_flops_per_iter": F,
-        "energies": hist,
-        "pt": pt,
-        "roofline": roof,
-        "phases_ms_per_iter": {k: round(v_ / args.steps, 2) for k, v_ in prof["phases"].items()},
-        "kernels": {k: {"count": v_["count"], "ms": round(v_["ms"], 3),
-                        "tflops": round(v_["flops"] / max(v_["ms"], 1e-9) / 1e9, 2)} for k, v_ in kern.items()},
+        "fp64": {"s_per_iter": round(spi_d, 4), "steps": args.fp64_steps, "ccsd_tflops": round(F / spi_d / 1e12, 3),
+                 "energies": hist_d, "roofline": roofline(prof_d, spi_d * args.fp64_steps, PEAK["fp64"], "FP64 matrix"),
+                 "phases_ms_per_iter": {k: round(v_ / args.fp64_steps, 2) for k, v_ in prof_d["phases"].items()},
+                 "kernels": kern_table(prof_d)},
+        "speedup_mixed_vs_fp64": round(spi_d / spi_m, 3),
+        "energies": hist_m,
+        "rel_dE_per_step_mixed_vs_fp64": [abs(a - b) / abs(b) for a, b in zip(hist_m, hist_d)],
+        "roofline": roofline(prof_m, spi_m * args.steps, round(PEAK["bf16"] / 3, 1),
+                             "BF16 dense matrix peak / 3 (three bf16 products per FP32-equivalent product)"),
+        "phases_ms_per_iter": {k: round(v_ / args.steps, 2) for k, v_ in prof_m["phases"].items()},
+        "kernels": kern_table(prof_m),
+        "pt": pt or None,
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline()

@@ -35,6 +35,7 @@ SIGNATURES = {
     "ccgpu_dgemm": (C.c_int, [C.c_char, C.c_char, C.c_int, C.c_int, C.c_int, C.c_double, _dp, C.c_int,
                               _dp, C.c_int, C.c_double, _dp, C.c_int]),
 }
+SIGNATURES["ccgpu_dgemm_mixed"] = SIGNATURES["ccgpu_dgemm"]
 
 _lib = None
 

@@ -150,7 +150,7 @@ def profile_stop() -> dict:
 
 
 def dgemm(transa: str, transb: str, alpha: float, A: np.ndarray, B: np.ndarray, beta: float = 0.0,
-          Cm: np.ndarray | None = None) -> np.ndarray:
+          Cm: np.ndarray | None = None, precision: int = FP64) -> np.ndarray:
     """BLAS DGEMM on the MFMA kernel; arrays are Fortran-ordered matrices."""
     A = np.asfortranarray(A, dtype=np.float64)
     B = np.asfortranarray(B, dtype=np.float64)
@@ -159,7 +159,8 @@ def dgemm(transa: str, transb: str, alpha: float, A: np.ndarray, B: np.ndarray, 
     n = B.shape[0] if transb.upper() == "T" else B.shape[1]
     Cm = np.zeros((m, n), order="F") if Cm is None else np.array(Cm, dtype=np.float64, order="F", copy=True)
     dp = C.POINTER(C.c_double)
-    check(lib().ccgpu_dgemm(transa.encode(), transb.encode(), m, n, k, alpha,
+    f = lib().ccgpu_dgemm if precision == FP64 else lib().ccgpu_dgemm_mixed
+    check(f(transa.encode(), transb.encode(), m, n, k, alpha,
                             A.ctypes.data_as(dp), max(1, A.shape[0]), B.ctypes.data_as(dp), max(1, B.shape[0]),
                             beta, Cm.ctypes.data_as(dp), max(1, m)))
     return Cm

@@ -29,8 +29,9 @@ int guarded(F&& f) {
   }
 }
 
-void check_prec(int precision) {
-  CCGPU_REQUIRE(precision == CCGPU_FP64, "precision not available in this build (FP64 only)");
+ccgpu::GemmPrec prec_of(int precision) {
+  CCGPU_REQUIRE(precision == CCGPU_FP64 || precision == CCGPU_MIXED, "unknown precision selector");
+  return precision == CCGPU_MIXED ? ccgpu::GemmPrec::BF16X3 : ccgpu::GemmPrec::F64;
 }
 }  // namespace
 
@@ -64,7 +65,7 @@ int ccgpu_rhs(ccgpu_ctx* ctx, const double* t1am, const double* t2am, double* om
               int precision) {
   return guarded([&] {
     CCGPU_REQUIRE(ctx && t1am && t2am && omega1 && omega2, "ccgpu_rhs: null argument");
-    check_prec(precision);
+    ctx->cc->set_precision(prec_of(precision));
     ctx->cc->residual_host(t1am, t2am, omega1, omega2);
   });
 }
@@ -80,7 +81,7 @@ int ccgpu_ccsd_guess(ccgpu_ctx* ctx, double* e_mp2) {
 int ccgpu_ccsd_iterate(ccgpu_ctx* ctx, int precision, double* e_corr, double* omega_norm) {
   return guarded([&] {
     CCGPU_REQUIRE(ctx, "ccgpu_ccsd_iterate: null context");
-    check_prec(precision);
+    ctx->cc->set_precision(prec_of(precision));
     double nrm = 0.0;
     const double e = ctx->cc->iterate(&nrm);
     if (e_corr) *e_corr = e;
@@ -92,17 +93,27 @@ int ccgpu_ccsd_energy(ccgpu_ctx* ctx, int maxit, double threnr, double thrvec, i
                       double* e_mp2, double* e_ccsd, int* niter, double* history) {
   return guarded([&] {
     CCGPU_REQUIRE(ctx && maxit > 0, "ccgpu_ccsd_energy: bad argument");
-    check_prec(precision);
+    const ccgpu::GemmPrec bulk = prec_of(precision);
     double en2 = ctx->cc->mp2_start();
     if (e_mp2) *e_mp2 = en2;
+    // Mixed precision: bulk contractions in BF16x3 until the iteration is
+    // close (|dE| < kSwitch and norm < sqrt(kSwitch)), then FP64; convergence is
+    // only accepted after at least two FP64 iterations.
+    const double kSwitch = 1e-6;
+    bool fp64 = bulk == ccgpu::GemmPrec::F64;
+    int nfp64 = 0;
     for (int it = 1;; ++it) {
       const double en1 = en2;
       double nrm = 0.0;
+      ctx->cc->set_precision(fp64 ? ccgpu::GemmPrec::F64 : bulk);
       en2 = ctx->cc->iterate(&nrm);
+      if (fp64) ++nfp64;
       if (history) history[it - 1] = en2;
       if (niter) *niter = it;
       if (e_ccsd) *e_ccsd = en2;
-      if (std::fabs(en2 - en1) < threnr && nrm < thrvec) break;
+      const bool conv = std::fabs(en2 - en1) < threnr && nrm < thrvec;
+      if (conv && (bulk == ccgpu::GemmPrec::F64 || nfp64 >= 2)) break;
+      if (!fp64 && std::fabs(en2 - en1) < kSwitch && nrm < std::sqrt(kSwitch)) fp64 = true;
       if (it + 1 > maxit) throw ccgpu::Error(4, "CC equations not converged.");
     }
   });
@@ -125,7 +136,7 @@ int ccgpu_set_amplitudes(ccgpu_ctx* ctx, const double* t1am, const double* t2am)
 int ccgpu_pt(ccgpu_ctx* ctx, int precision, double* e4, double* e5) {
   return guarded([&] {
     CCGPU_REQUIRE(ctx, "ccgpu_pt: null context");
-    check_prec(precision);
+    ctx->cc->set_precision(prec_of(precision));
     ctx->cc->pt_energy(0, ctx->cc->pt_ntriples(), 1, e4, e5, nullptr);
   });
 }
@@ -134,7 +145,7 @@ int ccgpu_pt_partial(ccgpu_ctx* ctx, int precision, long long first, long long c
                      double* e4, double* e5, double* flops, long long* ntriples) {
   return guarded([&] {
     CCGPU_REQUIRE(ctx, "ccgpu_pt_partial: null context");
-    check_prec(precision);
+    ctx->cc->set_precision(prec_of(precision));
     if (ntriples) *ntriples = ctx->cc->pt_ntriples();
     ctx->cc->pt_energy(first, count, stride, e4, e5, flops);
   });
@@ -156,8 +167,9 @@ int ccgpu_profile_stop(char* buf, int len) {
   return static_cast<int>(js.size()) + 1;
 }
 
-int ccgpu_dgemm(char transa, char transb, int m, int n, int k, double alpha, const double* a, int lda,
-                const double* b, int ldb, double beta, double* c, int ldc) {
+namespace {
+int dgemm_impl(ccgpu::GemmPrec prec, char transa, char transb, int m, int n, int k, double alpha,
+               const double* a, int lda, const double* b, int ldb, double beta, double* c, int ldc) {
   return guarded([&] {
     const bool ta = transa == 'T' || transa == 't' || transa == 'C' || transa == 'c';
     const bool tb = transb == 'T' || transb == 't' || transb == 'C' || transb == 'c';
@@ -177,10 +189,22 @@ int ccgpu_dgemm(char transa, char transb, int m, int n, int k, double alpha, con
     if (!tb) { d.b_k = 1; d.b_n = ldb; } else { d.b_k = ldb; d.b_n = 1; }
     d.c_m = 1; d.c_n = ldc;
     if (k == 0) { d.K = 0; }
-    ccgpu::gemm(d, ccgpu::GemmPrec::F64, nullptr);
+    ccgpu::gemm(d, prec, nullptr);
     CCGPU_HIPX(hipDeviceSynchronize());
     CCGPU_HIPX(hipMemcpy(c, C.p, nc * sizeof(double), hipMemcpyDeviceToHost));
   });
+}
+
+}  // namespace
+
+int ccgpu_dgemm(char transa, char transb, int m, int n, int k, double alpha, const double* a, int lda,
+                const double* b, int ldb, double beta, double* c, int ldc) {
+  return dgemm_impl(ccgpu::GemmPrec::F64, transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc);
+}
+
+int ccgpu_dgemm_mixed(char transa, char transb, int m, int n, int k, double alpha, const double* a, int lda,
+                      const double* b, int ldb, double beta, double* c, int ldc) {
+  return dgemm_impl(ccgpu::GemmPrec::BF16X3, transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc);
 }
 
 }  // extern "C"

@@ -23,6 +23,11 @@ class CCSD {
   const Dims& dims() const { return dm_; }
   hipStream_t stream() const { return st_; }
 
+  // Arithmetic of the bulk contractions (ladder, rings, hole ladder, DF
+  // integrals, (T) W): FP64, or BF16x3 split products for the mixed path.
+  void set_precision(GemmPrec p) { bulk_ = p; }
+  GemmPrec precision() const { return bulk_; }
+
   // CCRHSN mirror on device arrays: t1 [i][a], T [(i,a),(j,b)] -> om1, Om.
   void residual(const double* t1, const double* T, double* om1, double* Om);
 
@@ -67,6 +72,7 @@ class CCSD {
   int iter_ = 0;
   int diis_off_ = 0;  // DIIS cycle restarts after the history was released
   bool work_ = false;
+  GemmPrec bulk_ = GemmPrec::F64;
   bool pt_ready_ = false;
   std::vector<int> tri_i_, tri_j_, tri_k_;
   hipStream_t st_ = nullptr;

@@ -223,15 +223,16 @@ void CCSD::ladder(const double* T, double* Om) {
   (void)T;
   const int o = dm_.o, v = dm_.v, naux = dm_.naux;
   const long vv = (long)v * v;
+  const Gemm bulk{bulk_, st_};
   for (size_t bi = 0; bi + 1 < lad_batches_.size(); ++bi) {
     const int a0 = lad_batches_[bi], a1 = lad_batches_[bi + 1];
     long rows = 0;
     for (int a = a0; a < a1; ++a) rows += v - a;
     const long M = (long)(a1 - a0) * v, Nn = (long)(v - a0) * v;
-    g(M, Nn, naux, 1.0, Bvv_.p + (long)a0 * v, 1, vv, Bvv_.p + (long)a0 * v, vv, 1, 0.0, Y_.p, Nn, 1, st_);
+    bulk(M, Nn, naux, 1.0, Bvv_.p + (long)a0 * v, 1, vv, Bvv_.p + (long)a0 * v, vv, 1, 0.0, Y_.p, Nn, 1);
     ladder_vpm(Vp_.p, Vm_.p, Y_.p, Nn, a0, a1, v, st_);
-    g(rows, dm_.npo, dm_.npv, 1.0, Vp_.p, dm_.npv, 1, taup_.p, 1, dm_.npv, 0.0, Sp_.p, dm_.npo, 1, st_);
-    g(rows, dm_.npo, dm_.npv, 1.0, Vm_.p, dm_.npv, 1, taum_.p, 1, dm_.npv, 0.0, Sm_.p, dm_.npo, 1, st_);
+    bulk(rows, dm_.npo, dm_.npv, 1.0, Vp_.p, dm_.npv, 1, taup_.p, 1, dm_.npv, 0.0, Sp_.p, dm_.npo, 1);
+    bulk(rows, dm_.npo, dm_.npv, 1.0, Vm_.p, dm_.npv, 1, taum_.p, 1, dm_.npv, 0.0, Sm_.p, dm_.npo, 1);
     ladder_scatter(Om, Sp_.p, Sm_.p, a0, a1, o, v, st_);
   }
 }
@@ -239,6 +240,7 @@ void CCSD::ladder(const double* T, double* Om) {
 // CCSD vector function (CCRHSN, cc/ccsd_rhs.F:20-2278), T1-dressed form.
 void CCSD::residual(const double* t1, const double* T, double* om1, double* Om) {
   ensure_work();
+  const Gemm bulk{bulk_, st_};
   const int o = dm_.o, v = dm_.v, naux = dm_.naux;
   const long N = dm_.N, nov = dm_.nov, nov2 = dm_.nov2, oo = (long)o * o, vv = (long)v * v;
   mark("start");
@@ -246,7 +248,7 @@ void CCSD::residual(const double* t1, const double* T, double* om1, double* Om) 
   mark("dress");
 
   // A2 integral part: (ai~|bj~)
-  g(nov, nov, naux, 1.0, BvoT_.p, 1, nov, BvoT_.p, nov, 1, 0.0, Om, nov, 1, st_);
+  bulk(nov, nov, naux, 1.0, BvoT_.p, 1, nov, BvoT_.p, nov, 1, 0.0, Om, nov, 1);
   mark("A2_int");
 
   // Too[(i,j),(a,b)] (into bufR), tau+/-, hole-hole ladder (CCRHS_A :2505)
@@ -255,9 +257,9 @@ void CCSD::residual(const double* t1, const double* T, double* om1, double* Om) 
   ladder_tau(taup_.p, taum_.p, Too, o, v, st_);
   g(oo, oo, naux, 1.0, Boo_.p, 1, oo, Boo_.p, oo, 1, 0.0, G1_.p, oo, 1, st_);  // [(k,i),(l,j)]
   perm(Gam_.p, G1_.p, o, o, o, o, oo * o, o, oo, 1, 1.0, 0.0, st_);               // [(k,l),(i,j)]
-  g(oo, oo, vv, 1.0, Joovv_.p, vv, 1, Too, 1, vv, 1.0, Gam_.p, oo, 1, st_);
+  bulk(oo, oo, vv, 1.0, Joovv_.p, vv, 1, Too, 1, vv, 1.0, Gam_.p, oo, 1);
   double* LADh = bufX2_.p;
-  g(oo, vv, oo, 1.0, Gam_.p, 1, oo, Too, vv, 1, 0.0, LADh, vv, 1, st_);           // [(i,j),(a,b)]
+  bulk(oo, vv, oo, 1.0, Gam_.p, 1, oo, Too, vv, 1, 0.0, LADh, vv, 1);           // [(i,j),(a,b)]
   perm(Om, LADh, o, v, o, v, (long)o * vv, v, vv, 1, 1.0, 1.0, st_);
   mark("B2_hole_ladder");
 
@@ -271,22 +273,22 @@ void CCSD::residual(const double* t1, const double* T, double* om1, double* Om) 
   perm(U, T, 1, 1, 1, nov2, 0, 0, 0, 1, 2.0, 0.0, st_);
   perm(U, Tx, 1, 1, 1, nov2, 0, 0, 0, 1, -1.0, 1.0, st_);
   double* R = bufR_.p;
-  g(vv, oo, naux, 1.0, Bvv_.p, 1, vv, Boo_.p, oo, 1, 0.0, R, oo, 1, st_);  // [(a,c),(k,i)] = (ki~|ac~)
+  bulk(vv, oo, naux, 1.0, Bvv_.p, 1, vv, Boo_.p, oo, 1, 0.0, R, oo, 1);  // [(a,c),(k,i)] = (ki~|ac~)
   double* C = bufC_.p;
   double* D = bufD_.p;
   perm(C, R, o, v, o, v, 1, (long)v * oo, o, oo, 1.0, 0.0, st_);   // C[(i,a),(k,c)]
   perm(D, R, o, v, o, v, 1, (long)v * oo, o, oo, -1.0, 0.0, st_);  // D = -(ac~|ki~)
   mark("CD_int");
-  g(nov, nov, nov, -0.5, Tx, nov, 1, K_.p, nov, 1, 1.0, C, nov, 1, st_);
-  g(nov, nov, naux, 2.0, BvoT_.p, 1, nov, Bov_.p, nov, 1, 1.0, D, nov, 1, st_);
-  g(nov, nov, nov, 0.5, U, nov, 1, L_.p, nov, 1, 1.0, D, nov, 1, st_);
+  bulk(nov, nov, nov, -0.5, Tx, nov, 1, K_.p, nov, 1, 1.0, C, nov, 1);
+  bulk(nov, nov, naux, 2.0, BvoT_.p, 1, nov, Bov_.p, nov, 1, 1.0, D, nov, 1);
+  bulk(nov, nov, nov, 0.5, U, nov, 1, L_.p, nov, 1, 1.0, D, nov, 1);
   mark("CD_intermediates");
   double* P = bufR_.p;
   double* X2 = bufX2_.p;
-  g(nov, nov, nov, 1.0, C, nov, 1, Tx, nov, 1, 0.0, P, nov, 1, st_);
+  bulk(nov, nov, nov, 1.0, C, nov, 1, Tx, nov, 1, 0.0, P, nov, 1);
   perm(X2, P, 1, 1, 1, nov2, 0, 0, 0, 1, -0.5, 0.0, st_);
   perm(X2, P, o, v, o, v, v, nov, (long)v * nov, 1, -1.0, 1.0, st_);
-  g(nov, nov, nov, 0.5, D, nov, 1, U, nov, 1, 1.0, X2, nov, 1, st_);
+  bulk(nov, nov, nov, 0.5, D, nov, 1, U, nov, 1, 1.0, X2, nov, 1);
   mark("CD_contract");
 
   // E terms (CCRHS_EI/EFCK/E :3158,8604,2280)
@@ -306,7 +308,7 @@ void CCSD::residual(const double* t1, const double* T, double* om1, double* Om) 
   mark("sym");
 
   // singles (CCRHS_G/H/I/J :2986,2858,2450,2686)
-  g(naux, nov, nov, 1.0, Bov_.p, nov, 1, U, nov, 1, 0.0, Z_.p, nov, 1, st_);
+  bulk(naux, nov, nov, 1.0, Bov_.p, nov, 1, U, nov, 1, 0.0, Z_.p, nov, 1);
   g(o, v, (long)naux * v, 1.0, Z_.p, v, 1, Bvv_.p, 1, v, 0.0, om1, v, 1, st_, 1, 0, 0, 0, v, nov, vv);
   g(oo, nov, naux, 1.0, Boo_.p, 1, oo, Bov_.p, nov, 1, 0.0, W_.p, nov, 1, st_);
   g(o, v, (long)o * nov, -1.0, W_.p, nov, 1, U, 1, nov, 1.0, om1, v, 1, st_, 1, 0, 0, 0, nov,

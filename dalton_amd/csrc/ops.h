@@ -19,6 +19,18 @@ inline void g(long M, long N, long K, double alpha, const double* A, long a_m, l
   gemm(d, prec, s);
 }
 
+// GEMM call with a fixed arithmetic and stream (used for the bulk contractions
+// whose precision follows the caller's choice).
+struct Gemm {
+  GemmPrec prec;
+  hipStream_t st;
+  void operator()(long M, long N, long K, double alpha, const double* A, long a_m, long a_k, const double* B,
+                  long b_k, long b_n, double beta, double* C, long c_m, long c_n, int batch = 1, long a_b = 0,
+                  long b_b = 0, long c_b = 0) const {
+    g(M, N, K, alpha, A, a_m, a_k, B, b_k, b_n, beta, C, c_m, c_n, st, batch, a_b, b_b, c_b, 0, 0, 0, prec);
+  }
+};
+
 inline void perm(double* out, const double* in, long n0, long n1, long n2, long n3, long s0, long s1,
                  long s2, long s3, double alpha, double beta, hipStream_t st) {
   const long n[4] = {n0, n1, n2, n3};

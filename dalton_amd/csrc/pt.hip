@@ -196,22 +196,23 @@ void CCSD::pt_energy(long first, long count, long stride, double* e4, double* e5
   auto Qrq = [&](int r, int q) { return Qp + ((long)r * o + q) * o * v; };       // [l][z]
   long idx = 0;
   const long npv = (vv + v) / 2;
+  const Gemm bulk{bulk_, st_};
   for (long t = first; t < ntr && idx < n; t += stride, ++idx) {
     const int i = tri_i_[t], j = tri_j_[t], k = tri_k_[t];
     // virtual terms
-    g(v, vv, v, 1.0, Tpq(i, j), nov, 1, G + (long)k * vvv, vv, 1, 0.0, W, vv, 1, st_);
-    g(v, vv, v, 1.0, Tpq(i, k), nov, 1, Gp + (long)j * vvv, vv, 1, 1.0, W, vv, 1, st_);
-    g(v, v, v, 1.0, Tpq(j, i), nov, 1, G + (long)k * vvv, vv, 1, 1.0, W, v, 1, st_, v, 0, v, vv);
-    g(v, v, v, 1.0, Tpq(j, k), nov, 1, Gp + (long)i * vvv, vv, 1, 1.0, W, v, 1, st_, v, 0, v, vv);
-    g(vv, v, v, 1.0, G + (long)j * vvv, 1, vv, Tpq(k, i), 1, nov, 1.0, W, v, 1, st_);
-    g(vv, v, v, 1.0, Gp + (long)i * vvv, 1, vv, Tpq(k, j), 1, nov, 1.0, W, v, 1, st_);
+    bulk(v, vv, v, 1.0, Tpq(i, j), nov, 1, G + (long)k * vvv, vv, 1, 0.0, W, vv, 1);
+    bulk(v, vv, v, 1.0, Tpq(i, k), nov, 1, Gp + (long)j * vvv, vv, 1, 1.0, W, vv, 1);
+    bulk(v, v, v, 1.0, Tpq(j, i), nov, 1, G + (long)k * vvv, vv, 1, 1.0, W, v, 1, v, 0, v, vv);
+    bulk(v, v, v, 1.0, Tpq(j, k), nov, 1, Gp + (long)i * vvv, vv, 1, 1.0, W, v, 1, v, 0, v, vv);
+    bulk(vv, v, v, 1.0, G + (long)j * vvv, 1, vv, Tpq(k, i), 1, nov, 1.0, W, v, 1);
+    bulk(vv, v, v, 1.0, Gp + (long)i * vvv, 1, vv, Tpq(k, j), 1, nov, 1.0, W, v, 1);
     // occupied terms
-    g(vv, v, o, -1.0, Tv + (long)i * vv * o, o, 1, Qrq(k, j), v, 1, 1.0, W, v, 1, st_);
-    g(v, v, o, -1.0, Qrq(j, k), 1, v, Tv + (long)i * vv * o, 1, o, 1.0, W, v, 1, st_, v, 0, vo, vv);
-    g(v, v, o, -1.0, Tv + (long)j * vv * o, o, 1, Qrq(k, i), v, 1, 1.0, W, vv, 1, st_, v, vo, 0, v);
-    g(v, v, o, -1.0, Qrq(i, k), 1, v, Tv + (long)j * vv * o, 1, o, 1.0, W, vv, 1, st_, v, 0, vo, v);
-    g(v, v, o, -1.0, Qrq(j, i), 1, v, Tv + (long)k * vv * o, 1, vo, 1.0, W, v, 1, st_, v, 0, o, vv);
-    g(v, v, o, -1.0, Qrq(i, j), 1, v, Tv + (long)k * vv * o, 1, vo, 1.0, W, vv, 1, st_, v, 0, o, v);
+    bulk(vv, v, o, -1.0, Tv + (long)i * vv * o, o, 1, Qrq(k, j), v, 1, 1.0, W, v, 1);
+    bulk(v, v, o, -1.0, Qrq(j, k), 1, v, Tv + (long)i * vv * o, 1, o, 1.0, W, v, 1, v, 0, vo, vv);
+    bulk(v, v, o, -1.0, Tv + (long)j * vv * o, o, 1, Qrq(k, i), v, 1, 1.0, W, vv, 1, v, vo, 0, v);
+    bulk(v, v, o, -1.0, Qrq(i, k), 1, v, Tv + (long)j * vv * o, 1, o, 1.0, W, vv, 1, v, 0, vo, v);
+    bulk(v, v, o, -1.0, Qrq(j, i), 1, v, Tv + (long)k * vv * o, 1, vo, 1.0, W, v, 1, v, 0, o, vv);
+    bulk(v, v, o, -1.0, Qrq(i, j), 1, v, Tv + (long)k * vv * o, 1, vo, 1.0, W, vv, 1, v, 0, o, v);
     // energy
     const double mult = (i == j || j == k) ? 3.0 : 6.0;
     check_span(W, 0, vvv - 1, sizeof(double), "pt W");

@@ -106,6 +106,14 @@ int ccgpu_profile_stop(char* buf, int len);
 int ccgpu_dgemm(char transa, char transb, int m, int n, int k, double alpha, const double* a,
                 int lda, const double* b, int ldb, double beta, double* c, int ldc);
 
+/* The same operation with the mixed-precision arithmetic of CCGPU_MIXED:
+ * operands rounded to FP32 and split into bf16 hi + lo, products
+ * hi*hi + hi*lo + lo*hi on the bf16 matrix cores, FP32 accumulation,
+ * alpha/beta applied in FP64 (no reference counterpart: the precision
+ * variant of the DGEMM drop-in above). */
+int ccgpu_dgemm_mixed(char transa, char transb, int m, int n, int k, double alpha, const double* a,
+                      int lda, const double* b, int ldb, double beta, double* c, int ldc);
+
 #ifdef __cplusplus
 }
 #endif

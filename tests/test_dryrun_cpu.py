@@ -41,6 +41,10 @@ for (o, v, naux) in {shapes!r}:
     assert rc == 0, L.ccgpu_last_error()
     rc = L.ccgpu_ccsd_iterate(h, 0, C.byref(e), C.byref(n))  # workspace comes back after (T)
     assert rc == 0, L.ccgpu_last_error()
+    rc = L.ccgpu_ccsd_iterate(h, 1, C.byref(e), C.byref(n))  # mixed-precision iteration
+    assert rc == 0, L.ccgpu_last_error()
+    rc = L.ccgpu_pt_partial(h, 1, 0, 2, 1, C.byref(e), C.byref(n), C.byref(fl), C.byref(nt))
+    assert rc == 0, L.ccgpu_last_error()
     assert L.ccgpu_finalize(h) == 0
     print("ok", o, v, naux, flush=True)
 """

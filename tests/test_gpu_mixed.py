@@ -1,0 +1,81 @@
+"""Mixed-precision path (BF16x3 split products, FP32 accumulation) against
+FP64 and the reference's printed energies.  Tolerance for energies: 1e-6 Eh
+(mixed path); for single contractions: relative 1e-4 of the result scale."""
+import numpy as np
+import pytest
+
+from conftest import load_molecule
+from oracle import ccsd as occ
+from oracle import triples
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("ta,tb,m,n,k", [("N", "N", 130, 257, 33), ("T", "N", 200, 77, 1029), ("N", "T", 64, 300, 17),
+                                          ("T", "T", 129, 129, 129), ("N", "N", 1, 1, 1)])
+def test_dgemm_mixed_matches_numpy(ta, tb, m, n, k):
+    import dalton_amd as d
+
+    print(f"dgemm_mixed {ta}{tb} m={m} n={n} k={k}", flush=True)
+    rng = np.random.default_rng(m + n + k)
+    A = rng.standard_normal((k, m) if ta == "T" else (m, k))
+    B = rng.standard_normal((n, k) if tb == "T" else (k, n))
+    C0 = rng.standard_normal((m, n))
+    ref = 1.5 * (A.T if ta == "T" else A) @ (B.T if tb == "T" else B) - 0.5 * C0
+    got = d.dgemm(ta, tb, 1.5, A, B, -0.5, C0, precision=d.MIXED)
+    # bf16x3: ~2^-16 relative per product, random-sign accumulation
+    assert np.abs(got - ref).max() < 1e-4 * np.sqrt(k) + 1e-12
+
+
+def test_mixed_residual_close_to_fp64():
+    import dalton_amd as d
+
+    eps, B, o, _ = load_molecule("h2o_ccpvdz_test")
+    v = eps.size - o
+    print("mixed rhs h2o", flush=True)
+    rng = np.random.default_rng(4)
+    t1 = 0.02 * rng.standard_normal((v, o))
+    t2 = 0.02 * rng.standard_normal((v, o, v, o))
+    t2 = t2 + t2.transpose(2, 3, 0, 1)
+    r1, r2 = occ.residual(occ.DFSystem(eps, B, o), t1, t2)
+    with d.CCContext(eps, B, o) as c:
+        m1, m2 = c.rhs(occ.pack_t1(t1), occ.pack_t2(t2), precision=d.MIXED)
+    s = np.abs(r2).max()
+    assert np.abs(m2 - occ.pack_t2(r2)).max() < 1e-4 * s
+    assert np.abs(m1 - occ.pack_t1(r1)).max() < 1e-4 * s
+
+
+@pytest.mark.parametrize("name,threnr", [("h2o_ccpvdz_test", 1e-10), ("hf_631gss", 1e-10)])
+def test_mixed_ccsd_t_energy_within_1uEh(name, threnr, golden):
+    import dalton_amd as d
+
+    eps, B, o, _ = load_molecule(name)
+    g = golden[name]
+    print(f"mixed CCSD(T) {name}", flush=True)
+    with d.CCContext(eps, B, o) as c:
+        res = c.ccsd_energy(maxit=80, threnr=threnr, precision=d.MIXED)
+        e4, e5 = c.pt(precision=d.MIXED)
+    e_ccsd = res["e_ccsd"] + g["e_scf"]
+    assert abs(e_ccsd - g["e_ccsd"]) < 1e-6
+    et_ref = g["e_t"] if "e_t" in g else g["e4"] + g["e5"]
+    assert abs(e4 + e5 - et_ref) < 1e-6
+    assert abs(e_ccsd + e4 + e5 - (g["e_ccsd"] + et_ref)) < 1e-6
+
+
+def test_mixed_pt_close_to_oracle():
+    import dalton_amd as d
+    from dalton_amd.synthetic import make_system
+
+    o, v, naux = 6, 30, 60
+    eps, B = make_system(o, v, naux, seed=5)
+    print("mixed (T) synthetic", flush=True)
+    rng = np.random.default_rng(9)
+    t1 = 0.05 * rng.standard_normal((v, o))
+    t2 = 0.05 * rng.standard_normal((v, o, v, o))
+    t2 = t2 + t2.transpose(2, 3, 0, 1)
+    r4, r5 = triples.pt_energy(eps, B, o, t1, t2)
+    with d.CCContext(eps, B, o) as c:
+        c.set_amplitudes(occ.pack_t1(t1), occ.pack_t2(t2))
+        e4, e5 = c.pt(precision=d.MIXED)
+    assert abs(e4 - r4) < 1e-4 * abs(r4)
+    assert abs(e5 - r5) < 1e-4 * abs(r5) + 1e-4 * abs(r4)
