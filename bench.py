@@ -157,9 +157,9 @@ def main():
     F = f_ccsd(o, v, naux)
     pt = {}
     if args.pt_triples > 0:
-        _, _, _, ntr = ctx.pt_partial(0, 0, 1)  # build (T) integral layouts (untimed setup)
-        stride = max(1, ntr // args.pt_triples)
         for label, prec in (("mixed", d.MIXED), ("fp64", d.FP64)):
+            _, _, _, ntr = ctx.pt_partial(0, 0, 1, precision=prec)  # integral layouts of this precision (untimed)
+            stride = max(1, ntr // args.pt_triples)
             sync()
             profile_start()
             t0 = time.perf_counter()

@@ -65,7 +65,8 @@ class CCSD {
   void ladder(const double* T, double* Om);
   void mark(const char* name);
   void ensure_work();
-  void pt_prepare();
+  void pt_prepare(GemmPrec mode);
+  void pt_triples_mixed(long first, long n, long stride);
 
   Dims dm_;
   int mxdiis_;
@@ -92,7 +93,12 @@ class CCSD {
   std::vector<double> Amat_;
   DBuf<double> scal_;
   // (T)
+  GemmPrec pt_mode_ = GemmPrec::F64;
   DBuf<double> G_, Gp_, Tvvo_, Q_, Wt_, ptpart_, etrip_;
+  // (T), mixed path: (yd|zk) packed as bf16 hi/lo [k][(y,z)][d] in both (y,z)
+  // orders, per-triple packed slices, FP32 W
+  DBuf<unsigned short> GTh_, GTl_, GpTh_, GpTl_, Psm_;
+  DBuf<float> W32_;
 };
 
 }  // namespace ccgpu

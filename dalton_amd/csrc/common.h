@@ -115,6 +115,30 @@ enum class GemmPrec { F64 = 0, F32 = 1, BF16X3 = 2 };
 
 void gemm(const GemmDesc& d, GemmPrec prec, hipStream_t s);
 
+// ---- BF16x3 building blocks (operands pre-split into bf16 hi/lo arrays laid
+// out [batch][row][k], row length Kp, rows padded to 128, Kp to 64).
+struct BTerm {
+  const unsigned short *Ah, *Al, *Bh, *Bl;
+  long Kp;         // row length (padded K) of this term's A and B
+  long a_bs, b_bs;  // batch strides (elements) of the packed A and B
+};
+struct BMulti {
+  BTerm t[4];
+  int nterm;
+  long M, N, Mp, Np;
+  double alpha, beta;
+  void* C;
+  long c_m, c_n, c_b;
+};
+long bf16x3_kpad(long K);
+long bf16x3_rpad(long R);
+// Pack scale * A(r,k) (element at src + r*s_r + koff(k) + b*s_b) into hi/lo.
+void bf16x3_pack(const double* src, long rows, long K, long s_r, long s_k, long s_b, int batch, long Rp, long Kp,
+                 float scale, unsigned short* hi, unsigned short* lo, hipStream_t s, long s_ko = 0, long Kin = 0);
+// C = alpha * sum_t A_t B_t^T + beta * C (C float or double, element strides)
+void bf16x3_multi(const BTerm* terms, int nterm, long M, long N, int batch, void* C, bool c_float, long c_m,
+                  long c_n, long c_b, double alpha, double beta, double flops, hipStream_t s);
+
 // Convenience: row-major C[M][N] (+)= alpha * op(A) op(B) for contiguous matrices.
 // ta=false: A is [M][lda]; ta=true: A is [K][lda] used transposed.
 void gemm_rm(bool ta, bool tb, long M, long N, long K, double alpha, const void* A, long lda,

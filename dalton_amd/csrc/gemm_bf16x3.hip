@@ -32,6 +32,7 @@ struct PackDesc {
   long s_r, s_k, s_ko, Kin;  // element (r,k) at src + r*s_r + koff(k) + b*s_b
   long s_b;
   long Rp, Kp;  // padded sizes of the packed arrays
+  float scale;  // multiplies every element
 };
 
 __device__ __forceinline__ long pk_koff(const PackDesc& p, long k) {
@@ -53,7 +54,7 @@ __global__ void k_pack_bf16x2(PackDesc p, unsigned short* __restrict__ hi, unsig
 #pragma unroll
     for (int q = 0; q < 32; q += 8) {
       const long r = tr * 32 + tx, k = tk * 32 + ty + q;
-      tile[ty + q][tx] = (r < p.rows && k < p.K) ? (float)src[r * p.s_r + pk_koff(p, k)] : 0.0f;  // [k][r]
+      tile[ty + q][tx] = (r < p.rows && k < p.K) ? p.scale * (float)src[r * p.s_r + pk_koff(p, k)] : 0.0f;  // [k][r]
     }
     __syncthreads();
 #pragma unroll
@@ -72,7 +73,7 @@ __global__ void k_pack_bf16x2(PackDesc p, unsigned short* __restrict__ hi, unsig
     for (int q = 0; q < 32; q += 8) {
       const long r = tr * 32 + ty + q, k = tk * 32 + tx;
       if (r < p.Rp && k < p.Kp) {
-        const float f = (r < p.rows && k < p.K) ? (float)src[r * p.s_r + pk_koff(p, k)] : 0.0f;
+        const float f = (r < p.rows && k < p.K) ? p.scale * (float)src[r * p.s_r + pk_koff(p, k)] : 0.0f;
         const unsigned short h = f2bf(f);
         const long o = (b * p.Rp + r) * p.Kp + k;
         hi[o] = h;
@@ -96,40 +97,54 @@ __device__ __forceinline__ void tile_of3(long b, long tiles_m, long tiles_n, lon
 }
 
 constexpr int TM = 128, TN = 128, TK = 64, LDK = TK + 8;
+
 constexpr int NCH = TM * TK / 8 / 256;  // 16-byte chunks per thread per array per K step
 
 // Block tile 128x128, K step 64 (two MFMA k-slices of 32): each K step
 // keeps every wave busy for 96 MFMAs, long enough to cover the latency of
-// the next step's global loads, which are issued before the MFMAs.
-__global__ __launch_bounds__(256, 2) void k_gemm_bf16x3(const unsigned short* __restrict__ Ah,
-                                                        const unsigned short* __restrict__ Al,
-                                                        const unsigned short* __restrict__ Bh,
-                                                        const unsigned short* __restrict__ Bl, long Mp, long Np,
-                                                        long Kp, GemmDesc d) {
+// the next step's global loads, which are issued before the MFMAs.  The
+// output tile accumulates up to four products sum_t A_t B_t (each term with
+// its own packed operands and K), so several contributions to one output
+// array cost a single read-modify-write of it.
+template <typename OutT>
+__global__ __launch_bounds__(256, 2) void k_gemm_bf16x3(BMulti p) {
   __shared__ __attribute__((aligned(16))) unsigned short sA[2][TM][LDK];
   __shared__ __attribute__((aligned(16))) unsigned short sB[2][TN][LDK];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
-  const long tiles_m = Mp / TM, tiles_n = Np / TN;
+  const long tiles_m = p.Mp / TM, tiles_n = p.Np / TN;
   long tm, tn;
   tile_of3(blockIdx.x, tiles_m, tiles_n, tm, tn);
   const long m0 = tm * TM, n0 = tn * TN, z = blockIdx.z;
-  const unsigned short* pAh = Ah + (z * Mp + m0) * Kp;
-  const unsigned short* pAl = Al + (z * Mp + m0) * Kp;
-  const unsigned short* pBh = Bh + (z * Np + n0) * Kp;
-  const unsigned short* pBl = Bl + (z * Np + n0) * Kp;
-  // thread t owns 16-byte chunks c = t + 256 q: row c / (TK/8), k-chunk c % (TK/8).
-  // Plain register arrays with compile-time indices (kept out of scratch).
   constexpr int CPR = TK / 8;
-  u32x4 rah[NCH], ral[NCH], rbh[NCH], rbl[NCH];
-  long goff[NCH];
+  f4 acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+  const int lr = lane & 15, lk = (lane >> 4) * 8;
   int loff[NCH];
 #pragma unroll
   for (int q = 0; q < NCH; ++q) {
     const int c = tid + q * 256;
-    goff[q] = (long)(c / CPR) * Kp + (c % CPR) * 8;
     loff[q] = (c / CPR) * LDK + (c % CPR) * 8;
   }
+  for (int t = 0; t < p.nterm; ++t) {
+    const BTerm& T = p.t[t];
+    const long Kp = T.Kp;
+    const unsigned short* pAh = T.Ah + z * T.a_bs + m0 * Kp;
+    const unsigned short* pAl = T.Al + z * T.a_bs + m0 * Kp;
+    const unsigned short* pBh = T.Bh + z * T.b_bs + n0 * Kp;
+    const unsigned short* pBl = T.Bl + z * T.b_bs + n0 * Kp;
+    // thread t owns 16-byte chunks c = t + 256 q: row c / (TK/8), k-chunk c % (TK/8).
+    // Plain register arrays with compile-time indices (kept out of scratch).
+    u32x4 rah[NCH], ral[NCH], rbh[NCH], rbl[NCH];
+    long goff[NCH];
+#pragma unroll
+    for (int q = 0; q < NCH; ++q) {
+      const int c = tid + q * 256;
+      goff[q] = (long)(c / CPR) * Kp + (c % CPR) * 8;
+    }
 #define CCGPU_BF3_LOAD(k0)                                                       \
   _Pragma("unroll") for (int q = 0; q < NCH; ++q) {                              \
     rah[q] = *reinterpret_cast<const u32x4*>(pAh + goff[q] + (k0));              \
@@ -144,49 +159,45 @@ __global__ __launch_bounds__(256, 2) void k_gemm_bf16x3(const unsigned short* __
     *reinterpret_cast<u32x4*>(&sB[0][0][0] + loff[q]) = rbh[q];                  \
     *reinterpret_cast<u32x4*>(&sB[1][0][0] + loff[q]) = rbl[q];                  \
   }
-  f4 acc[4][4];
-#pragma unroll
-  for (int i = 0; i < 4; ++i)
-#pragma unroll
-    for (int j = 0; j < 4; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
-  const int lr = lane & 15, lk = (lane >> 4) * 8;
-  CCGPU_BF3_LOAD(0)
-  for (long k0 = 0; k0 < Kp; k0 += TK) {
-    __syncthreads();
-    CCGPU_BF3_STORE()
-    __syncthreads();
-    if (k0 + TK < Kp) {
-      CCGPU_BF3_LOAD(k0 + TK)
-    }
-#pragma unroll
-    for (int ks = 0; ks < TK; ks += 32) {
-      bf16x8 ah[4], al[4], bh[4], bl[4];
-#pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        ah[t] = *reinterpret_cast<const bf16x8*>(&sA[0][wm * 64 + t * 16 + lr][ks + lk]);
-        al[t] = *reinterpret_cast<const bf16x8*>(&sA[1][wm * 64 + t * 16 + lr][ks + lk]);
-        bh[t] = *reinterpret_cast<const bf16x8*>(&sB[0][wn * 64 + t * 16 + lr][ks + lk]);
-        bl[t] = *reinterpret_cast<const bf16x8*>(&sB[1][wn * 64 + t * 16 + lr][ks + lk]);
+    CCGPU_BF3_LOAD(0)
+    for (long k0 = 0; k0 < Kp; k0 += TK) {
+      __syncthreads();
+      CCGPU_BF3_STORE()
+      __syncthreads();
+      if (k0 + TK < Kp) {
+        CCGPU_BF3_LOAD(k0 + TK)
       }
-      // the three split products of one accumulator are 16 MFMAs apart
 #pragma unroll
-      for (int i = 0; i < 4; ++i)
+      for (int ks = 0; ks < TK; ks += 32) {
+        bf16x8 ah[4], al[4], bh[4], bl[4];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
+        for (int u = 0; u < 4; ++u) {
+          ah[u] = *reinterpret_cast<const bf16x8*>(&sA[0][wm * 64 + u * 16 + lr][ks + lk]);
+          al[u] = *reinterpret_cast<const bf16x8*>(&sA[1][wm * 64 + u * 16 + lr][ks + lk]);
+          bh[u] = *reinterpret_cast<const bf16x8*>(&sB[0][wn * 64 + u * 16 + lr][ks + lk]);
+          bl[u] = *reinterpret_cast<const bf16x8*>(&sB[1][wn * 64 + u * 16 + lr][ks + lk]);
+        }
+        // the three split products of one accumulator are 16 MFMAs apart
 #pragma unroll
-      for (int i = 0; i < 4; ++i)
+        for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+          for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
 #pragma unroll
-      for (int i = 0; i < 4; ++i)
+        for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+          for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+          for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+      }
     }
-  }
 #undef CCGPU_BF3_LOAD
 #undef CCGPU_BF3_STORE
+    __syncthreads();
+  }
   // D[i][j] of a 16x16 f32 MFMA tile: lane = j + 16*(i/4), element i%4.
-  double* C = static_cast<double*>(d.C) + z * d.c_b;
+  OutT* C = static_cast<OutT*>(p.C) + z * p.c_b;
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
@@ -195,11 +206,11 @@ __global__ __launch_bounds__(256, 2) void k_gemm_bf16x3(const unsigned short* __
       for (int r = 0; r < 4; ++r) {
         const long m = m0 + wm * 64 + i * 16 + (lane >> 4) * 4 + r;
         const long n = n0 + wn * 64 + j * 16 + lr;
-        if (m < d.M && n < d.N) {
-          double* p = C + m * d.c_m + n * d.c_n;
-          double v = d.alpha * (double)acc[i][j][r];
-          if (d.beta != 0.0) v += d.beta * *p;
-          *p = v;
+        if (m < p.M && n < p.N) {
+          OutT* q = C + m * p.c_m + n * p.c_n;
+          double v = p.alpha * (double)acc[i][j][r];
+          if (p.beta != 0.0) v += p.beta * (double)*q;
+          *q = static_cast<OutT>(v);
         }
       }
 }
@@ -210,10 +221,33 @@ DBuf<unsigned short>& pack_buf(int i) {
   return p[i];
 }
 
-void pack(const PackDesc& p, int batch, unsigned short* hi, unsigned short* lo, bool rfast, hipStream_t s) {
-  const long rt = (p.Rp + 31) / 32, kt = (p.Kp + 31) / 32;
-  CCGPU_REQUIRE(kt < 65536 && batch < 65536, "bf16x3 pack: grid too large");
+template <typename OutT>
+void launch_multi(const BMulti& p, int batch, hipStream_t s) {
+  const long ntiles = (p.Mp / TM) * (p.Np / TN);
+  CCGPU_REQUIRE(ntiles < (1L << 31) && batch < 65536, "bf16x3 gemm: grid too large");
+  CCGPU_LAUNCH(k_gemm_bf16x3<OutT>, dim3((unsigned)ntiles, 1, (unsigned)batch), dim3(256), 0, s, p);
+  CCGPU_HIPX(hipGetLastError());
+}
+
+}  // namespace
+
+long bf16x3_kpad(long K) { return std::max<long>(TK, (K + TK - 1) / TK * TK); }
+long bf16x3_rpad(long R) { return (R + TM - 1) / TM * TM; }
+
+void bf16x3_pack(const double* src, long rows, long K, long s_r, long s_k, long s_b, int batch, long Rp, long Kp,
+                 float scale, unsigned short* hi, unsigned short* lo, hipStream_t s, long s_ko, long Kin) {
+  PackDesc p{src, rows, K, s_r, s_k, s_ko, Kin, s_b, Rp, Kp, scale};
+  if (rows > 0 && K > 0) {
+    long hiofs = (rows - 1) * s_r + (batch - 1) * s_b;
+    hiofs += (Kin > 0 && Kin < K) ? ((K - 1) / Kin) * s_ko + (Kin - 1) * s_k : (K - 1) * s_k;
+    check_span(src, 0, hiofs, sizeof(double), "bf16x3 pack source");
+  }
+  check_span(hi, 0, (long)batch * Rp * Kp - 1, 2, "bf16x3 pack hi");
+  check_span(lo, 0, (long)batch * Rp * Kp - 1, 2, "bf16x3 pack lo");
+  const long rt = (Rp + 31) / 32, kt = (Kp + 31) / 32;
+  CCGPU_REQUIRE(kt < 65536 && batch < 65536 && rt < (1L << 31), "bf16x3 pack: grid too large");
   dim3 grid((unsigned)rt, (unsigned)kt, (unsigned)batch);
+  const bool rfast = s_r == 1 && s_k != 1;
   if (rfast)
     CCGPU_LAUNCH(k_pack_bf16x2<true>, grid, dim3(32, 8), 0, s, p, hi, lo);
   else
@@ -221,31 +255,53 @@ void pack(const PackDesc& p, int batch, unsigned short* hi, unsigned short* lo, 
   CCGPU_HIPX(hipGetLastError());
 }
 
-}  // namespace
+// Multi-term product into a float or double output (see k_gemm_bf16x3).
+// Every term's packed A must cover Mp rows (per batch, at stride a_bs) and
+// B Np rows, both with row length Kp; rows past M/N are read, never used.
+void bf16x3_multi(const BTerm* terms, int nterm, long M, long N, int batch, void* C, bool c_float, long c_m,
+                  long c_n, long c_b, double alpha, double beta, double flops, hipStream_t s) {
+  CCGPU_REQUIRE(nterm >= 1 && nterm <= 4, "bf16x3_multi: 1..4 terms");
+  BMulti p{};
+  p.nterm = nterm;
+  for (int t = 0; t < nterm; ++t) {
+    p.t[t] = terms[t];
+    CCGPU_REQUIRE(terms[t].Kp % TK == 0, "bf16x3_multi: Kp must be a multiple of 64");
+  }
+  p.M = M; p.N = N; p.Mp = bf16x3_rpad(M); p.Np = bf16x3_rpad(N);
+  p.alpha = alpha; p.beta = beta; p.C = C; p.c_m = c_m; p.c_n = c_n; p.c_b = c_b;
+  const size_t esz = c_float ? sizeof(float) : sizeof(double);
+  for (int t = 0; t < nterm; ++t) {
+    const BTerm& T = terms[t];
+    check_span(T.Ah, 0, (batch - 1) * T.a_bs + p.Mp * T.Kp - 1, 2, "bf16x3 A");
+    check_span(T.Al, 0, (batch - 1) * T.a_bs + p.Mp * T.Kp - 1, 2, "bf16x3 A lo");
+    check_span(T.Bh, 0, (batch - 1) * T.b_bs + p.Np * T.Kp - 1, 2, "bf16x3 B");
+    check_span(T.Bl, 0, (batch - 1) * T.b_bs + p.Np * T.Kp - 1, 2, "bf16x3 B lo");
+  }
+  check_span(C, 0, (M - 1) * c_m + (N - 1) * c_n + (batch - 1) * c_b, esz, "bf16x3 C");
+  const int id = prof_begin("k_gemm_bf16x3", flops, s);
+  if (c_float)
+    launch_multi<float>(p, batch, s);
+  else
+    launch_multi<double>(p, batch, s);
+  prof_end(id, s);
+}
 
 void check_gemm(const GemmDesc& d, size_t esz);
 
 void gemm_bf16x3(const GemmDesc& d, hipStream_t s) {
   if (d.M <= 0 || d.N <= 0 || d.batch <= 0) return;
   check_gemm(d, sizeof(double));
-  const long Mp = (d.M + TM - 1) / TM * TM, Np = (d.N + TN - 1) / TN * TN;
-  const long Kp = std::max<long>(TK, (d.K + TK - 1) / TK * TK);
+  const long Mp = bf16x3_rpad(d.M), Np = bf16x3_rpad(d.N), Kp = bf16x3_kpad(d.K);
   const long na = (long)d.batch * Mp * Kp, nb = (long)d.batch * Np * Kp;
   for (int i = 0; i < 2; ++i) pack_buf(i).ensure(na);
   for (int i = 2; i < 4; ++i) pack_buf(i).ensure(nb);
-  PackDesc pa{static_cast<const double*>(d.A), d.M, d.K, d.a_m, d.a_k, d.a_ko, d.Kin, d.a_b, Mp, Kp};
-  PackDesc pb{static_cast<const double*>(d.B), d.N, d.K, d.b_n, d.b_k, d.b_ko, d.Kin, d.b_b, Np, Kp};
-  pack(pa, d.batch, pack_buf(0).p, pack_buf(1).p, d.a_m == 1 && d.a_k != 1, s);
-  pack(pb, d.batch, pack_buf(2).p, pack_buf(3).p, d.b_n == 1 && d.b_k != 1, s);
-  check_span(pack_buf(0).p, 0, na - 1, 2, "bf16x3 Ah");
-  check_span(pack_buf(2).p, 0, nb - 1, 2, "bf16x3 Bh");
-  const long ntiles = (Mp / TM) * (Np / TN);
-  CCGPU_REQUIRE(ntiles < (1L << 31) && d.batch < 65536, "bf16x3 gemm: grid too large");
-  const int id = prof_begin("k_gemm_bf16x3", 2.0 * d.M * d.N * d.K * d.batch, s);
-  CCGPU_LAUNCH(k_gemm_bf16x3, dim3((unsigned)ntiles, 1, (unsigned)d.batch), dim3(256), 0, s, pack_buf(0).p,
-               pack_buf(1).p, pack_buf(2).p, pack_buf(3).p, Mp, Np, Kp, d);
-  prof_end(id, s);
-  CCGPU_HIPX(hipGetLastError());
+  bf16x3_pack(static_cast<const double*>(d.A), d.M, d.K, d.a_m, d.a_k, d.a_b, d.batch, Mp, Kp, 1.0f,
+              pack_buf(0).p, pack_buf(1).p, s, d.a_ko, d.Kin);
+  bf16x3_pack(static_cast<const double*>(d.B), d.N, d.K, d.b_n, d.b_k, d.b_b, d.batch, Np, Kp, 1.0f,
+              pack_buf(2).p, pack_buf(3).p, s, d.b_ko, d.Kin);
+  BTerm t{pack_buf(0).p, pack_buf(1).p, pack_buf(2).p, pack_buf(3).p, Kp, Mp * Kp, Np * Kp};
+  bf16x3_multi(&t, 1, d.M, d.N, d.batch, d.C, false, d.c_m, d.c_n, d.c_b, d.alpha, d.beta,
+               2.0 * d.M * d.N * d.K * d.batch, s);
 }
 
 }  // namespace ccgpu

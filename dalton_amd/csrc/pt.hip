@@ -34,7 +34,8 @@ __device__ __forceinline__ void tri_dec(long p, long& lo, long& hi) {
 
 // One block per virtual pair a <= b, threads over c in [b, v).
 // Writes (E4, E5) block partials, already scaled by mult/3.
-__global__ void k_pt_energy(const double* __restrict__ W, const double* __restrict__ J,
+template <typename WT>
+__global__ void k_pt_energy(const WT* __restrict__ W, const double* __restrict__ J,
                             const double* __restrict__ t1, const double* __restrict__ eps, int i, int j,
                             int k, int o, int v, double scale, double* part) {
   __shared__ double sh[2][8];
@@ -51,7 +52,7 @@ __global__ void k_pt_energy(const double* __restrict__ W, const double* __restri
 #pragma unroll
     for (int s = 0; s < 6; ++s) {
       const int x = X[s], yy = Y[s], z = Z[s];
-      w[s] = W[(long)x * vv + (long)yy * v + z];
+      w[s] = (double)W[(long)x * vv + (long)yy * v + z];
       y[s] = t1[(long)i * v + x] * J[((long)j * v + yy) * nov + (long)k * v + z] +
              t1[(long)j * v + yy] * J[((long)i * v + x) * nov + (long)k * v + z] +
              t1[(long)k * v + z] * J[((long)i * v + x) * nov + (long)j * v + yy];
@@ -132,34 +133,65 @@ long CCSD::pt_ntriples() const {
 
 void CCSD::pt_release() {
   for (DBuf<double>* b : {&G_, &Gp_, &Tvvo_, &Q_, &Wt_, &ptpart_, &etrip_}) b->release();
+  for (DBuf<unsigned short>* b : {&GTh_, &GTl_, &GpTh_, &GpTl_, &Psm_}) b->release();
+  W32_.release();
   pt_ready_ = false;
 }
 
 // Integral and amplitude layouts for the triple loop (built once per call
 // sequence; the iteration workspace is released first to make room).
-void CCSD::pt_prepare() {
-  if (pt_ready_) return;
+void CCSD::pt_prepare(GemmPrec mode) {
+  if (pt_ready_ && pt_mode_ == mode) return;
+  pt_release();
   release_work();
   const int o = dm_.o, v = dm_.v, naux = dm_.naux;
   const long N = dm_.N, NN = N * N, nov = dm_.nov, vv = (long)v * v, vvv = vv * v;
-  G_.alloc((size_t)o * vvv);
-  Gp_.alloc((size_t)o * vvv);
+  const bool mixed = mode == GemmPrec::BF16X3;
   Tvvo_.alloc((size_t)o * vv * o);
   Q_.alloc((size_t)o * o * o * v);
-  Wt_.alloc(vvv);
-  ptpart_.alloc(2 * (size_t)(vv + v) / 2 + 2);
+  ptpart_.alloc((size_t)(vv + v) + 2);
   {
     // undressed virtual-virtual and occupied-occupied blocks of B
     DBuf<double> Bvv0((size_t)naux * vv), Boo0((size_t)naux * o * o), R((size_t)nov * o * o);
     perm(Bvv0.p, B_.p + o * N + o, 1, naux, v, v, 0, NN, N, 1, 1.0, 0.0, st_);
     perm(Boo0.p, B_.p, 1, naux, o, o, 0, NN, N, 1, 1.0, 0.0, st_);
-    // G[k][d][y][z] = (yd|zk) = sum_P B[P][d][y] B[P][k][z], batched over k
-    g(vv, v, naux, 1.0, Bvv0.p, 1, vv, Bov_.p, nov, 1, 0.0, G_.p, v, 1, st_, o, 0, v, vvv);
-    // Gp[k][d][y][z] = G[k][d][z][y]
-    perm(Gp_.p, G_.p, 1, (long)o * v, v, v, 0, vv, 1, v, 1.0, 0.0, st_);
+    if (!mixed) {
+      G_.alloc((size_t)o * vvv);
+      Gp_.alloc((size_t)o * vvv);
+      Wt_.alloc(vvv);
+      // G[k][d][y][z] = (yd|zk) = sum_P B[P][d][y] B[P][k][z], batched over k
+      g(vv, v, naux, 1.0, Bvv0.p, 1, vv, Bov_.p, nov, 1, 0.0, G_.p, v, 1, st_, o, 0, v, vvv);
+      // Gp[k][d][y][z] = G[k][d][z][y]
+      perm(Gp_.p, G_.p, 1, (long)o * v, v, v, 0, vv, 1, v, 1.0, 0.0, st_);
+    } else {
+      // packed transposes GT[k][(y,z)][d] = G[k][d][y][z] and GpT = the same of Gp,
+      // built one k at a time; 128 zero rows at the end absorb tile over-reads
+      const long Kpv = bf16x3_kpad(v);
+      const size_t ng = ((size_t)o * vv + 128) * Kpv;
+      GTh_.alloc(ng);
+      GTl_.alloc(ng);
+      GpTh_.alloc(ng);
+      GpTl_.alloc(ng);
+      W32_.alloc(vvv);
+      DBuf<double> Gk(vvv), Gpk(vvv);
+      for (int k = 0; k < o; ++k) {
+        g(vv, v, naux, 1.0, Bvv0.p, 1, vv, Bov_.p + (long)k * v, nov, 1, 0.0, Gk.p, v, 1, st_);
+        perm(Gpk.p, Gk.p, 1, v, v, v, 0, vv, 1, v, 1.0, 0.0, st_);
+        const long rp = (k == o - 1) ? vv + 128 : vv;
+        const size_t off = (size_t)k * vv * Kpv;
+        // element (r=(y,z), d) of the operand at Gk[d*vv + r]
+        bf16x3_pack(Gk.p, vv, v, 1, vv, 0, 1, rp, Kpv, 1.0f, GTh_.p + off, GTl_.p + off, st_);
+        bf16x3_pack(Gpk.p, vv, v, 1, vv, 0, 1, rp, Kpv, 1.0f, GpTh_.p + off, GpTl_.p + off, st_);
+      }
+      // per-triple slices: 6 T(p,q) [x][d], 6 Q_rq [z][l], 3 Tvvo slices [rows][l]
+      const long Rv = bf16x3_rpad(v), Kpo = bf16x3_kpad(o), Rvv = bf16x3_rpad(vv) + 128;
+      Psm_.alloc(2 * (6 * (size_t)Rv * Kpv + 6 * (size_t)Rv * Kpo + 3 * (size_t)Rvv * Kpo));
+      CCGPU_HIPX(hipStreamSynchronize(st_));
+    }
     // R[(k,c)][(l,j)] = (ck|lj);  Q[k][j][l][c] = R
     g(nov, (long)o * o, naux, 1.0, Bov_.p, 1, nov, Boo0.p, (long)o * o, 1, 0.0, R.p, (long)o * o, 1, st_);
     perm(Q_.p, R.p, o, o, o, v, (long)v * o * o, 1, o, (long)o * o, 1.0, 0.0, st_);
+    CCGPU_HIPX(hipStreamSynchronize(st_));
   }
   // Tvvo[i][a][b][l] = t_il^ab = T[(i,a),(l,b)]
   perm(Tvvo_.p, T_.p, o, v, v, o, (long)v * nov, nov, 1, v, 1.0, 0.0, st_);
@@ -174,11 +206,73 @@ void CCSD::pt_prepare() {
         tri_j_.push_back(j);
         tri_k_.push_back(k);
       }
+  pt_mode_ = mode;
   pt_ready_ = true;
 }
 
+// W for one triple, mixed path: the 12 terms as 4 multi-term BF16x3 GEMMs
+// grouped by output orientation, into the FP32 buffer W32.
+void CCSD::pt_triples_mixed(long t, long, long) {
+  const int o = dm_.o, v = dm_.v;
+  const long nov = dm_.nov, vv = (long)v * v;
+  const int i = tri_i_[t], j = tri_j_[t], k = tri_k_[t];
+  const long Kpv = bf16x3_kpad(v), Kpo = bf16x3_kpad(o), Rv = bf16x3_rpad(v), Rvv = bf16x3_rpad(vv) + 128;
+  const size_t sT = (size_t)Rv * Kpv, sQ = (size_t)Rv * Kpo, sP = (size_t)Rvv * Kpo;
+  unsigned short* base = Psm_.p;
+  const size_t half = 6 * sT + 6 * sQ + 3 * sP;  // hi block, then lo block
+  auto hiT = [&](int n) { return base + n * sT; };
+  auto hiQ = [&](int n) { return base + 6 * sT + n * sQ; };
+  auto hiP = [&](int n) { return base + 6 * sT + 6 * sQ + n * sP; };
+  // T(p,q)[x][d] = t_pq^{xd}: 0 (i,j) 1 (i,k) 2 (j,i) 3 (j,k) 4 (k,i) 5 (k,j)
+  const int tp[6][2] = {{i, j}, {i, k}, {j, i}, {j, k}, {k, i}, {k, j}};
+  for (int n = 0; n < 6; ++n)
+    bf16x3_pack(T_.p + (long)tp[n][0] * v * nov + (long)tp[n][1] * v, v, v, nov, 1, 0, 1, Rv, Kpv, 1.0f, hiT(n),
+                hiT(n) + half, st_);
+  // Q_rq[z][l] = (zr|lq): 0 (k,j) 1 (j,k) 2 (j,i) 3 (k,i) 4 (i,k) 5 (i,j)
+  const int qp[6][2] = {{k, j}, {j, k}, {j, i}, {k, i}, {i, k}, {i, j}};
+  for (int n = 0; n < 6; ++n)
+    bf16x3_pack(Q_.p + ((long)qp[n][0] * o + qp[n][1]) * o * v, v, o, 1, v, 0, 1, Rv, Kpo, 1.0f, hiQ(n),
+                hiQ(n) + half, st_);
+  // -t_pl^{xy} as [(x,y)][l]: 0 Tvvo_i, 1 Tvvo_j, 2 Tvvo_k with (x,y) swapped
+  bf16x3_pack(Tvvo_.p + (long)i * vv * o, vv, o, o, 1, 0, 1, Rvv, Kpo, -1.0f, hiP(0), hiP(0) + half, st_);
+  bf16x3_pack(Tvvo_.p + (long)j * vv * o, vv, o, o, 1, 0, 1, Rvv, Kpo, -1.0f, hiP(1), hiP(1) + half, st_);
+  // [x][c][l] = -Tvvo_k[c][x][l]: v rows per x, contiguous (the tail padding of the
+  // slot absorbs the over-read of the last x)
+  bf16x3_pack(Tvvo_.p + (long)k * vv * o, v, o, (long)v * o, 1, o, v, v, Kpo, -1.0f, hiP(2), hiP(2) + half,
+              st_);
+  const long gk = (long)vv * Kpv;  // rows of one k slice of GT
+  auto GT = [&](int r) { return BTerm{GTh_.p + r * gk, GTl_.p + r * gk, nullptr, nullptr, Kpv, 0, 0}; };
+  auto GpT = [&](int r) { return BTerm{GpTh_.p + r * gk, GpTl_.p + r * gk, nullptr, nullptr, Kpv, 0, 0}; };
+  auto mk = [&](const unsigned short* ah, const unsigned short* bh, long Kp, long abs_, long bbs) {
+    return BTerm{ah, ah + half, bh, bh + half, Kp, abs_, bbs};
+  };
+  auto mkGA = [&](BTerm g_, const unsigned short* bh, long Kp) {  // G slice as A operand
+    return BTerm{g_.Ah, g_.Al, bh, bh + half, Kp, 0, 0};
+  };
+  auto mkGB = [&](const unsigned short* ah, BTerm g_, long bbs) {  // G slice as B operand
+    return BTerm{ah, ah + half, g_.Ah, g_.Al, Kpv, 0, bbs};
+  };
+  float* W = W32_.p;
+  const double fv = 2.0 * (double)vv * vv, fo = 2.0 * (double)vv * v * o;
+  // G1: W[a][(b,c)] = t_ij^{ad} (bd|ck) + t_ik^{ad} (cd|bj)
+  BTerm g1[2] = {mkGB(hiT(0), GT(k), 0), mkGB(hiT(1), GpT(j), 0)};
+  bf16x3_multi(g1, 2, v, vv, 1, W, true, vv, 1, 0, 1.0, 0.0, 2 * fv, st_);
+  // G2 (batch a): W[a][b][c] += t_ji^{bd} (ad|ck) + t_jk^{bd} (cd|ai) - t_il^{ac} (bj|lk) - t_kl^{ca} (bj|li)
+  BTerm g2[4] = {mkGB(hiT(2), GT(k), v * Kpv), mkGB(hiT(3), GpT(i), v * Kpv),
+                 mk(hiQ(1), hiP(0), Kpo, 0, v * Kpo), mk(hiQ(2), hiP(2), Kpo, 0, v * Kpo)};
+  bf16x3_multi(g2, 4, v, v, v, W, true, v, 1, vv, 1.0, 1.0, 2 * fv + 2 * fo, st_);
+  // G3: W[(a,b)][c] += (ad|bj) t_ki^{cd} + (bd|ai) t_kj^{cd} - t_il^{ab} (ck|lj)
+  BTerm g3[3] = {mkGA(GT(j), hiT(4), Kpv), mkGA(GpT(i), hiT(5), Kpv), mk(hiP(0), hiQ(0), Kpo, 0, 0)};
+  bf16x3_multi(g3, 3, vv, v, 1, W, true, v, 1, 0, 1.0, 1.0, 2 * fv + fo, st_);
+  // G4 (batch b): W[a][b][c] += -t_jl^{ba} (ck|li) - (ai|lk) t_jl^{bc} - (ai|lj) t_kl^{cb}
+  BTerm g4[3] = {mk(hiP(1), hiQ(3), Kpo, v * Kpo, 0), mk(hiQ(4), hiP(1), Kpo, 0, v * Kpo),
+                 mk(hiQ(5), hiP(2), Kpo, 0, v * Kpo)};
+  bf16x3_multi(g4, 3, v, v, v, W, true, vv, 1, v, 1.0, 1.0, 3 * fo, st_);
+}
+
 void CCSD::pt_energy(long first, long count, long stride, double* e4, double* e5, double* flops) {
-  pt_prepare();
+  const bool mixed = bulk_ == GemmPrec::BF16X3;
+  pt_prepare(mixed ? GemmPrec::BF16X3 : GemmPrec::F64);
   const int o = dm_.o, v = dm_.v;
   const long nov = dm_.nov, vv = (long)v * v, vvv = vv * v, vo = (long)v * o;
   const long ntr = pt_ntriples();
@@ -196,30 +290,40 @@ void CCSD::pt_energy(long first, long count, long stride, double* e4, double* e5
   auto Qrq = [&](int r, int q) { return Qp + ((long)r * o + q) * o * v; };       // [l][z]
   long idx = 0;
   const long npv = (vv + v) / 2;
-  const Gemm bulk{bulk_, st_};
+  const Gemm bulk{GemmPrec::F64, st_};
   for (long t = first; t < ntr && idx < n; t += stride, ++idx) {
     const int i = tri_i_[t], j = tri_j_[t], k = tri_k_[t];
-    // virtual terms
-    bulk(v, vv, v, 1.0, Tpq(i, j), nov, 1, G + (long)k * vvv, vv, 1, 0.0, W, vv, 1);
-    bulk(v, vv, v, 1.0, Tpq(i, k), nov, 1, Gp + (long)j * vvv, vv, 1, 1.0, W, vv, 1);
-    bulk(v, v, v, 1.0, Tpq(j, i), nov, 1, G + (long)k * vvv, vv, 1, 1.0, W, v, 1, v, 0, v, vv);
-    bulk(v, v, v, 1.0, Tpq(j, k), nov, 1, Gp + (long)i * vvv, vv, 1, 1.0, W, v, 1, v, 0, v, vv);
-    bulk(vv, v, v, 1.0, G + (long)j * vvv, 1, vv, Tpq(k, i), 1, nov, 1.0, W, v, 1);
-    bulk(vv, v, v, 1.0, Gp + (long)i * vvv, 1, vv, Tpq(k, j), 1, nov, 1.0, W, v, 1);
-    // occupied terms
-    bulk(vv, v, o, -1.0, Tv + (long)i * vv * o, o, 1, Qrq(k, j), v, 1, 1.0, W, v, 1);
-    bulk(v, v, o, -1.0, Qrq(j, k), 1, v, Tv + (long)i * vv * o, 1, o, 1.0, W, v, 1, v, 0, vo, vv);
-    bulk(v, v, o, -1.0, Tv + (long)j * vv * o, o, 1, Qrq(k, i), v, 1, 1.0, W, vv, 1, v, vo, 0, v);
-    bulk(v, v, o, -1.0, Qrq(i, k), 1, v, Tv + (long)j * vv * o, 1, o, 1.0, W, vv, 1, v, 0, vo, v);
-    bulk(v, v, o, -1.0, Qrq(j, i), 1, v, Tv + (long)k * vv * o, 1, vo, 1.0, W, v, 1, v, 0, o, vv);
-    bulk(v, v, o, -1.0, Qrq(i, j), 1, v, Tv + (long)k * vv * o, 1, vo, 1.0, W, vv, 1, v, 0, o, v);
+    if (mixed) {
+      pt_triples_mixed(t, 0, 0);
+    } else {
+      // virtual terms
+      bulk(v, vv, v, 1.0, Tpq(i, j), nov, 1, G + (long)k * vvv, vv, 1, 0.0, W, vv, 1);
+      bulk(v, vv, v, 1.0, Tpq(i, k), nov, 1, Gp + (long)j * vvv, vv, 1, 1.0, W, vv, 1);
+      bulk(v, v, v, 1.0, Tpq(j, i), nov, 1, G + (long)k * vvv, vv, 1, 1.0, W, v, 1, v, 0, v, vv);
+      bulk(v, v, v, 1.0, Tpq(j, k), nov, 1, Gp + (long)i * vvv, vv, 1, 1.0, W, v, 1, v, 0, v, vv);
+      bulk(vv, v, v, 1.0, G + (long)j * vvv, 1, vv, Tpq(k, i), 1, nov, 1.0, W, v, 1);
+      bulk(vv, v, v, 1.0, Gp + (long)i * vvv, 1, vv, Tpq(k, j), 1, nov, 1.0, W, v, 1);
+      // occupied terms
+      bulk(vv, v, o, -1.0, Tv + (long)i * vv * o, o, 1, Qrq(k, j), v, 1, 1.0, W, v, 1);
+      bulk(v, v, o, -1.0, Qrq(j, k), 1, v, Tv + (long)i * vv * o, 1, o, 1.0, W, v, 1, v, 0, vo, vv);
+      bulk(v, v, o, -1.0, Tv + (long)j * vv * o, o, 1, Qrq(k, i), v, 1, 1.0, W, vv, 1, v, vo, 0, v);
+      bulk(v, v, o, -1.0, Qrq(i, k), 1, v, Tv + (long)j * vv * o, 1, o, 1.0, W, vv, 1, v, 0, vo, v);
+      bulk(v, v, o, -1.0, Qrq(j, i), 1, v, Tv + (long)k * vv * o, 1, vo, 1.0, W, v, 1, v, 0, o, vv);
+      bulk(v, v, o, -1.0, Qrq(i, j), 1, v, Tv + (long)k * vv * o, 1, vo, 1.0, W, vv, 1, v, 0, o, v);
+    }
     // energy
     const double mult = (i == j || j == k) ? 3.0 : 6.0;
-    check_span(W, 0, vvv - 1, sizeof(double), "pt W");
     check_span(ptpart_.p, 0, 2 * npv - 1, sizeof(double), "pt part");
     check_span(J_.p, 0, dm_.nov2 - 1, sizeof(double), "pt J");
-    CCGPU_LAUNCH(k_pt_energy, dim3((unsigned)npv), dim3(256), 0, st_, W, J_.p, t1_.p, eps_.p, i, j, k, o, v,
-                 mult / 3.0, ptpart_.p);
+    if (mixed) {
+      check_span(W32_.p, 0, vvv - 1, sizeof(float), "pt W32");
+      CCGPU_LAUNCH(k_pt_energy<float>, dim3((unsigned)npv), dim3(256), 0, st_, W32_.p, J_.p, t1_.p, eps_.p, i, j,
+                   k, o, v, mult / 3.0, ptpart_.p);
+    } else {
+      check_span(W, 0, vvv - 1, sizeof(double), "pt W");
+      CCGPU_LAUNCH(k_pt_energy<double>, dim3((unsigned)npv), dim3(256), 0, st_, W, J_.p, t1_.p, eps_.p, i, j, k,
+                   o, v, mult / 3.0, ptpart_.p);
+    }
     CCGPU_LAUNCH(k_pt_reduce, dim3(1), dim3(256), 0, st_, ptpart_.p, npv, etrip_.p + 2 * idx);
     CCGPU_HIPX(hipGetLastError());
   }
