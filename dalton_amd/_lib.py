@@ -21,6 +21,7 @@ SIGNATURES = {
     "ccgpu_init": (C.c_int, [C.c_int, C.c_int, C.c_int, _dp, _dp, C.c_int, C.POINTER(C.c_void_p)]),
     "ccgpu_finalize": (C.c_int, [C.c_void_p]),
     "ccgpu_rhs": (C.c_int, [C.c_void_p, _dp, _dp, _dp, _dp, C.c_int]),
+    "ccgpu_lhtr": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _dp, _dp, _dp, _dp]),
     "ccgpu_ccsd_guess": (C.c_int, [C.c_void_p, _dp]),
     "ccgpu_ccsd_iterate": (C.c_int, [C.c_void_p, C.c_int, _dp, _dp]),
     "ccgpu_ccsd_energy": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_int, _dp, _dp,

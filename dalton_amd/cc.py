@@ -88,6 +88,23 @@ class CCContext:
         check(lib().ccgpu_rhs(self._h, ptr(t1), ptr(t2p), ptr(om1), ptr(om2), precision))
         return om1, om2
 
+    def lhtr(self, ctr1: np.ndarray, ctr2: np.ndarray, precision: int = FP64):
+        """CC_LHTR at the current amplitudes: rho = ctr A for one vector (1-D
+        inputs) or a stack of vectors (2-D, one per row), packed storage."""
+        c1 = np.ascontiguousarray(ctr1, dtype=np.float64)
+        c2 = np.ascontiguousarray(ctr2, dtype=np.float64)
+        single = c1.ndim == 1
+        c1 = c1.reshape(-1, self.nt1) if c1.size else c1.reshape(0, self.nt1)
+        c2 = c2.reshape(-1, self.nt2) if c2.size else c2.reshape(0, self.nt2)
+        if c1.shape[0] != c2.shape[0]:
+            raise ValueError("ctr1 and ctr2 hold different numbers of vectors")
+        n = c1.shape[0]
+        r1 = np.empty((n, self.nt1))
+        r2 = np.empty((n, self.nt2))
+        if n:
+            check(lib().ccgpu_lhtr(self._h, precision, n, ptr(c1), ptr(c2), ptr(r1), ptr(r2)))
+        return (r1[0], r2[0]) if single else (r1, r2)
+
     def guess(self) -> float:
         e = C.c_double()
         check(lib().ccgpu_ccsd_guess(self._h, C.byref(e)))

@@ -70,6 +70,16 @@ int ccgpu_rhs(ccgpu_ctx* ctx, const double* t1am, const double* t2am, double* om
   });
 }
 
+int ccgpu_lhtr(ccgpu_ctx* ctx, int precision, int nsimtr, const double* ctr1, const double* ctr2, double* rho1,
+               double* rho2) {
+  return guarded([&] {
+    CCGPU_REQUIRE(ctx, "ccgpu_lhtr: null context");
+    CCGPU_REQUIRE(nsimtr == 0 || (ctr1 && ctr2 && rho1 && rho2), "ccgpu_lhtr: null argument");
+    ctx->cc->set_precision(prec_of(precision));
+    ctx->cc->lhtr_host(nsimtr, ctr1, ctr2, rho1, rho2);
+  });
+}
+
 int ccgpu_ccsd_guess(ccgpu_ctx* ctx, double* e_mp2) {
   return guarded([&] {
     CCGPU_REQUIRE(ctx, "ccgpu_ccsd_guess: null context");

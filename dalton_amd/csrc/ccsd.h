@@ -54,6 +54,12 @@ class CCSD {
   void pt_release();
   void release_work();
 
+  // Left transformation rho = lambda A at the current amplitudes (CC_LHTR):
+  // nvec trial vectors in Dalton's layouts (host arrays, vector-major).  The
+  // forward intermediates are formed once per amplitude set and precision
+  // and reused for every vector; the DIIS history is released first.
+  void lhtr_host(int nvec, const double* l1, const double* l2p, double* rho1, double* rho2p);
+
   const double* t1() const { return t1_.p; }
   const double* T() const { return T_.p; }
   const double* eps() const { return eps_.p; }
@@ -67,6 +73,11 @@ class CCSD {
   void ensure_work();
   void pt_prepare(GemmPrec mode);
   void pt_triples_mixed(long first, long n, long stride);
+  void ensure_diis();
+  void ensure_lhtr();
+  void lhtr_prepare();
+  void lhtr_apply(const double* l1, double* Xb, double* rho1, double* rho2s);
+  void lhtr_ladder(const double* Xb);
 
   Dims dm_;
   int mxdiis_;
@@ -99,6 +110,11 @@ class CCSD {
   // orders, per-triple packed slices, FP32 W
   DBuf<unsigned short> GTh_, GTl_, GpTh_, GpTl_, Psm_;
   DBuf<float> W32_;
+  // left transformation: cotangents (suffix b) of the residual's intermediates
+  bool lhtr_ready_ = false;
+  GemmPrec lhtr_prec_ = GemmPrec::F64;
+  DBuf<double> Tb_, Ub_, Txb_, Db_, Btb_, Bvvb_, Boob_, BvoTb_, Zb_, Wb_, Fb_, E1b_, E2b_, Jdb_, Gamb_, G1b_,
+      taupb_, taumb_, t1b_, vecb_;
 };
 
 }  // namespace ccgpu

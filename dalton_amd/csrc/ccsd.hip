@@ -10,6 +10,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 
 #include "kernels.h"
@@ -112,7 +113,6 @@ void CCSD::ensure_work() {
   if (work_) return;
   pt_release();
   const int o = dm_.o, v = dm_.v, naux = dm_.naux;
-  const int mxdiis = mxdiis_;
   const long N = dm_.N, nov = dm_.nov, nov2 = dm_.nov2;
   (void)N;
   // per-iteration buffers
@@ -140,7 +140,12 @@ void CCSD::ensure_work() {
 
   // particle-ladder batches over a: V+/- rows (a,b>=a) x npv and the
   // Y[(a,c),(b,d)] block must fit a fixed budget; rows < 65536 (grid.y).
-  const double budget = 10.0e9;  // bytes per V+/- / Y buffer
+  // bytes per V+/- / Y buffer; CCGPU_LADDER_BUDGET_MB overrides the 10 GB default
+  double budget = 10.0e9;
+  if (const char* e = std::getenv("CCGPU_LADDER_BUDGET_MB")) {
+    const double mb = std::atof(e);
+    if (mb > 0.0) budget = mb * 1.0e6;
+  }
   long maxrows = std::max<long>(1, std::min<long>(65535, (long)(budget / (8.0 * dm_.npv))));
   lad_batches_.clear();
   lad_batches_.push_back(0);
@@ -167,19 +172,28 @@ void CCSD::ensure_work() {
   Sp_.alloc(maxr * dm_.npo);
   Sm_.alloc(maxr * dm_.npo);
 
-  // DIIS history (Dalton DIIST/DIISS files)
-  diisS_.alloc((size_t)mxdiis * dm_.ntamp);
-  diisE_.alloc((size_t)mxdiis * dm_.ntamp);
   work_ = true;
+}
+
+// DIIS history (Dalton's DIIST/DIISS files).  Released by the left
+// transformation; reallocating it restarts the DIIS cycle.
+void CCSD::ensure_diis() {
+  if (diisS_.p) return;
+  diisS_.alloc((size_t)mxdiis_ * dm_.ntamp);
+  diisE_.alloc((size_t)mxdiis_ * dm_.ntamp);
+  diis_off_ = iter_ > 0 ? iter_ - 1 : 0;
 }
 
 void CCSD::release_work() {
   if (!work_) return;
   for (DBuf<double>* b : {&Bt_, &Bvv_, &Boo_, &BvoT_, &F_, &ht_, &hx_, &G1_, &Gam_, &Z_, &W_, &E1_, &E2_,
                           &bufR_, &bufX2_, &bufTx_, &bufU_, &bufC_, &bufD_, &taup_, &taum_, &Y_, &Vp_,
-                          &Vm_, &Sp_, &Sm_, &diisS_, &diisE_})
+                          &Vm_, &Sp_, &Sm_, &diisS_, &diisE_, &Tb_, &Ub_, &Txb_, &Db_, &Btb_, &Bvvb_, &Boob_,
+                          &BvoTb_, &Zb_, &Wb_, &Fb_, &E1b_, &E2b_, &Jdb_, &Gamb_, &G1b_, &taupb_, &taumb_, &t1b_,
+                          &vecb_})
     b->release();
   work_ = false;
+  lhtr_ready_ = false;
   diis_off_ = iter_ > 0 ? iter_ - 1 : 0;  // the DIIS history is gone: restart the cycle
 }
 
@@ -240,6 +254,7 @@ void CCSD::ladder(const double* T, double* Om) {
 // CCSD vector function (CCRHSN, cc/ccsd_rhs.F:20-2278), T1-dressed form.
 void CCSD::residual(const double* t1, const double* T, double* om1, double* Om) {
   ensure_work();
+  lhtr_ready_ = false;
   const Gemm bulk{bulk_, st_};
   const int o = dm_.o, v = dm_.v, naux = dm_.naux;
   const long N = dm_.N, nov = dm_.nov, nov2 = dm_.nov2, oo = (long)o * o, vv = (long)v * v;
@@ -329,6 +344,7 @@ double CCSD::mp2_start() {
   CCGPU_HIPX(hipStreamSynchronize(st_));
   iter_ = 1;
   diis_off_ = 0;
+  lhtr_ready_ = false;
   return e;
 }
 
@@ -339,6 +355,7 @@ double CCSD::iterate(double* omnm) {
   const long nov = dm_.nov, nt = dm_.ntamp;
   CCGPU_REQUIRE(iter_ >= 1, "ccgpu: call mp2_start (or set amplitudes) before iterating");
   residual(t1_.p, T_.p, om1_.p, Om_.p);
+  ensure_diis();
   nxtam(om1_.p, Om_.p, t1_.p, T_.p, eps_.p, o, v, scal_.p, st_);
   const int mx = mxdiis_;
   const int it = iter_ - diis_off_;
@@ -380,6 +397,7 @@ double CCSD::iterate(double* omnm) {
   CCGPU_HIPX(hipStreamSynchronize(st_));
   if (omnm) *omnm = std::sqrt(nrm2);
   ++iter_;
+  lhtr_ready_ = false;  // the amplitudes moved
   return e;
 }
 
@@ -392,6 +410,7 @@ void CCSD::set_amplitudes(const double* t1_host, const double* t2p_host) {
   CCGPU_HIPX(hipStreamSynchronize(st_));
   iter_ = 1;
   diis_off_ = 0;
+  lhtr_ready_ = false;
 }
 
 void CCSD::get_amplitudes(double* t1_host, double* t2p_host) {

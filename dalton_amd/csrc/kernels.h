@@ -63,3 +63,24 @@ namespace ccgpu {
 // MP2 guess (CCSD_GUESS): T = J / (e_i + e_j - e_a - e_b)
 void mp2_guess(double* T, const double* J, const double* eps, int o, int v, hipStream_t st);
 }  // namespace ccgpu
+
+namespace ccgpu {
+// ---- left transformation (adjoint) helpers
+// M[i][i] *= f for i < n (row stride ld)
+void scale_diag(double* M, long n, long ld, double f, hipStream_t st);
+// Bt[P][k][k] += f * x[P] for k < o
+void add_occ_diag(double* Bt, const double* x, long naux, int N, int o, double f, hipStream_t st);
+// S+/-bar[r][q] gathered from the full cotangent L (adjoint of ladder_scatter)
+void ladder_gather(double* spb, double* smb, const double* L, int a0, int a1, int o, int v, hipStream_t st);
+// Ybar[(a,c),(b,d)] from V+/-bar (adjoint of ladder_vpm); entries with b < a untouched
+void ladder_vpm_adj(double* Yb, long ldY, const double* vpb, const double* vmb, int a0, int a1, int v,
+                    hipStream_t st);
+// Too_bar[(i,j)][(c,d)] += adjoint of ladder_tau for i <= j
+void ladder_tau_adj(double* Toob, const double* taupb, const double* taumb, int o, int v, hipStream_t st);
+}  // namespace ccgpu
+
+namespace ccgpu {
+// out[j0*os0 + j1*os1 + j2*os2] += alpha * in[(j0*n1 + j1)*n2 + j2]
+void add_strided3(double* out, long os0, long os1, long os2, const double* in, long n0, long n1, long n2,
+                  double alpha, hipStream_t st);
+}  // namespace ccgpu

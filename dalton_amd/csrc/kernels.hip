@@ -599,3 +599,178 @@ void mp2_guess(double* T, const double* J, const double* eps, int o, int v, hipS
   CCGPU_HIPX(hipGetLastError());
 }
 }  // namespace ccgpu
+
+namespace ccgpu {
+__global__ void k_scale_diag(double* M, long n, long ld, double f) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) M[i * ld + i] *= f;
+}
+
+void scale_diag(double* M, long n, long ld, double f, hipStream_t st) {
+  if (n <= 0) return;
+  check_span(M, 0, (n - 1) * ld + n - 1, sizeof(double), "scale_diag");
+  CCGPU_LAUNCH(k_scale_diag, dim3(cdiv(n, 256)), dim3(256), 0, st, M, n, ld, f);
+  CCGPU_HIPX(hipGetLastError());
+}
+
+__global__ void k_add_occ_diag(double* Bt, const double* x, long naux, int N, int o, double f) {
+  const long P = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (P >= naux) return;
+  for (int k = 0; k < o; ++k) Bt[P * N * N + (long)k * N + k] += f * x[P];
+}
+
+void add_occ_diag(double* Bt, const double* x, long naux, int N, int o, double f, hipStream_t st) {
+  check_span(Bt, 0, naux * N * N - 1, sizeof(double), "add_occ_diag Bt");
+  check_span(x, 0, naux - 1, sizeof(double), "add_occ_diag x");
+  CCGPU_LAUNCH(k_add_occ_diag, dim3(cdiv(naux, 256)), dim3(256), 0, st, Bt, x, naux, N, o, f);
+  CCGPU_HIPX(hipGetLastError());
+}
+
+__global__ void k_ladder_gather(double* __restrict__ spb, double* __restrict__ smb, const double* __restrict__ L,
+                                int a0, int a1, int o, int v) {
+  const long r = blockIdx.y;
+  int a, b;
+  batch_row_decode(r, a0, a1, v, a, b);
+  const long npo = (long)o * (o + 1) / 2, nov = (long)o * v;
+  for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < npo; q += (long)gridDim.x * blockDim.x) {
+    long i, j;
+    tri_decode(q, i, j);
+    const double e1 = L[(i * v + a) * nov + j * v + b];
+    const double e2 = (i != j) ? L[(j * v + a) * nov + i * v + b] : 0.0;
+    const double e3 = (a != b) ? L[(i * v + b) * nov + j * v + a] : 0.0;
+    const double e4 = (i != j && a != b) ? L[(j * v + b) * nov + i * v + a] : 0.0;
+    spb[r * npo + q] = e1 + e2 + e3 + e4;
+    smb[r * npo + q] = e1 - e2 - e3 + e4;
+  }
+}
+
+void ladder_gather(double* spb, double* smb, const double* L, int a0, int a1, int o, int v, hipStream_t st) {
+  long rows = 0;
+  for (int a = a0; a < a1; ++a) rows += v - a;
+  const long npo = (long)o * (o + 1) / 2;
+  CCGPU_REQUIRE(rows < 65536, "ladder_gather: batch too large");
+  check_span(spb, 0, rows * npo - 1, sizeof(double), "ladder_gather spb");
+  check_span(smb, 0, rows * npo - 1, sizeof(double), "ladder_gather smb");
+  check_span(L, 0, (long)o * v * o * v - 1, sizeof(double), "ladder_gather L");
+  const unsigned nbx = (unsigned)std::min<long>(64, (npo + 255) / 256);
+  CCGPU_LAUNCH(k_ladder_gather, dim3(nbx, (unsigned)rows), dim3(256), 0, st, spb, smb, L, a0, a1, o, v);
+  CCGPU_HIPX(hipGetLastError());
+}
+
+// one block per (row r = (a,b), tile pair tc <= td); each Ybar element with
+// b >= a is written by exactly one thread (from p = (min, max) of its c, d)
+__global__ void k_ladder_vpm_adj(double* __restrict__ Yb, long ldY, const double* __restrict__ vpb,
+                                 const double* __restrict__ vmb, int a0, int a1, int v) {
+  __shared__ double Pp[32][33];
+  __shared__ double Pm[32][33];
+  const long r = blockIdx.y;
+  int a, b;
+  batch_row_decode(r, a0, a1, v, a, b);
+  long tc, td;
+  tri_decode(blockIdx.x, tc, td);
+  const int tx = threadIdx.x, ty = threadIdx.y;
+  const long c0 = tc * 32, d0 = td * 32;
+  const long npv = (long)v * (v + 1) / 2;
+  double* Ya = Yb + (long)(a - a0) * v * ldY + (long)(b - a0) * v;  // Ybar[(a,x),(b,y)] = Ya[x*ldY + y]
+  // load vbar for (c, d), c in tile tc (tx), d in tile td (ty)
+#pragma unroll
+  for (int k = 0; k < 32; k += 8) {
+    const long c = c0 + tx, d = d0 + ty + k;
+    double xp = 0.0, xm = 0.0;
+    if (c < v && d < v && c <= d) {
+      const long p = d * (d + 1) / 2 + c;
+      xp = vpb[r * npv + p];
+      xm = vmb[r * npv + p];
+    }
+    Pp[ty + k][tx] = xp;  // [d_l][c_l]
+    Pm[ty + k][tx] = xm;
+  }
+  __syncthreads();
+  // Ybar[(a,c),(b,d)] = vp + vm   (c <= d; c == d gives 2 vp)
+#pragma unroll
+  for (int k = 0; k < 32; k += 8) {
+    const long c = c0 + ty + k, d = d0 + tx;
+    if (c < v && d < v && c <= d) {
+      const double xp = Pp[tx][ty + k], xm = Pm[tx][ty + k];
+      Ya[c * ldY + d] = (c == d) ? 2.0 * xp : xp + xm;
+    }
+  }
+  // Ybar[(a,d),(b,c)] = vp - vm   (c < d)
+#pragma unroll
+  for (int k = 0; k < 32; k += 8) {
+    const long d = d0 + ty + k, c = c0 + tx;
+    if (c < v && d < v && c < d) {
+      const double xp = Pp[ty + k][tx], xm = Pm[ty + k][tx];
+      Ya[d * ldY + c] = xp - xm;
+    }
+  }
+}
+
+void ladder_vpm_adj(double* Yb, long ldY, const double* vpb, const double* vmb, int a0, int a1, int v,
+                    hipStream_t st) {
+  long rows = 0;
+  for (int a = a0; a < a1; ++a) rows += v - a;
+  CCGPU_REQUIRE(rows < 65536, "ladder_vpm_adj: batch too large");
+  const long npv_ = (long)v * (v + 1) / 2;
+  check_span(vpb, 0, rows * npv_ - 1, sizeof(double), "ladder_vpm_adj vpb");
+  check_span(vmb, 0, rows * npv_ - 1, sizeof(double), "ladder_vpm_adj vmb");
+  check_span(Yb, 0, ((long)(a1 - a0) * v - 1) * ldY + (long)(v - a0) * v - 1, sizeof(double), "ladder_vpm_adj Y");
+  CCGPU_REQUIRE(ldY >= (long)(v - a0) * v, "ladder_vpm_adj: ldY too small");
+  const long nt = (v + 31) / 32;
+  CCGPU_LAUNCH(k_ladder_vpm_adj, dim3((unsigned)(nt * (nt + 1) / 2), (unsigned)rows), dim3(32, 8), 0, st, Yb, ldY,
+               vpb, vmb, a0, a1, v);
+  CCGPU_HIPX(hipGetLastError());
+}
+
+__global__ void k_ladder_tau_adj(double* __restrict__ Toob, const double* __restrict__ taupb,
+                                 const double* __restrict__ taumb, int o, int v) {
+  const long q = blockIdx.y;
+  long i, j;
+  tri_decode(q, i, j);
+  const long npv = (long)v * (v + 1) / 2;
+  double* t = Toob + (i * o + j) * (long)v * v;
+  for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < npv; p += (long)gridDim.x * blockDim.x) {
+    long c, d;
+    tri_decode(p, c, d);
+    const double xp = taupb[q * npv + p], xm = taumb[q * npv + p];
+    if (c == d) {
+      t[c * v + c] += 0.5 * xp;
+    } else {
+      t[c * v + d] += 0.5 * (xp + xm);
+      t[d * v + c] += 0.5 * (xp - xm);
+    }
+  }
+}
+
+void ladder_tau_adj(double* Toob, const double* taupb, const double* taumb, int o, int v, hipStream_t st) {
+  const long npo = (long)o * (o + 1) / 2, npv = (long)v * (v + 1) / 2;
+  CCGPU_REQUIRE(npo < 65536, "ladder_tau_adj: too many occupied pairs");
+  check_span(taupb, 0, npo * npv - 1, sizeof(double), "ladder_tau_adj taup");
+  check_span(taumb, 0, npo * npv - 1, sizeof(double), "ladder_tau_adj taum");
+  check_span(Toob, 0, (long)o * o * v * v - 1, sizeof(double), "ladder_tau_adj Too");
+  const unsigned nbx = (unsigned)std::min<long>(256, (npv + 255) / 256);
+  CCGPU_LAUNCH(k_ladder_tau_adj, dim3(nbx, (unsigned)npo), dim3(256), 0, st, Toob, taupb, taumb, o, v);
+  CCGPU_HIPX(hipGetLastError());
+}
+}  // namespace ccgpu
+
+namespace ccgpu {
+__global__ void k_add_strided3(double* __restrict__ out, long os0, long os1, long os2, const double* __restrict__ in,
+                               long n1, long n2, long total, double alpha) {
+  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
+    const long j2 = e % n2, t = e / n2, j1 = t % n1, j0 = t / n1;
+    out[j0 * os0 + j1 * os1 + j2 * os2] += alpha * in[e];
+  }
+}
+
+void add_strided3(double* out, long os0, long os1, long os2, const double* in, long n0, long n1, long n2,
+                  double alpha, hipStream_t st) {
+  const long total = n0 * n1 * n2;
+  if (total <= 0) return;
+  check_span(in, 0, total - 1, sizeof(double), "add_strided3 in");
+  check_span(out, 0, (n0 - 1) * os0 + (n1 - 1) * os1 + (n2 - 1) * os2, sizeof(double), "add_strided3 out");
+  const unsigned nb = (unsigned)std::min<long>(8192, (total + 255) / 256);
+  CCGPU_LAUNCH(k_add_strided3, dim3(nb), dim3(256), 0, st, out, os0, os1, os2, in, n1, n2, total, alpha);
+  CCGPU_HIPX(hipGetLastError());
+}
+}  // namespace ccgpu

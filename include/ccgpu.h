@@ -54,6 +54,20 @@ int ccgpu_finalize(ccgpu_ctx* ctx);
 int ccgpu_rhs(ccgpu_ctx* ctx, const double* t1am, const double* t2am, double* omega1,
               double* omega2, int precision);
 
+/* Left-hand transformation of nsimtr trial vectors at the context's current
+ * amplitudes: replaces CC_LHTR (cc/cc_lhtr.F:20) for MODEL = CCSD
+ * (closed shell, C1 symmetry, no R12/CC3/PE terms), whose CTR1/CTR2 trial
+ * vectors and RHO1/RHO2 results are held NSIMTR at a time.  In packed
+ * storage on both sides,
+ *   rho_mu = sum_nu ctr_nu dOmega_nu/dt_mu,
+ * Omega the vector function of ccgpu_rhs (CCRHSN).  ctr1/rho1 are
+ * nsimtr x NT1AMX, ctr2/rho2 nsimtr x NT2AMX, vector-major.  The forward
+ * intermediates at the current amplitudes are built on the first call and
+ * reused until the amplitudes or the precision change; the DIIS history is
+ * released (a later iterate restarts the DIIS cycle). */
+int ccgpu_lhtr(ccgpu_ctx* ctx, int precision, int nsimtr, const double* ctr1, const double* ctr2, double* rho1,
+               double* rho2);
+
 /* MP2 starting guess and its energy: CCSD_GUESS (cc/ccsd_energy.F:1654) followed
  * by CCSD_ECCSD (cc/ccsd_energy.F:1964).  e_mp2 is the correlation energy. */
 int ccgpu_ccsd_guess(ccgpu_ctx* ctx, double* e_mp2);

@@ -45,6 +45,12 @@ for (o, v, naux) in {shapes!r}:
     assert rc == 0, L.ccgpu_last_error()
     rc = L.ccgpu_pt_partial(h, 1, 0, 2, 1, C.byref(e), C.byref(n), C.byref(fl), C.byref(nt))
     assert rc == 0, L.ccgpu_last_error()
+    d = np.zeros(1)  # host arrays are never touched in a dry run
+    for prec, nvec in ((0, 2), (1, 1), (0, 1)):  # left transformation, both precisions
+        rc = L.ccgpu_lhtr(h, prec, nvec, _lib.ptr(d), _lib.ptr(d), _lib.ptr(d), _lib.ptr(d))
+        assert rc == 0, L.ccgpu_last_error()
+    rc = L.ccgpu_ccsd_iterate(h, 0, C.byref(e), C.byref(n))  # DIIS history comes back after lhtr
+    assert rc == 0, L.ccgpu_last_error()
     assert L.ccgpu_finalize(h) == 0
     print("ok", o, v, naux, flush=True)
 """

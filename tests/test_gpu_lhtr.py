@@ -1,0 +1,214 @@
+"""HIP left transformation (CC_LHTR) against the autograd oracle and against
+central finite differences of the CC Lagrangian lambda . Omega(t) formed with
+the HIP vector function (ccgpu_rhs)."""
+import numpy as np
+import pytest
+
+from conftest import load_molecule
+from oracle import ccsd as occ
+from oracle import lhtr as olh
+
+pytestmark = pytest.mark.gpu
+
+
+def _system(name):
+    if name.startswith("synthetic"):
+        from dalton_amd.synthetic import make_system
+
+        o, v, naux = (int(x[1:]) for x in name.split("_")[1:])
+        eps, B = make_system(o, v, naux, seed=11)
+        return eps, B, o
+    eps, B, o, _ = load_molecule(name)
+    return eps, B, o
+
+
+def _vectors(nt1, nt2, rng, scale_t=0.03):
+    t1 = scale_t * rng.standard_normal(nt1)
+    t2 = scale_t * rng.standard_normal(nt2)
+    l1 = rng.standard_normal(nt1)
+    l2 = rng.standard_normal(nt2)
+    return t1, t2, l1, l2
+
+
+def _lagrangian(c, t1, t2, l1, l2, prec=0):
+    om1, om2 = c.rhs(t1, t2, prec)
+    return float(np.dot(om1, l1) + np.dot(om2, l2))
+
+
+@pytest.mark.parametrize("name", ["h2o_ccpvdz_test", "synthetic_o4_v23_n60"])
+def test_lhtr_matches_oracle(name):
+    import dalton_amd as d
+
+    eps, B, o = _system(name)
+    print(f"lhtr vs oracle {name}", flush=True)
+    v = eps.size - o
+    sysm = occ.DFSystem(eps, B, o)
+    rng = np.random.default_rng(5)
+    with d.CCContext(eps, B, o) as c:
+        t1, t2, l1, l2 = _vectors(c.nt1, c.nt2, rng)
+        c.set_amplitudes(t1, t2)
+        r1, r2 = c.lhtr(l1, l2)
+        q1, q2 = olh.left_transform(sysm, t1, t2, l1, l2)
+    s = max(1.0, np.abs(q1).max(), np.abs(q2).max())
+    assert np.abs(r1 - q1).max() < 1e-12 * s, np.abs(r1 - q1).max()
+    assert np.abs(r2 - q2).max() < 1e-12 * s, np.abs(r2 - q2).max()
+    assert v > 0
+
+
+def test_lhtr_at_converged_ccsd_matches_oracle():
+    """At Dalton's converged H2O/cc-pVDZ ground state (the state CC_LHTR is
+    called at), several trial vectors in one call."""
+    import dalton_amd as d
+
+    eps, B, o, _ = load_molecule("h2o_ccpvdz_test")
+    print("lhtr at converged H2O", flush=True)
+    sysm = occ.DFSystem(eps, B, o)
+    rng = np.random.default_rng(6)
+    with d.CCContext(eps, B, o) as c:
+        c.ccsd_energy(maxit=60, threnr=1e-12)
+        t1, t2 = c.amplitudes()
+        L1 = rng.standard_normal((3, c.nt1))
+        L2 = rng.standard_normal((3, c.nt2))
+        R1, R2 = c.lhtr(L1, L2)
+    for k in range(3):
+        q1, q2 = olh.left_transform(sysm, t1, t2, L1[k], L2[k])
+        s = max(1.0, np.abs(q2).max())
+        assert np.abs(R1[k] - q1).max() < 1e-12 * s
+        assert np.abs(R2[k] - q2).max() < 1e-12 * s
+
+
+@pytest.mark.parametrize("name", ["h2o_ccpvdz_test", "synthetic_o6_v40_n90"])
+def test_lhtr_matches_finite_differences(name):
+    """32 random directions, central differences with h = 1e-5 (HIP rhs)."""
+    import dalton_amd as d
+
+    eps, B, o = _system(name)
+    print(f"lhtr vs finite differences {name}", flush=True)
+    rng = np.random.default_rng(7)
+    with d.CCContext(eps, B, o) as c:
+        t1, t2, l1, l2 = _vectors(c.nt1, c.nt2, rng)
+        c.set_amplitudes(t1, t2)
+        r1, r2 = c.lhtr(l1, l2)
+        h = 1e-5
+        worst = 0.0
+        for _ in range(32):
+            e1 = rng.standard_normal(c.nt1)
+            e2 = rng.standard_normal(c.nt2)
+            nrm = np.sqrt(np.dot(e1, e1) + np.dot(e2, e2))
+            e1 /= nrm
+            e2 /= nrm
+            fd = (_lagrangian(c, t1 + h * e1, t2 + h * e2, l1, l2)
+                  - _lagrangian(c, t1 - h * e1, t2 - h * e2, l1, l2)) / (2 * h)
+            an = float(np.dot(r1, e1) + np.dot(r2, e2))
+            worst = max(worst, abs(fd - an))
+    print(f"  max |FD - rho.d| = {worst:.3e}", flush=True)
+    assert worst < 1e-7
+
+
+def test_lhtr_c4_shape_sixteen_vectors():
+    """o=32, v=256, naux=1024 at the MP2 amplitudes (plus small singles): 16
+    unit trial vectors in one call, each checked along its own random unit
+    direction by central differences of lambda . Omega (h = 1e-5, relative
+    1e-7); the batched call equals one-vector calls.  The particle ladder runs
+    in several a-batches at this size."""
+    import dalton_amd as d
+    from dalton_amd.synthetic import make_system
+
+    o, v, naux = 32, 256, 1024
+    eps, B = make_system(o, v, naux, seed=2)
+    print("lhtr o32 v256 naux1024, 16 vectors", flush=True)
+    rng = np.random.default_rng(8)
+    with d.CCContext(eps, B, o) as c:
+        c.guess()
+        t1, t2 = c.amplitudes()
+        t1 = 0.01 * rng.standard_normal(c.nt1)
+        nv = 16
+        L1 = rng.standard_normal((nv, c.nt1))
+        L2 = rng.standard_normal((nv, c.nt2))
+        nrm = np.sqrt((L1 * L1).sum(1) + (L2 * L2).sum(1))
+        L1 /= nrm[:, None]
+        L2 /= nrm[:, None]
+        c.set_amplitudes(t1, t2)
+        R1, R2 = c.lhtr(L1, L2)
+        s1, s2 = c.lhtr(L1[5], L2[5])
+        assert np.abs(s1 - R1[5]).max() == 0.0 and np.abs(s2 - R2[5]).max() == 0.0
+        h = 1e-5
+        worst = 0.0
+        for k in range(nv):
+            e1 = rng.standard_normal(c.nt1)
+            e2 = rng.standard_normal(c.nt2)
+            en = np.sqrt(np.dot(e1, e1) + np.dot(e2, e2))
+            e1 /= en
+            e2 /= en
+            fd = (_lagrangian(c, t1 + h * e1, t2 + h * e2, L1[k], L2[k])
+                  - _lagrangian(c, t1 - h * e1, t2 - h * e2, L1[k], L2[k])) / (2 * h)
+            an = float(np.dot(R1[k], e1) + np.dot(R2[k], e2))
+            rel = abs(fd - an) / max(1.0, abs(an))
+            print(f"  vector {k}: rho.d = {an:.12e} FD = {fd:.12e} rel = {rel:.2e}", flush=True)
+            worst = max(worst, rel)
+    assert worst < 1e-7
+
+
+@pytest.mark.parametrize("budget_mb", ["2", "0.05"])
+def test_lhtr_and_rhs_with_batched_ladder(budget_mb, monkeypatch):
+    """The particle ladder split into several a-batches (small memory budget):
+    rhs and the left transformation against the oracle."""
+    import dalton_amd as d
+
+    monkeypatch.setenv("CCGPU_LADDER_BUDGET_MB", budget_mb)
+    eps, B, o = _system("synthetic_o6_v40_n90")
+    print(f"batched ladder, budget {budget_mb} MB", flush=True)
+    sysm = occ.DFSystem(eps, B, o)
+    rng = np.random.default_rng(12)
+    with d.CCContext(eps, B, o) as c:
+        t1, t2, l1, l2 = _vectors(c.nt1, c.nt2, rng)
+        om1, om2 = c.rhs(t1, t2)
+        c.set_amplitudes(t1, t2)
+        r1, r2 = c.lhtr(l1, l2)
+    v = eps.size - o
+    p1, p2 = occ.residual(sysm, occ.unpack_t1(t1, o, v), occ.unpack_t2(t2, o, v))
+    p1, p2 = occ.pack_t1(p1), occ.pack_t2(p2)
+    s = max(1.0, np.abs(p2).max())
+    assert np.abs(om1 - p1).max() < 1e-12 * s and np.abs(om2 - p2).max() < 1e-12 * s
+    q1, q2 = olh.left_transform(sysm, t1, t2, l1, l2)
+    s = max(1.0, np.abs(q2).max())
+    assert np.abs(r1 - q1).max() < 1e-12 * s and np.abs(r2 - q2).max() < 1e-12 * s
+
+
+def test_lhtr_mixed_close_to_fp64():
+    import dalton_amd as d
+
+    eps, B, o = _system("synthetic_o6_v40_n90")
+    print("lhtr mixed vs fp64", flush=True)
+    rng = np.random.default_rng(9)
+    with d.CCContext(eps, B, o) as c:
+        t1, t2, l1, l2 = _vectors(c.nt1, c.nt2, rng)
+        c.set_amplitudes(t1, t2)
+        a1, a2 = c.lhtr(l1, l2, d.FP64)
+        b1, b2 = c.lhtr(l1, l2, d.MIXED)
+    s = max(np.abs(a1).max(), np.abs(a2).max())
+    err = max(np.abs(a1 - b1).max(), np.abs(a2 - b2).max()) / s
+    print(f"  max rel deviation mixed vs fp64 = {err:.3e}", flush=True)
+    assert err < 1e-4
+
+
+def test_autograd_function_backward_is_lhtr():
+    import torch
+
+    import dalton_amd as d
+    from dalton_amd.autograd import ccsd_residual
+
+    eps, B, o = _system("synthetic_o4_v23_n60")
+    print("autograd backward", flush=True)
+    sysm = occ.DFSystem(eps, B, o)
+    rng = np.random.default_rng(10)
+    with d.CCContext(eps, B, o) as c:
+        t1, t2, l1, l2 = _vectors(c.nt1, c.nt2, rng)
+        x1 = torch.tensor(t1, requires_grad=True)
+        x2 = torch.tensor(t2, requires_grad=True)
+        om1, om2 = ccsd_residual(c, x1, x2)
+        (om1 @ torch.tensor(l1) + om2 @ torch.tensor(l2)).backward()
+    q1, q2 = olh.left_transform(sysm, t1, t2, l1, l2)
+    s = max(1.0, np.abs(q2).max())
+    assert np.abs(x1.grad.numpy() - q1).max() < 1e-12 * s
+    assert np.abs(x2.grad.numpy() - q2).max() < 1e-12 * s
