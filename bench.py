@@ -85,6 +85,128 @@ def cpu_baseline(sample=(16, 128, 512), seed=3):
     }
 
 
+def lhtr_leg(nvec, log, kern_table, o=32, v=256, naux=1024, seed=4):
+    """Left transformation of nvec random unit trial vectors at the MP2
+    amplitudes of the o=32 v=256 naux=1024 synthetic system, one call per
+    precision (the forward intermediates are built in an untimed first call)."""
+    import dalton_amd as d
+    from dalton_amd.cc import profile_start, profile_stop
+    from dalton_amd.synthetic import make_system
+
+    eps, B = make_system(o, v, naux, seed)
+    ctx = d.CCContext(eps, B, o)
+    del B
+    ctx.guess()
+    rng = np.random.default_rng(seed)
+    L1 = rng.standard_normal((nvec, ctx.nt1))
+    L2 = rng.standard_normal((nvec, ctx.nt2))
+    nrm = np.sqrt((L1 * L1).sum(1) + (L2 * L2).sum(1))
+    L1 /= nrm[:, None]
+    L2 /= nrm[:, None]
+    res = {"workload": f"synthetic o={o} v={v} naux={naux} seed {seed}, MP2 amplitudes, {nvec} unit vectors",
+           "flops_per_vector_nominal": 2.0 * f_ccsd(o, v, naux)}
+    out = {}
+    for label, prec in (("mixed", d.MIXED), ("fp64", d.FP64)):
+        ctx.lhtr(L1[0], L2[0], prec)  # forward intermediates at this precision (untimed)
+        sync()
+        profile_start()
+        t0 = time.perf_counter()
+        R1, R2 = ctx.lhtr(L1, L2, prec)
+        sync()
+        dt = time.perf_counter() - t0
+        prof = profile_stop()
+        gem = sum(k_["flops"] for k_ in prof["kernels"].values())
+        gms = sum(k_["ms"] for k_ in prof["kernels"].values())
+        res[label] = {"seconds": round(dt, 4), "s_per_vector": round(dt / nvec, 4),
+                      "gemm_tflops": round(gem / max(gms, 1e-9) / 1e9, 2),
+                      "gemm_flops_per_vector": gem / nvec,
+                      "phases_ms_per_vector": {k_: round(v_ / nvec, 2) for k_, v_ in prof["phases"].items()},
+                      "kernels": kern_table(prof)}
+        out[label] = (R1, R2)
+        log(f"lhtr {label}: {nvec} vectors in {dt:.3f} s")
+    a1, a2 = out["fp64"]
+    b1, b2 = out["mixed"]
+    res["max_rel_dev_mixed_vs_fp64"] = float(max(np.abs(a1 - b1).max(), np.abs(a2 - b2).max())
+                                             / max(np.abs(a1).max(), np.abs(a2).max()))
+    res["speedup_mixed_vs_fp64"] = round(res["fp64"]["seconds"] / res["mixed"]["seconds"], 3)
+    ctx.close()
+    return res
+
+
+def _log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def pt48_workload(o=48, v=384, naux=1536, seed=5):
+    """(T) over every i>=j>=k triple at o=48 v=384 naux=1536, FP64 and mixed,
+    after converging CCSD in the same precision.  The integral scale is the
+    convergent one of dalton_amd.synthetic.batch_sigma, so the energies are
+    finite and |dE_CCSD(T)| between the precisions is meaningful."""
+    import dalton_amd as d
+    from dalton_amd.synthetic import batch_sigma, make_system
+
+    sigma = batch_sigma(v, naux)
+    eps, B = make_system(o, v, naux, seed, sigma=sigma)
+    res = {"metric": "(T) over all i>=j>=k at o=48 v=384, FP64 vs mixed", "unit": "s",
+           "workload": f"synthetic o={o} v={v} naux={naux} seed {seed} sigma {sigma:.5f}",
+           "flops_pt": 2.0 * o**3 * v**4 + 2.0 * o**4 * v**3}
+    for label, prec in (("fp64", d.FP64), ("mixed", d.MIXED)):
+        with d.CCContext(eps, B, o) as c:
+            t0 = time.perf_counter()
+            r = c.ccsd_energy(maxit=80, threnr=1e-9, precision=prec)
+            sync()
+            tc = time.perf_counter() - t0
+            _log(f"{label}: CCSD {r['niter']} it in {tc:.1f} s, E = {r['e_ccsd']:.12f}")
+            c.pt_partial(0, 0, 1, precision=prec)  # (T) integral layouts (untimed)
+            sync()
+            t0 = time.perf_counter()
+            e4, e5 = c.pt(prec)
+            sync()
+            tp = time.perf_counter() - t0
+            _log(f"{label}: (T) {tp:.1f} s, E4 = {e4:.12f} E5 = {e5:.12f}")
+        res[label] = {"ccsd_seconds": round(tc, 2), "ccsd_iterations": r["niter"], "e_ccsd": r["e_ccsd"],
+                      "pt_seconds": round(tp, 3), "pt_tflops": round(res["flops_pt"] / tp / 1e12, 2),
+                      "e4": e4, "e5": e5}
+    m, f = res["mixed"], res["fp64"]
+    res["abs_dE_ccsd"] = abs(m["e_ccsd"] - f["e_ccsd"])
+    res["abs_dE_pt"] = abs(m["e4"] + m["e5"] - f["e4"] - f["e5"])
+    res["abs_dE_ccsd_t"] = abs(m["e_ccsd"] + m["e4"] + m["e5"] - f["e_ccsd"] - f["e4"] - f["e5"])
+    res["pt_speedup_mixed_vs_fp64"] = round(f["pt_seconds"] / m["pt_seconds"], 3)
+    return res
+
+
+def batch_workload(nsys=32, seed=5):
+    """32 uneven systems (o ~ U{5..30}, v = 8..12 o, naux = 4v, convergent
+    integral scale) converged together: FP64, one HIP graph replay per
+    iteration after an autotuning pass; then the same work issued eagerly
+    (no graph) for comparison.  Energies of both runs must agree."""
+    import dalton_amd as d
+    from dalton_amd.synthetic import batch_shapes, make_batch
+
+    shapes = batch_shapes(nsys, seed)
+    t0 = time.perf_counter()
+    systems = make_batch(shapes, seed)
+    _log(f"{nsys} systems generated in {time.perf_counter() - t0:.1f} s")
+    out = {"metric": "32 uneven CCSD systems converged together (FP64)", "unit": "s",
+           "workload": f"batch_shapes({nsys}, seed {seed}): " + " ".join(f"{o}/{v}/{n}" for o, v, n in shapes),
+           "flops_per_round_all_systems": sum(f_ccsd(o, v, n) for o, v, n in shapes)}
+    with d.CCBatch(systems, ladder_budget_bytes=5e8) as b:
+        runs = {}
+        for label, graph in (("graph", True), ("eager", False)):
+            sync()
+            t0 = time.perf_counter()
+            r = b.energy(maxit=100, threnr=1e-8, graph=graph, tune=(label == "graph"))
+            sync()
+            dt = time.perf_counter() - t0
+            runs[label] = r
+            out[label] = {"seconds": round(dt, 3), "rounds": int(r["niter"].max()), **b.stats()}
+            _log(f"batch {label}: {dt:.2f} s, rounds {int(r['niter'].max())}")
+    out["niter"] = [int(x) for x in runs["graph"]["niter"]]
+    out["e_ccsd"] = [float(x) for x in runs["graph"]["e_ccsd"]]
+    out["max_abs_dE_graph_vs_eager"] = float(np.abs(runs["graph"]["e_ccsd"] - runs["eager"]["e_ccsd"]).max())
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -98,11 +220,23 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--pt-triples", type=int, default=24,
                     help="(T) triples timed, spread evenly over the i>=j>=k list (0: skip)")
+    ap.add_argument("--lhtr-vectors", type=int, default=16,
+                    help="left-transformation trial vectors timed at o=32 v=256 naux=1024 (0: skip)")
+    ap.add_argument("--workload", choices=["ccsd", "pt48", "batch"], default="ccsd",
+                    help="ccsd: the headline line (default); pt48: full (T) at o=48 v=384 in both "
+                         "precisions on a convergent system; batch: 32 uneven systems converged in "
+                         "one graph replay per iteration")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; --gpus must be 1 "
               "(how the path would shard is described in DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "pt48":
+        print(json.dumps(pt48_workload()), flush=True)
+        return
+    if args.workload == "batch":
+        print(json.dumps(batch_workload()), flush=True)
+        return
 
     import dalton_amd as d
     from dalton_amd.cc import profile_start, profile_stop
@@ -179,6 +313,11 @@ def main():
         pt["rel_err_e5_mixed_vs_fp64"] = abs(pt["mixed"]["e5"] - pt["fp64"]["e5"]) / abs(pt["fp64"]["e5"])
         pt["speedup_mixed_vs_fp64"] = round(pt["fp64"]["seconds"] / pt["mixed"]["seconds"], 3)
     ctx.close()
+    del ctx
+
+    lh = None
+    if args.lhtr_vectors > 0:
+        lh = lhtr_leg(args.lhtr_vectors, log, kern_table)
 
     out = {
         "metric": "CCSD s/iter + (T) TFLOP/s at o64/v512: mixed vs FP64 MFMA, |dE|<=1uEh",
@@ -210,6 +349,7 @@ def main():
         "phases_ms_per_iter": {k: round(v_ / args.steps, 2) for k, v_ in prof_m["phases"].items()},
         "kernels": kern_table(prof_m),
         "pt": pt or None,
+        "lhtr": lh,
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline()

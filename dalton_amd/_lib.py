@@ -26,6 +26,14 @@ SIGNATURES = {
     "ccgpu_ccsd_iterate": (C.c_int, [C.c_void_p, C.c_int, _dp, _dp]),
     "ccgpu_ccsd_energy": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_int, _dp, _dp,
                                     C.POINTER(C.c_int), _dp]),
+    "ccgpu_batch_init": (C.c_int, [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                   C.POINTER(_dp), C.POINTER(_dp), C.c_int, C.c_double,
+                                   C.POINTER(C.c_void_p)]),
+    "ccgpu_batch_energy": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int, _dp, _dp,
+                                     C.POINTER(C.c_int)]),
+    "ccgpu_batch_get_amplitudes": (C.c_int, [C.c_void_p, C.c_int, _dp, _dp]),
+    "ccgpu_batch_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "ccgpu_batch_finalize": (C.c_int, [C.c_void_p]),
     "ccgpu_get_amplitudes": (C.c_int, [C.c_void_p, _dp, _dp]),
     "ccgpu_set_amplitudes": (C.c_int, [C.c_void_p, _dp, _dp]),
     "ccgpu_pt": (C.c_int, [C.c_void_p, C.c_int, _dp, _dp]),

@@ -13,6 +13,7 @@ import ctypes as C
 import numpy as np
 
 from ._lib import CCGPUError, check, lib, ptr  # noqa: F401
+from ._lib import _dp as _lib_dp
 
 FP64 = 0
 MIXED = 1
@@ -181,3 +182,76 @@ def dgemm(transa: str, transb: str, alpha: float, A: np.ndarray, B: np.ndarray, 
                             A.ctypes.data_as(dp), max(1, A.shape[0]), B.ctypes.data_as(dp), max(1, B.shape[0]),
                             beta, Cm.ctypes.data_as(dp), max(1, m)))
     return Cm
+
+
+BATCH_EAGER = 1
+BATCH_NO_TUNE = 2
+
+
+class CCBatch:
+    """Independent closed-shell systems converged together: one HIP graph
+    replay per iteration of every still-active system (ccgpu_batch_*).
+    systems: sequence of (eps, B, nocc)."""
+
+    def __init__(self, systems, mxdiis: int = 8, ladder_budget_bytes: float = 0.0):
+        n = len(systems)
+        self._keep = []
+        eps_p = (_lib_dp * n)()
+        B_p = (_lib_dp * n)()
+        o = (C.c_int * n)()
+        v = (C.c_int * n)()
+        naux = (C.c_int * n)()
+        self.shapes = []
+        for k, (eps, B, nocc) in enumerate(systems):
+            eps = np.ascontiguousarray(eps, dtype=np.float64)
+            B = np.ascontiguousarray(B, dtype=np.float64)
+            N = eps.shape[0]
+            if B.ndim != 3 or B.shape[1] != N or B.shape[2] != N:
+                raise ValueError(f"system {k}: B must have shape (naux, N, N)")
+            self._keep += [eps, B]
+            eps_p[k] = ptr(eps)
+            B_p[k] = ptr(B)
+            o[k], v[k], naux[k] = nocc, N - nocc, B.shape[0]
+            self.shapes.append((nocc, N - nocc, B.shape[0]))
+        h = C.c_void_p()
+        check(lib().ccgpu_batch_init(n, o, v, naux, eps_p, B_p, mxdiis, float(ladder_budget_bytes), C.byref(h)))
+        self._keep = []  # the library copied the inputs to the device
+        self._h = h
+        self.n = n
+
+    def energy(self, maxit: int = 100, threnr: float = 1e-10, thrvec: float = 1e9, precision: int = FP64,
+               graph: bool = True, tune: bool = True):
+        e_mp2, e = np.zeros(self.n), np.zeros(self.n)
+        it = (C.c_int * self.n)()
+        flags = (0 if graph else BATCH_EAGER) | (0 if tune else BATCH_NO_TUNE)
+        check(lib().ccgpu_batch_energy(self._h, maxit, threnr, thrvec, precision, flags, ptr(e_mp2), ptr(e), it))
+        return {"e_mp2": e_mp2, "e_ccsd": e, "niter": np.array(list(it))}
+
+    def amplitudes(self, k: int):
+        o, v, _ = self.shapes[k]
+        t1 = np.empty(o * v)
+        t2 = np.empty(npack(o, v))
+        check(lib().ccgpu_batch_get_amplitudes(self._h, k, ptr(t1), ptr(t2)))
+        return t1, t2
+
+    def stats(self):
+        c, r = C.c_int(), C.c_int()
+        check(lib().ccgpu_batch_stats(self._h, C.byref(c), C.byref(r)))
+        return {"captures": c.value, "replays": r.value}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            check(lib().ccgpu_batch_finalize(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()

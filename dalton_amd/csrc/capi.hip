@@ -6,10 +6,15 @@
 #include <string>
 
 #include "../../include/ccgpu.h"
+#include "batch.h"
 #include "ccsd.h"
 
 struct ccgpu_ctx {
   std::unique_ptr<ccgpu::CCSD> cc;
+};
+
+struct ccgpu_batch {
+  std::unique_ptr<ccgpu::Batch> b;
 };
 
 namespace {
@@ -33,6 +38,15 @@ ccgpu::GemmPrec prec_of(int precision) {
   CCGPU_REQUIRE(precision == CCGPU_FP64 || precision == CCGPU_MIXED, "unknown precision selector");
   return precision == CCGPU_MIXED ? ccgpu::GemmPrec::BF16X3 : ccgpu::GemmPrec::F64;
 }
+// CCGPU_DRYRUN=1 (read once, at the first context or batch creation)
+bool dryrun_env() {
+  static const bool dry = [] {
+    const char* e = std::getenv("CCGPU_DRYRUN");
+    return e != nullptr && std::atoi(e) != 0;
+  }();
+  ccgpu::dryrun() = dry;
+  return dry;
+}
 }  // namespace
 
 extern "C" {
@@ -44,11 +58,7 @@ const char* ccgpu_last_error(void) { return g_err.c_str(); }
 int ccgpu_init(int nocc, int nvir, int naux, const double* eps, const double* B, int mxdiis,
                ccgpu_ctx** ctx) {
   return guarded([&] {
-    static const bool dry = [] {
-      const char* e = std::getenv("CCGPU_DRYRUN");
-      return e != nullptr && std::atoi(e) != 0;
-    }();
-    ccgpu::dryrun() = dry;
+    const bool dry = dryrun_env();
     CCGPU_REQUIRE(ctx != nullptr && (dry || (eps != nullptr && B != nullptr)), "ccgpu_init: null argument");
     *ctx = nullptr;
     auto c = std::make_unique<ccgpu_ctx>();
@@ -127,6 +137,51 @@ int ccgpu_ccsd_energy(ccgpu_ctx* ctx, int maxit, double threnr, double thrvec, i
       if (it + 1 > maxit) throw ccgpu::Error(4, "CC equations not converged.");
     }
   });
+}
+
+int ccgpu_batch_init(int nsys, const int* nocc, const int* nvir, const int* naux, const double* const* eps,
+                     const double* const* B, int mxdiis, double ladder_budget_bytes, ccgpu_batch** batch) {
+  return guarded([&] {
+    CCGPU_REQUIRE(batch && nsys > 0 && nocc && nvir && naux && eps && B, "ccgpu_batch_init: bad argument");
+    for (int k = 0; k < nsys; ++k)
+      CCGPU_REQUIRE(dryrun_env() || (eps[k] && B[k]), "ccgpu_batch_init: null system arrays");
+    auto* h = new ccgpu_batch;
+    try {
+      h->b.reset(new ccgpu::Batch(nsys, nocc, nvir, naux, eps, B, mxdiis, ladder_budget_bytes));
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    *batch = h;
+  });
+}
+
+int ccgpu_batch_energy(ccgpu_batch* batch, int maxit, double threnr, double thrvec, int precision, int flags,
+                       double* e_mp2, double* e_ccsd, int* niter) {
+  return guarded([&] {
+    CCGPU_REQUIRE(batch, "ccgpu_batch_energy: null batch");
+    const bool graph = !(flags & CCGPU_BATCH_EAGER), tune = !(flags & CCGPU_BATCH_NO_TUNE);
+    batch->b->solve(maxit, threnr, thrvec, prec_of(precision), graph, tune, e_mp2, e_ccsd, niter);
+  });
+}
+
+int ccgpu_batch_get_amplitudes(ccgpu_batch* batch, int k, double* t1am, double* t2am) {
+  return guarded([&] {
+    CCGPU_REQUIRE(batch && k >= 0 && k < batch->b->size(), "ccgpu_batch_get_amplitudes: bad argument");
+    batch->b->system(k).get_amplitudes(t1am, t2am);
+  });
+}
+
+int ccgpu_batch_stats(ccgpu_batch* batch, int* captures, int* replays) {
+  return guarded([&] {
+    CCGPU_REQUIRE(batch, "ccgpu_batch_stats: null batch");
+    if (captures) *captures = batch->b->captures();
+    if (replays) *replays = batch->b->replays();
+  });
+}
+
+int ccgpu_batch_finalize(ccgpu_batch* batch) {
+  return guarded([&] { delete batch; });
 }
 
 int ccgpu_get_amplitudes(ccgpu_ctx* ctx, double* t1am, double* t2am) {

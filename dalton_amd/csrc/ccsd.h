@@ -30,12 +30,27 @@ class CCSD {
 
   // CCRHSN mirror on device arrays: t1 [i][a], T [(i,a),(j,b)] -> om1, Om.
   void residual(const double* t1, const double* T, double* om1, double* Om);
+  // residual of the current amplitudes into the internal Omega buffers
+  void residual_current() { residual(t1_.p, T_.p, om1_.p, Om_.p); }
 
   // Device-resident iteration (CCSD_ENERGY loop body).
   double mp2_start();  // t1 = 0, t2 = MP2; returns E_MP2 (correlation)
   // One iteration: residual, Jacobi step, DIIS, energy.  Returns the new
   // correlation energy; *omnm = 2-norm of the incoming residual.
   double iterate(double* omnm);
+  // The same iteration split for graph capture: enqueue issues every launch
+  // on stream() with no host synchronisation or host-dependent argument
+  // (report, if given, receives [norm^2, energy, singular flag, cycle count]
+  // on the device); finish_iteration reads the result back afterwards.
+  // prepare_iteration sizes every buffer an iteration needs (call eagerly).
+  void prepare_iteration();
+  void enqueue_iteration(double* report);
+  double finish_iteration(double* omnm);
+  // Host bookkeeping after an iteration enqueued elsewhere (graph replay).
+  void note_iteration() { ++iter_; lhtr_ready_ = false; }
+  // Bytes per particle-ladder batch buffer (V+/-, Y); applies from the next
+  // workspace allocation.
+  void set_ladder_budget(double bytes) { ladder_budget_ = bytes; }
   int iteration() const { return iter_; }
 
   // Host <-> device amplitude exchange in Dalton's layouts.
@@ -74,6 +89,7 @@ class CCSD {
   void pt_prepare(GemmPrec mode);
   void pt_triples_mixed(long first, long n, long stride);
   void ensure_diis();
+  void reset_cycle();
   void ensure_lhtr();
   void lhtr_prepare();
   void lhtr_apply(const double* l1, double* Xb, double* rho1, double* rho2s);
@@ -100,9 +116,10 @@ class CCSD {
   DBuf<double> taup_, taum_, Y_, Vp_, Vm_, Sp_, Sm_;
   std::vector<int> lad_batches_;
   // DIIS
-  DBuf<double> diisS_, diisE_, dvec_, dots_, coef_;
-  std::vector<double> Amat_;
-  DBuf<double> scal_;
+  DBuf<double> diisS_, diisE_, dvec_, dots_, coef_, Amat_, diiswork_;
+  DBuf<int> ctr_;  // [0] iteration of the DIIS cycle, [1] singular-matrix flag
+  DBuf<double> scal_;  // [0] squared residual norm, [1] energy
+  double ladder_budget_ = 10.0e9;
   // (T)
   GemmPrec pt_mode_ = GemmPrec::F64;
   DBuf<double> G_, Gp_, Tvvo_, Q_, Wt_, ptpart_, etrip_;

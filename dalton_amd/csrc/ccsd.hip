@@ -20,33 +20,6 @@ namespace ccgpu {
 
 namespace {
 
-// Small dense LU solve with partial pivoting (the DIIS system).
-std::vector<double> lu_solve(std::vector<double> A, std::vector<double> b, int n) {
-  std::vector<int> piv(n);
-  for (int j = 0; j < n; ++j) {
-    int p = j;
-    for (int i = j + 1; i < n; ++i)
-      if (std::fabs(A[i * n + j]) > std::fabs(A[p * n + j])) p = i;
-    if (A[p * n + j] == 0.0) throw Error(3, "DIIS: ZERO PIVOT - MATRIX SINGULAR");
-    if (p != j) {
-      for (int k = 0; k < n; ++k) std::swap(A[j * n + k], A[p * n + k]);
-      std::swap(b[j], b[p]);
-    }
-    for (int i = j + 1; i < n; ++i) {
-      const double f = A[i * n + j] / A[j * n + j];
-      for (int k = j; k < n; ++k) A[i * n + k] -= f * A[j * n + k];
-      b[i] -= f * b[j];
-    }
-  }
-  std::vector<double> x(n);
-  for (int i = n - 1; i >= 0; --i) {
-    double s = b[i];
-    for (int k = i + 1; k < n; ++k) s -= A[i * n + k] * x[k];
-    x[i] = s / A[i * n + i];
-  }
-  return x;
-}
-
 constexpr int kMaxVec = 80;  // Dalton MAXVEC, cc/ccsd_diis.F
 
 }  // namespace
@@ -104,8 +77,14 @@ CCSD::CCSD(int o, int v, int naux, const double* eps_host, const double* B_host,
   Om_.alloc(nov2);
   dots_.alloc(kMaxVec + 1);
   coef_.alloc(kMaxVec + 1);
-  Amat_.assign((kMaxVec + 1) * (kMaxVec + 1), 0.0);
+  Amat_.alloc((kMaxVec + 1) * (kMaxVec + 1));
+  diiswork_.alloc((kMaxVec + 2) * (kMaxVec + 2));
+  ctr_.alloc(4);
   scal_.alloc(4);
+  if (const char* e = std::getenv("CCGPU_LADDER_BUDGET_MB")) {
+    const double mb = std::atof(e);
+    if (mb > 0.0) ladder_budget_ = mb * 1.0e6;
+  }
   CCGPU_HIPX(hipStreamSynchronize(st_));
 }
 
@@ -140,12 +119,8 @@ void CCSD::ensure_work() {
 
   // particle-ladder batches over a: V+/- rows (a,b>=a) x npv and the
   // Y[(a,c),(b,d)] block must fit a fixed budget; rows < 65536 (grid.y).
-  // bytes per V+/- / Y buffer; CCGPU_LADDER_BUDGET_MB overrides the 10 GB default
-  double budget = 10.0e9;
-  if (const char* e = std::getenv("CCGPU_LADDER_BUDGET_MB")) {
-    const double mb = std::atof(e);
-    if (mb > 0.0) budget = mb * 1.0e6;
-  }
+  // bytes per V+/- / Y buffer (10 GB unless set; CCGPU_LADDER_BUDGET_MB overrides)
+  const double budget = ladder_budget_;
   long maxrows = std::max<long>(1, std::min<long>(65535, (long)(budget / (8.0 * dm_.npv))));
   lad_batches_.clear();
   lad_batches_.push_back(0);
@@ -182,6 +157,14 @@ void CCSD::ensure_diis() {
   diisS_.alloc((size_t)mxdiis_ * dm_.ntamp);
   diisE_.alloc((size_t)mxdiis_ * dm_.ntamp);
   diis_off_ = iter_ > 0 ? iter_ - 1 : 0;
+  reset_cycle();
+}
+
+// The next iteration starts a new DIIS cycle (device counter = 1).
+void CCSD::reset_cycle() {
+  const int c[4] = {1, 0, 0, 0};
+  CCGPU_HIPX(hipMemcpyAsync(ctr_.p, c, sizeof c, hipMemcpyHostToDevice, st_));
+  CCGPU_HIPX(hipStreamSynchronize(st_));
 }
 
 void CCSD::release_work() {
@@ -199,7 +182,11 @@ void CCSD::release_work() {
 
 CCSD::~CCSD() {
   if (dryrun()) return;
-  if (st_) (void)hipStreamDestroy(st_);
+  if (st_) {
+    (void)hipStreamSynchronize(st_);
+    release_scratch(st_);
+    (void)hipStreamDestroy(st_);
+  }
 }
 
 void CCSD::mark(const char* name) { prof_phase(name, st_); }
@@ -345,60 +332,54 @@ double CCSD::mp2_start() {
   iter_ = 1;
   diis_off_ = 0;
   lhtr_ready_ = false;
+  reset_cycle();
   return e;
 }
 
 // One pass of the CCSD_ENERGY loop (cc/ccsd_energy.F:742-982): residual,
-// CCSD_NXTAM, CCSD_DIIS (cc/ccsd_diis.F:251-365), CCSD_ECCSD.
-double CCSD::iterate(double* omnm) {
+// CCSD_NXTAM, CCSD_DIIS (cc/ccsd_diis.F:251-365), CCSD_ECCSD, all on the
+// device: the DIIS slot, the B-matrix extension and the bordered solve take
+// the cycle count from the device counter.
+void CCSD::prepare_iteration() {
+  CCGPU_REQUIRE(iter_ >= 1, "ccgpu: call mp2_start (or set amplitudes) before iterating");
+  ensure_work();
+  ensure_diis();
+}
+
+void CCSD::enqueue_iteration(double* report) {
   const int o = dm_.o, v = dm_.v;
   const long nov = dm_.nov, nt = dm_.ntamp;
-  CCGPU_REQUIRE(iter_ >= 1, "ccgpu: call mp2_start (or set amplitudes) before iterating");
-  residual(t1_.p, T_.p, om1_.p, Om_.p);
-  ensure_diis();
-  nxtam(om1_.p, Om_.p, t1_.p, T_.p, eps_.p, o, v, scal_.p, st_);
   const int mx = mxdiis_;
-  const int it = iter_ - diis_off_;
-  const int ivec = it > mx ? it - ((it - 1) / mx) * mx : it;
-  const int nloop = ivec;
-  double* svec = diisS_.p + (long)(ivec - 1) * nt;
-  double* evec = diisE_.p + (long)(ivec - 1) * nt;
-  pack_diis(svec, evec, om1_.p, Om_.p, t1_.p, T_.p, nov, st_);
-  multidot(dots_.p, evec, diisE_.p, nt, nloop, nt, st_);
-  std::vector<double> dots(nloop + 1);
-  double nrm2 = 0.0;
-  CCGPU_HIPX(hipMemcpyAsync(dots.data(), dots_.p, nloop * sizeof(double), hipMemcpyDeviceToHost, st_));
-  CCGPU_HIPX(hipMemcpyAsync(&nrm2, scal_.p, sizeof(double), hipMemcpyDeviceToHost, st_));
-  CCGPU_HIPX(hipStreamSynchronize(st_));
-  if (dryrun())
-    for (int i = 0; i < nloop; ++i) dots[i] = (i == ivec - 1) ? 1.0 : 0.0;
-  const int LD = kMaxVec + 1;
-  for (int i = 0; i < nloop; ++i) {
-    Amat_[i * LD + (ivec - 1)] = dots[i];
-    Amat_[(ivec - 1) * LD + i] = dots[i];
-  }
-  const int n = nloop + 1;
-  std::vector<double> M(n * n, 0.0), rhs(n, 0.0);
-  for (int i = 0; i < nloop; ++i) {
-    for (int j = 0; j < nloop; ++j) M[i * n + j] = Amat_[i * LD + j];
-    M[i * n + nloop] = -1.0;
-    M[nloop * n + i] = -1.0;
-  }
-  rhs[nloop] = -1.0;
-  std::vector<double> c = lu_solve(M, rhs, n);
-  CCGPU_HIPX(hipMemcpyAsync(coef_.p, c.data(), nloop * sizeof(double), hipMemcpyHostToDevice, st_));
+  residual(t1_.p, T_.p, om1_.p, Om_.p);
+  nxtam(om1_.p, Om_.p, t1_.p, T_.p, eps_.p, o, v, scal_.p, st_);
+  pack_diis_slot(diisS_.p, diisE_.p, ctr_.p, mx, om1_.p, Om_.p, t1_.p, T_.p, nov, st_);
+  multidot_slot(dots_.p, diisE_.p, nt, ctr_.p, mx, nt, st_);
+  diis_solve(Amat_.p, kMaxVec + 1, dots_.p, coef_.p, ctr_.p, mx, diiswork_.p, st_);
   double* newv = bufX2_.p;
-  lincomb(newv, diisS_.p, nt, coef_.p, nloop, nt, st_);
+  lincomb_slot(newv, diisS_.p, nt, coef_.p, ctr_.p, mx, nt, st_);
   CCGPU_HIPX(hipMemcpyAsync(t1_.p, newv, nov * sizeof(double), hipMemcpyDeviceToDevice, st_));
   unpack_sym(T_.p, newv + nov, nov, st_);
   ccsd_energy(scal_.p + 1, L_.p, T_.p, t1_.p, nov, st_);
-  double e = 0.0;
-  CCGPU_HIPX(hipMemcpyAsync(&e, scal_.p + 1, sizeof(double), hipMemcpyDeviceToHost, st_));
+  ctr_advance(ctr_.p, st_);
+  if (report) report4(report, scal_.p, ctr_.p, st_);
+}
+
+double CCSD::finish_iteration(double* omnm) {
+  double sc[2] = {0.0, 0.0};
+  int flag[2] = {0, 0};
+  CCGPU_HIPX(hipMemcpyAsync(sc, scal_.p, 2 * sizeof(double), hipMemcpyDeviceToHost, st_));
+  CCGPU_HIPX(hipMemcpyAsync(flag, ctr_.p, 2 * sizeof(int), hipMemcpyDeviceToHost, st_));
   CCGPU_HIPX(hipStreamSynchronize(st_));
-  if (omnm) *omnm = std::sqrt(nrm2);
-  ++iter_;
-  lhtr_ready_ = false;  // the amplitudes moved
-  return e;
+  note_iteration();
+  if (flag[1]) throw Error(3, "DIIS: ZERO PIVOT - MATRIX SINGULAR");
+  if (omnm) *omnm = std::sqrt(sc[0]);
+  return sc[1];
+}
+
+double CCSD::iterate(double* omnm) {
+  prepare_iteration();
+  enqueue_iteration(nullptr);
+  return finish_iteration(omnm);
 }
 
 void CCSD::set_amplitudes(const double* t1_host, const double* t2p_host) {
@@ -411,6 +392,7 @@ void CCSD::set_amplitudes(const double* t1_host, const double* t2p_host) {
   iter_ = 1;
   diis_off_ = 0;
   lhtr_ready_ = false;
+  reset_cycle();
 }
 
 void CCSD::get_amplitudes(double* t1_host, double* t2p_host) {

@@ -93,6 +93,30 @@ struct DBuf {
 
 inline unsigned cdiv(long a, long b) { return static_cast<unsigned>((a + b - 1) / b); }
 
+// True while the stream is being captured into a HIP graph (never in a dry run).
+bool capturing(hipStream_t s);
+
+// Scratch memory owned per stream (split-K partials, BF16x3 operand packing,
+// reduction partials), so work issued on different streams never shares
+// scratch.  Buffers only grow; growth is refused while the stream is being
+// captured (a graph must not allocate).  Never destroyed before release_scratch.
+struct StreamScratch {
+  DBuf<double> splitk;
+  DBuf<unsigned short> pack[4];
+  DBuf<double> part;
+};
+StreamScratch& scratch_for(hipStream_t s);
+void release_scratch(hipStream_t s);
+
+template <typename T>
+T* grow_scratch(DBuf<T>& b, size_t n, hipStream_t s) {
+  if (n > b.n) {
+    CCGPU_REQUIRE(!capturing(s), "scratch would grow during HIP graph capture (run the workload once eagerly)");
+    b.alloc(n);
+  }
+  return b.p;
+}
+
 // ------------------------------------------------------------------ GEMM API
 // C(m,n) = alpha * sum_k A(m,k) B(k,n) + beta * C(m,n), all operands given by
 // element strides, optionally batched (blockIdx.z) and with a two-level K
@@ -114,6 +138,15 @@ struct GemmDesc {
 enum class GemmPrec { F64 = 0, F32 = 1, BF16X3 = 2 };
 
 void gemm(const GemmDesc& d, GemmPrec prec, hipStream_t s);
+
+// FP64 GEMM autotuner.  While enabled, the first call of every new shape
+// class (M, N, K, two-level K, batch, operand orders) times each candidate
+// launch configuration (block tile 128 or 64, split-K on or off) on its
+// stream and caches the fastest; later calls of that shape use it, enabled
+// or not.  Timing runs the GEMM for real, so a tuning pass gives throwaway
+// results.  Never tunes during graph capture or a dry run.
+void autotune_enable(bool on);
+long autotune_entries();
 
 // ---- BF16x3 building blocks (operands pre-split into bf16 hi/lo arrays laid
 // out [batch][row][k], row length Kp, rows padded to 128, Kp to 64).

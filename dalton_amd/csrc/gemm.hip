@@ -10,6 +10,10 @@
 // row-major, transposed, batched and two-level-K (k = ko*Kin + ki)
 // contractions.  Which dimension the loader walks fastest (for coalescing)
 // is a template parameter chosen from the strides on the host.
+#include <algorithm>
+#include <map>
+#include <mutex>
+
 #include "common.h"
 
 namespace ccgpu {
@@ -46,25 +50,30 @@ __device__ __forceinline__ void tile_of(long b, long tiles_m, long tiles_n, long
 
 // LDS staging: an operand loaded k-fastest is stored [row][k] (stride TK+1,
 // conflict-free for both the stores and the 16x4 fragment reads); one loaded
-// row-fastest is stored [k][row] (stride 128+16).
-template <int MODE>
+// row-fastest is stored [k][row] (stride T+16).
+template <int MODE, int T>
 struct Stage;
-template <>
-struct Stage<KFAST> {
-  double s[128][17];
+template <int T>
+struct Stage<KFAST, T> {
+  double s[T][17];
   __device__ __forceinline__ double& at(int r, int k) { return s[r][k]; }
 };
-template <>
-struct Stage<RFAST> {
-  double s[16][144];
+template <int T>
+struct Stage<RFAST, T> {
+  double s[16][T + 16];
   __device__ __forceinline__ double& at(int r, int k) { return s[k][r]; }
 };
 
-template <int AMODE, int BMODE, bool K2>
+// Block tile T x T (T = 128 or 64), 2x2 waves of (T/2)x(T/2), each wave
+// (T/32)x(T/32) MFMA 16x16 tiles.
+template <int AMODE, int BMODE, bool K2, int T>
 __global__ __launch_bounds__(256, 2) void k_dgemm(GemmDesc d, long kchunk, double* ws) {
-  constexpr int TM = 128, TN = 128, TK = 16;
-  __shared__ Stage<AMODE> As;
-  __shared__ Stage<BMODE> Bs;
+  constexpr int TM = T, TN = T, TK = 16;
+  constexpr int NL = T / 16;      // elements per thread per operand and K tile
+  constexpr int RS = 256 / T;     // row-fastest loader: K columns per pass
+  constexpr int IW = T / 32;      // MFMA tiles per wave and dimension
+  __shared__ Stage<AMODE, T> As;
+  __shared__ Stage<BMODE, T> Bs;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
   const long tiles_m = (d.M + TM - 1) / TM, tiles_n = (d.N + TN - 1) / TN;
@@ -82,9 +91,9 @@ __global__ __launch_bounds__(256, 2) void k_dgemm(GemmDesc d, long kchunk, doubl
     B += (long)blockIdx.z * d.b_b;
   }
 
-  double ra[8], rb[8];
+  double ra[NL], rb[NL];
   // k-fastest: thread owns column c = tid % 16 and rows tid/16 + 16 i;
-  // row-fastest: thread owns row tid % 128 and columns tid/128 + 2 i.
+  // row-fastest: thread owns row tid % T and columns tid/T + RS i.
   auto load = [&](long k0) {
     if (AMODE == KFAST) {
       const long k = k0 + (tid % TK);
@@ -92,7 +101,7 @@ __global__ __launch_bounds__(256, 2) void k_dgemm(GemmDesc d, long kchunk, doubl
       const long ko = kok ? koff<K2>(k, d.a_k, d.a_ko, d.Kin) : 0;
       const long mb = m0 + tid / TK;
 #pragma unroll
-      for (int i = 0; i < 8; ++i) {
+      for (int i = 0; i < NL; ++i) {
         const long m = mb + 16 * i;
         ra[i] = (kok && m < d.M) ? A[m * d.a_m + ko] : 0.0;
       }
@@ -102,8 +111,8 @@ __global__ __launch_bounds__(256, 2) void k_dgemm(GemmDesc d, long kchunk, doubl
       const double* Ar = A + (mok ? m : 0) * d.a_m;
       const long kb = k0 + tid / TM;
 #pragma unroll
-      for (int i = 0; i < 8; ++i) {
-        const long k = kb + 2 * i;
+      for (int i = 0; i < NL; ++i) {
+        const long k = kb + RS * i;
         ra[i] = (mok && k < kend) ? Ar[koff<K2>(k, d.a_k, d.a_ko, d.Kin)] : 0.0;
       }
     }
@@ -113,7 +122,7 @@ __global__ __launch_bounds__(256, 2) void k_dgemm(GemmDesc d, long kchunk, doubl
       const long ko = kok ? koff<K2>(k, d.b_k, d.b_ko, d.Kin) : 0;
       const long nb = n0 + tid / TK;
 #pragma unroll
-      for (int i = 0; i < 8; ++i) {
+      for (int i = 0; i < NL; ++i) {
         const long n = nb + 16 * i;
         rb[i] = (kok && n < d.N) ? B[n * d.b_n + ko] : 0.0;
       }
@@ -123,22 +132,22 @@ __global__ __launch_bounds__(256, 2) void k_dgemm(GemmDesc d, long kchunk, doubl
       const double* Br = B + (nok ? n : 0) * d.b_n;
       const long kb = k0 + tid / TN;
 #pragma unroll
-      for (int i = 0; i < 8; ++i) {
-        const long k = kb + 2 * i;
+      for (int i = 0; i < NL; ++i) {
+        const long k = kb + RS * i;
         rb[i] = (nok && k < kend) ? Br[koff<K2>(k, d.b_k, d.b_ko, d.Kin)] : 0.0;
       }
     }
   };
   auto store = [&]() {
 #pragma unroll
-    for (int i = 0; i < 8; ++i) {
+    for (int i = 0; i < NL; ++i) {
       const int e = tid + i * 256;
       int r, c;
       if (AMODE == KFAST) { r = e / TK; c = e % TK; } else { r = e % TM; c = e / TM; }
       As.at(r, c) = ra[i];
     }
 #pragma unroll
-    for (int i = 0; i < 8; ++i) {
+    for (int i = 0; i < NL; ++i) {
       const int e = tid + i * 256;
       int r, c;
       if (BMODE == KFAST) { r = e / TK; c = e % TK; } else { r = e % TN; c = e / TN; }
@@ -146,11 +155,11 @@ __global__ __launch_bounds__(256, 2) void k_dgemm(GemmDesc d, long kchunk, doubl
     }
   };
 
-  d4 acc[4][4];
+  d4 acc[IW][IW];
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < IW; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
+    for (int j = 0; j < IW; ++j) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
 
   const int lr = lane & 15, lk = lane >> 4;
   load(kbeg);
@@ -161,15 +170,15 @@ __global__ __launch_bounds__(256, 2) void k_dgemm(GemmDesc d, long kchunk, doubl
     if (k0 + TK < kend) load(k0 + TK);
 #pragma unroll
     for (int kk = 0; kk < TK / 4; ++kk) {
-      double a[4], b[4];
+      double a[IW], b[IW];
 #pragma unroll
-      for (int t = 0; t < 4; ++t) a[t] = As.at(wm * 64 + t * 16 + lr, kk * 4 + lk);
+      for (int t = 0; t < IW; ++t) a[t] = As.at(wm * (T / 2) + t * 16 + lr, kk * 4 + lk);
 #pragma unroll
-      for (int t = 0; t < 4; ++t) b[t] = Bs.at(wn * 64 + t * 16 + lr, kk * 4 + lk);
+      for (int t = 0; t < IW; ++t) b[t] = Bs.at(wn * (T / 2) + t * 16 + lr, kk * 4 + lk);
 #pragma unroll
-      for (int i = 0; i < 4; ++i)
+      for (int i = 0; i < IW; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
+        for (int j = 0; j < IW; ++j)
           acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
     }
   }
@@ -179,26 +188,26 @@ __global__ __launch_bounds__(256, 2) void k_dgemm(GemmDesc d, long kchunk, doubl
     // split-K partial: ws[z][m][n] (dense M x N)
     double* W = ws + (long)blockIdx.z * d.M * d.N;
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < IW; ++i)
 #pragma unroll
-      for (int j = 0; j < 4; ++j)
+      for (int j = 0; j < IW; ++j)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const long m = m0 + wm * 64 + i * 16 + r * 4 + lk;
-          const long n = n0 + wn * 64 + j * 16 + lr;
+          const long m = m0 + wm * (T / 2) + i * 16 + r * 4 + lk;
+          const long n = n0 + wn * (T / 2) + j * 16 + lr;
           if (m < d.M && n < d.N) W[m * d.N + n] = acc[i][j][r];
         }
     return;
   }
   double* C = static_cast<double*>(d.C) + (long)blockIdx.z * d.c_b;
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < IW; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
+    for (int j = 0; j < IW; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const long m = m0 + wm * 64 + i * 16 + r * 4 + lk;
-        const long n = n0 + wn * 64 + j * 16 + lr;
+        const long m = m0 + wm * (T / 2) + i * 16 + r * 4 + lk;
+        const long n = n0 + wn * (T / 2) + j * 16 + lr;
         if (m < d.M && n < d.N) {
           double* p = C + m * d.c_m + n * d.c_n;
           double v = d.alpha * acc[i][j][r];
@@ -223,29 +232,60 @@ __global__ void k_splitk_reduce_f64(GemmDesc d, int nz, const double* ws) {
 }
 
 namespace {
-// Split-K workspace, grown on demand (one per process; calls on one stream).
-// Never destroyed (a static destructor would run after the HIP runtime exits).
-DBuf<double>& g_ws_ref() {
-  static auto* p = new DBuf<double>();
-  return *p;
-}
-#define g_ws g_ws_ref()
+const char* const kDgemmName[2][2][2][2] = {
+    {{{"k_dgemm<0,0,false,128>", "k_dgemm<0,0,false,64>"}, {"k_dgemm<0,0,true,128>", "k_dgemm<0,0,true,64>"}},
+     {{"k_dgemm<0,1,false,128>", "k_dgemm<0,1,false,64>"}, {"k_dgemm<0,1,true,128>", "k_dgemm<0,1,true,64>"}}},
+    {{{"k_dgemm<1,0,false,128>", "k_dgemm<1,0,false,64>"}, {"k_dgemm<1,0,true,128>", "k_dgemm<1,0,true,64>"}},
+     {{"k_dgemm<1,1,false,128>", "k_dgemm<1,1,false,64>"}, {"k_dgemm<1,1,true,128>", "k_dgemm<1,1,true,64>"}}}};
 
-const char* const kDgemmName[2][2][2] = {
-    {{"k_dgemm<0,0,false>", "k_dgemm<0,0,true>"}, {"k_dgemm<0,1,false>", "k_dgemm<0,1,true>"}},
-    {{"k_dgemm<1,0,false>", "k_dgemm<1,0,true>"}, {"k_dgemm<1,1,false>", "k_dgemm<1,1,true>"}}};
-
-template <int AM, int BM>
+template <int AM, int BM, int T>
 void launch_f64(const GemmDesc& d, dim3 grid, long kchunk, double* ws, hipStream_t s) {
   const bool k2 = d.Kin > 0 && d.Kin < d.K;
-  const int id = prof_begin(kDgemmName[AM][BM][k2 ? 1 : 0], 2.0 * d.M * d.N * d.K * d.batch, s);
+  const int id =
+      prof_begin(kDgemmName[AM][BM][k2 ? 1 : 0][T == 64 ? 1 : 0], 2.0 * d.M * d.N * d.K * d.batch, s);
   if (k2)
-    CCGPU_LAUNCH((k_dgemm<AM, BM, true>), grid, dim3(256), 0, s, d, kchunk, ws);
+    CCGPU_LAUNCH((k_dgemm<AM, BM, true, T>), grid, dim3(256), 0, s, d, kchunk, ws);
   else
-    CCGPU_LAUNCH((k_dgemm<AM, BM, false>), grid, dim3(256), 0, s, d, kchunk, ws);
+    CCGPU_LAUNCH((k_dgemm<AM, BM, false, T>), grid, dim3(256), 0, s, d, kchunk, ws);
   prof_end(id, s);
 }
+
+// Launch configuration of one FP64 GEMM: block tile and split-K on/off.
+struct F64Cfg {
+  int tile = 128;
+  bool splitk = true;
+};
+
+// ---- autotuner: per GEMM shape class, the fastest configuration measured
+struct ShapeKey {
+  long M, N, K, Kin;
+  int batch, am, bm;
+  bool operator<(const ShapeKey& o) const {
+    if (M != o.M) return M < o.M;
+    if (N != o.N) return N < o.N;
+    if (K != o.K) return K < o.K;
+    if (Kin != o.Kin) return Kin < o.Kin;
+    if (batch != o.batch) return batch < o.batch;
+    if (am != o.am) return am < o.am;
+    return bm < o.bm;
+  }
+};
+struct Tuner {
+  bool on = false;
+  std::map<ShapeKey, F64Cfg> best;
+  std::mutex mu;
+};
+Tuner& tuner() {
+  static auto* t = new Tuner;
+  return *t;
+}
 }  // namespace
+
+void autotune_enable(bool on) { tuner().on = on; }
+long autotune_entries() {
+  std::lock_guard<std::mutex> lk(tuner().mu);
+  return (long)tuner().best.size();
+}
 
 namespace {
 long kspan(const GemmDesc& d, long s_k, long s_ko) {
@@ -267,39 +307,44 @@ void check_gemm(const GemmDesc& d, size_t esz) {
   check_span(d.C, 0, (d.M - 1) * d.c_m + (d.N - 1) * d.c_n + (d.batch - 1) * d.c_b, esz, "gemm C");
 }
 
-void gemm_f64(const GemmDesc& d, hipStream_t s) {
-  if (d.M <= 0 || d.N <= 0 || d.batch <= 0) return;
-  check_gemm(d, sizeof(double));
-  const int am = (d.a_m == 1 && d.a_k != 1) ? RFAST : KFAST;
-  const int bm = (d.b_n == 1 && d.b_k != 1) ? RFAST : KFAST;
-  const long tiles = (long)cdiv(d.M, 128) * cdiv(d.N, 128);
+namespace {
+void run_f64(const GemmDesc& d, const F64Cfg& cfg, int am, int bm, hipStream_t s) {
+  const int T = cfg.tile;
+  const long tiles = (long)cdiv(d.M, T) * cdiv(d.N, T);
   long kchunk = 0;
   int nz = d.batch;
   double* ws = nullptr;
-  if (d.batch == 1 && tiles < 512 && d.K >= 2048) {
+  const long few = T == 128 ? 512 : 2048;  // below this many tiles, split K
+  if (cfg.splitk && d.batch == 1 && tiles < few && d.K >= 2048) {
     // split K over blockIdx.z so the launch has enough workgroups
-    long want = std::min<long>((1024 + tiles - 1) / tiles, d.K / 1024);
+    long want = std::min<long>((4 * few / 2 + tiles - 1) / tiles, d.K / 1024);
+    want = std::min<long>(want, (1024 * 128 / T + tiles - 1) / tiles);
     if (want >= 2) {
       long chunk = (d.K + want - 1) / want;
       chunk = (chunk + 15) / 16 * 16;  // K tiles never straddle two chunks
       nz = static_cast<int>((d.K + chunk - 1) / chunk);
       if (nz >= 2) {
         kchunk = chunk;
-        g_ws.ensure(static_cast<size_t>(nz) * d.M * d.N);
-        ws = g_ws.p;
+        ws = grow_scratch(scratch_for(s).splitk, static_cast<size_t>(nz) * d.M * d.N, s);
       } else {
         nz = 1;
       }
     }
   }
   CCGPU_REQUIRE(nz <= 65535, "gemm: too many batches");
-  const long ntiles = (long)cdiv(d.M, 128) * cdiv(d.N, 128);
-  CCGPU_REQUIRE(ntiles < (1L << 31), "gemm: too many tiles");
-  dim3 grid((unsigned)ntiles, 1, nz);
-  if (am == KFAST && bm == KFAST) launch_f64<KFAST, KFAST>(d, grid, kchunk, ws, s);
-  else if (am == KFAST) launch_f64<KFAST, RFAST>(d, grid, kchunk, ws, s);
-  else if (bm == KFAST) launch_f64<RFAST, KFAST>(d, grid, kchunk, ws, s);
-  else launch_f64<RFAST, RFAST>(d, grid, kchunk, ws, s);
+  CCGPU_REQUIRE(tiles < (1L << 31), "gemm: too many tiles");
+  dim3 grid((unsigned)tiles, 1, nz);
+#define CCGPU_F64_LAUNCH(TT)                                                   \
+  if (am == KFAST && bm == KFAST) launch_f64<KFAST, KFAST, TT>(d, grid, kchunk, ws, s); \
+  else if (am == KFAST) launch_f64<KFAST, RFAST, TT>(d, grid, kchunk, ws, s);          \
+  else if (bm == KFAST) launch_f64<RFAST, KFAST, TT>(d, grid, kchunk, ws, s);          \
+  else launch_f64<RFAST, RFAST, TT>(d, grid, kchunk, ws, s);
+  if (T == 64) {
+    CCGPU_F64_LAUNCH(64)
+  } else {
+    CCGPU_F64_LAUNCH(128)
+  }
+#undef CCGPU_F64_LAUNCH
   CCGPU_HIPX(hipGetLastError());
   if (kchunk > 0) {
     const long mn = d.M * d.N;
@@ -307,6 +352,61 @@ void gemm_f64(const GemmDesc& d, hipStream_t s) {
     CCGPU_LAUNCH(k_splitk_reduce_f64, dim3(nb), dim3(256), 0, s, d, nz, ws);
     CCGPU_HIPX(hipGetLastError());
   }
+}
+
+// Times every candidate configuration of this shape (the GEMM runs for real,
+// so tuning passes produce throwaway results) and returns the fastest.
+F64Cfg tune_f64(const GemmDesc& d, int am, int bm, hipStream_t s) {
+  const F64Cfg cands[4] = {{128, true}, {128, false}, {64, true}, {64, false}};
+  const bool can_split = d.batch == 1 && d.K >= 2048;
+  hipEvent_t e0, e1;
+  CCGPU_HIP(hipEventCreate(&e0));
+  CCGPU_HIP(hipEventCreate(&e1));
+  F64Cfg best;
+  float best_ms = 1e30f;
+  for (const F64Cfg& c : cands) {
+    if (!c.splitk && !can_split) continue;  // identical to the split variant
+    run_f64(d, c, am, bm, s);                // warm-up (also sizes scratch)
+    CCGPU_HIP(hipEventRecord(e0, s));
+    for (int r = 0; r < 3; ++r) run_f64(d, c, am, bm, s);
+    CCGPU_HIP(hipEventRecord(e1, s));
+    CCGPU_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    CCGPU_HIP(hipEventElapsedTime(&ms, e0, e1));
+    if (ms < best_ms) {
+      best_ms = ms;
+      best = c;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return best;
+}
+}  // namespace
+
+void gemm_f64(const GemmDesc& d, hipStream_t s) {
+  if (d.M <= 0 || d.N <= 0 || d.batch <= 0) return;
+  check_gemm(d, sizeof(double));
+  const int am = (d.a_m == 1 && d.a_k != 1) ? RFAST : KFAST;
+  const int bm = (d.b_n == 1 && d.b_k != 1) ? RFAST : KFAST;
+  F64Cfg cfg;  // default: 128 x 128 tiles, split K when the launch is small
+  Tuner& tu = tuner();
+  const ShapeKey key{d.M, d.N, d.K, (d.Kin > 0 && d.Kin < d.K) ? d.Kin : 0, d.batch, am, bm};
+  bool found = false;
+  {
+    std::lock_guard<std::mutex> lk(tu.mu);
+    auto it = tu.best.find(key);
+    if (it != tu.best.end()) {
+      cfg = it->second;
+      found = true;
+    }
+  }
+  if (!found && tu.on && !dryrun() && !capturing(s)) {
+    cfg = tune_f64(d, am, bm, s);
+    std::lock_guard<std::mutex> lk(tu.mu);
+    tu.best[key] = cfg;
+  }
+  run_f64(d, cfg, am, bm, s);
 }
 
 void gemm_bf16x3(const GemmDesc& d, hipStream_t s);

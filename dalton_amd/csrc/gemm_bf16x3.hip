@@ -215,11 +215,6 @@ __global__ __launch_bounds__(256, 2) void k_gemm_bf16x3(BMulti p) {
       }
 }
 
-// packed operand scratch (grown on demand, never destroyed: see gemm.hip)
-DBuf<unsigned short>& pack_buf(int i) {
-  static auto* p = new DBuf<unsigned short>[4];
-  return p[i];
-}
 
 template <typename OutT>
 void launch_multi(const BMulti& p, int batch, hipStream_t s) {
@@ -293,13 +288,14 @@ void gemm_bf16x3(const GemmDesc& d, hipStream_t s) {
   check_gemm(d, sizeof(double));
   const long Mp = bf16x3_rpad(d.M), Np = bf16x3_rpad(d.N), Kp = bf16x3_kpad(d.K);
   const long na = (long)d.batch * Mp * Kp, nb = (long)d.batch * Np * Kp;
-  for (int i = 0; i < 2; ++i) pack_buf(i).ensure(na);
-  for (int i = 2; i < 4; ++i) pack_buf(i).ensure(nb);
+  StreamScratch& sc = scratch_for(s);
+  unsigned short* pk[4];
+  for (int i = 0; i < 4; ++i) pk[i] = grow_scratch(sc.pack[i], i < 2 ? na : nb, s);
   bf16x3_pack(static_cast<const double*>(d.A), d.M, d.K, d.a_m, d.a_k, d.a_b, d.batch, Mp, Kp, 1.0f,
-              pack_buf(0).p, pack_buf(1).p, s, d.a_ko, d.Kin);
+              pk[0], pk[1], s, d.a_ko, d.Kin);
   bf16x3_pack(static_cast<const double*>(d.B), d.N, d.K, d.b_n, d.b_k, d.b_b, d.batch, Np, Kp, 1.0f,
-              pack_buf(2).p, pack_buf(3).p, s, d.b_ko, d.Kin);
-  BTerm t{pack_buf(0).p, pack_buf(1).p, pack_buf(2).p, pack_buf(3).p, Kp, Mp * Kp, Np * Kp};
+              pk[2], pk[3], s, d.b_ko, d.Kin);
+  BTerm t{pk[0], pk[1], pk[2], pk[3], Kp, Mp * Kp, Np * Kp};
   bf16x3_multi(&t, 1, d.M, d.N, d.batch, d.C, false, d.c_m, d.c_n, d.c_b, d.alpha, d.beta,
                2.0 * d.M * d.N * d.K * d.batch, s);
 }

@@ -84,3 +84,27 @@ namespace ccgpu {
 void add_strided3(double* out, long os0, long os1, long os2, const double* in, long n0, long n1, long n2,
                   double alpha, hipStream_t st);
 }  // namespace ccgpu
+
+namespace ccgpu {
+// ---- device-resident DIIS (CCSD_DIIS, cc/ccsd_diis.F:251-365).  ctr[0] is the
+// iteration count of the current DIIS cycle (1-based), so every launch is
+// independent of host state and an iteration can be captured in a graph.
+// Slot ivec = it for it <= mx, else it - ((it-1)/mx)*mx; nloop = ivec.
+// svec/evec of slot ivec from S = [s1; pack(s2)] and T = [t1; pack(t2)]
+void pack_diis_slot(double* diisS, double* diisE, const int* ctr, int mx, const double* s1, const double* s2,
+                    const double* t1, const double* t2, long nov, hipStream_t st);
+// dots[i] = <E_ivec, E_i> for i < nloop (deterministic two-pass)
+void multidot_slot(double* dots, const double* diisE, long ld, const int* ctr, int mx, long n, hipStream_t st);
+// Extends the B matrix (Amat, leading dimension ld) by the new dots and
+// solves Dalton's bordered system with partial pivoting; coef[0..nloop).
+// A zero pivot sets ctr[1] = 1 and takes the newest vector alone.
+void diis_solve(double* Amat, int ld, const double* dots, double* coef, int* ctr, int mx, double* work,
+                hipStream_t st);
+// out = sum_{i < nloop} coef[i] S_i
+void lincomb_slot(double* out, const double* S, long ld, const double* coef, const int* ctr, int mx, long n,
+                  hipStream_t st);
+// ctr[0] += 1
+void ctr_advance(int* ctr, hipStream_t st);
+// out[0..3] = scal[0], scal[1], ctr[1], ctr[0]
+void report4(double* out, const double* scal, const int* ctr, hipStream_t st);
+}  // namespace ccgpu

@@ -139,16 +139,8 @@ __global__ void k_sum_partials(double* out, const double* part, int n, int strid
 }
 
 namespace {
-// partial-sum workspace (stream-ordered reuse); never destroyed
-DBuf<double>& g_part_ref() {
-  static auto* p = new DBuf<double>();
-  return *p;
-}
-#define g_part g_part_ref()
-double* partials(long n) {
-  g_part.ensure(n);
-  return g_part.p;
-}
+// partial-sum workspace of the stream (stream-ordered reuse)
+double* partials(long n, hipStream_t s) { return grow_scratch(scratch_for(s).part, (size_t)n, s); }
 }  // namespace
 
 // ------------------------------------------------------------------ gemv
@@ -183,7 +175,7 @@ void gemv_mc(long M, long K, double alpha, const double* A, long lda, const doub
   const long kchunk = (K + nk - 1) / nk;
   nk = (K + kchunk - 1) / kchunk;
   if (nk < 1) nk = 1;
-  double* part = partials(nk * M);
+  double* part = partials(nk * M, st);
   CCGPU_LAUNCH(k_gemv_mc, dim3((unsigned)mb, (unsigned)nk), dim3(256), 0, st, M, K, kchunk,
                      alpha, A, lda, x, part);
   CCGPU_LAUNCH(k_gemv_finish, dim3((unsigned)mb), dim3(256), 0, st, M, (int)nk, part, beta, y);
@@ -285,7 +277,7 @@ void nxtam(double* om1, double* om2, const double* t1, const double* t2, const d
   check_span(t2, 0, nov * nov - 1, sizeof(double), "nxtam t2");
   check_span(eps, 0, o + v - 1, sizeof(double), "nxtam eps");
   check_span(norm2, 0, 0, sizeof(double), "nxtam norm2");
-  double* part = partials(nov + 1);
+  double* part = partials(nov + 1, st);
   CCGPU_LAUNCH(k_nxtam1, dim3(1), dim3(256), 0, st, om1, t1, eps, o, v, part + nov);
   CCGPU_LAUNCH(k_nxtam2, dim3((unsigned)nov), dim3(256), 0, st, om2, t2, eps, o, v, part);
   CCGPU_LAUNCH(k_sum_partials, dim3(1), dim3(64), 0, st, norm2, part, (int)(nov + 1), 1, 1);
@@ -352,7 +344,7 @@ void multidot(double* dots, const double* e, const double* E, long ld, int nvec,
   check_span(e, 0, n - 1, sizeof(double), "multidot e");
   check_span(E, 0, (long)(nvec - 1) * ld + n - 1, sizeof(double), "multidot E");
   const int nb = (int)std::min<long>(2048, std::max<long>(1, (n + 255) / 256));
-  double* part = partials((long)nb * kDotGroup);
+  double* part = partials((long)nb * kDotGroup, st);
   for (int g0 = 0; g0 < nvec; g0 += kDotGroup) {
     const int ng = std::min(kDotGroup, nvec - g0);
     CCGPU_LAUNCH(k_multidot, dim3(nb), dim3(256), 0, st, e, E, ld, g0, ng, n, part);
@@ -397,7 +389,7 @@ void ccsd_energy(double* out, const double* L, const double* t2, const double* t
   check_span(L, 0, nov * nov - 1, sizeof(double), "energy L");
   check_span(t2, 0, nov * nov - 1, sizeof(double), "energy t2");
   check_span(t1, 0, nov - 1, sizeof(double), "energy t1");
-  double* part = partials(nov);
+  double* part = partials(nov, st);
   CCGPU_LAUNCH(k_energy, dim3((unsigned)nov), dim3(256), 0, st, L, t2, t1, nov, part);
   CCGPU_LAUNCH(k_sum_partials, dim3(1), dim3(64), 0, st, out, part, (int)nov, 1, 1);
   CCGPU_HIPX(hipGetLastError());
@@ -771,6 +763,201 @@ void add_strided3(double* out, long os0, long os1, long os2, const double* in, l
   check_span(out, 0, (n0 - 1) * os0 + (n1 - 1) * os1 + (n2 - 1) * os2, sizeof(double), "add_strided3 out");
   const unsigned nb = (unsigned)std::min<long>(8192, (total + 255) / 256);
   CCGPU_LAUNCH(k_add_strided3, dim3(nb), dim3(256), 0, st, out, os0, os1, os2, in, n1, n2, total, alpha);
+  CCGPU_HIPX(hipGetLastError());
+}
+}  // namespace ccgpu
+
+namespace ccgpu {
+__device__ __forceinline__ int diis_ivec(int it, int mx) { return it > mx ? it - ((it - 1) / mx) * mx : it; }
+
+__global__ void k_diff1_slot(double* diisS, double* diisE, long ld, const int* ctr, int mx, const double* s,
+                             const double* t, long n) {
+  const long off = (long)(diis_ivec(ctr[0], mx) - 1) * ld;
+  double* svec = diisS + off;
+  double* evec = diisE + off;
+  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
+    svec[e] = s[e];
+    evec[e] = s[e] - t[e];
+  }
+}
+
+__global__ void k_pack_diis2_slot(double* diisS, double* diisE, long ld, long head, const int* ctr, int mx,
+                                  const double* __restrict__ S, const double* __restrict__ T, long n) {
+  const long off = (long)(diis_ivec(ctr[0], mx) - 1) * ld + head;
+  double* svec = diisS + off;
+  double* evec = diisE + off;
+  const long bj = blockIdx.x;
+  const long base = bj * (bj + 1) / 2;
+  for (long ai = threadIdx.x; ai <= bj; ai += blockDim.x) {
+    const double s = S[bj * n + ai];
+    svec[base + ai] = s;
+    evec[base + ai] = s - T[bj * n + ai];
+  }
+}
+
+void pack_diis_slot(double* diisS, double* diisE, const int* ctr, int mx, const double* s1, const double* s2,
+                    const double* t1, const double* t2, long nov, hipStream_t st) {
+  const long ntot = nov + nov * (nov + 1) / 2;
+  check_span(diisS, 0, (long)mx * ntot - 1, sizeof(double), "pack_diis_slot S");
+  check_span(diisE, 0, (long)mx * ntot - 1, sizeof(double), "pack_diis_slot E");
+  check_span(ctr, 0, 1, sizeof(int), "pack_diis_slot ctr");
+  check_span(s1, 0, nov - 1, sizeof(double), "pack_diis_slot s1");
+  check_span(t1, 0, nov - 1, sizeof(double), "pack_diis_slot t1");
+  check_span(s2, 0, nov * nov - 1, sizeof(double), "pack_diis_slot s2");
+  check_span(t2, 0, nov * nov - 1, sizeof(double), "pack_diis_slot t2");
+  CCGPU_LAUNCH(k_diff1_slot, dim3(cdiv(nov, 256)), dim3(256), 0, st, diisS, diisE, ntot, ctr, mx, s1, t1, nov);
+  CCGPU_LAUNCH(k_pack_diis2_slot, dim3((unsigned)nov), dim3(256), 0, st, diisS, diisE, ntot, nov, ctr, mx, s2,
+               t2, nov);
+  CCGPU_HIPX(hipGetLastError());
+}
+
+__global__ void k_multidot_slot(const double* __restrict__ E, long ld, const int* ctr, int mx, int g0, long n,
+                                double* part) {
+  __shared__ double sh[16];
+  const int ivec = diis_ivec(ctr[0], mx), nloop = ivec;
+  const double* e = E + (long)(ivec - 1) * ld;
+  double s[kDotGroup];
+#pragma unroll
+  for (int i = 0; i < kDotGroup; ++i) s[i] = 0.0;
+  for (long x = (long)blockIdx.x * blockDim.x + threadIdx.x; x < n; x += (long)gridDim.x * blockDim.x) {
+    const double ex = e[x];
+#pragma unroll
+    for (int i = 0; i < kDotGroup; ++i)
+      if (g0 + i < nloop) s[i] += ex * E[(long)(g0 + i) * ld + x];
+  }
+  for (int i = 0; i < kDotGroup; ++i) {
+    const double t = block_sum(s[i], sh);
+    if (threadIdx.x == 0) part[(long)blockIdx.x * kDotGroup + i] = t;
+  }
+}
+
+void multidot_slot(double* dots, const double* diisE, long ld, const int* ctr, int mx, long n, hipStream_t st) {
+  check_span(dots, 0, mx - 1, sizeof(double), "multidot_slot dots");
+  check_span(diisE, 0, (long)(mx - 1) * ld + n - 1, sizeof(double), "multidot_slot E");
+  check_span(ctr, 0, 1, sizeof(int), "multidot_slot ctr");
+  const int nb = (int)std::min<long>(2048, std::max<long>(1, (n + 255) / 256));
+  double* part = partials((long)nb * kDotGroup, st);
+  for (int g0 = 0; g0 < mx; g0 += kDotGroup) {
+    const int ng = std::min(kDotGroup, mx - g0);
+    CCGPU_LAUNCH(k_multidot_slot, dim3(nb), dim3(256), 0, st, diisE, ld, ctr, mx, g0, n, part);
+    CCGPU_LAUNCH(k_sum_partials, dim3(1), dim3(64), 0, st, dots + g0, part, nb, kDotGroup, ng);
+  }
+  CCGPU_HIPX(hipGetLastError());
+}
+
+// One thread: Dalton's bordered DIIS system
+//   [ B  -1 ] [c]   [ 0]
+//   [-1   0 ] [l] = [-1]
+// solved by Gaussian elimination with partial pivoting.
+__global__ void k_diis_solve(double* Amat, int LD, const double* dots, double* coef, int* ctr, int mx,
+                             double* M) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const int ivec = diis_ivec(ctr[0], mx), nloop = ivec, n = nloop + 1;
+  for (int i = 0; i < nloop; ++i) {
+    Amat[i * LD + (ivec - 1)] = dots[i];
+    Amat[(ivec - 1) * LD + i] = dots[i];
+  }
+  double* b = M + n * n;
+  for (int i = 0; i < n; ++i) {
+    for (int j = 0; j < n; ++j) M[i * n + j] = 0.0;
+    b[i] = 0.0;
+  }
+  for (int i = 0; i < nloop; ++i) {
+    for (int j = 0; j < nloop; ++j) M[i * n + j] = Amat[i * LD + j];
+    M[i * n + nloop] = -1.0;
+    M[nloop * n + i] = -1.0;
+  }
+  b[nloop] = -1.0;
+  for (int j = 0; j < n; ++j) {
+    int p = j;
+    for (int i = j + 1; i < n; ++i)
+      if (fabs(M[i * n + j]) > fabs(M[p * n + j])) p = i;
+    if (M[p * n + j] == 0.0) {
+      ctr[1] = 1;
+      for (int i = 0; i < nloop; ++i) coef[i] = (i == ivec - 1) ? 1.0 : 0.0;
+      return;
+    }
+    if (p != j) {
+      for (int k = 0; k < n; ++k) {
+        const double t = M[j * n + k];
+        M[j * n + k] = M[p * n + k];
+        M[p * n + k] = t;
+      }
+      const double t = b[j];
+      b[j] = b[p];
+      b[p] = t;
+    }
+    for (int i = j + 1; i < n; ++i) {
+      const double f = M[i * n + j] / M[j * n + j];
+      for (int k = j; k < n; ++k) M[i * n + k] -= f * M[j * n + k];
+      b[i] -= f * b[j];
+    }
+  }
+  for (int i = n - 1; i >= 0; --i) {
+    double s = b[i];
+    for (int k = i + 1; k < n; ++k) s -= M[i * n + k] * b[k];
+    b[i] = s / M[i * n + i];
+  }
+  for (int i = 0; i < nloop; ++i) coef[i] = b[i];
+}
+
+void diis_solve(double* Amat, int ld, const double* dots, double* coef, int* ctr, int mx, double* work,
+                hipStream_t st) {
+  CCGPU_REQUIRE(mx + 1 <= ld, "diis_solve: leading dimension too small");
+  check_span(Amat, 0, (long)ld * ld - 1, sizeof(double), "diis_solve Amat");
+  check_span(dots, 0, mx - 1, sizeof(double), "diis_solve dots");
+  check_span(coef, 0, mx - 1, sizeof(double), "diis_solve coef");
+  check_span(work, 0, (long)(mx + 1) * (mx + 2) - 1, sizeof(double), "diis_solve work");
+  check_span(ctr, 0, 1, sizeof(int), "diis_solve ctr");
+  CCGPU_LAUNCH(k_diis_solve, dim3(1), dim3(64), 0, st, Amat, ld, dots, coef, ctr, mx, work);
+  CCGPU_HIPX(hipGetLastError());
+}
+
+__global__ void k_lincomb_slot(double* __restrict__ out, const double* __restrict__ S, long ld,
+                               const double* __restrict__ c, const int* ctr, int mx, long n) {
+  const int nvec = diis_ivec(ctr[0], mx);
+  for (long x = (long)blockIdx.x * blockDim.x + threadIdx.x; x < n; x += (long)gridDim.x * blockDim.x) {
+    double s = 0.0;
+    for (int i = 0; i < nvec; ++i) s += c[i] * S[(long)i * ld + x];
+    out[x] = s;
+  }
+}
+
+void lincomb_slot(double* out, const double* S, long ld, const double* coef, const int* ctr, int mx, long n,
+                  hipStream_t st) {
+  check_span(out, 0, n - 1, sizeof(double), "lincomb_slot out");
+  check_span(S, 0, (long)(mx - 1) * ld + n - 1, sizeof(double), "lincomb_slot S");
+  check_span(coef, 0, mx - 1, sizeof(double), "lincomb_slot c");
+  check_span(ctr, 0, 1, sizeof(int), "lincomb_slot ctr");
+  const unsigned nb = (unsigned)std::min<long>(65536, (n + 255) / 256);
+  CCGPU_LAUNCH(k_lincomb_slot, dim3(nb), dim3(256), 0, st, out, S, ld, coef, ctr, mx, n);
+  CCGPU_HIPX(hipGetLastError());
+}
+
+__global__ void k_ctr_advance(int* ctr) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) ctr[0] += 1;
+}
+
+void ctr_advance(int* ctr, hipStream_t st) {
+  check_span(ctr, 0, 0, sizeof(int), "ctr_advance");
+  CCGPU_LAUNCH(k_ctr_advance, dim3(1), dim3(64), 0, st, ctr);
+  CCGPU_HIPX(hipGetLastError());
+}
+
+__global__ void k_report4(double* out, const double* scal, const int* ctr) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    out[0] = scal[0];
+    out[1] = scal[1];
+    out[2] = (double)ctr[1];
+    out[3] = (double)ctr[0];
+  }
+}
+
+void report4(double* out, const double* scal, const int* ctr, hipStream_t st) {
+  check_span(out, 0, 3, sizeof(double), "report4 out");
+  check_span(scal, 0, 1, sizeof(double), "report4 scal");
+  check_span(ctr, 0, 1, sizeof(int), "report4 ctr");
+  CCGPU_LAUNCH(k_report4, dim3(1), dim3(64), 0, st, out, scal, ctr);
   CCGPU_HIPX(hipGetLastError());
 }
 }  // namespace ccgpu

@@ -21,6 +21,39 @@ std::map<uintptr_t, size_t>& allocs() {
 uintptr_t g_next = uintptr_t(1) << 44;  // fake device addresses for dry runs
 }  // namespace
 
+bool capturing(hipStream_t s) {
+  if (dryrun()) return false;
+  hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+  CCGPU_HIP(hipStreamIsCapturing(s, &st));
+  return st == hipStreamCaptureStatusActive;
+}
+
+namespace {
+std::map<hipStream_t, StreamScratch*>& scratch_map() {
+  static auto* m = new std::map<hipStream_t, StreamScratch*>;
+  return *m;
+}
+}  // namespace
+
+StreamScratch& scratch_for(hipStream_t s) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  StreamScratch*& p = scratch_map()[s];
+  if (!p) p = new StreamScratch;
+  return *p;
+}
+
+void release_scratch(hipStream_t s) {
+  StreamScratch* p = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto it = scratch_map().find(s);
+    if (it == scratch_map().end()) return;
+    p = it->second;
+    scratch_map().erase(it);
+  }
+  delete p;
+}
+
 bool& dryrun() {
   static bool flag = false;
   return flag;
@@ -106,7 +139,7 @@ void prof_start() {
 }
 
 int prof_begin(const char* kernel, double flops, hipStream_t s) {
-  if (!profiling()) return -1;
+  if (!profiling() || capturing(s)) return -1;
   Prof& p = prof();
   p.launches.push_back({kernel, flops, new_event(s), nullptr});
   return static_cast<int>(p.launches.size()) - 1;
@@ -118,7 +151,7 @@ void prof_end(int id, hipStream_t s) {
 }
 
 void prof_phase(const char* name, hipStream_t s) {
-  if (!profiling()) return;
+  if (!profiling() || capturing(s)) return;
   prof().phases.push_back({name, new_event(s)});
 }
 

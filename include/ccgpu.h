@@ -85,6 +85,31 @@ int ccgpu_ccsd_iterate(ccgpu_ctx* ctx, int precision, double* e_corr, double* om
 int ccgpu_ccsd_energy(ccgpu_ctx* ctx, int maxit, double threnr, double thrvec, int precision,
                       double* e_mp2, double* e_ccsd, int* niter, double* history);
 
+/* Many independent closed-shell systems converged together (no reference
+ * counterpart: it runs the CCSD_ENERGY loop of cc/ccsd_energy.F:742-982 for
+ * nsys systems at once).  System k has nocc[k], nvir[k], naux[k] and host
+ * arrays eps[k], B[k] as for ccgpu_init.  ladder_budget_bytes bounds each
+ * system's particle-ladder batch buffers (<= 0: the single-system default,
+ * 10 GB).  ccgpu_batch_energy converges every system from its MP2 start with
+ * Dalton's test (|dE| < threnr and norm < thrvec; mixed precision switches to
+ * FP64 as in ccgpu_ccsd_energy): one iteration of all still-active systems is
+ * one HIP graph replay, systems running as concurrent graph branches, after
+ * an autotuning pass over the FP64 GEMM shapes.  e_mp2/e_ccsd/niter receive
+ * nsys values.  flags: CCGPU_BATCH_EAGER issues the same work without a
+ * graph, CCGPU_BATCH_NO_TUNE skips the autotuning pass. */
+typedef struct ccgpu_batch ccgpu_batch;
+#define CCGPU_BATCH_EAGER 1
+#define CCGPU_BATCH_NO_TUNE 2
+int ccgpu_batch_init(int nsys, const int* nocc, const int* nvir, const int* naux, const double* const* eps,
+                     const double* const* B, int mxdiis, double ladder_budget_bytes, ccgpu_batch** batch);
+int ccgpu_batch_energy(ccgpu_batch* batch, int maxit, double threnr, double thrvec, int precision, int flags,
+                       double* e_mp2, double* e_ccsd, int* niter);
+/* Converged amplitudes of system k (layouts of ccgpu_get_amplitudes). */
+int ccgpu_batch_get_amplitudes(ccgpu_batch* batch, int k, double* t1am, double* t2am);
+/* Number of graph captures and replays of the last ccgpu_batch_energy. */
+int ccgpu_batch_stats(ccgpu_batch* batch, int* captures, int* replays);
+int ccgpu_batch_finalize(ccgpu_batch* batch);
+
 /* Amplitudes in Dalton's layouts (CC_WRRSP/CC_RDRSP 'R0' content, cc/cc_wrrsp.F:20). */
 int ccgpu_get_amplitudes(ccgpu_ctx* ctx, double* t1am, double* t2am);
 int ccgpu_set_amplitudes(ccgpu_ctx* ctx, const double* t1am, const double* t2am);

@@ -67,3 +67,37 @@ def test_dryrun_bounds(shapes):
                        env=env, capture_output=True, text=True, timeout=600)
     assert r.returncode == 0, r.stdout + r.stderr
     assert r.stdout.count("ok") == len(shapes)
+
+
+BATCH_SCRIPT = r"""
+import ctypes as C, sys
+import numpy as np
+sys.path.insert(0, {root!r})
+from dalton_amd import _lib
+L = _lib.lib()
+shapes = {shapes!r}
+n = len(shapes)
+o = (C.c_int * n)(*[s[0] for s in shapes]); v = (C.c_int * n)(*[s[1] for s in shapes])
+na = (C.c_int * n)(*[s[2] for s in shapes])
+nul = (_lib._dp * n)()
+h = C.c_void_p()
+assert L.ccgpu_batch_init(n, o, v, na, nul, nul, 8, 2.0e6, C.byref(h)) == 0, L.ccgpu_last_error()
+e0 = np.zeros(n); e1 = np.zeros(n); it = (C.c_int * n)()
+for prec in (0, 1):
+    for flags in (0, 1, 3):
+        rc = L.ccgpu_batch_energy(h, 5, 1e-8, 1e9, prec, flags, _lib.ptr(e0), _lib.ptr(e1), it)
+        assert rc == 0, L.ccgpu_last_error()
+assert L.ccgpu_batch_finalize(h) == 0
+print("ok batch", flush=True)
+"""
+
+
+def test_dryrun_batch_bounds():
+    """The batched solver's per-system launches (graph path falls back to the
+    same enqueue sequence in a dry run) at uneven shapes, both precisions."""
+    shapes = [(3, 20, 40), (5, 40, 160), (2, 7, 9), (1, 1, 1), (30, 360, 1440)]
+    env = dict(os.environ, CCGPU_DRYRUN="1")
+    r = subprocess.run([sys.executable, "-c", BATCH_SCRIPT.format(root=ROOT, shapes=shapes)],
+                       env=env, capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "ok batch" in r.stdout

@@ -79,3 +79,27 @@ def test_mixed_pt_close_to_oracle():
         e4, e5 = c.pt(precision=d.MIXED)
     assert abs(e4 - r4) < 1e-4 * abs(r4)
     assert abs(e5 - r5) < 1e-4 * abs(r5) + 1e-4 * abs(r4)
+
+
+def test_mixed_ccsd_t_within_1uEh_convergent_synthetic_o32_v256():
+    """A synthetic o=32, v=256, naux=1024 system with an integral scale at
+    which CCSD converges (dalton_amd.synthetic.batch_sigma): the mixed path's
+    CCSD(T) energy within 1 uEh of the FP64 path's, with the full (T) sum."""
+    import dalton_amd as d
+    from dalton_amd.synthetic import batch_sigma, make_system
+
+    o, v, naux = 32, 256, 1024
+    eps, B = make_system(o, v, naux, 6, sigma=batch_sigma(v, naux))
+    print("convergent synthetic o32 v256: fp64 then mixed CCSD(T)", flush=True)
+    out = {}
+    for label, prec in (("fp64", d.FP64), ("mixed", d.MIXED)):
+        with d.CCContext(eps, B, o) as c:
+            r = c.ccsd_energy(maxit=60, threnr=1e-9, precision=prec)
+            e4, e5 = c.pt(prec)
+        out[label] = (r["e_ccsd"], e4 + e5, r["niter"])
+        print(f"  {label}: E_CCSD = {r['e_ccsd']:.12f} ({r['niter']} it), E_(T) = {e4 + e5:.12f}", flush=True)
+    dccsd = abs(out["mixed"][0] - out["fp64"][0])
+    dt = abs(out["mixed"][1] - out["fp64"][1])
+    dtot = abs(out["mixed"][0] + out["mixed"][1] - out["fp64"][0] - out["fp64"][1])
+    print(f"  |dE_CCSD| = {dccsd:.2e}  |dE_(T)| = {dt:.2e}  |dE_CCSD(T)| = {dtot:.2e}", flush=True)
+    assert dtot < 1e-6
