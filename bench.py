@@ -207,6 +207,24 @@ def batch_workload(nsys=32, seed=5):
     return out
 
 
+def copy_workload(nbytes=2 << 30, reps=3):
+    """A plain device copy of known size (torch's copy kernel): the
+    calibration run for the FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+
+    n = nbytes // 8
+    x = torch.ones(n, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    return {"workload": "device copy", "bytes_read": nbytes, "bytes_written": nbytes, "reps": reps,
+            "GBps_read_plus_write": round(2 * nbytes / dt / 1e9, 1)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -222,7 +240,7 @@ def main():
                     help="(T) triples timed, spread evenly over the i>=j>=k list (0: skip)")
     ap.add_argument("--lhtr-vectors", type=int, default=16,
                     help="left-transformation trial vectors timed at o=32 v=256 naux=1024 (0: skip)")
-    ap.add_argument("--workload", choices=["ccsd", "pt48", "batch"], default="ccsd",
+    ap.add_argument("--workload", choices=["ccsd", "pt48", "batch", "copy"], default="ccsd",
                     help="ccsd: the headline line (default); pt48: full (T) at o=48 v=384 in both "
                          "precisions on a convergent system; batch: 32 uneven systems converged in "
                          "one graph replay per iteration")
@@ -236,6 +254,9 @@ def main():
         return
     if args.workload == "batch":
         print(json.dumps(batch_workload()), flush=True)
+        return
+    if args.workload == "copy":
+        print(json.dumps(copy_workload()), flush=True)
         return
 
     import dalton_amd as d
@@ -272,7 +293,9 @@ def main():
         return dt / nsteps, hist, profile_stop()
 
     spi_m, hist_m, prof_m = timed_steps(d.MIXED, args.steps, "mixed")
-    spi_d, hist_d, prof_d = timed_steps(d.FP64, args.fp64_steps, "fp64")
+    fp64_leg = args.fp64_steps > 0
+    if fp64_leg:
+        spi_d, hist_d, prof_d = timed_steps(d.FP64, args.fp64_steps, "fp64")
 
     def kern_table(prof):
         return {k: {"count": v_["count"], "ms": round(v_["ms"], 3),
@@ -340,10 +363,10 @@ def main():
         "fp64": {"s_per_iter": round(spi_d, 4), "steps": args.fp64_steps, "ccsd_tflops": round(F / spi_d / 1e12, 3),
                  "energies": hist_d, "roofline": roofline(prof_d, spi_d * args.fp64_steps, PEAK["fp64"], "FP64 matrix"),
                  "phases_ms_per_iter": {k: round(v_ / args.fp64_steps, 2) for k, v_ in prof_d["phases"].items()},
-                 "kernels": kern_table(prof_d)},
-        "speedup_mixed_vs_fp64": round(spi_d / spi_m, 3),
+                 "kernels": kern_table(prof_d)} if fp64_leg else None,
+        "speedup_mixed_vs_fp64": round(spi_d / spi_m, 3) if fp64_leg else None,
         "energies": hist_m,
-        "rel_dE_per_step_mixed_vs_fp64": [abs(a - b) / abs(b) for a, b in zip(hist_m, hist_d)],
+        "rel_dE_per_step_mixed_vs_fp64": [abs(a - b) / abs(b) for a, b in zip(hist_m, hist_d)] if fp64_leg else None,
         "roofline": roofline(prof_m, spi_m * args.steps, round(PEAK["bf16"] / 3, 1),
                              "BF16 dense matrix peak / 3 (three bf16 products per FP32-equivalent product)"),
         "phases_ms_per_iter": {k: round(v_ / args.steps, 2) for k, v_ in prof_m["phases"].items()},

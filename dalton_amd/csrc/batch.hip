@@ -114,16 +114,21 @@ void Batch::solve(int maxit, double threnr, double thrvec, GemmPrec prec, bool u
   std::vector<int> active;
   for (int k = 0; k < n; ++k) active.push_back(k);
   bool dirty = true;
+  // A round that launches shapes or precisions not seen before runs eagerly:
+  // it sizes the per-stream scratch, which must not grow inside a capture.
+  bool fresh = true;
   for (int round = 1; !active.empty(); ++round) {
     for (int k : active) sys_[k]->set_precision(fp64[k] ? GemmPrec::F64 : prec);
-    if (use_graph && !dryrun()) {
+    if (use_graph && !dryrun() && !fresh) {
       if (dirty || !exec_) capture(active);
       dirty = false;
       CCGPU_HIP(hipGraphLaunch(exec_, master_));
       ++replays_;
     } else {
       enqueue_active(active);
+      dirty = true;
     }
+    fresh = false;
     CCGPU_HIPX(hipMemcpyAsync(host_report_.data(), report_.p, report_.bytes(), hipMemcpyDeviceToHost, master_));
     CCGPU_HIPX(hipStreamSynchronize(master_));
     std::vector<int> next;
@@ -146,6 +151,7 @@ void Batch::solve(int maxit, double threnr, double thrvec, GemmPrec prec, bool u
       if (!fp64[k] && std::fabs(en[k] - e_old) < kSwitch && nrm < std::sqrt(kSwitch)) {
         fp64[k] = 1;
         dirty = true;
+        fresh = true;
       }
       if (round + 1 > maxit) throw Error(4, "CC equations not converged (system " + std::to_string(k) + ")");
       next.push_back(k);

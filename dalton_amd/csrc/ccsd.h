@@ -84,6 +84,7 @@ class CCSD {
  private:
   void dress(const double* t1);
   void ladder(const double* T, double* Om);
+  void ladder_mixed(double* Om);
   void mark(const char* name);
   void ensure_work();
   void pt_prepare(GemmPrec mode);
@@ -115,6 +116,10 @@ class CCSD {
   // ladder
   DBuf<double> taup_, taum_, Y_, Vp_, Vm_, Sp_, Sm_;
   std::vector<int> lad_batches_;
+  // mixed-precision ladder: bf16 hi/lo of Bvv [(a,c)][P], tau+/- [s][ij][cd],
+  // V+/- [s][r][cd]; S+/- [s][r][ij]
+  DBuf<unsigned short> lpBh_, lpBl_, lpTh_, lpTl_, lpVh_, lpVl_;
+  DBuf<double> S2_;
   // DIIS
   DBuf<double> diisS_, diisE_, dvec_, dots_, coef_, Amat_, diiswork_;
   DBuf<int> ctr_;  // [0] iteration of the DIIS cycle, [1] singular-matrix flag

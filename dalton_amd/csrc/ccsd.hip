@@ -171,10 +171,11 @@ void CCSD::release_work() {
   if (!work_) return;
   for (DBuf<double>* b : {&Bt_, &Bvv_, &Boo_, &BvoT_, &F_, &ht_, &hx_, &G1_, &Gam_, &Z_, &W_, &E1_, &E2_,
                           &bufR_, &bufX2_, &bufTx_, &bufU_, &bufC_, &bufD_, &taup_, &taum_, &Y_, &Vp_,
-                          &Vm_, &Sp_, &Sm_, &diisS_, &diisE_, &Tb_, &Ub_, &Txb_, &Db_, &Btb_, &Bvvb_, &Boob_,
+                          &Vm_, &Sp_, &Sm_, &S2_, &diisS_, &diisE_, &Tb_, &Ub_, &Txb_, &Db_, &Btb_, &Bvvb_, &Boob_,
                           &BvoTb_, &Zb_, &Wb_, &Fb_, &E1b_, &E2b_, &Jdb_, &Gamb_, &G1b_, &taupb_, &taumb_, &t1b_,
                           &vecb_})
     b->release();
+  for (DBuf<unsigned short>* b : {&lpBh_, &lpBl_, &lpTh_, &lpTl_, &lpVh_, &lpVl_}) b->release();
   work_ = false;
   lhtr_ready_ = false;
   diis_off_ = iter_ > 0 ? iter_ - 1 : 0;  // the DIIS history is gone: restart the cycle
@@ -222,6 +223,10 @@ void CCSD::dress(const double* t1) {
 // the dressed DF factor (v^4 never stored), contracted with tau+/- on MFMA.
 void CCSD::ladder(const double* T, double* Om) {
   (void)T;
+  if (bulk_ == GemmPrec::BF16X3) {
+    ladder_mixed(Om);
+    return;
+  }
   const int o = dm_.o, v = dm_.v, naux = dm_.naux;
   const long vv = (long)v * v;
   const Gemm bulk{bulk_, st_};
@@ -235,6 +240,53 @@ void CCSD::ladder(const double* T, double* Om) {
     bulk(rows, dm_.npo, dm_.npv, 1.0, Vp_.p, dm_.npv, 1, taup_.p, 1, dm_.npv, 0.0, Sp_.p, dm_.npo, 1);
     bulk(rows, dm_.npo, dm_.npv, 1.0, Vm_.p, dm_.npv, 1, taum_.p, 1, dm_.npv, 0.0, Sm_.p, dm_.npo, 1);
     ladder_scatter(Om, Sp_.p, Sm_.p, a0, a1, o, v, st_);
+  }
+}
+
+// The same ladder with BF16x3 products.  The dressed factor Bvv and tau+/-
+// are split into bf16 hi/lo once per call (not per batch); each batch forms
+// Y in FP32 straight from the packed factor, and V+/- goes from Y directly
+// into the packed operand of one batched (+ and -) S GEMM.
+void CCSD::ladder_mixed(double* Om) {
+  const int o = dm_.o, v = dm_.v, naux = dm_.naux;
+  const long vv = (long)v * v, npo = dm_.npo, npv = dm_.npv;
+  long maxr = 0;
+  for (size_t bi = 0; bi + 1 < lad_batches_.size(); ++bi) {
+    long rows = 0;
+    for (int a = lad_batches_[bi]; a < lad_batches_[bi + 1]; ++a) rows += v - a;
+    maxr = std::max(maxr, rows);
+  }
+  // rows of the packed factor: every batch reads a0*v + rpad(...) <= vv + 127 rows
+  const long RpB = bf16x3_rpad(vv + 127), KpB = bf16x3_kpad(naux);
+  const long RpT = bf16x3_rpad(npo), KpT = bf16x3_kpad(npv);
+  const long RpV = bf16x3_rpad(maxr);
+  lpBh_.ensure((size_t)RpB * KpB);
+  lpBl_.ensure((size_t)RpB * KpB);
+  lpTh_.ensure((size_t)2 * RpT * KpT);
+  lpTl_.ensure((size_t)2 * RpT * KpT);
+  lpVh_.ensure((size_t)2 * RpV * KpT);
+  lpVl_.ensure((size_t)2 * RpV * KpT);
+  S2_.ensure((size_t)2 * maxr * npo);
+  bf16x3_pack(Bvv_.p, vv, naux, 1, vv, 0, 1, RpB, KpB, 1.0f, lpBh_.p, lpBl_.p, st_);
+  bf16x3_pack(taup_.p, npo, npv, npv, 1, 0, 1, RpT, KpT, 1.0f, lpTh_.p, lpTl_.p, st_);
+  bf16x3_pack(taum_.p, npo, npv, npv, 1, 0, 1, RpT, KpT, 1.0f, lpTh_.p + RpT * KpT, lpTl_.p + RpT * KpT, st_);
+  float* Y32 = reinterpret_cast<float*>(Y_.p);
+  double* Sp = S2_.p;
+  double* Sm = S2_.p + maxr * npo;
+  for (size_t bi = 0; bi + 1 < lad_batches_.size(); ++bi) {
+    const int a0 = lad_batches_[bi], a1 = lad_batches_[bi + 1];
+    long rows = 0;
+    for (int a = a0; a < a1; ++a) rows += v - a;
+    const long M = (long)(a1 - a0) * v, Nn = (long)(v - a0) * v;
+    CCGPU_REQUIRE((size_t)(M * Nn) <= 2 * Y_.n, "ladder_mixed: Y block exceeds its buffer");
+    const BTerm ty{lpBh_.p + a0 * v * KpB, lpBl_.p + a0 * v * KpB, lpBh_.p + a0 * v * KpB,
+                   lpBl_.p + a0 * v * KpB, KpB, 0, 0};
+    bf16x3_multi(&ty, 1, M, Nn, 1, Y32, true, Nn, 1, 0, 1.0, 0.0, 2.0 * M * Nn * naux, st_);
+    const long Rp = bf16x3_rpad(rows);
+    ladder_vpm_pack(lpVh_.p, lpVl_.p, Y32, Nn, a0, a1, v, Rp, KpT, st_);
+    const BTerm ts{lpVh_.p, lpVl_.p, lpTh_.p, lpTl_.p, KpT, Rp * KpT, RpT * KpT};
+    bf16x3_multi(&ts, 1, rows, npo, 2, Sp, false, npo, 1, maxr * npo, 1.0, 0.0, 4.0 * rows * npo * npv, st_);
+    ladder_scatter(Om, Sp, Sm, a0, a1, o, v, st_);
   }
 }
 

@@ -172,6 +172,11 @@ void bf16x3_pack(const double* src, long rows, long K, long s_r, long s_k, long 
 void bf16x3_multi(const BTerm* terms, int nterm, long M, long N, int batch, void* C, bool c_float, long c_m,
                   long c_n, long c_b, double alpha, double beta, double flops, hipStream_t s);
 
+// V+/- of one particle-ladder a-batch from an FP32 Y block straight into the
+// split-bf16 operand layout hi/lo[2][Rp][Kp] (V+ then V-), zero padded.
+void ladder_vpm_pack(unsigned short* hi, unsigned short* lo, const float* Y, long ldY, int a0, int a1, int v,
+                     long Rp, long Kp, hipStream_t st);
+
 // Convenience: row-major C[M][N] (+)= alpha * op(A) op(B) for contiguous matrices.
 // ta=false: A is [M][lda]; ta=true: A is [K][lda] used transposed.
 void gemm_rm(bool ta, bool tb, long M, long N, long K, double alpha, const void* A, long lda,

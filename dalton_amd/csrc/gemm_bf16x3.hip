@@ -301,3 +301,97 @@ void gemm_bf16x3(const GemmDesc& d, hipStream_t s) {
 }
 
 }  // namespace ccgpu
+
+namespace ccgpu {
+// V+/-_{ab,cd} = Y[(a,c),(b,d)] +/- Y[(a,d),(b,c)] of one ladder a-batch,
+// from the FP32 Y block straight into the split-bf16 operand layout of the
+// S GEMM: hi/lo[s][r][p] (s = 0: V+, 1: V-; p = d(d+1)/2 + c), row length
+// Kp; the K padding p in [npv, Kp) is written as zeros by the first tile
+// pair of each row.
+__global__ void k_ladder_vpm_pack(unsigned short* __restrict__ hi, unsigned short* __restrict__ lo,
+                                  const float* __restrict__ Y, long ldY, int a0, int a1, int v, long Rp, long Kp) {
+  __shared__ float Mcd[32][33];
+  __shared__ float Mdc[32][33];
+  const long r = blockIdx.y;
+  int a = a0;
+  long off = 0;
+  while (a < a1 && off + (v - a) <= r) {
+    off += v - a;
+    ++a;
+  }
+  const int b = a + (int)(r - off);
+  const long bx = blockIdx.x;
+  long td = (long)((sqrt(8.0 * (double)bx + 1.0) - 1.0) * 0.5);
+  while (td * (td + 1) / 2 > bx) --td;
+  while ((td + 1) * (td + 2) / 2 <= bx) ++td;
+  const long tc = bx - td * (td + 1) / 2;
+  const int tx = threadIdx.x, ty = threadIdx.y;
+  const long c0 = tc * 32, d0 = td * 32;
+  const float* Ya = Y + (long)(a - a0) * v * ldY + (long)(b - a0) * v;
+#pragma unroll
+  for (int k = 0; k < 32; k += 8) {
+    const long c = c0 + ty + k, d = d0 + tx;
+    Mcd[ty + k][tx] = (c < v && d < v) ? Ya[c * ldY + d] : 0.0f;
+    const long dd = d0 + ty + k, cc = c0 + tx;
+    Mdc[ty + k][tx] = (dd < v && cc < v) ? Ya[dd * ldY + cc] : 0.0f;
+  }
+  __syncthreads();
+  const long npv = (long)v * (v + 1) / 2;
+  unsigned short* hp = hi + r * Kp;
+  unsigned short* lp = lo + r * Kp;
+  unsigned short* hm = hi + (Rp + r) * Kp;
+  unsigned short* lm = lo + (Rp + r) * Kp;
+#pragma unroll
+  for (int k = 0; k < 32; k += 8) {
+    const long c = c0 + tx, d = d0 + ty + k;
+    if (c < v && d < v && c <= d) {
+      const float x = Mcd[tx][ty + k], y = Mdc[ty + k][tx];
+      const long p = d * (d + 1) / 2 + c;
+      const float vp = x + y, vm = x - y;
+      const unsigned short h1 = f2bf(vp), h2 = f2bf(vm);
+      hp[p] = h1;
+      lp[p] = f2bf(vp - bf2f(h1));
+      hm[p] = h2;
+      lm[p] = f2bf(vm - bf2f(h2));
+    }
+  }
+  if (bx == 0) {
+    const int t = ty * 32 + tx;
+    for (long p = npv + t; p < Kp; p += 256) {
+      hp[p] = 0;
+      lp[p] = 0;
+      hm[p] = 0;
+      lm[p] = 0;
+    }
+  }
+}
+
+__global__ void k_fill_u16(unsigned short* p, long n) {
+  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) p[e] = 0;
+}
+
+void ladder_vpm_pack(unsigned short* hi, unsigned short* lo, const float* Y, long ldY, int a0, int a1, int v,
+                     long Rp, long Kp, hipStream_t st) {
+  long rows = 0;
+  for (int a = a0; a < a1; ++a) rows += v - a;
+  const long npv = (long)v * (v + 1) / 2;
+  CCGPU_REQUIRE(rows < 65536 && rows <= Rp && npv <= Kp, "ladder_vpm_pack: bad batch");
+  check_span(hi, 0, 2 * Rp * Kp - 1, sizeof(unsigned short), "ladder_vpm_pack hi");
+  check_span(lo, 0, 2 * Rp * Kp - 1, sizeof(unsigned short), "ladder_vpm_pack lo");
+  check_span(Y, 0, ((long)(a1 - a0) * v - 1) * ldY + (long)(v - a0) * v - 1, sizeof(float), "ladder_vpm_pack Y");
+  CCGPU_REQUIRE(ldY >= (long)(v - a0) * v, "ladder_vpm_pack: ldY too small");
+  const long nt = (v + 31) / 32;
+  CCGPU_LAUNCH(k_ladder_vpm_pack, dim3((unsigned)(nt * (nt + 1) / 2), (unsigned)rows), dim3(32, 8), 0, st, hi, lo,
+               Y, ldY, a0, a1, v, Rp, Kp);
+  // padding rows of both halves
+  const long pad = (Rp - rows) * Kp;
+  if (pad > 0) {
+    const unsigned nb = (unsigned)std::min<long>(4096, (pad + 255) / 256);
+    for (int s = 0; s < 2; ++s) {
+      CCGPU_LAUNCH(k_fill_u16, dim3(nb), dim3(256), 0, st, hi + (s * Rp + rows) * Kp, pad);
+      CCGPU_LAUNCH(k_fill_u16, dim3(nb), dim3(256), 0, st, lo + (s * Rp + rows) * Kp, pad);
+    }
+  }
+  CCGPU_HIPX(hipGetLastError());
+}
+}  // namespace ccgpu

@@ -42,8 +42,8 @@ def test_batch_fp64_matches_individual_solves(graph):
         assert res["niter"][k] == r["niter"], (k, res["niter"][k], r["niter"])
         assert abs(res["e_ccsd"][k] - r["e_ccsd"]) < 1e-10, (k, res["e_ccsd"][k], r["e_ccsd"])
         assert np.abs(amps[k][0] - t1).max() < 1e-9 and np.abs(amps[k][1] - t2).max() < 1e-9
-    if graph:
-        assert st["replays"] == int(max(res["niter"])) and st["captures"] >= 1
+    if graph:  # the first round runs eagerly (it sizes the scratch), the rest are replays
+        assert st["replays"] == int(max(res["niter"])) - 1 and st["captures"] >= 1
     else:
         assert st["replays"] == 0
 
