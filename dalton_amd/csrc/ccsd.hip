@@ -256,8 +256,8 @@ void CCSD::ladder_mixed(double* Om) {
     for (int a = lad_batches_[bi]; a < lad_batches_[bi + 1]; ++a) rows += v - a;
     maxr = std::max(maxr, rows);
   }
-  // rows of the packed factor: every batch reads a0*v + rpad(...) <= vv + 127 rows
-  const long RpB = bf16x3_rpad(vv + 127), KpB = bf16x3_kpad(naux);
+  // rows of the packed factor: every batch reads a0*v + rpad(...) < vv + (row padding) rows
+  const long RpB = bf16x3_rpad(vv + bf16x3_rpad(1) - 1), KpB = bf16x3_kpad(naux);
   const long RpT = bf16x3_rpad(npo), KpT = bf16x3_kpad(npv);
   const long RpV = bf16x3_rpad(maxr);
   lpBh_.ensure((size_t)RpB * KpB);

@@ -9,6 +9,9 @@
 // [batch][row][k] hi and lo arrays padded to the tile sizes; the GEMM kernel
 // then streams 16-byte chunks straight into LDS.  The epilogue applies
 // alpha/beta in FP64 to the FP64 output with arbitrary strides.
+#include <cmath>
+#include <cstdlib>
+
 #include "common.h"
 
 namespace ccgpu {
@@ -216,18 +219,146 @@ __global__ __launch_bounds__(256, 2) void k_gemm_bf16x3(BMulti p) {
 }
 
 
+// Large-tile variant for launches with many tiles and long K: block tile
+// 256x256 (4 waves of 128x128, 8x8 MFMA tiles each, accumulators held in
+// the accumulation registers), K step 32.  Every staged operand element
+// feeds twice as many products as in the 128x128 kernel, halving the
+// operand traffic from L2 and beyond per flop; one block per CU.
+constexpr int BT = 256, BTK = 32, BLDK = BTK + 8;
+constexpr int BNCH = BT * BTK / 8 / 256;  // 16-byte chunks per thread per array per K step
+
+template <typename OutT>
+__global__ __launch_bounds__(256, 1) void k_gemm_bf16x3_256(BMulti p) {
+  __shared__ __attribute__((aligned(16))) unsigned short sA[2][BT][BLDK];
+  __shared__ __attribute__((aligned(16))) unsigned short sB[2][BT][BLDK];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  const long tiles_m = p.Mp / BT, tiles_n = p.Np / BT;
+  long tm, tn;
+  tile_of3(blockIdx.x, tiles_m, tiles_n, tm, tn);
+  const long m0 = tm * BT, n0 = tn * BT, z = blockIdx.z;
+  constexpr int CPR = BTK / 8;
+  f4 acc[8][8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i)
+#pragma unroll
+    for (int j = 0; j < 8; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+  const int lr = lane & 15, lk = (lane >> 4) * 8;
+  int loff[BNCH];
+#pragma unroll
+  for (int q = 0; q < BNCH; ++q) {
+    const int c = tid + q * 256;
+    loff[q] = (c / CPR) * BLDK + (c % CPR) * 8;
+  }
+  for (int t = 0; t < p.nterm; ++t) {
+    const BTerm& T = p.t[t];
+    const long Kp = T.Kp;
+    const unsigned short* pAh = T.Ah + z * T.a_bs + m0 * Kp;
+    const unsigned short* pAl = T.Al + z * T.a_bs + m0 * Kp;
+    const unsigned short* pBh = T.Bh + z * T.b_bs + n0 * Kp;
+    const unsigned short* pBl = T.Bl + z * T.b_bs + n0 * Kp;
+    u32x4 rah[BNCH], ral[BNCH], rbh[BNCH], rbl[BNCH];
+    long goff[BNCH];
+#pragma unroll
+    for (int q = 0; q < BNCH; ++q) {
+      const int c = tid + q * 256;
+      goff[q] = (long)(c / CPR) * Kp + (c % CPR) * 8;
+    }
+#define CCGPU_BF3B_LOAD(k0)                                                      \
+  _Pragma("unroll") for (int q = 0; q < BNCH; ++q) {                             \
+    rah[q] = *reinterpret_cast<const u32x4*>(pAh + goff[q] + (k0));              \
+    ral[q] = *reinterpret_cast<const u32x4*>(pAl + goff[q] + (k0));              \
+    rbh[q] = *reinterpret_cast<const u32x4*>(pBh + goff[q] + (k0));              \
+    rbl[q] = *reinterpret_cast<const u32x4*>(pBl + goff[q] + (k0));              \
+  }
+#define CCGPU_BF3B_STORE()                                                       \
+  _Pragma("unroll") for (int q = 0; q < BNCH; ++q) {                             \
+    *reinterpret_cast<u32x4*>(&sA[0][0][0] + loff[q]) = rah[q];                  \
+    *reinterpret_cast<u32x4*>(&sA[1][0][0] + loff[q]) = ral[q];                  \
+    *reinterpret_cast<u32x4*>(&sB[0][0][0] + loff[q]) = rbh[q];                  \
+    *reinterpret_cast<u32x4*>(&sB[1][0][0] + loff[q]) = rbl[q];                  \
+  }
+    CCGPU_BF3B_LOAD(0)
+    for (long k0 = 0; k0 < Kp; k0 += BTK) {
+      __syncthreads();
+      CCGPU_BF3B_STORE()
+      __syncthreads();
+      if (k0 + BTK < Kp) {
+        CCGPU_BF3B_LOAD(k0 + BTK)
+      }
+      bf16x8 bh[8], bl[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        bh[u] = *reinterpret_cast<const bf16x8*>(&sB[0][wn * 128 + u * 16 + lr][lk]);
+        bl[u] = *reinterpret_cast<const bf16x8*>(&sB[1][wn * 128 + u * 16 + lr][lk]);
+      }
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        const bf16x8 ah = *reinterpret_cast<const bf16x8*>(&sA[0][wm * 128 + i * 16 + lr][lk]);
+        const bf16x8 al = *reinterpret_cast<const bf16x8*>(&sA[1][wm * 128 + i * 16 + lr][lk]);
+        // the three split products of one accumulator are 8 MFMAs apart
+#pragma unroll
+        for (int j = 0; j < 8; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[j], acc[i][j], 0, 0, 0);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[j], acc[i][j], 0, 0, 0);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[j], acc[i][j], 0, 0, 0);
+      }
+    }
+#undef CCGPU_BF3B_LOAD
+#undef CCGPU_BF3B_STORE
+    __syncthreads();
+  }
+  // D[i][j] of a 16x16 f32 MFMA tile: lane = j + 16*(i/4), element i%4.
+  OutT* C = static_cast<OutT*>(p.C) + z * p.c_b;
+#pragma unroll
+  for (int i = 0; i < 8; ++i)
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const long m = m0 + wm * 128 + i * 16 + (lane >> 4) * 4 + r;
+        const long n = n0 + wn * 128 + j * 16 + lr;
+        if (m < p.M && n < p.N) {
+          OutT* q = C + m * p.c_m + n * p.c_n;
+          double v = p.alpha * (double)acc[i][j][r];
+          if (p.beta != 0.0) v += p.beta * (double)*q;
+          *q = static_cast<OutT>(v);
+        }
+      }
+}
+
+// Block tile for one launch.  128 unless CCGPU_BF16X3_TILE=256: measured at
+// o=64 v=512 the 256-tile kernel (one block per CU) runs the residual GEMMs
+// at 269 TFLOP/s against 319 for the 128-tile kernel (two blocks per CU),
+// its halved operand traffic not making up for the exposed load latency.
+int pick_tile(long Mp, long Np, int batch) {
+  (void)batch;
+  static const int forced = [] {
+    const char* e = std::getenv("CCGPU_BF16X3_TILE");
+    return e ? std::atoi(e) : 0;
+  }();
+  return (forced == 256 && Mp % BT == 0 && Np % BT == 0) ? 256 : 128;
+}
+
 template <typename OutT>
 void launch_multi(const BMulti& p, int batch, hipStream_t s) {
-  const long ntiles = (p.Mp / TM) * (p.Np / TN);
-  CCGPU_REQUIRE(ntiles < (1L << 31) && batch < 65536, "bf16x3 gemm: grid too large");
-  CCGPU_LAUNCH(k_gemm_bf16x3<OutT>, dim3((unsigned)ntiles, 1, (unsigned)batch), dim3(256), 0, s, p);
+  if (pick_tile(p.Mp, p.Np, batch) == 256) {
+    const long ntiles = (p.Mp / BT) * (p.Np / BT);
+    CCGPU_REQUIRE(ntiles < (1L << 31) && batch < 65536, "bf16x3 gemm: grid too large");
+    CCGPU_LAUNCH(k_gemm_bf16x3_256<OutT>, dim3((unsigned)ntiles, 1, (unsigned)batch), dim3(256), 0, s, p);
+  } else {
+    const long ntiles = (p.Mp / TM) * (p.Np / TN);
+    CCGPU_REQUIRE(ntiles < (1L << 31) && batch < 65536, "bf16x3 gemm: grid too large");
+    CCGPU_LAUNCH(k_gemm_bf16x3<OutT>, dim3((unsigned)ntiles, 1, (unsigned)batch), dim3(256), 0, s, p);
+  }
   CCGPU_HIPX(hipGetLastError());
 }
 
 }  // namespace
 
 long bf16x3_kpad(long K) { return std::max<long>(TK, (K + TK - 1) / TK * TK); }
-long bf16x3_rpad(long R) { return (R + TM - 1) / TM * TM; }
+long bf16x3_rpad(long R) { return (R + BT - 1) / BT * BT; }
 
 void bf16x3_pack(const double* src, long rows, long K, long s_r, long s_k, long s_b, int batch, long Rp, long Kp,
                  float scale, unsigned short* hi, unsigned short* lo, hipStream_t s, long s_ko, long Kin) {

@@ -165,9 +165,9 @@ void CCSD::pt_prepare(GemmPrec mode) {
       perm(Gp_.p, G_.p, 1, (long)o * v, v, v, 0, vv, 1, v, 1.0, 0.0, st_);
     } else {
       // packed transposes GT[k][(y,z)][d] = G[k][d][y][z] and GpT = the same of Gp,
-      // built one k at a time; 128 zero rows at the end absorb tile over-reads
+      // built one k at a time; 256 zero rows at the end absorb tile over-reads
       const long Kpv = bf16x3_kpad(v);
-      const size_t ng = ((size_t)o * vv + 128) * Kpv;
+      const size_t ng = ((size_t)o * vv + 256) * Kpv;
       GTh_.alloc(ng);
       GTl_.alloc(ng);
       GpTh_.alloc(ng);
@@ -177,14 +177,14 @@ void CCSD::pt_prepare(GemmPrec mode) {
       for (int k = 0; k < o; ++k) {
         g(vv, v, naux, 1.0, Bvv0.p, 1, vv, Bov_.p + (long)k * v, nov, 1, 0.0, Gk.p, v, 1, st_);
         perm(Gpk.p, Gk.p, 1, v, v, v, 0, vv, 1, v, 1.0, 0.0, st_);
-        const long rp = (k == o - 1) ? vv + 128 : vv;
+        const long rp = (k == o - 1) ? vv + 256 : vv;
         const size_t off = (size_t)k * vv * Kpv;
         // element (r=(y,z), d) of the operand at Gk[d*vv + r]
         bf16x3_pack(Gk.p, vv, v, 1, vv, 0, 1, rp, Kpv, 1.0f, GTh_.p + off, GTl_.p + off, st_);
         bf16x3_pack(Gpk.p, vv, v, 1, vv, 0, 1, rp, Kpv, 1.0f, GpTh_.p + off, GpTl_.p + off, st_);
       }
       // per-triple slices: 6 T(p,q) [x][d], 6 Q_rq [z][l], 3 Tvvo slices [rows][l]
-      const long Rv = bf16x3_rpad(v), Kpo = bf16x3_kpad(o), Rvv = bf16x3_rpad(vv) + 128;
+      const long Rv = bf16x3_rpad(v), Kpo = bf16x3_kpad(o), Rvv = bf16x3_rpad(vv) + 256;
       Psm_.alloc(2 * (6 * (size_t)Rv * Kpv + 6 * (size_t)Rv * Kpo + 3 * (size_t)Rvv * Kpo));
       CCGPU_HIPX(hipStreamSynchronize(st_));
     }
@@ -216,7 +216,7 @@ void CCSD::pt_triples_mixed(long t, long, long) {
   const int o = dm_.o, v = dm_.v;
   const long nov = dm_.nov, vv = (long)v * v;
   const int i = tri_i_[t], j = tri_j_[t], k = tri_k_[t];
-  const long Kpv = bf16x3_kpad(v), Kpo = bf16x3_kpad(o), Rv = bf16x3_rpad(v), Rvv = bf16x3_rpad(vv) + 128;
+  const long Kpv = bf16x3_kpad(v), Kpo = bf16x3_kpad(o), Rv = bf16x3_rpad(v), Rvv = bf16x3_rpad(vv) + 256;
   const size_t sT = (size_t)Rv * Kpv, sQ = (size_t)Rv * Kpo, sP = (size_t)Rvv * Kpo;
   unsigned short* base = Psm_.p;
   const size_t half = 6 * sT + 6 * sQ + 3 * sP;  // hi block, then lo block

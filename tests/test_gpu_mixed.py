@@ -27,6 +27,75 @@ def test_dgemm_mixed_matches_numpy(ta, tb, m, n, k):
     assert np.abs(got - ref).max() < 1e-4 * np.sqrt(k) + 1e-12
 
 
+@pytest.mark.parametrize("ta,tb", [("N", "N"), ("T", "N")])
+def test_dgemm_mixed_large_tile_shape(ta, tb):
+    """4096 x 4096 output, K = 600 (a launch of 1024 tiles)."""
+    import dalton_amd as d
+
+    m = n = 4096
+    k = 600
+    print(f"dgemm_mixed large {ta}{tb} m={m} n={n} k={k}", flush=True)
+    rng = np.random.default_rng(77)
+    A = rng.standard_normal((k, m) if ta == "T" else (m, k))
+    B = rng.standard_normal((n, k) if tb == "T" else (k, n))
+    C0 = rng.standard_normal((m, n))
+    ref = 0.75 * (A.T if ta == "T" else A) @ (B.T if tb == "T" else B) + 2.0 * C0
+    got = d.dgemm(ta, tb, 0.75, A, B, 2.0, C0, precision=d.MIXED)
+    assert np.abs(got - ref).max() < 1e-4 * np.sqrt(k)
+
+
+FORCED_TILE_SCRIPT = r"""
+import sys
+import numpy as np
+sys.path.insert(0, {root!r}); sys.path.insert(0, {tests!r})
+import dalton_amd as d
+from conftest import load_molecule
+from oracle import ccsd as occ
+worst = 0.0
+for (ta, tb, m, n, k) in [("N", "N", 130, 257, 33), ("T", "T", 129, 300, 1029), ("N", "T", 1, 513, 64)]:
+    rng = np.random.default_rng(m + n + k)
+    A = rng.standard_normal((k, m) if ta == "T" else (m, k))
+    B = rng.standard_normal((n, k) if tb == "T" else (k, n))
+    C0 = rng.standard_normal((m, n))
+    ref = 1.5 * (A.T if ta == "T" else A) @ (B.T if tb == "T" else B) - 0.5 * C0
+    got = d.dgemm(ta, tb, 1.5, A, B, -0.5, C0, precision=d.MIXED)
+    err = np.abs(got - ref).max() / np.sqrt(k)
+    print("gemm", ta, tb, m, n, k, err, flush=True)
+    assert err < 1e-4
+eps, B, o, _ = load_molecule("h2o_ccpvdz_test")
+v = eps.size - o
+rng = np.random.default_rng(4)
+t1 = 0.02 * rng.standard_normal((v, o))
+t2 = 0.02 * rng.standard_normal((v, o, v, o)); t2 = t2 + t2.transpose(2, 3, 0, 1)
+r1, r2 = occ.residual(occ.DFSystem(eps, B, o), t1, t2)
+with d.CCContext(eps, B, o) as c:
+    m1, m2 = c.rhs(occ.pack_t1(t1), occ.pack_t2(t2), precision=d.MIXED)
+    e4, e5 = c.pt(d.MIXED)
+s = np.abs(r2).max()
+print("rhs", np.abs(m2 - occ.pack_t2(r2)).max() / s, flush=True)
+assert np.abs(m2 - occ.pack_t2(r2)).max() < 1e-4 * s and np.abs(m1 - occ.pack_t1(r1)).max() < 1e-4 * s
+print("ok", e4, e5, flush=True)
+"""
+
+
+def test_mixed_path_with_forced_large_tile():
+    """Every mixed GEMM forced onto the 256x256-tile kernel (odd shapes, the
+    residual and (T) at H2O): same results as the numpy / oracle references."""
+    import os
+    import subprocess
+    import sys
+
+    from conftest import ROOT
+
+    print("mixed path, 256 tiles forced (subprocess)", flush=True)
+    env = dict(os.environ, CCGPU_BF16X3_TILE="256")
+    r = subprocess.run([sys.executable, "-u", "-c",
+                        FORCED_TILE_SCRIPT.format(root=ROOT, tests=os.path.join(ROOT, "tests"))],
+                       env=env, capture_output=True, text=True, timeout=240)
+    print(r.stdout[-2000:], r.stderr[-2000:], flush=True)
+    assert r.returncode == 0 and "ok" in r.stdout
+
+
 def test_mixed_residual_close_to_fp64():
     import dalton_amd as d
 
