@@ -29,6 +29,12 @@ sys.path.insert(0, ROOT)
 
 PEAK = {"fp64": 78.6, "fp32": 157.3, "bf16": 2500.0}  # MI355X dense matrix TFLOP/s
 
+# HBM-side bytes per launch of the dominant kernels at o=64 v=512 naux=2048,
+# from two rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE) scaled by the same
+# counters read on a copy of known size; the summaries and commands are in
+# profiles/gemm_bf16x3_o64_v512_traffic.md and profiles/dgemm_o64_v512_traffic.md.
+TRAFFIC_BYTES = {"k_gemm_bf16x3": 99.3e9, "k_dgemm<0,1,false,128>": 1.4979e12}
+
 
 def f_ccsd(o: int, v: int, naux: int) -> float:
     """Algorithmic flops of one closed-shell DF-CCSD iteration (multiply-add = 2):
@@ -133,6 +139,11 @@ def lhtr_leg(nvec, log, kern_table, o=32, v=256, naux=1024, seed=4):
     return res
 
 
+def kern_table_of(prof):
+    return {k: {"count": v_["count"], "ms": round(v_["ms"], 3),
+                "tflops": round(v_["flops"] / max(v_["ms"], 1e-9) / 1e9, 2)} for k, v_ in prof["kernels"].items()}
+
+
 def _log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -191,16 +202,22 @@ def batch_workload(nsys=32, seed=5):
            "workload": f"batch_shapes({nsys}, seed {seed}): " + " ".join(f"{o}/{v}/{n}" for o, v, n in shapes),
            "flops_per_round_all_systems": sum(f_ccsd(o, v, n) for o, v, n in shapes)}
     with d.CCBatch(systems, ladder_budget_bytes=5e8) as b:
+        sync()
+        t0 = time.perf_counter()
+        b.tune()
+        out["tune_seconds"] = round(time.perf_counter() - t0, 3)
+        _log(f"batch autotuning pass: {out['tune_seconds']:.2f} s")
         runs = {}
         for label, graph in (("graph", True), ("eager", False)):
             sync()
             t0 = time.perf_counter()
-            r = b.energy(maxit=100, threnr=1e-8, graph=graph, tune=(label == "graph"))
+            r = b.energy(maxit=100, threnr=1e-8, graph=graph, tune=False)
             sync()
             dt = time.perf_counter() - t0
             runs[label] = r
             out[label] = {"seconds": round(dt, 3), "rounds": int(r["niter"].max()), **b.stats()}
             _log(f"batch {label}: {dt:.2f} s, rounds {int(r['niter'].max())}")
+        out["speedup_graph_vs_eager"] = round(out["eager"]["seconds"] / out["graph"]["seconds"], 3)
     out["niter"] = [int(x) for x in runs["graph"]["niter"]]
     out["e_ccsd"] = [float(x) for x in runs["graph"]["e_ccsd"]]
     out["max_abs_dE_graph_vs_eager"] = float(np.abs(runs["graph"]["e_ccsd"] - runs["eager"]["e_ccsd"]).max())
@@ -292,6 +309,11 @@ def main():
         dt = time.perf_counter() - t0
         return dt / nsteps, hist, profile_stop()
 
+    if args.steps < 1:  # profiling runs of the FP64 leg alone
+        spi_d, hist_d, prof_d = timed_steps(d.FP64, max(1, args.fp64_steps), "fp64")
+        print(json.dumps({"fp64_s_per_iter": spi_d, "kernels": kern_table_of(prof_d)}), flush=True)
+        ctx.close()
+        return
     spi_m, hist_m, prof_m = timed_steps(d.MIXED, args.steps, "mixed")
     fp64_leg = args.fp64_steps > 0
     if fp64_leg:
@@ -306,8 +328,10 @@ def main():
         dom = max(kern, key=lambda k: kern[k]["ms"])
         kd = kern[dom]
         ach = kd["flops"] / (kd["ms"] * 1e-3) / 1e12
+        tb = TRAFFIC_BYTES.get(dom) if (o, v, naux) == (64, 512, 2048) else None
         return {"bound": "mfma", "achieved": round(ach, 3), "peak": peak, "unit": "TFLOP/s",
-                "frac": round(ach / peak, 4), "traffic": None, "kernel": dom, "launches": kd["count"],
+                "frac": round(ach / peak, 4), "traffic": tb, "traffic_unit": "bytes per launch",
+                "kernel": dom, "launches": kd["count"],
                 "avg_launch_ms": round(kd["ms"] / kd["count"], 4),
                 "share_of_step": round(kd["ms"] * 1e-3 / dt_total, 4), "peak_note": peak_note}
 

@@ -186,6 +186,7 @@ def dgemm(transa: str, transb: str, alpha: float, A: np.ndarray, B: np.ndarray, 
 
 BATCH_EAGER = 1
 BATCH_NO_TUNE = 2
+BATCH_TUNE_ONLY = 4
 
 
 class CCBatch:
@@ -226,6 +227,12 @@ class CCBatch:
         flags = (0 if graph else BATCH_EAGER) | (0 if tune else BATCH_NO_TUNE)
         check(lib().ccgpu_batch_energy(self._h, maxit, threnr, thrvec, precision, flags, ptr(e_mp2), ptr(e), it))
         return {"e_mp2": e_mp2, "e_ccsd": e, "niter": np.array(list(it))}
+
+    def tune(self):
+        """Autotuning pass alone (GEMM launch configurations per shape)."""
+        z = np.zeros(self.n)
+        it = (C.c_int * self.n)()
+        check(lib().ccgpu_batch_energy(self._h, 1, 0.0, 0.0, FP64, BATCH_TUNE_ONLY, ptr(z), ptr(z), it))
 
     def amplitudes(self, k: int):
         o, v, _ = self.shapes[k]

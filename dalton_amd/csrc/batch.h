@@ -23,6 +23,9 @@ class Batch {
   void solve(int maxit, double threnr, double thrvec, GemmPrec prec, bool use_graph, bool tune, double* e_mp2,
              double* e_ccsd, int* niter);
 
+  // Autotuning pass alone (the one solve() runs first when tune is set).
+  void tune();
+
   int size() const { return (int)sys_.size(); }
   CCSD& system(int k) { return *sys_[k]; }
   int captures() const { return captures_; }

@@ -82,22 +82,31 @@ void Batch::capture(const std::vector<int>& active) {
   ++captures_;
 }
 
+// Autotuning pass: one throwaway FP64 residual per system at its MP2
+// amplitudes, every new GEMM shape timed in each launch configuration.
+void Batch::tune() {
+  autotune_enable(true);
+  try {
+    for (auto& s : sys_) {
+      s->set_precision(GemmPrec::F64);
+      s->mp2_start();
+      s->residual_current();
+    }
+  } catch (...) {
+    autotune_enable(false);
+    throw;
+  }
+  autotune_enable(false);
+  CCGPU_HIPX(hipDeviceSynchronize());
+}
+
 void Batch::solve(int maxit, double threnr, double thrvec, GemmPrec prec, bool use_graph, bool tune,
                   double* e_mp2, double* e_ccsd, int* niter) {
   const int n = (int)sys_.size();
   CCGPU_REQUIRE(maxit > 0, "ccgpu_batch_energy: maxit must be positive");
   captures_ = 0;
   replays_ = 0;
-  // tuning pass (throwaway residuals), then the real MP2 start
-  if (tune) {
-    autotune_enable(true);
-    for (auto& s : sys_) {
-      s->set_precision(GemmPrec::F64);
-      s->mp2_start();
-      s->residual_current();
-    }
-    autotune_enable(false);
-  }
+  if (tune) this->tune();
   std::vector<double> en(n), emp2(n);
   std::vector<int> it(n, 0), nfp64(n, 0);
   std::vector<char> fp64(n, prec == GemmPrec::F64), done(n, 0);

@@ -160,6 +160,10 @@ int ccgpu_batch_energy(ccgpu_batch* batch, int maxit, double threnr, double thrv
                        double* e_mp2, double* e_ccsd, int* niter) {
   return guarded([&] {
     CCGPU_REQUIRE(batch, "ccgpu_batch_energy: null batch");
+    if (flags & CCGPU_BATCH_TUNE_ONLY) {
+      batch->b->tune();
+      return;
+    }
     const bool graph = !(flags & CCGPU_BATCH_EAGER), tune = !(flags & CCGPU_BATCH_NO_TUNE);
     batch->b->solve(maxit, threnr, thrvec, prec_of(precision), graph, tune, e_mp2, e_ccsd, niter);
   });

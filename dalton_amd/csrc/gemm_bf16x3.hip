@@ -332,13 +332,17 @@ __global__ __launch_bounds__(256, 1) void k_gemm_bf16x3_256(BMulti p) {
 // o=64 v=512 the 256-tile kernel (one block per CU) runs the residual GEMMs
 // at 269 TFLOP/s against 319 for the 128-tile kernel (two blocks per CU),
 // its halved operand traffic not making up for the exposed load latency.
-int pick_tile(long Mp, long Np, int batch) {
-  (void)batch;
+int forced_tile() {
   static const int forced = [] {
     const char* e = std::getenv("CCGPU_BF16X3_TILE");
     return e ? std::atoi(e) : 0;
   }();
-  return (forced == 256 && Mp % BT == 0 && Np % BT == 0) ? 256 : 128;
+  return forced;
+}
+
+int pick_tile(long Mp, long Np, int batch) {
+  (void)batch;
+  return (forced_tile() == 256 && Mp % BT == 0 && Np % BT == 0) ? 256 : 128;
 }
 
 template <typename OutT>
@@ -358,7 +362,10 @@ void launch_multi(const BMulti& p, int batch, hipStream_t s) {
 }  // namespace
 
 long bf16x3_kpad(long K) { return std::max<long>(TK, (K + TK - 1) / TK * TK); }
-long bf16x3_rpad(long R) { return (R + BT - 1) / BT * BT; }
+long bf16x3_rpad(long R) {
+  const long t = forced_tile() == 256 ? BT : TM;  // rows padded to the block tile in use
+  return (R + t - 1) / t * t;
+}
 
 void bf16x3_pack(const double* src, long rows, long K, long s_r, long s_k, long s_b, int batch, long Rp, long Kp,
                  float scale, unsigned short* hi, unsigned short* lo, hipStream_t s, long s_ko, long Kin) {

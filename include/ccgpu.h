@@ -96,10 +96,13 @@ int ccgpu_ccsd_energy(ccgpu_ctx* ctx, int maxit, double threnr, double thrvec, i
  * one HIP graph replay, systems running as concurrent graph branches, after
  * an autotuning pass over the FP64 GEMM shapes.  e_mp2/e_ccsd/niter receive
  * nsys values.  flags: CCGPU_BATCH_EAGER issues the same work without a
- * graph, CCGPU_BATCH_NO_TUNE skips the autotuning pass. */
+ * graph, CCGPU_BATCH_NO_TUNE skips the autotuning pass, CCGPU_BATCH_TUNE_ONLY
+ * runs only the autotuning pass (outputs untouched; a later call with
+ * CCGPU_BATCH_NO_TUNE then uses its choices). */
 typedef struct ccgpu_batch ccgpu_batch;
 #define CCGPU_BATCH_EAGER 1
 #define CCGPU_BATCH_NO_TUNE 2
+#define CCGPU_BATCH_TUNE_ONLY 4
 int ccgpu_batch_init(int nsys, const int* nocc, const int* nvir, const int* naux, const double* const* eps,
                      const double* const* B, int mxdiis, double ladder_budget_bytes, ccgpu_batch** batch);
 int ccgpu_batch_energy(ccgpu_batch* batch, int maxit, double threnr, double thrvec, int precision, int flags,
