@@ -35,6 +35,10 @@ PEAK = {"fp64": 78.6, "fp32": 157.3, "bf16": 2500.0}  # MI355X dense matrix TFLO
 # profiles/gemm_bf16x3_o64_v512_traffic.md and profiles/dgemm_o64_v512_traffic.md.
 TRAFFIC_BYTES = {"k_gemm_bf16x3": 99.3e9, "k_dgemm<0,1,false,128>": 1.4979e12}
 
+# Attainable dense rates of the vendor BLAS on this card (torch.matmul 8192^3,
+# profiles/vendor_blas_attainable_rates.md): bf16 1086, fp64 71.5 TFLOP/s.
+ATTAINABLE = {"bf16x3": 1086.2 / 3, "fp64": 71.5}
+
 
 def f_ccsd(o: int, v: int, naux: int) -> float:
     """Algorithmic flops of one closed-shell DF-CCSD iteration (multiply-add = 2):
@@ -242,6 +246,28 @@ def copy_workload(nbytes=2 << 30, reps=3):
             "GBps_read_plus_write": round(2 * nbytes / dt / 1e9, 1)}
 
 
+def blas_workload(n=8192, reps=5):
+    """The vendor BLAS (torch.matmul -> hipBLASLt / rocBLAS) on square
+    products, as the attainable-rate calibration for this card: bf16 and fp64
+    dense GEMM TFLOP/s."""
+    import torch
+
+    out = {"workload": f"torch.matmul {n}^3", "reps": reps}
+    for name, dt in (("bf16", torch.bfloat16), ("fp32", torch.float32), ("fp64", torch.float64)):
+        a = torch.randn(n, n, device="cuda", dtype=torch.float32).to(dt)
+        b = torch.randn(n, n, device="cuda", dtype=torch.float32).to(dt)
+        c = a @ b
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            c = a @ b
+        torch.cuda.synchronize()
+        dt_ = (time.perf_counter() - t0) / reps
+        out[name + "_tflops"] = round(2.0 * n**3 / dt_ / 1e12, 1)
+        del a, b, c
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -257,7 +283,7 @@ def main():
                     help="(T) triples timed, spread evenly over the i>=j>=k list (0: skip)")
     ap.add_argument("--lhtr-vectors", type=int, default=16,
                     help="left-transformation trial vectors timed at o=32 v=256 naux=1024 (0: skip)")
-    ap.add_argument("--workload", choices=["ccsd", "pt48", "batch", "copy"], default="ccsd",
+    ap.add_argument("--workload", choices=["ccsd", "pt48", "batch", "copy", "blas"], default="ccsd",
                     help="ccsd: the headline line (default); pt48: full (T) at o=48 v=384 in both "
                          "precisions on a convergent system; batch: 32 uneven systems converged in "
                          "one graph replay per iteration")
@@ -274,6 +300,9 @@ def main():
         return
     if args.workload == "copy":
         print(json.dumps(copy_workload()), flush=True)
+        return
+    if args.workload == "blas":
+        print(json.dumps(blas_workload()), flush=True)
         return
 
     import dalton_amd as d
@@ -323,7 +352,7 @@ def main():
         return {k: {"count": v_["count"], "ms": round(v_["ms"], 3),
                     "tflops": round(v_["flops"] / max(v_["ms"], 1e-9) / 1e9, 2)} for k, v_ in prof["kernels"].items()}
 
-    def roofline(prof, dt_total, peak, peak_note):
+    def roofline(prof, dt_total, peak, peak_note, attainable=None):
         kern = prof["kernels"]
         dom = max(kern, key=lambda k: kern[k]["ms"])
         kd = kern[dom]
@@ -333,7 +362,9 @@ def main():
                 "frac": round(ach / peak, 4), "traffic": tb, "traffic_unit": "bytes per launch",
                 "kernel": dom, "launches": kd["count"],
                 "avg_launch_ms": round(kd["ms"] / kd["count"], 4),
-                "share_of_step": round(kd["ms"] * 1e-3 / dt_total, 4), "peak_note": peak_note}
+                "share_of_step": round(kd["ms"] * 1e-3 / dt_total, 4), "peak_note": peak_note,
+                "attainable": attainable,
+                "frac_attainable": round(ach / attainable, 4) if attainable else None}
 
     F = f_ccsd(o, v, naux)
     pt = {}
@@ -385,14 +416,16 @@ def main():
         "ccsd_tflops": round(F / spi_m / 1e12, 3),
         "ccsd_flops_per_iter": F,
         "fp64": {"s_per_iter": round(spi_d, 4), "steps": args.fp64_steps, "ccsd_tflops": round(F / spi_d / 1e12, 3),
-                 "energies": hist_d, "roofline": roofline(prof_d, spi_d * args.fp64_steps, PEAK["fp64"], "FP64 matrix"),
+                 "energies": hist_d, "roofline": roofline(prof_d, spi_d * args.fp64_steps, PEAK["fp64"], "FP64 matrix",
+                                                     ATTAINABLE["fp64"]),
                  "phases_ms_per_iter": {k: round(v_ / args.fp64_steps, 2) for k, v_ in prof_d["phases"].items()},
                  "kernels": kern_table(prof_d)} if fp64_leg else None,
         "speedup_mixed_vs_fp64": round(spi_d / spi_m, 3) if fp64_leg else None,
         "energies": hist_m,
         "rel_dE_per_step_mixed_vs_fp64": [abs(a - b) / abs(b) for a, b in zip(hist_m, hist_d)] if fp64_leg else None,
         "roofline": roofline(prof_m, spi_m * args.steps, round(PEAK["bf16"] / 3, 1),
-                             "BF16 dense matrix peak / 3 (three bf16 products per FP32-equivalent product)"),
+                             "BF16 dense matrix peak / 3 (three bf16 products per FP32-equivalent product)",
+                             round(ATTAINABLE["bf16x3"], 1)),
         "phases_ms_per_iter": {k: round(v_ / args.steps, 2) for k, v_ in prof_m["phases"].items()},
         "kernels": kern_table(prof_m),
         "pt": pt or None,

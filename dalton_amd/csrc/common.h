@@ -158,6 +158,7 @@ struct BTerm {
 struct BMulti {
   BTerm t[4];
   int nterm;
+  int gm = 8;  // tile-rasterisation group (m-tiles per group)
   long M, N, Mp, Np;
   double alpha, beta;
   void* C;

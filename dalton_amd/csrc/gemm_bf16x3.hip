@@ -86,11 +86,10 @@ __global__ void k_pack_bf16x2(PackDesc p, unsigned short* __restrict__ hi, unsig
   }
 }
 
-__device__ __forceinline__ void tile_of3(long b, long tiles_m, long tiles_n, long& tm, long& tn) {
+__device__ __forceinline__ void tile_of3(long b, long tiles_m, long tiles_n, long& tm, long& tn, long GM = 8) {
   const long nt = tiles_m * tiles_n;
   const long per = nt / 8, full = per * 8;
   const long lg = (b < full) ? (b % 8) * per + b / 8 : b;
-  constexpr long GM = 8;
   const long grp = lg / (GM * tiles_n);
   const long fm = grp * GM;
   const long gsz = min(tiles_m - fm, GM);
@@ -101,24 +100,26 @@ __device__ __forceinline__ void tile_of3(long b, long tiles_m, long tiles_n, lon
 
 constexpr int TM = 128, TN = 128, TK = 64, LDK = TK + 8;
 
-constexpr int NCH = TM * TK / 8 / 256;  // 16-byte chunks per thread per array per K step
-
-// Block tile 128x128, K step 64 (two MFMA k-slices of 32): each K step
-// keeps every wave busy for 96 MFMAs, long enough to cover the latency of
-// the next step's global loads, which are issued before the MFMAs.  The
-// output tile accumulates up to four products sum_t A_t B_t (each term with
-// its own packed operands and K), so several contributions to one output
-// array cost a single read-modify-write of it.
-template <typename OutT>
-__global__ __launch_bounds__(256, 2) void k_gemm_bf16x3(BMulti p) {
-  __shared__ __attribute__((aligned(16))) unsigned short sA[2][TM][LDK];
+// Block tile (64 RW) x 128, K step 64 (two MFMA k-slices of 32), 2 RW waves
+// of 64x64 (4x4 MFMA tiles each): RW = 2 is the 128x128 kernel (two blocks
+// per CU), RW = 4 the 256x128 one (one block of eight waves per CU, a
+// quarter less operand traffic per flop).  Each K step keeps every wave busy
+// for 96 MFMAs, covering the latency of the next step's global loads, which
+// are issued before the MFMAs.  The output tile accumulates up to four
+// products sum_t A_t B_t (each term with its own packed operands and K), so
+// several contributions to one output array cost a single read-modify-write.
+template <typename OutT, int RW>
+__global__ __launch_bounds__(128 * RW, 4 / RW) void k_gemm_bf16x3(BMulti p) {
+  constexpr int TMr = 64 * RW, NT = 128 * RW;
+  constexpr int NCHA = TMr * TK / 8 / NT, NCHB = TN * TK / 8 / NT;  // 16-byte chunks per thread
+  __shared__ __attribute__((aligned(16))) unsigned short sA[2][TMr][LDK];
   __shared__ __attribute__((aligned(16))) unsigned short sB[2][TN][LDK];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
-  const long tiles_m = p.Mp / TM, tiles_n = p.Np / TN;
+  const long tiles_m = p.Mp / TMr, tiles_n = p.Np / TN;
   long tm, tn;
-  tile_of3(blockIdx.x, tiles_m, tiles_n, tm, tn);
-  const long m0 = tm * TM, n0 = tn * TN, z = blockIdx.z;
+  tile_of3(blockIdx.x, tiles_m, tiles_n, tm, tn, p.gm);
+  const long m0 = tm * TMr, n0 = tn * TN, z = blockIdx.z;
   constexpr int CPR = TK / 8;
   f4 acc[4][4];
 #pragma unroll
@@ -126,11 +127,16 @@ __global__ __launch_bounds__(256, 2) void k_gemm_bf16x3(BMulti p) {
 #pragma unroll
     for (int j = 0; j < 4; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
   const int lr = lane & 15, lk = (lane >> 4) * 8;
-  int loff[NCH];
+  int loffa[NCHA], loffb[NCHB];
 #pragma unroll
-  for (int q = 0; q < NCH; ++q) {
-    const int c = tid + q * 256;
-    loff[q] = (c / CPR) * LDK + (c % CPR) * 8;
+  for (int q = 0; q < NCHA; ++q) {
+    const int c = tid + q * NT;
+    loffa[q] = (c / CPR) * LDK + (c % CPR) * 8;
+  }
+#pragma unroll
+  for (int q = 0; q < NCHB; ++q) {
+    const int c = tid + q * NT;
+    loffb[q] = (c / CPR) * LDK + (c % CPR) * 8;
   }
   for (int t = 0; t < p.nterm; ++t) {
     const BTerm& T = p.t[t];
@@ -139,28 +145,37 @@ __global__ __launch_bounds__(256, 2) void k_gemm_bf16x3(BMulti p) {
     const unsigned short* pAl = T.Al + z * T.a_bs + m0 * Kp;
     const unsigned short* pBh = T.Bh + z * T.b_bs + n0 * Kp;
     const unsigned short* pBl = T.Bl + z * T.b_bs + n0 * Kp;
-    // thread t owns 16-byte chunks c = t + 256 q: row c / (TK/8), k-chunk c % (TK/8).
+    // thread t owns 16-byte chunks c = t + NT q: row c / (TK/8), k-chunk c % (TK/8).
     // Plain register arrays with compile-time indices (kept out of scratch).
-    u32x4 rah[NCH], ral[NCH], rbh[NCH], rbl[NCH];
-    long goff[NCH];
+    u32x4 rah[NCHA], ral[NCHA], rbh[NCHB], rbl[NCHB];
+    long goffa[NCHA], goffb[NCHB];
 #pragma unroll
-    for (int q = 0; q < NCH; ++q) {
-      const int c = tid + q * 256;
-      goff[q] = (long)(c / CPR) * Kp + (c % CPR) * 8;
+    for (int q = 0; q < NCHA; ++q) {
+      const int c = tid + q * NT;
+      goffa[q] = (long)(c / CPR) * Kp + (c % CPR) * 8;
+    }
+#pragma unroll
+    for (int q = 0; q < NCHB; ++q) {
+      const int c = tid + q * NT;
+      goffb[q] = (long)(c / CPR) * Kp + (c % CPR) * 8;
     }
 #define CCGPU_BF3_LOAD(k0)                                                       \
-  _Pragma("unroll") for (int q = 0; q < NCH; ++q) {                              \
-    rah[q] = *reinterpret_cast<const u32x4*>(pAh + goff[q] + (k0));              \
-    ral[q] = *reinterpret_cast<const u32x4*>(pAl + goff[q] + (k0));              \
-    rbh[q] = *reinterpret_cast<const u32x4*>(pBh + goff[q] + (k0));              \
-    rbl[q] = *reinterpret_cast<const u32x4*>(pBl + goff[q] + (k0));              \
+  _Pragma("unroll") for (int q = 0; q < NCHA; ++q) {                             \
+    rah[q] = *reinterpret_cast<const u32x4*>(pAh + goffa[q] + (k0));             \
+    ral[q] = *reinterpret_cast<const u32x4*>(pAl + goffa[q] + (k0));             \
+  }                                                                              \
+  _Pragma("unroll") for (int q = 0; q < NCHB; ++q) {                             \
+    rbh[q] = *reinterpret_cast<const u32x4*>(pBh + goffb[q] + (k0));             \
+    rbl[q] = *reinterpret_cast<const u32x4*>(pBl + goffb[q] + (k0));             \
   }
 #define CCGPU_BF3_STORE()                                                        \
-  _Pragma("unroll") for (int q = 0; q < NCH; ++q) {                              \
-    *reinterpret_cast<u32x4*>(&sA[0][0][0] + loff[q]) = rah[q];                  \
-    *reinterpret_cast<u32x4*>(&sA[1][0][0] + loff[q]) = ral[q];                  \
-    *reinterpret_cast<u32x4*>(&sB[0][0][0] + loff[q]) = rbh[q];                  \
-    *reinterpret_cast<u32x4*>(&sB[1][0][0] + loff[q]) = rbl[q];                  \
+  _Pragma("unroll") for (int q = 0; q < NCHA; ++q) {                             \
+    *reinterpret_cast<u32x4*>(&sA[0][0][0] + loffa[q]) = rah[q];                 \
+    *reinterpret_cast<u32x4*>(&sA[1][0][0] + loffa[q]) = ral[q];                 \
+  }                                                                              \
+  _Pragma("unroll") for (int q = 0; q < NCHB; ++q) {                             \
+    *reinterpret_cast<u32x4*>(&sB[0][0][0] + loffb[q]) = rbh[q];                 \
+    *reinterpret_cast<u32x4*>(&sB[1][0][0] + loffb[q]) = rbl[q];                 \
   }
     CCGPU_BF3_LOAD(0)
     for (long k0 = 0; k0 < Kp; k0 += TK) {
@@ -235,7 +250,7 @@ __global__ __launch_bounds__(256, 1) void k_gemm_bf16x3_256(BMulti p) {
   const int wm = wave >> 1, wn = wave & 1;
   const long tiles_m = p.Mp / BT, tiles_n = p.Np / BT;
   long tm, tn;
-  tile_of3(blockIdx.x, tiles_m, tiles_n, tm, tn);
+  tile_of3(blockIdx.x, tiles_m, tiles_n, tm, tn, p.gm);
   const long m0 = tm * BT, n0 = tn * BT, z = blockIdx.z;
   constexpr int CPR = BTK / 8;
   f4 acc[8][8];
@@ -332,29 +347,167 @@ __global__ __launch_bounds__(256, 1) void k_gemm_bf16x3_256(BMulti p) {
 // o=64 v=512 the 256-tile kernel (one block per CU) runs the residual GEMMs
 // at 269 TFLOP/s against 319 for the 128-tile kernel (two blocks per CU),
 // its halved operand traffic not making up for the exposed load latency.
-int forced_tile() {
-  static const int forced = [] {
-    const char* e = std::getenv("CCGPU_BF16X3_TILE");
-    return e ? std::atoi(e) : 0;
-  }();
-  return forced;
+// Direct-to-LDS large-tile variant: block 256x256, 4 waves of 128x128 (8x8
+// MFMA tiles, accumulators in the accumulation registers), K step 32.
+// Operands go global -> LDS with global_load_lds_dwordx4 (no staging
+// registers, no LDS store instructions) into two LDS stages, so the next
+// step's loads are in flight during the current step's 192 MFMAs per wave,
+// with one barrier per step.  LDS rows are 64 B (four 16-byte chunks); the
+// chunk of row r sits in slot c ^ ((r >> 1) & 3), which keeps the 16-byte
+// fragment reads of eight consecutive lanes on distinct bank groups.  Each
+// wave instruction fills 1 KiB = 16 rows; per stage and array a wave issues
+// four of them.  The two stages are separate LDS objects and the K loop is
+// unrolled by two, so every LDS read names its stage statically.
+constexpr int DT = 256, DTK = 32;
+__device__ __forceinline__ int swz(int r) { return (r >> 1) & 3; }
+
+typedef __attribute__((address_space(3))) void lds_void;
+typedef const __attribute__((address_space(1))) void glb_void;
+
+template <typename OutT>
+__global__ __launch_bounds__(256, 1) void k_gemm_bf16x3_dl(BMulti p) {
+  __shared__ __attribute__((aligned(16))) unsigned short s0[4][DT * DTK];
+  __shared__ __attribute__((aligned(16))) unsigned short s1[4][DT * DTK];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  const long tiles_m = p.Mp / DT, tiles_n = p.Np / DT;
+  long tm, tn;
+  tile_of3(blockIdx.x, tiles_m, tiles_n, tm, tn, p.gm);
+  const long m0 = tm * DT, n0 = tn * DT, z = blockIdx.z;
+  f4 acc[8][8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i)
+#pragma unroll
+    for (int j = 0; j < 8; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+  const int lr = lane & 15, kc = lane >> 4;
+  const int lrow = lane >> 2, lslot = lane & 3;
+  // fragment offsets (elements) of this lane within one 256 x 32 array
+  int offB[8], offA[8];
+#pragma unroll
+  for (int u = 0; u < 8; ++u) {
+    const int rb = wn * 128 + u * 16 + lr, ra = wm * 128 + u * 16 + lr;
+    offB[u] = rb * DTK + 8 * (kc ^ swz(rb));
+    offA[u] = ra * DTK + 8 * (kc ^ swz(ra));
+  }
+  for (int t = 0; t < p.nterm; ++t) {
+    const BTerm& T = p.t[t];
+    const long Kp = T.Kp;
+    const unsigned short* src[4] = {T.Ah + z * T.a_bs + m0 * Kp, T.Al + z * T.a_bs + m0 * Kp,
+                                    T.Bh + z * T.b_bs + n0 * Kp, T.Bl + z * T.b_bs + n0 * Kp};
+    long goff[4];
+#pragma unroll
+    for (int ins = 0; ins < 4; ++ins) {
+      const int row = (wave * 4 + ins) * 16 + lrow;
+      goff[ins] = (long)row * Kp + 8 * (lslot ^ swz(row));
+    }
+    const long nsteps = Kp / DTK;
+#define CCGPU_DL_ISSUE(S, k0)                                                                       \
+  _Pragma("unroll") for (int a = 0; a < 4; ++a)                                                    \
+    _Pragma("unroll") for (int ins = 0; ins < 4; ++ins)                                            \
+      __builtin_amdgcn_global_load_lds((glb_void*)(src[a] + goff[ins] + (k0)),                     \
+                                       (lds_void*)(&S[a][(wave * 4 + ins) * 16 * DTK]), 16, 0, 0);
+#define CCGPU_DL_COMPUTE(S)                                                                        \
+  {                                                                                                \
+    bf16x8 bh[8], bl[8];                                                                           \
+    _Pragma("unroll") for (int u = 0; u < 8; ++u) {                                                \
+      bh[u] = *reinterpret_cast<const bf16x8*>(&S[2][offB[u]]);                                    \
+      bl[u] = *reinterpret_cast<const bf16x8*>(&S[3][offB[u]]);                                    \
+    }                                                                                              \
+    _Pragma("unroll") for (int i = 0; i < 8; ++i) {                                                \
+      const bf16x8 ah = *reinterpret_cast<const bf16x8*>(&S[0][offA[i]]);                          \
+      const bf16x8 al = *reinterpret_cast<const bf16x8*>(&S[1][offA[i]]);                          \
+      _Pragma("unroll") for (int j = 0; j < 8; ++j)                                                \
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[j], acc[i][j], 0, 0, 0);        \
+      _Pragma("unroll") for (int j = 0; j < 8; ++j)                                                \
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[j], acc[i][j], 0, 0, 0);        \
+      _Pragma("unroll") for (int j = 0; j < 8; ++j)                                                \
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[j], acc[i][j], 0, 0, 0);        \
+    }                                                                                              \
+  }
+    CCGPU_DL_ISSUE(s0, 0)
+    for (long st = 0; st < nsteps; st += 2) {
+      // even step: stage 0 (its loads are the only ones outstanding)
+      __builtin_amdgcn_s_waitcnt(0);
+      __syncthreads();
+      if (st + 1 < nsteps) {
+        CCGPU_DL_ISSUE(s1, (st + 1) * DTK)
+      }
+      CCGPU_DL_COMPUTE(s0)
+      if (st + 1 < nsteps) {
+        // odd step: stage 1
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        if (st + 2 < nsteps) {
+          CCGPU_DL_ISSUE(s0, (st + 2) * DTK)
+        }
+        CCGPU_DL_COMPUTE(s1)
+      }
+    }
+#undef CCGPU_DL_ISSUE
+#undef CCGPU_DL_COMPUTE
+    __syncthreads();  // the next term's first loads overwrite stage 0
+  }
+  // D[i][j] of a 16x16 f32 MFMA tile: lane = j + 16*(i/4), element i%4.
+  OutT* C = static_cast<OutT*>(p.C) + z * p.c_b;
+#pragma unroll
+  for (int i = 0; i < 8; ++i)
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const long m = m0 + wm * 128 + i * 16 + (lane >> 4) * 4 + r;
+        const long n = n0 + wn * 128 + j * 16 + lr;
+        if (m < p.M && n < p.N) {
+          OutT* q = C + m * p.c_m + n * p.c_n;
+          double v = p.alpha * (double)acc[i][j][r];
+          if (p.beta != 0.0) v += p.beta * (double)*q;
+          *q = static_cast<OutT>(v);
+        }
+      }
 }
 
-int pick_tile(long Mp, long Np, int batch) {
-  (void)batch;
-  return (forced_tile() == 256 && Mp % BT == 0 && Np % BT == 0) ? 256 : 128;
+int variant() {
+  static const int v = [] {
+    const char* e = std::getenv("CCGPU_BF16X3_VARIANT");
+    return e ? std::atoi(e) : 0;
+  }();
+  return v;
+}
+
+int group_m() {
+  static const int g = [] {
+    const char* e = std::getenv("CCGPU_BF16X3_GM");
+    const int x = e ? std::atoi(e) : 8;
+    return x > 0 ? x : 8;
+  }();
+  return g;
 }
 
 template <typename OutT>
-void launch_multi(const BMulti& p, int batch, hipStream_t s) {
-  if (pick_tile(p.Mp, p.Np, batch) == 256) {
+void launch_multi(BMulti p, int batch, hipStream_t s) {
+  // Block tile: CCGPU_BF16X3_VARIANT = 0 (default) 128x128, 1 256x128,
+  // 2 256x256, 3 256x256 with direct-to-LDS double-buffered loads.  Measured at o=64 v=512, the 256x256 kernel (one block of
+  // four waves per CU) runs the residual GEMMs at 269 TFLOP/s against 319
+  // for 128x128 (two blocks per CU).
+  p.gm = group_m();
+  const int var = variant();
+  CCGPU_REQUIRE(batch < 65536, "bf16x3 gemm: grid too large");
+  if (var == 3 && p.Mp % DT == 0 && p.Np % DT == 0) {
+    const long ntiles = (p.Mp / DT) * (p.Np / DT);
+    CCGPU_REQUIRE(ntiles < (1L << 31), "bf16x3 gemm: grid too large");
+    CCGPU_LAUNCH(k_gemm_bf16x3_dl<OutT>, dim3((unsigned)ntiles, 1, (unsigned)batch), dim3(256), 0, s, p);
+  } else if (var == 2 && p.Mp % BT == 0 && p.Np % BT == 0) {
     const long ntiles = (p.Mp / BT) * (p.Np / BT);
-    CCGPU_REQUIRE(ntiles < (1L << 31) && batch < 65536, "bf16x3 gemm: grid too large");
+    CCGPU_REQUIRE(ntiles < (1L << 31), "bf16x3 gemm: grid too large");
     CCGPU_LAUNCH(k_gemm_bf16x3_256<OutT>, dim3((unsigned)ntiles, 1, (unsigned)batch), dim3(256), 0, s, p);
+  } else if (var == 1 && p.Mp % 256 == 0) {
+    const long ntiles = (p.Mp / 256) * (p.Np / TN);
+    CCGPU_REQUIRE(ntiles < (1L << 31), "bf16x3 gemm: grid too large");
+    CCGPU_LAUNCH((k_gemm_bf16x3<OutT, 4>), dim3((unsigned)ntiles, 1, (unsigned)batch), dim3(512), 0, s, p);
   } else {
     const long ntiles = (p.Mp / TM) * (p.Np / TN);
-    CCGPU_REQUIRE(ntiles < (1L << 31) && batch < 65536, "bf16x3 gemm: grid too large");
-    CCGPU_LAUNCH(k_gemm_bf16x3<OutT>, dim3((unsigned)ntiles, 1, (unsigned)batch), dim3(256), 0, s, p);
+    CCGPU_REQUIRE(ntiles < (1L << 31), "bf16x3 gemm: grid too large");
+    CCGPU_LAUNCH((k_gemm_bf16x3<OutT, 2>), dim3((unsigned)ntiles, 1, (unsigned)batch), dim3(256), 0, s, p);
   }
   CCGPU_HIPX(hipGetLastError());
 }
@@ -363,7 +516,7 @@ void launch_multi(const BMulti& p, int batch, hipStream_t s) {
 
 long bf16x3_kpad(long K) { return std::max<long>(TK, (K + TK - 1) / TK * TK); }
 long bf16x3_rpad(long R) {
-  const long t = forced_tile() == 256 ? BT : TM;  // rows padded to the block tile in use
+  const long t = variant() != 0 ? BT : TM;  // rows padded to the block tile in use
   return (R + t - 1) / t * t;
 }
 

@@ -78,17 +78,20 @@ print("ok", e4, e5, flush=True)
 """
 
 
-def test_mixed_path_with_forced_large_tile():
-    """Every mixed GEMM forced onto the 256x256-tile kernel (odd shapes, the
-    residual and (T) at H2O): same results as the numpy / oracle references."""
+@pytest.mark.parametrize("variant", ["1", "2", "3"])
+def test_mixed_path_with_forced_large_tile(variant):
+    """Every mixed GEMM forced onto the 256x128 (variant 1), 256x256 (2) or
+    256x256 direct-to-LDS (3) block-tile kernel, with a different tile-rasterisation group
+    (odd shapes, the residual and (T) at H2O): same results as the numpy /
+    oracle references."""
     import os
     import subprocess
     import sys
 
     from conftest import ROOT
 
-    print("mixed path, 256 tiles forced (subprocess)", flush=True)
-    env = dict(os.environ, CCGPU_BF16X3_TILE="256")
+    print(f"mixed path, GEMM variant {variant} forced (subprocess)", flush=True)
+    env = dict(os.environ, CCGPU_BF16X3_VARIANT=variant, CCGPU_BF16X3_GM="4")
     r = subprocess.run([sys.executable, "-u", "-c",
                         FORCED_TILE_SCRIPT.format(root=ROOT, tests=os.path.join(ROOT, "tests"))],
                        env=env, capture_output=True, text=True, timeout=240)
