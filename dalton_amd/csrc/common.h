@@ -140,10 +140,10 @@ enum class GemmPrec { F64 = 0, F32 = 1, BF16X3 = 2 };
 void gemm(const GemmDesc& d, GemmPrec prec, hipStream_t s);
 
 // FP64 GEMM autotuner.  While enabled, the first call of every new shape
-// class (M, N, K, two-level K, batch, operand orders) times each candidate
-// launch configuration (block tile 128 or 64, split-K on or off) on its
-// stream and caches the fastest; later calls of that shape use it, enabled
-// or not.  Timing runs the GEMM for real, so a tuning pass gives throwaway
+// class (M, N, K, two-level K, batch, operand orders) whose default launch
+// has fewer than 1024 128 x 128 tiles times each candidate launch
+// configuration (block tile 128 or 64, split-K on or off) on its stream and
+// caches the fastest; later calls of that shape use it, enabled or not.  Timing runs the GEMM for real, so a tuning pass gives throwaway
 // results.  Never tunes during graph capture or a dry run.
 void autotune_enable(bool on);
 long autotune_entries();

@@ -368,7 +368,7 @@ F64Cfg tune_f64(const GemmDesc& d, int am, int bm, hipStream_t s) {
     if (!c.splitk && !can_split) continue;  // identical to the split variant
     run_f64(d, c, am, bm, s);                // warm-up (also sizes scratch)
     CCGPU_HIP(hipEventRecord(e0, s));
-    for (int r = 0; r < 3; ++r) run_f64(d, c, am, bm, s);
+    for (int r = 0; r < 2; ++r) run_f64(d, c, am, bm, s);
     CCGPU_HIP(hipEventRecord(e1, s));
     CCGPU_HIP(hipEventSynchronize(e1));
     float ms = 0.f;
@@ -401,7 +401,10 @@ void gemm_f64(const GemmDesc& d, hipStream_t s) {
       found = true;
     }
   }
-  if (!found && tu.on && !dryrun() && !capturing(s)) {
+  // Launches of 1024 or more 128 x 128 tiles fill the chip several times
+  // over with the default configuration: only smaller ones are tuned.
+  const long tiles128 = (long)cdiv(d.M, 128) * cdiv(d.N, 128) * d.batch;
+  if (!found && tu.on && tiles128 < 1024 && !dryrun() && !capturing(s)) {
     cfg = tune_f64(d, am, bm, s);
     std::lock_guard<std::mutex> lk(tu.mu);
     tu.best[key] = cfg;
