@@ -314,11 +314,28 @@ void run_f64(const GemmDesc& d, const F64Cfg& cfg, int am, int bm, hipStream_t s
   long kchunk = 0;
   int nz = d.batch;
   double* ws = nullptr;
-  const long few = T == 128 ? 512 : 2048;  // below this many tiles, split K
-  if (cfg.splitk && d.batch == 1 && tiles < few && d.K >= 2048) {
-    // split K over blockIdx.z so the launch has enough workgroups
-    long want = std::min<long>((4 * few / 2 + tiles - 1) / tiles, d.K / 1024);
-    want = std::min<long>(want, (1024 * 128 / T + tiles - 1) / tiles);
+  // Resident blocks when the chip is full: 2 per CU for 128-tiles, 4 for 64.
+  const long slots = T == 128 ? 512 : 1024;
+  if (cfg.splitk && d.batch == 1 && d.K >= 2048) {
+    // Split K over blockIdx.z so the launch fills whole waves of resident
+    // blocks: under one wave, enough chunks for two waves; otherwise the
+    // factor (<= 8, K chunks >= 1024) with the best fraction
+    // tiles*nz / (slots * waves) of occupied slots, the smallest on ties.
+    long want = 1;
+    if (tiles < slots) {
+      // under one wave: aim at two waves of blocks
+      want = std::min<long>((2 * slots + tiles - 1) / tiles, d.K / 1024);
+    } else {
+      double best = (double)tiles / (double)(((tiles + slots - 1) / slots) * slots);
+      for (long f = 2; f <= 8 && d.K / f >= 1024; ++f) {
+        const long tf = tiles * f;
+        const double e = (double)tf / (double)(((tf + slots - 1) / slots) * slots);
+        if (e > best + 0.02) {
+          best = e;
+          want = f;
+        }
+      }
+    }
     if (want >= 2) {
       long chunk = (d.K + want - 1) / want;
       chunk = (chunk + 15) / 16 * 16;  // K tiles never straddle two chunks
