@@ -74,3 +74,30 @@ def test_synthetic_iterations_match_oracle():
     for (e, n), (ge, gn) in zip(hist, got):
         assert abs(e - ge) < 1e-10 * max(1.0, abs(e))
         assert abs(n - gn) < 1e-9 * max(1.0, n)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_baseline_h2o_geometry_energy_and_amplitudes(precision):
+    """H2O/cc-pVDZ at O(0,0,0), H(0,+-0.757,0.586) A (no shipped Dalton output):
+    CCSD converged to 1e-10 in norm on the GPU against the oracle's solve on
+    the same integrals -- energies, (T) and the converged amplitudes.  FP64
+    to 1e-9 Eh / 1e-8 in amplitudes; mixed to 1e-6 Eh / 1e-5."""
+    import dalton_amd as d
+    from oracle import triples
+
+    eps, B, o, _ = load_molecule("h2o_ccpvdz_base")
+    print(f"baseline H2O geometry, precision {precision}", flush=True)
+    sysm = occ.DFSystem(eps, B, o)
+    ref = occ.solve(sysm, maxiter=80, threnr=1e-12, thrvec=1e-10)
+    e4r, e5r = triples.pt_energy(eps, B, o, ref["t1"], ref["t2"])
+    with _ctx(eps, B, o) as c:
+        res = c.ccsd_energy(maxit=80, threnr=1e-12, thrvec=1e-10, precision=precision)
+        t1, t2 = c.amplitudes()
+        e4, e5 = c.pt(precision)
+    tol_e, tol_t = (1e-9, 1e-8) if precision == 0 else (1e-6, 1e-5)
+    assert abs(res["e_ccsd"] - ref["e_ccsd"]) < tol_e
+    assert abs(e4 + e5 - (e4r + e5r)) < tol_e
+    assert np.abs(t1 - occ.pack_t1(ref["t1"])).max() < tol_t
+    assert np.abs(t2 - occ.pack_t2(ref["t2"])).max() < tol_t
+    if precision == 0:
+        assert res["niter"] == len(ref["history"])
