@@ -127,11 +127,11 @@ class CCSD {
   double ladder_budget_ = 10.0e9;
   // (T)
   GemmPrec pt_mode_ = GemmPrec::F64;
-  DBuf<double> G_, Gp_, Tvvo_, Q_, Wt_, ptpart_, etrip_;
+  DBuf<double> G_, Gp_, Tvvo_, Q_, Wt_, Wr_, ptpart_, etrip_;
   // (T), mixed path: (yd|zk) packed as bf16 hi/lo [k][(y,z)][d] in both (y,z)
   // orders, per-triple packed slices, FP32 W
   DBuf<unsigned short> GTh_, GTl_, GpTh_, GpTl_, Psm_;
-  DBuf<float> W32_;
+  DBuf<float> W32_, W32r_;  // W and its two re-ordered copies (energy reads)
   // left transformation: cotangents (suffix b) of the residual's intermediates
   bool lhtr_ready_ = false;
   GemmPrec lhtr_prec_ = GemmPrec::F64;

@@ -32,10 +32,50 @@ __device__ __forceinline__ void tri_dec(long p, long& lo, long& hi) {
   lo = p - h * (h + 1) / 2;
 }
 
+// Batched 2D transpose through LDS: out[b*osb + j*osj + i*osi] =
+// in[b*isb + i*isi + j*isj], i < ni, j < nj; reads run along j (isj = 1),
+// writes along i (osi = 1).  Builds the two re-ordered copies of W that let
+// the energy kernel read every permutation slot along the same index.
+template <typename WT>
+__global__ void k_transpose_batched(WT* __restrict__ out, const WT* __restrict__ in, long ni, long nj, long isb,
+                                    long isi, long osb, long osj) {
+  __shared__ WT tile[32][33];
+  const long b = blockIdx.z, i0 = (long)blockIdx.y * 32, j0 = (long)blockIdx.x * 32;
+  const int tx = threadIdx.x, ty = threadIdx.y;
+#pragma unroll
+  for (int k = 0; k < 32; k += 8) {
+    const long i = i0 + ty + k, j = j0 + tx;
+    if (i < ni && j < nj) tile[ty + k][tx] = in[b * isb + i * isi + j];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < 32; k += 8) {
+    const long j = j0 + ty + k, i = i0 + tx;
+    if (i < ni && j < nj) out[b * osb + j * osj + i] = tile[tx][ty + k];
+  }
+}
+
+template <typename WT>
+void w_reorder(WT* W1, WT* W2, const WT* W, int v, hipStream_t st) {
+  const long vv = (long)v * v, vvv = vv * v;
+  check_span(W, 0, vvv - 1, sizeof(WT), "pt W");
+  check_span(W1, 0, vvv - 1, sizeof(WT), "pt W1");
+  check_span(W2, 0, vvv - 1, sizeof(WT), "pt W2");
+  const dim3 grid(cdiv(v, 32), cdiv(v, 32), (unsigned)v);
+  // W2[x][z][y] = W[x][y][z]: batch x, i = y, j = z
+  CCGPU_LAUNCH(k_transpose_batched<WT>, grid, dim3(32, 8), 0, st, W2, W, (long)v, (long)v, vv, (long)v, vv,
+               (long)v);
+  // W1[z][y][x] = W[x][y][z]: batch y, i = x, j = z
+  CCGPU_LAUNCH(k_transpose_batched<WT>, grid, dim3(32, 8), 0, st, W1, W, (long)v, (long)v, (long)v, vv,
+               (long)v, vv);
+  CCGPU_HIPX(hipGetLastError());
+}
+
 // One block per virtual pair a <= b, threads over c in [b, v).
 // Writes (E4, E5) block partials, already scaled by mult/3.
 template <typename WT>
-__global__ void k_pt_energy(const WT* __restrict__ W, const double* __restrict__ J,
+__global__ void k_pt_energy(const WT* __restrict__ W, const WT* __restrict__ W1, const WT* __restrict__ W2,
+                            const double* __restrict__ J,
                             const double* __restrict__ t1, const double* __restrict__ eps, int i, int j,
                             int k, int o, int v, double scale, double* part) {
   __shared__ double sh[2][8];
@@ -49,10 +89,16 @@ __global__ void k_pt_energy(const WT* __restrict__ W, const double* __restrict__
     // slots: 0 abc, 1 acb, 2 bac, 3 bca, 4 cab, 5 cba; even class {0,3,4}, odd {1,2,5}
     const int X[6] = {a, a, b, b, c, c}, Y[6] = {b, c, a, c, a, b}, Z[6] = {c, b, c, a, b, a};
     double w[6], y[6];
+    // every slot read runs along c: W1[z][y][x] = W[x][y][z], W2[x][z][y] = W[x][y][z]
+    w[0] = (double)W[(long)a * vv + (long)b * v + c];
+    w[1] = (double)W2[(long)a * vv + (long)b * v + c];
+    w[2] = (double)W[(long)b * vv + (long)a * v + c];
+    w[3] = (double)W2[(long)b * vv + (long)a * v + c];
+    w[4] = (double)W1[(long)b * vv + (long)a * v + c];
+    w[5] = (double)W1[(long)a * vv + (long)b * v + c];
 #pragma unroll
     for (int s = 0; s < 6; ++s) {
       const int x = X[s], yy = Y[s], z = Z[s];
-      w[s] = (double)W[(long)x * vv + (long)yy * v + z];
       y[s] = t1[(long)i * v + x] * J[((long)j * v + yy) * nov + (long)k * v + z] +
              t1[(long)j * v + yy] * J[((long)i * v + x) * nov + (long)k * v + z] +
              t1[(long)k * v + z] * J[((long)i * v + x) * nov + (long)j * v + yy];
@@ -132,9 +178,10 @@ long CCSD::pt_ntriples() const {
 }
 
 void CCSD::pt_release() {
-  for (DBuf<double>* b : {&G_, &Gp_, &Tvvo_, &Q_, &Wt_, &ptpart_, &etrip_}) b->release();
+  for (DBuf<double>* b : {&G_, &Gp_, &Tvvo_, &Q_, &Wt_, &Wr_, &ptpart_, &etrip_}) b->release();
   for (DBuf<unsigned short>* b : {&GTh_, &GTl_, &GpTh_, &GpTl_, &Psm_}) b->release();
   W32_.release();
+  W32r_.release();
   pt_ready_ = false;
 }
 
@@ -159,6 +206,7 @@ void CCSD::pt_prepare(GemmPrec mode) {
       G_.alloc((size_t)o * vvv);
       Gp_.alloc((size_t)o * vvv);
       Wt_.alloc(vvv);
+      Wr_.alloc(2 * (size_t)vvv);
       // G[k][d][y][z] = (yd|zk) = sum_P B[P][d][y] B[P][k][z], batched over k
       g(vv, v, naux, 1.0, Bvv0.p, 1, vv, Bov_.p, nov, 1, 0.0, G_.p, v, 1, st_, o, 0, v, vvv);
       // Gp[k][d][y][z] = G[k][d][z][y]
@@ -173,6 +221,7 @@ void CCSD::pt_prepare(GemmPrec mode) {
       GpTh_.alloc(ng);
       GpTl_.alloc(ng);
       W32_.alloc(vvv);
+      W32r_.alloc(2 * (size_t)vvv);
       DBuf<double> Gk(vvv), Gpk(vvv);
       for (int k = 0; k < o; ++k) {
         g(vv, v, naux, 1.0, Bvv0.p, 1, vv, Bov_.p + (long)k * v, nov, 1, 0.0, Gk.p, v, 1, st_);
@@ -316,13 +365,17 @@ void CCSD::pt_energy(long first, long count, long stride, double* e4, double* e5
     check_span(ptpart_.p, 0, 2 * npv - 1, sizeof(double), "pt part");
     check_span(J_.p, 0, dm_.nov2 - 1, sizeof(double), "pt J");
     if (mixed) {
-      check_span(W32_.p, 0, vvv - 1, sizeof(float), "pt W32");
-      CCGPU_LAUNCH(k_pt_energy<float>, dim3((unsigned)npv), dim3(256), 0, st_, W32_.p, J_.p, t1_.p, eps_.p, i, j,
-                   k, o, v, mult / 3.0, ptpart_.p);
+      float* W1 = W32r_.p;
+      float* W2 = W32r_.p + vvv;
+      w_reorder(W1, W2, W32_.p, v, st_);
+      CCGPU_LAUNCH(k_pt_energy<float>, dim3((unsigned)npv), dim3(256), 0, st_, W32_.p, W1, W2, J_.p, t1_.p, eps_.p,
+                   i, j, k, o, v, mult / 3.0, ptpart_.p);
     } else {
-      check_span(W, 0, vvv - 1, sizeof(double), "pt W");
-      CCGPU_LAUNCH(k_pt_energy<double>, dim3((unsigned)npv), dim3(256), 0, st_, W, J_.p, t1_.p, eps_.p, i, j, k,
-                   o, v, mult / 3.0, ptpart_.p);
+      double* W1 = Wr_.p;
+      double* W2 = Wr_.p + vvv;
+      w_reorder(W1, W2, W, v, st_);
+      CCGPU_LAUNCH(k_pt_energy<double>, dim3((unsigned)npv), dim3(256), 0, st_, W, W1, W2, J_.p, t1_.p, eps_.p, i,
+                   j, k, o, v, mult / 3.0, ptpart_.p);
     }
     CCGPU_LAUNCH(k_pt_reduce, dim3(1), dim3(256), 0, st_, ptpart_.p, npv, etrip_.p + 2 * idx);
     CCGPU_HIPX(hipGetLastError());
