@@ -101,3 +101,37 @@ def test_baseline_h2o_geometry_energy_and_amplitudes(precision):
     assert np.abs(t2 - occ.pack_t2(ref["t2"])).max() < tol_t
     if precision == 0:
         assert res["niter"] == len(ref["history"])
+
+
+@pytest.mark.parametrize("o,v,naux", [(1, 1, 1), (2, 1, 3), (1, 6, 4), (4, 2, 7), (3, 5, 2)])
+def test_degenerate_shapes_match_oracle(o, v, naux):
+    """Edge shapes (a single occupied or virtual orbital, more occupied than
+    virtual, fewer auxiliary functions than orbitals): residual, left
+    transformation and (T) against the oracle."""
+    import dalton_amd as d
+    from dalton_amd.synthetic import make_system
+    from oracle import lhtr as olh
+    from oracle import triples
+
+    eps, B = make_system(o, v, naux, seed=o * 100 + v * 10 + naux, sigma=0.2)
+    print(f"edge shape o={o} v={v} naux={naux}", flush=True)
+    sysm = occ.DFSystem(eps, B, o)
+    rng = np.random.default_rng(3)
+    t1 = 0.05 * rng.standard_normal((v, o))
+    t2 = 0.05 * rng.standard_normal((v, o, v, o))
+    t2 = t2 + t2.transpose(2, 3, 0, 1)
+    p1, p2 = occ.pack_t1(t1), occ.pack_t2(t2)
+    l1, l2 = rng.standard_normal(p1.size), rng.standard_normal(p2.size)
+    r1, r2 = occ.residual(sysm, t1, t2)
+    q1, q2 = olh.left_transform(sysm, p1, p2, l1, l2)
+    e4r, e5r = triples.pt_energy(eps, B, o, t1, t2)
+    with d.CCContext(eps, B, o) as c:
+        om1, om2 = c.rhs(p1, p2)
+        c.set_amplitudes(p1, p2)
+        g1, g2 = c.lhtr(l1, l2)
+        e4, e5 = c.pt()
+    s = max(1.0, np.abs(occ.pack_t2(r2)).max())
+    assert np.abs(om1 - occ.pack_t1(r1)).max() < 1e-12 * s and np.abs(om2 - occ.pack_t2(r2)).max() < 1e-12 * s
+    s = max(1.0, np.abs(q2).max(), np.abs(q1).max())
+    assert np.abs(g1 - q1).max() < 1e-12 * s and np.abs(g2 - q2).max() < 1e-12 * s
+    assert abs(e4 - e4r) < 1e-12 * max(1.0, abs(e4r)) and abs(e5 - e5r) < 1e-12 * max(1.0, abs(e5r))

@@ -238,3 +238,24 @@ def test_autograd_function_backward_is_lhtr():
     s = max(1.0, np.abs(q2).max())
     assert np.abs(x1.grad.numpy() - q1).max() < 1e-12 * s
     assert np.abs(x2.grad.numpy() - q2).max() < 1e-12 * s
+
+
+def test_lhtr_hf_molecule_matches_oracle():
+    """HF/6-31G** (the reference's cc_energy_triples molecule) at its converged
+    ground state: the left transformation of two vectors against the oracle."""
+    import dalton_amd as d
+
+    eps, B, o = _system("hf_631gss")
+    print("lhtr HF at the converged ground state", flush=True)
+    sysm = occ.DFSystem(eps, B, o)
+    rng = np.random.default_rng(13)
+    with d.CCContext(eps, B, o) as c:
+        c.ccsd_energy(maxit=60, threnr=1e-10)
+        t1, t2 = c.amplitudes()
+        L1 = rng.standard_normal((2, c.nt1))
+        L2 = rng.standard_normal((2, c.nt2))
+        R1, R2 = c.lhtr(L1, L2)
+    for k in range(2):
+        q1, q2 = olh.left_transform(sysm, t1, t2, L1[k], L2[k])
+        s = max(1.0, np.abs(q2).max())
+        assert np.abs(R1[k] - q1).max() < 1e-12 * s and np.abs(R2[k] - q2).max() < 1e-12 * s
