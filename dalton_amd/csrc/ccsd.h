@@ -136,7 +136,11 @@ class CCSD {
   bool lhtr_ready_ = false;
   GemmPrec lhtr_prec_ = GemmPrec::F64;
   DBuf<double> Tb_, Ub_, Txb_, Db_, Btb_, Bvvb_, Boob_, BvoTb_, Zb_, Wb_, Fb_, E1b_, E2b_, Jdb_, Gamb_, G1b_,
-      taupb_, taumb_, t1b_, vecb_;
+      taupb_, taumb_, vecb_;
+  // trial-vector staging (two in, two out slots) and the copy stream
+  DBuf<double> lst_;
+  hipStream_t cst_ = nullptr;
+  hipEvent_t ev_in_[2] = {nullptr, nullptr}, ev_done_[2] = {nullptr, nullptr};
 };
 
 }  // namespace ccgpu

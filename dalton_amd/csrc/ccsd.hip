@@ -172,8 +172,8 @@ void CCSD::release_work() {
   for (DBuf<double>* b : {&Bt_, &Bvv_, &Boo_, &BvoT_, &F_, &ht_, &hx_, &G1_, &Gam_, &Z_, &W_, &E1_, &E2_,
                           &bufR_, &bufX2_, &bufTx_, &bufU_, &bufC_, &bufD_, &taup_, &taum_, &Y_, &Vp_,
                           &Vm_, &Sp_, &Sm_, &S2_, &diisS_, &diisE_, &Tb_, &Ub_, &Txb_, &Db_, &Btb_, &Bvvb_, &Boob_,
-                          &BvoTb_, &Zb_, &Wb_, &Fb_, &E1b_, &E2b_, &Jdb_, &Gamb_, &G1b_, &taupb_, &taumb_, &t1b_,
-                          &vecb_})
+                          &BvoTb_, &Zb_, &Wb_, &Fb_, &E1b_, &E2b_, &Jdb_, &Gamb_, &G1b_, &taupb_, &taumb_, &vecb_,
+                          &lst_})
     b->release();
   for (DBuf<unsigned short>* b : {&lpBh_, &lpBl_, &lpTh_, &lpTl_, &lpVh_, &lpVl_}) b->release();
   work_ = false;
@@ -183,6 +183,14 @@ void CCSD::release_work() {
 
 CCSD::~CCSD() {
   if (dryrun()) return;
+  if (cst_) {
+    (void)hipStreamSynchronize(cst_);
+    for (int q = 0; q < 2; ++q) {
+      if (ev_in_[q]) (void)hipEventDestroy(ev_in_[q]);
+      if (ev_done_[q]) (void)hipEventDestroy(ev_done_[q]);
+    }
+    (void)hipStreamDestroy(cst_);
+  }
   if (st_) {
     (void)hipStreamSynchronize(st_);
     release_scratch(st_);

@@ -44,7 +44,6 @@ void CCSD::ensure_lhtr() {
   G1b_.alloc((size_t)o * o * o * o);
   taupb_.alloc(dm_.npo * dm_.npv);
   taumb_.alloc(dm_.npo * dm_.npv);
-  t1b_.alloc(nov);
   vecb_.alloc(2 * nov);
 }
 
@@ -213,23 +212,62 @@ void CCSD::lhtr_apply(const double* l1, double* Xb, double* rho1, double* rho2s)
   mark("lhtr_fold");
 }
 
+// Vectors are pipelined: while vector k is transformed on the compute
+// stream, vector k+1 is copied in and the result of vector k-1 copied out
+// on a copy stream, through two sets of device staging buffers.
 void CCSD::lhtr_host(int nvec, const double* l1, const double* l2p, double* rho1, double* rho2p) {
   CCGPU_REQUIRE(nvec >= 0, "ccgpu_lhtr: nvec < 0");
   CCGPU_REQUIRE(iter_ >= 1, "ccgpu_lhtr: no amplitudes (run the CCSD iterations or set amplitudes first)");
   if (nvec == 0) return;
   const long nov = dm_.nov, np2 = dm_.ntamp - nov;
   lhtr_prepare();
-  for (int iv = 0; iv < nvec; ++iv) {
-    CCGPU_HIPX(hipMemcpyAsync(vecb_.p, l1 + iv * nov, nov * sizeof(double), hipMemcpyHostToDevice, st_));
-    CCGPU_HIPX(hipMemcpyAsync(bufX2_.p, l2p + iv * np2, np2 * sizeof(double), hipMemcpyHostToDevice, st_));
-    unpack_sym(Om_.p, bufX2_.p, nov, st_);
-    scale_diag(Om_.p, nov, nov, 2.0, st_);  // Xb = Lf + diag(Lf)
-    lhtr_apply(vecb_.p, Om_.p, t1b_.p, bufX2_.p);
-    pack_sym(Om_.p, bufX2_.p, nov, st_);
-    CCGPU_HIPX(hipMemcpyAsync(rho1 + iv * nov, t1b_.p, nov * sizeof(double), hipMemcpyDeviceToHost, st_));
-    CCGPU_HIPX(hipMemcpyAsync(rho2p + iv * np2, Om_.p, np2 * sizeof(double), hipMemcpyDeviceToHost, st_));
-    CCGPU_HIPX(hipStreamSynchronize(st_));
+  if (!lst_.p) {
+    lst_.alloc(4 * (size_t)(np2 + nov));  // [in slot 0, in slot 1, out slot 0, out slot 1] x (l1/rho1, l2/rho2)
+    if (!dryrun()) {
+      CCGPU_HIP(hipStreamCreateWithFlags(&cst_, hipStreamNonBlocking));
+      for (int q = 0; q < 2; ++q) {
+        CCGPU_HIP(hipEventCreateWithFlags(&ev_in_[q], hipEventDisableTiming));
+        CCGPU_HIP(hipEventCreateWithFlags(&ev_done_[q], hipEventDisableTiming));
+      }
+    }
   }
+  const long slot = np2 + nov;
+  auto in1 = [&](int q) { return lst_.p + q * slot; };
+  auto in2 = [&](int q) { return lst_.p + q * slot + nov; };
+  auto out1 = [&](int q) { return lst_.p + (2 + q) * slot; };
+  auto out2 = [&](int q) { return lst_.p + (2 + q) * slot + nov; };
+  auto copy_in = [&](int iv) {
+    const int q = iv & 1;
+    CCGPU_HIPX(hipMemcpyAsync(in1(q), l1 + iv * nov, nov * sizeof(double), hipMemcpyHostToDevice, cst_));
+    CCGPU_HIPX(hipMemcpyAsync(in2(q), l2p + iv * np2, np2 * sizeof(double), hipMemcpyHostToDevice, cst_));
+    CCGPU_HIPX(hipEventRecord(ev_in_[q], cst_));
+  };
+  auto compute = [&](int iv) {
+    const int q = iv & 1;
+    CCGPU_HIPX(hipStreamWaitEvent(st_, ev_in_[q], 0));
+    unpack_sym(Om_.p, in2(q), nov, st_);
+    scale_diag(Om_.p, nov, nov, 2.0, st_);  // Xb = Lf + diag(Lf)
+    lhtr_apply(in1(q), Om_.p, out1(q), bufX2_.p);
+    pack_sym(out2(q), bufX2_.p, nov, st_);
+    CCGPU_HIPX(hipEventRecord(ev_done_[q], st_));
+  };
+  auto copy_out = [&](int iv) {
+    const int q = iv & 1;
+    CCGPU_HIPX(hipStreamWaitEvent(cst_, ev_done_[q], 0));
+    CCGPU_HIPX(hipMemcpyAsync(rho1 + iv * nov, out1(q), nov * sizeof(double), hipMemcpyDeviceToHost, cst_));
+    CCGPU_HIPX(hipMemcpyAsync(rho2p + iv * np2, out2(q), np2 * sizeof(double), hipMemcpyDeviceToHost, cst_));
+    CCGPU_HIPX(hipStreamSynchronize(cst_));  // slot q (in and out) is free again
+  };
+  copy_in(0);
+  compute(0);
+  for (int iv = 0; iv < nvec; ++iv) {
+    if (iv + 1 < nvec) {
+      copy_in(iv + 1);  // slot of iv+1 was released by copy_out(iv-1)
+      compute(iv + 1);
+    }
+    copy_out(iv);
+  }
+  CCGPU_HIPX(hipStreamSynchronize(st_));
 }
 
 }  // namespace ccgpu
