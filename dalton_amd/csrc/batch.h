@@ -11,7 +11,7 @@ namespace ccgpu {
 class Batch {
  public:
   // ladder_budget: bytes per particle-ladder batch buffer of each system
-  // (<= 0: the CCSD default).
+  // (<= 0: 128 v^3 bytes, room for 16 a-rows, clamped to 256 MB .. 10 GB).
   Batch(int n, const int* o, const int* v, const int* naux, const double* const* eps, const double* const* B,
         int mxdiis, double ladder_budget);
   ~Batch();

@@ -25,7 +25,10 @@ Batch::Batch(int n, const int* o, const int* v, const int* naux, const double* c
   sys_.reserve(n);
   for (int k = 0; k < n; ++k) {
     sys_.emplace_back(new CCSD(o[k], v[k], naux[k], eps[k], B[k], mxdiis));
-    if (ladder_budget > 0.0) sys_.back()->set_ladder_budget(ladder_budget);
+    // default: room for 16 a-rows of the ladder tile block, 256 MB .. 10 GB
+    const double vk = v[k];
+    const double auto_budget = std::min(10.0e9, std::max(2.56e8, 128.0 * vk * vk * vk));
+    sys_.back()->set_ladder_budget(ladder_budget > 0.0 ? ladder_budget : auto_budget);
   }
   CCGPU_HIPX(hipStreamCreateWithFlags(&master_, hipStreamNonBlocking));
   report_.alloc(4 * (size_t)n);

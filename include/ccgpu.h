@@ -89,8 +89,8 @@ int ccgpu_ccsd_energy(ccgpu_ctx* ctx, int maxit, double threnr, double thrvec, i
  * counterpart: it runs the CCSD_ENERGY loop of cc/ccsd_energy.F:742-982 for
  * nsys systems at once).  System k has nocc[k], nvir[k], naux[k] and host
  * arrays eps[k], B[k] as for ccgpu_init.  ladder_budget_bytes bounds each
- * system's particle-ladder batch buffers (<= 0: the single-system default,
- * 10 GB).  ccgpu_batch_energy converges every system from its MP2 start with
+ * system's particle-ladder batch buffers (<= 0: 128 v^3 bytes per system,
+ * clamped to 256 MB .. 10 GB).  ccgpu_batch_energy converges every system from its MP2 start with
  * Dalton's test (|dE| < threnr and norm < thrvec; mixed precision switches to
  * FP64 as in ccgpu_ccsd_energy): one iteration of all still-active systems is
  * one HIP graph replay, systems running as concurrent graph branches, after
