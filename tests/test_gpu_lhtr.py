@@ -110,8 +110,9 @@ def _c4_check(sigma, label, strict):
     unit trial vectors in one call, each checked along its own random unit
     direction by central differences of lambda . Omega with h = 1e-5; the
     batched call equals one-vector calls.  The particle ladder runs in several
-    a-batches at this size.  The FD round-off floor is estimated from a second
-    difference with 2h: an adjoint error would show in both estimates alike."""
+    a-batches at this size.  The FD round-off floor is estimated from the
+    spread of the differences at h, 2h and h/2: an adjoint error would show in
+    all of them alike."""
     import dalton_amd as d
     from dalton_amd.synthetic import make_system
 
@@ -152,9 +153,9 @@ def _c4_check(sigma, label, strict):
             rel = abs(f1 - an) / scale
             if strict:
                 bound = 1e-7
-            else:
-                f2 = fd(2 * h)
-                bound = max(1e-7, 4.0 * abs(f1 - f2) / scale)
+            else:  # FD noise floor from the spread of three step sizes
+                f2, f3 = fd(2 * h), fd(0.5 * h)
+                bound = max(1e-7, 4.0 * max(abs(f1 - f2), abs(f1 - f3)) / scale)
             print(f"  vector {k}: rho.d = {an:.12e} FD = {f1:.12e} rel = {rel:.2e} bound = {bound:.1e}", flush=True)
             assert rel < bound, (k, rel, bound)
             worst = max(worst, rel)
