@@ -71,7 +71,7 @@ void w_reorder(WT* W1, WT* W2, const WT* W, int v, hipStream_t st) {
   CCGPU_HIPX(hipGetLastError());
 }
 
-// One block per virtual pair a <= b, threads over c in [b, v).
+// Blocks stride over the virtual pairs a <= b, threads over c in [b, v).
 // Writes (E4, E5) block partials, already scaled by mult/3.
 template <typename WT>
 __global__ void k_pt_energy(const WT* __restrict__ W, const WT* __restrict__ W1, const WT* __restrict__ W2,
@@ -79,46 +79,48 @@ __global__ void k_pt_energy(const WT* __restrict__ W, const WT* __restrict__ W1,
                             const double* __restrict__ t1, const double* __restrict__ eps, int i, int j,
                             int k, int o, int v, double scale, double* part) {
   __shared__ double sh[2][8];
-  long a_, b_;
-  tri_dec(blockIdx.x, a_, b_);
-  const int a = (int)a_, b = (int)b_;
-  const long nov = (long)o * v, vv = (long)v * v;
+  const long nov = (long)o * v, vv = (long)v * v, npairs = (long)v * (v + 1) / 2;
   const double eijk = eps[i] + eps[j] + eps[k];
   double s4 = 0.0, s5 = 0.0;
-  for (int c = b + (int)threadIdx.x; c < v; c += blockDim.x) {
-    // slots: 0 abc, 1 acb, 2 bac, 3 bca, 4 cab, 5 cba; even class {0,3,4}, odd {1,2,5}
-    const int X[6] = {a, a, b, b, c, c}, Y[6] = {b, c, a, c, a, b}, Z[6] = {c, b, c, a, b, a};
-    double w[6], y[6];
-    // every slot read runs along c: W1[z][y][x] = W[x][y][z], W2[x][z][y] = W[x][y][z]
-    w[0] = (double)W[(long)a * vv + (long)b * v + c];
-    w[1] = (double)W2[(long)a * vv + (long)b * v + c];
-    w[2] = (double)W[(long)b * vv + (long)a * v + c];
-    w[3] = (double)W2[(long)b * vv + (long)a * v + c];
-    w[4] = (double)W1[(long)b * vv + (long)a * v + c];
-    w[5] = (double)W1[(long)a * vv + (long)b * v + c];
-#pragma unroll
-    for (int s = 0; s < 6; ++s) {
-      const int x = X[s], yy = Y[s], z = Z[s];
-      y[s] = t1[(long)i * v + x] * J[((long)j * v + yy) * nov + (long)k * v + z] +
-             t1[(long)j * v + yy] * J[((long)i * v + x) * nov + (long)k * v + z] +
-             t1[(long)k * v + z] * J[((long)i * v + x) * nov + (long)j * v + yy];
+  for (long pr = blockIdx.x; pr < npairs; pr += gridDim.x) {
+    long a_, b_;
+    tri_dec(pr, a_, b_);
+    const int a = (int)a_, b = (int)b_;
+    for (int c = b + (int)threadIdx.x; c < v; c += blockDim.x) {
+      // slots: 0 abc, 1 acb, 2 bac, 3 bca, 4 cab, 5 cba; even class {0,3,4}, odd {1,2,5}
+      const int X[6] = {a, a, b, b, c, c}, Y[6] = {b, c, a, c, a, b}, Z[6] = {c, b, c, a, b, a};
+      double w[6], y[6];
+      // every slot read runs along c: W1[z][y][x] = W[x][y][z], W2[x][z][y] = W[x][y][z]
+      w[0] = (double)W[(long)a * vv + (long)b * v + c];
+      w[1] = (double)W2[(long)a * vv + (long)b * v + c];
+      w[2] = (double)W[(long)b * vv + (long)a * v + c];
+      w[3] = (double)W2[(long)b * vv + (long)a * v + c];
+      w[4] = (double)W1[(long)b * vv + (long)a * v + c];
+      w[5] = (double)W1[(long)a * vv + (long)b * v + c];
+  #pragma unroll
+      for (int s = 0; s < 6; ++s) {
+        const int x = X[s], yy = Y[s], z = Z[s];
+        y[s] = t1[(long)i * v + x] * J[((long)j * v + yy) * nov + (long)k * v + z] +
+               t1[(long)j * v + yy] * J[((long)i * v + x) * nov + (long)k * v + z] +
+               t1[(long)k * v + z] * J[((long)i * v + x) * nov + (long)j * v + yy];
+      }
+      const double wE = w[0] + w[3] + w[4], wO = w[1] + w[2] + w[5];
+      const double yE = y[0] + y[3] + y[4], yO = y[1] + y[2] + y[5];
+      double ww = 0.0, wy = 0.0;
+  #pragma unroll
+      for (int s = 0; s < 6; ++s) {
+        ww += w[s] * w[s];
+        wy += w[s] * y[s];
+      }
+      // sum over the 6 slots of W_p (4X_p + X_cyc1 + X_cyc2 - 2 X_swaps)
+      const double f4 = 3.0 * ww + 3.0 * (wE * wE + wO * wO) - 2.0 * (wE + wO) * (wE + wO);
+      const double f5 = 3.0 * wy + 3.0 * (wE * yE + wO * yO) - 2.0 * (wE + wO) * (yE + yO);
+      // each distinct tuple is counted once: slots alias when indices coincide
+      const double wt = (a == b && b == c) ? (1.0 / 6.0) : ((a == b || b == c) ? 0.5 : 1.0);
+      const double D = eijk - eps[o + a] - eps[o + b] - eps[o + c];
+      s4 += wt * f4 / D;
+      s5 += wt * f5 / D;
     }
-    const double wE = w[0] + w[3] + w[4], wO = w[1] + w[2] + w[5];
-    const double yE = y[0] + y[3] + y[4], yO = y[1] + y[2] + y[5];
-    double ww = 0.0, wy = 0.0;
-#pragma unroll
-    for (int s = 0; s < 6; ++s) {
-      ww += w[s] * w[s];
-      wy += w[s] * y[s];
-    }
-    // sum over the 6 slots of W_p (4X_p + X_cyc1 + X_cyc2 - 2 X_swaps)
-    const double f4 = 3.0 * ww + 3.0 * (wE * wE + wO * wO) - 2.0 * (wE + wO) * (wE + wO);
-    const double f5 = 3.0 * wy + 3.0 * (wE * yE + wO * yO) - 2.0 * (wE + wO) * (yE + yO);
-    // each distinct tuple is counted once: slots alias when indices coincide
-    const double wt = (a == b && b == c) ? (1.0 / 6.0) : ((a == b || b == c) ? 0.5 : 1.0);
-    const double D = eijk - eps[o + a] - eps[o + b] - eps[o + c];
-    s4 += wt * f4 / D;
-    s5 += wt * f5 / D;
   }
   for (int off = 32; off > 0; off >>= 1) {
     s4 += __shfl_down(s4, off, 64);
@@ -234,7 +236,7 @@ void CCSD::pt_prepare(GemmPrec mode) {
       }
       // per-triple slices: 6 T(p,q) [x][d], 6 Q_rq [z][l], 3 Tvvo slices [rows][l]
       const long Rv = bf16x3_rpad(v), Kpo = bf16x3_kpad(o), Rvv = bf16x3_rpad(vv) + 256;
-      Psm_.alloc(2 * (6 * (size_t)Rv * Kpv + 6 * (size_t)Rv * Kpo + 3 * (size_t)Rvv * Kpo));
+      Psm_.alloc(2 * (6 * (size_t)Rv * Kpv + 6 * (size_t)Rv * Kpo + 4 * (size_t)Rvv * Kpo));
       CCGPU_HIPX(hipStreamSynchronize(st_));
     }
     // R[(k,c)][(l,j)] = (ck|lj);  Q[k][j][l][c] = R
@@ -268,7 +270,7 @@ void CCSD::pt_triples_mixed(long t, long, long) {
   const long Kpv = bf16x3_kpad(v), Kpo = bf16x3_kpad(o), Rv = bf16x3_rpad(v), Rvv = bf16x3_rpad(vv) + 256;
   const size_t sT = (size_t)Rv * Kpv, sQ = (size_t)Rv * Kpo, sP = (size_t)Rvv * Kpo;
   unsigned short* base = Psm_.p;
-  const size_t half = 6 * sT + 6 * sQ + 3 * sP;  // hi block, then lo block
+  const size_t half = 6 * sT + 6 * sQ + 4 * sP;  // hi block, then lo block
   auto hiT = [&](int n) { return base + n * sT; };
   auto hiQ = [&](int n) { return base + 6 * sT + n * sQ; };
   auto hiP = [&](int n) { return base + 6 * sT + 6 * sQ + n * sP; };
@@ -289,6 +291,9 @@ void CCSD::pt_triples_mixed(long t, long, long) {
   // slot absorbs the over-read of the last x)
   bf16x3_pack(Tvvo_.p + (long)k * vv * o, v, o, (long)v * o, 1, o, v, v, Kpo, -1.0f, hiP(2), hiP(2) + half,
               st_);
+  // [a][b][l] = -Tvvo_j[b][a][l]
+  bf16x3_pack(Tvvo_.p + (long)j * vv * o, v, o, (long)v * o, 1, o, v, v, Kpo, -1.0f, hiP(3), hiP(3) + half,
+              st_);
   const long gk = (long)vv * Kpv;  // rows of one k slice of GT
   auto GT = [&](int r) { return BTerm{GTh_.p + r * gk, GTl_.p + r * gk, nullptr, nullptr, Kpv, 0, 0}; };
   auto GpT = [&](int r) { return BTerm{GpTh_.p + r * gk, GpTl_.p + r * gk, nullptr, nullptr, Kpv, 0, 0}; };
@@ -303,20 +308,19 @@ void CCSD::pt_triples_mixed(long t, long, long) {
   };
   float* W = W32_.p;
   const double fv = 2.0 * (double)vv * vv, fo = 2.0 * (double)vv * v * o;
-  // G1: W[a][(b,c)] = t_ij^{ad} (bd|ck) + t_ik^{ad} (cd|bj)
-  BTerm g1[2] = {mkGB(hiT(0), GT(k), 0), mkGB(hiT(1), GpT(j), 0)};
-  bf16x3_multi(g1, 2, v, vv, 1, W, true, vv, 1, 0, 1.0, 0.0, 2 * fv, st_);
+  // Three passes over W, four terms each (two with K = v, two with K = o).
+  // G1: W[a][(b,c)] = t_ij^{ad} (bd|ck) + t_ik^{ad} (cd|bj) - (ai|lk) t_jl^{bc} - (ai|lj) t_kl^{cb}
+  BTerm g1[4] = {mkGB(hiT(0), GT(k), 0), mkGB(hiT(1), GpT(j), 0), mk(hiQ(4), hiP(1), Kpo, 0, 0),
+                 mk(hiQ(5), hiP(2), Kpo, 0, 0)};
+  bf16x3_multi(g1, 4, v, vv, 1, W, true, vv, 1, 0, 1.0, 0.0, 2 * fv + 2 * fo, st_);
   // G2 (batch a): W[a][b][c] += t_ji^{bd} (ad|ck) + t_jk^{bd} (cd|ai) - t_il^{ac} (bj|lk) - t_kl^{ca} (bj|li)
   BTerm g2[4] = {mkGB(hiT(2), GT(k), v * Kpv), mkGB(hiT(3), GpT(i), v * Kpv),
                  mk(hiQ(1), hiP(0), Kpo, 0, v * Kpo), mk(hiQ(2), hiP(2), Kpo, 0, v * Kpo)};
   bf16x3_multi(g2, 4, v, v, v, W, true, v, 1, vv, 1.0, 1.0, 2 * fv + 2 * fo, st_);
-  // G3: W[(a,b)][c] += (ad|bj) t_ki^{cd} + (bd|ai) t_kj^{cd} - t_il^{ab} (ck|lj)
-  BTerm g3[3] = {mkGA(GT(j), hiT(4), Kpv), mkGA(GpT(i), hiT(5), Kpv), mk(hiP(0), hiQ(0), Kpo, 0, 0)};
-  bf16x3_multi(g3, 3, vv, v, 1, W, true, v, 1, 0, 1.0, 1.0, 2 * fv + fo, st_);
-  // G4 (batch b): W[a][b][c] += -t_jl^{ba} (ck|li) - (ai|lk) t_jl^{bc} - (ai|lj) t_kl^{cb}
-  BTerm g4[3] = {mk(hiP(1), hiQ(3), Kpo, v * Kpo, 0), mk(hiQ(4), hiP(1), Kpo, 0, v * Kpo),
-                 mk(hiQ(5), hiP(2), Kpo, 0, v * Kpo)};
-  bf16x3_multi(g4, 3, v, v, v, W, true, vv, 1, v, 1.0, 1.0, 3 * fo, st_);
+  // G3: W[(a,b)][c] += (ad|bj) t_ki^{cd} + (bd|ai) t_kj^{cd} - t_il^{ab} (ck|lj) - t_jl^{ba} (ck|li)
+  BTerm g3[4] = {mkGA(GT(j), hiT(4), Kpv), mkGA(GpT(i), hiT(5), Kpv), mk(hiP(0), hiQ(0), Kpo, 0, 0),
+                 mk(hiP(3), hiQ(3), Kpo, 0, 0)};
+  bf16x3_multi(g3, 4, vv, v, 1, W, true, v, 1, 0, 1.0, 1.0, 2 * fv + 2 * fo, st_);
 }
 
 void CCSD::pt_energy(long first, long count, long stride, double* e4, double* e5, double* flops) {
@@ -362,22 +366,23 @@ void CCSD::pt_energy(long first, long count, long stride, double* e4, double* e5
     }
     // energy
     const double mult = (i == j || j == k) ? 3.0 : 6.0;
+    const long nblk = std::min<long>(npv, 2048);  // energy partials (blocks stride over the pairs)
     check_span(ptpart_.p, 0, 2 * npv - 1, sizeof(double), "pt part");
     check_span(J_.p, 0, dm_.nov2 - 1, sizeof(double), "pt J");
     if (mixed) {
       float* W1 = W32r_.p;
       float* W2 = W32r_.p + vvv;
       w_reorder(W1, W2, W32_.p, v, st_);
-      CCGPU_LAUNCH(k_pt_energy<float>, dim3((unsigned)npv), dim3(256), 0, st_, W32_.p, W1, W2, J_.p, t1_.p, eps_.p,
+      CCGPU_LAUNCH(k_pt_energy<float>, dim3((unsigned)nblk), dim3(256), 0, st_, W32_.p, W1, W2, J_.p, t1_.p, eps_.p,
                    i, j, k, o, v, mult / 3.0, ptpart_.p);
     } else {
       double* W1 = Wr_.p;
       double* W2 = Wr_.p + vvv;
       w_reorder(W1, W2, W, v, st_);
-      CCGPU_LAUNCH(k_pt_energy<double>, dim3((unsigned)npv), dim3(256), 0, st_, W, W1, W2, J_.p, t1_.p, eps_.p, i,
+      CCGPU_LAUNCH(k_pt_energy<double>, dim3((unsigned)nblk), dim3(256), 0, st_, W, W1, W2, J_.p, t1_.p, eps_.p, i,
                    j, k, o, v, mult / 3.0, ptpart_.p);
     }
-    CCGPU_LAUNCH(k_pt_reduce, dim3(1), dim3(256), 0, st_, ptpart_.p, npv, etrip_.p + 2 * idx);
+    CCGPU_LAUNCH(k_pt_reduce, dim3(1), dim3(256), 0, st_, ptpart_.p, nblk, etrip_.p + 2 * idx);
     CCGPU_HIPX(hipGetLastError());
   }
   std::vector<double> et(2 * std::max<long>(n, 1), 0.0);
