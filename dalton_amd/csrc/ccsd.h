@@ -87,8 +87,9 @@ class CCSD {
   void ladder_mixed(double* Om);
   void mark(const char* name);
   void ensure_work();
-  void pt_prepare(GemmPrec mode);
+  void pt_prepare(GemmPrec mode, bool persistent);
   void pt_triples_mixed(long first, long n, long stride);
+  void pt_energy_persistent(long first, long n, long stride, double* e4, double* e5);
   void ensure_diis();
   void reset_cycle();
   void ensure_lhtr();
@@ -132,6 +133,14 @@ class CCSD {
   // orders, per-triple packed slices, FP32 W
   DBuf<unsigned short> GTh_, GTl_, GpTh_, GpTl_, Psm_;
   DBuf<float> W32_, W32r_;  // W and its two re-ordered copies (energy reads)
+  // (T), persistent kernel: bf16 hi/lo of every T(p,q) [x][d] and Q_rq [z][l]
+  // slice and of -Tvvo_p in both (x,y) orders (mixed), Tvvo with (x,y)
+  // swapped (FP64); four W slots (in W32_ / Wt_); launch tables and counters
+  bool pt_persist_ = true;
+  DBuf<unsigned short> ppH_[4], ppL_[4];
+  DBuf<double> TvvoS_;
+  DBuf<int> ptctr_, pttri_, ptctab_;
+  DBuf<unsigned char> ptdesc_;
   // left transformation: cotangents (suffix b) of the residual's intermediates
   bool lhtr_ready_ = false;
   GemmPrec lhtr_prec_ = GemmPrec::F64;

@@ -172,6 +172,10 @@ void bf16x3_pack(const double* src, long rows, long K, long s_r, long s_k, long 
 // C = alpha * sum_t A_t B_t^T + beta * C (C float or double, element strides)
 void bf16x3_multi(const BTerm* terms, int nterm, long M, long N, int batch, void* C, bool c_float, long c_m,
                   long c_n, long c_b, double alpha, double beta, double flops, hipStream_t s);
+// The launch descriptor of that product, every operand span checked (tiles
+// of 128 over Mp = bf16x3_rpad(M) rows of A and Np rows of B per batch).
+BMulti bf16x3_desc(const BTerm* terms, int nterm, long M, long N, int batch, void* C, bool c_float, long c_m,
+                   long c_n, long c_b, double alpha, double beta);
 
 // V+/- of one particle-ladder a-batch from an FP32 Y block straight into the
 // split-bf16 operand layout hi/lo[2][Rp][Kp] (V+ then V-), zero padded.

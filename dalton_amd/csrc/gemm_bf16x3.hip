@@ -12,13 +12,11 @@
 #include <cmath>
 #include <cstdlib>
 
+#include "bf16x3_tile.h"
 #include "common.h"
 
 namespace ccgpu {
 
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-typedef float f4 __attribute__((ext_vector_type(4)));
-typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
 namespace {
 
@@ -98,139 +96,16 @@ __device__ __forceinline__ void tile_of3(long b, long tiles_m, long tiles_n, lon
   tn = r / gsz;
 }
 
-constexpr int TM = 128, TN = 128, TK = 64, LDK = TK + 8;
+constexpr int TM = 128, TN = bf3::TN, TK = bf3::TK;
 
-// Block tile (64 RW) x 128, K step 64 (two MFMA k-slices of 32), 2 RW waves
-// of 64x64 (4x4 MFMA tiles each): RW = 2 is the 128x128 kernel (two blocks
-// per CU), RW = 4 the 256x128 one (one block of eight waves per CU, a
-// quarter less operand traffic per flop).  Each K step keeps every wave busy
-// for 96 MFMAs, covering the latency of the next step's global loads, which
-// are issued before the MFMAs.  The output tile accumulates up to four
-// products sum_t A_t B_t (each term with its own packed operands and K), so
-// several contributions to one output array cost a single read-modify-write.
+// One block per output tile, rasterised by tile_of3.
 template <typename OutT, int RW>
 __global__ __launch_bounds__(128 * RW, 4 / RW) void k_gemm_bf16x3(BMulti p) {
-  constexpr int TMr = 64 * RW, NT = 128 * RW;
-  constexpr int NCHA = TMr * TK / 8 / NT, NCHB = TN * TK / 8 / NT;  // 16-byte chunks per thread
-  __shared__ __attribute__((aligned(16))) unsigned short sA[2][TMr][LDK];
-  __shared__ __attribute__((aligned(16))) unsigned short sB[2][TN][LDK];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wm = wave >> 1, wn = wave & 1;
-  const long tiles_m = p.Mp / TMr, tiles_n = p.Np / TN;
+  __shared__ bf3::Smem<RW> S;
+  constexpr int TMr = 64 * RW;
   long tm, tn;
-  tile_of3(blockIdx.x, tiles_m, tiles_n, tm, tn, p.gm);
-  const long m0 = tm * TMr, n0 = tn * TN, z = blockIdx.z;
-  constexpr int CPR = TK / 8;
-  f4 acc[4][4];
-#pragma unroll
-  for (int i = 0; i < 4; ++i)
-#pragma unroll
-    for (int j = 0; j < 4; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
-  const int lr = lane & 15, lk = (lane >> 4) * 8;
-  int loffa[NCHA], loffb[NCHB];
-#pragma unroll
-  for (int q = 0; q < NCHA; ++q) {
-    const int c = tid + q * NT;
-    loffa[q] = (c / CPR) * LDK + (c % CPR) * 8;
-  }
-#pragma unroll
-  for (int q = 0; q < NCHB; ++q) {
-    const int c = tid + q * NT;
-    loffb[q] = (c / CPR) * LDK + (c % CPR) * 8;
-  }
-  for (int t = 0; t < p.nterm; ++t) {
-    const BTerm& T = p.t[t];
-    const long Kp = T.Kp;
-    const unsigned short* pAh = T.Ah + z * T.a_bs + m0 * Kp;
-    const unsigned short* pAl = T.Al + z * T.a_bs + m0 * Kp;
-    const unsigned short* pBh = T.Bh + z * T.b_bs + n0 * Kp;
-    const unsigned short* pBl = T.Bl + z * T.b_bs + n0 * Kp;
-    // thread t owns 16-byte chunks c = t + NT q: row c / (TK/8), k-chunk c % (TK/8).
-    // Plain register arrays with compile-time indices (kept out of scratch).
-    u32x4 rah[NCHA], ral[NCHA], rbh[NCHB], rbl[NCHB];
-    long goffa[NCHA], goffb[NCHB];
-#pragma unroll
-    for (int q = 0; q < NCHA; ++q) {
-      const int c = tid + q * NT;
-      goffa[q] = (long)(c / CPR) * Kp + (c % CPR) * 8;
-    }
-#pragma unroll
-    for (int q = 0; q < NCHB; ++q) {
-      const int c = tid + q * NT;
-      goffb[q] = (long)(c / CPR) * Kp + (c % CPR) * 8;
-    }
-#define CCGPU_BF3_LOAD(k0)                                                       \
-  _Pragma("unroll") for (int q = 0; q < NCHA; ++q) {                             \
-    rah[q] = *reinterpret_cast<const u32x4*>(pAh + goffa[q] + (k0));             \
-    ral[q] = *reinterpret_cast<const u32x4*>(pAl + goffa[q] + (k0));             \
-  }                                                                              \
-  _Pragma("unroll") for (int q = 0; q < NCHB; ++q) {                             \
-    rbh[q] = *reinterpret_cast<const u32x4*>(pBh + goffb[q] + (k0));             \
-    rbl[q] = *reinterpret_cast<const u32x4*>(pBl + goffb[q] + (k0));             \
-  }
-#define CCGPU_BF3_STORE()                                                        \
-  _Pragma("unroll") for (int q = 0; q < NCHA; ++q) {                             \
-    *reinterpret_cast<u32x4*>(&sA[0][0][0] + loffa[q]) = rah[q];                 \
-    *reinterpret_cast<u32x4*>(&sA[1][0][0] + loffa[q]) = ral[q];                 \
-  }                                                                              \
-  _Pragma("unroll") for (int q = 0; q < NCHB; ++q) {                             \
-    *reinterpret_cast<u32x4*>(&sB[0][0][0] + loffb[q]) = rbh[q];                 \
-    *reinterpret_cast<u32x4*>(&sB[1][0][0] + loffb[q]) = rbl[q];                 \
-  }
-    CCGPU_BF3_LOAD(0)
-    for (long k0 = 0; k0 < Kp; k0 += TK) {
-      __syncthreads();
-      CCGPU_BF3_STORE()
-      __syncthreads();
-      if (k0 + TK < Kp) {
-        CCGPU_BF3_LOAD(k0 + TK)
-      }
-#pragma unroll
-      for (int ks = 0; ks < TK; ks += 32) {
-        bf16x8 ah[4], al[4], bh[4], bl[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          ah[u] = *reinterpret_cast<const bf16x8*>(&sA[0][wm * 64 + u * 16 + lr][ks + lk]);
-          al[u] = *reinterpret_cast<const bf16x8*>(&sA[1][wm * 64 + u * 16 + lr][ks + lk]);
-          bh[u] = *reinterpret_cast<const bf16x8*>(&sB[0][wn * 64 + u * 16 + lr][ks + lk]);
-          bl[u] = *reinterpret_cast<const bf16x8*>(&sB[1][wn * 64 + u * 16 + lr][ks + lk]);
-        }
-        // the three split products of one accumulator are 16 MFMAs apart
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-#pragma unroll
-          for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-#pragma unroll
-          for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-#pragma unroll
-          for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
-      }
-    }
-#undef CCGPU_BF3_LOAD
-#undef CCGPU_BF3_STORE
-    __syncthreads();
-  }
-  // D[i][j] of a 16x16 f32 MFMA tile: lane = j + 16*(i/4), element i%4.
-  OutT* C = static_cast<OutT*>(p.C) + z * p.c_b;
-#pragma unroll
-  for (int i = 0; i < 4; ++i)
-#pragma unroll
-    for (int j = 0; j < 4; ++j)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const long m = m0 + wm * 64 + i * 16 + (lane >> 4) * 4 + r;
-        const long n = n0 + wn * 64 + j * 16 + lr;
-        if (m < p.M && n < p.N) {
-          OutT* q = C + m * p.c_m + n * p.c_n;
-          double v = p.alpha * (double)acc[i][j][r];
-          if (p.beta != 0.0) v += p.beta * (double)*q;
-          *q = static_cast<OutT>(v);
-        }
-      }
+  tile_of3(blockIdx.x, p.Mp / TMr, p.Np / TN, tm, tn, p.gm);
+  bf3::tile<OutT, RW>(p, tm * TMr, tn * TN, blockIdx.z, S);
 }
 
 
@@ -544,8 +419,8 @@ void bf16x3_pack(const double* src, long rows, long K, long s_r, long s_k, long 
 // Multi-term product into a float or double output (see k_gemm_bf16x3).
 // Every term's packed A must cover Mp rows (per batch, at stride a_bs) and
 // B Np rows, both with row length Kp; rows past M/N are read, never used.
-void bf16x3_multi(const BTerm* terms, int nterm, long M, long N, int batch, void* C, bool c_float, long c_m,
-                  long c_n, long c_b, double alpha, double beta, double flops, hipStream_t s) {
+BMulti bf16x3_desc(const BTerm* terms, int nterm, long M, long N, int batch, void* C, bool c_float, long c_m,
+                   long c_n, long c_b, double alpha, double beta) {
   CCGPU_REQUIRE(nterm >= 1 && nterm <= 4, "bf16x3_multi: 1..4 terms");
   BMulti p{};
   p.nterm = nterm;
@@ -564,6 +439,12 @@ void bf16x3_multi(const BTerm* terms, int nterm, long M, long N, int batch, void
     check_span(T.Bl, 0, (batch - 1) * T.b_bs + p.Np * T.Kp - 1, 2, "bf16x3 B lo");
   }
   check_span(C, 0, (M - 1) * c_m + (N - 1) * c_n + (batch - 1) * c_b, esz, "bf16x3 C");
+  return p;
+}
+
+void bf16x3_multi(const BTerm* terms, int nterm, long M, long N, int batch, void* C, bool c_float, long c_m,
+                  long c_n, long c_b, double alpha, double beta, double flops, hipStream_t s) {
+  const BMulti p = bf16x3_desc(terms, nterm, M, N, batch, C, c_float, c_m, c_n, c_b, alpha, beta);
   const int id = prof_begin("k_gemm_bf16x3", flops, s);
   if (c_float)
     launch_multi<float>(p, batch, s);

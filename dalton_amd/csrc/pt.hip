@@ -14,7 +14,9 @@
 // The energy kernel evaluates the permutation-symmetrised functional of
 // w_prod (:1547-1640) per orbit {a,b,c}: each W element is read once.
 // E4 takes V = W, E5 takes the t1 part of V (v_calc :1345, :1483-1527).
+#include <algorithm>
 #include <cmath>
+#include <cstdlib>
 
 #include "ccsd.h"
 #include "kernels.h"
@@ -182,15 +184,25 @@ long CCSD::pt_ntriples() const {
 void CCSD::pt_release() {
   for (DBuf<double>* b : {&G_, &Gp_, &Tvvo_, &Q_, &Wt_, &Wr_, &ptpart_, &etrip_}) b->release();
   for (DBuf<unsigned short>* b : {&GTh_, &GTl_, &GpTh_, &GpTl_, &Psm_}) b->release();
+  for (int q = 0; q < 4; ++q) {
+    ppH_[q].release();
+    ppL_[q].release();
+  }
+  for (DBuf<int>* b : {&ptctr_, &pttri_, &ptctab_}) b->release();
+  TvvoS_.release();
+  ptdesc_.release();
   W32_.release();
   W32r_.release();
   pt_ready_ = false;
 }
 
 // Integral and amplitude layouts for the triple loop (built once per call
-// sequence; the iteration workspace is released first to make room).
-void CCSD::pt_prepare(GemmPrec mode) {
-  if (pt_ready_ && pt_mode_ == mode) return;
+// sequence; the iteration workspace is released first to make room).  The
+// persistent kernel takes every per-triple operand pre-packed and four W
+// slots; the per-triple launches pack per triple and keep one W with its two
+// re-ordered copies.
+void CCSD::pt_prepare(GemmPrec mode, bool persistent) {
+  if (pt_ready_ && pt_mode_ == mode && pt_persist_ == persistent) return;
   pt_release();
   release_work();
   const int o = dm_.o, v = dm_.v, naux = dm_.naux;
@@ -198,7 +210,7 @@ void CCSD::pt_prepare(GemmPrec mode) {
   const bool mixed = mode == GemmPrec::BF16X3;
   Tvvo_.alloc((size_t)o * vv * o);
   Q_.alloc((size_t)o * o * o * v);
-  ptpart_.alloc((size_t)(vv + v) + 2);
+  ptpart_.alloc(std::max<size_t>((size_t)(vv + v) + 2, 8 * 2048));
   {
     // undressed virtual-virtual and occupied-occupied blocks of B
     DBuf<double> Bvv0((size_t)naux * vv), Boo0((size_t)naux * o * o), R((size_t)nov * o * o);
@@ -207,8 +219,13 @@ void CCSD::pt_prepare(GemmPrec mode) {
     if (!mixed) {
       G_.alloc((size_t)o * vvv);
       Gp_.alloc((size_t)o * vvv);
-      Wt_.alloc(vvv);
-      Wr_.alloc(2 * (size_t)vvv);
+      if (persistent) {
+        Wt_.alloc(4 * (size_t)vvv);
+        TvvoS_.alloc((size_t)o * vv * o);
+      } else {
+        Wt_.alloc(vvv);
+        Wr_.alloc(2 * (size_t)vvv);
+      }
       // G[k][d][y][z] = (yd|zk) = sum_P B[P][d][y] B[P][k][z], batched over k
       g(vv, v, naux, 1.0, Bvv0.p, 1, vv, Bov_.p, nov, 1, 0.0, G_.p, v, 1, st_, o, 0, v, vvv);
       // Gp[k][d][y][z] = G[k][d][z][y]
@@ -222,8 +239,12 @@ void CCSD::pt_prepare(GemmPrec mode) {
       GTl_.alloc(ng);
       GpTh_.alloc(ng);
       GpTl_.alloc(ng);
-      W32_.alloc(vvv);
-      W32r_.alloc(2 * (size_t)vvv);
+      if (persistent) {
+        W32_.alloc(4 * (size_t)vvv);
+      } else {
+        W32_.alloc(vvv);
+        W32r_.alloc(2 * (size_t)vvv);
+      }
       DBuf<double> Gk(vvv), Gpk(vvv);
       for (int k = 0; k < o; ++k) {
         g(vv, v, naux, 1.0, Bvv0.p, 1, vv, Bov_.p + (long)k * v, nov, 1, 0.0, Gk.p, v, 1, st_);
@@ -234,9 +255,9 @@ void CCSD::pt_prepare(GemmPrec mode) {
         bf16x3_pack(Gk.p, vv, v, 1, vv, 0, 1, rp, Kpv, 1.0f, GTh_.p + off, GTl_.p + off, st_);
         bf16x3_pack(Gpk.p, vv, v, 1, vv, 0, 1, rp, Kpv, 1.0f, GpTh_.p + off, GpTl_.p + off, st_);
       }
-      // per-triple slices: 6 T(p,q) [x][d], 6 Q_rq [z][l], 3 Tvvo slices [rows][l]
+      // per-triple slices: 6 T(p,q) [x][d], 6 Q_rq [z][l], 4 Tvvo slices [rows][l]
       const long Rv = bf16x3_rpad(v), Kpo = bf16x3_kpad(o), Rvv = bf16x3_rpad(vv) + 256;
-      Psm_.alloc(2 * (6 * (size_t)Rv * Kpv + 6 * (size_t)Rv * Kpo + 4 * (size_t)Rvv * Kpo));
+      if (!persistent) Psm_.alloc(2 * (6 * (size_t)Rv * Kpv + 6 * (size_t)Rv * Kpo + 4 * (size_t)Rvv * Kpo));
       CCGPU_HIPX(hipStreamSynchronize(st_));
     }
     // R[(k,c)][(l,j)] = (ck|lj);  Q[k][j][l][c] = R
@@ -246,6 +267,48 @@ void CCSD::pt_prepare(GemmPrec mode) {
   }
   // Tvvo[i][a][b][l] = t_il^ab = T[(i,a),(l,b)]
   perm(Tvvo_.p, T_.p, o, v, v, o, (long)v * nov, nov, 1, v, 1.0, 0.0, st_);
+  if (persistent) {
+    if (!mixed) {
+      // TvvoS[p][x][y][l] = Tvvo[p][y][x][l]
+      perm(TvvoS_.p, Tvvo_.p, o, v, v, o, vv * o, o, (long)v * o, 1, 1.0, 0.0, st_);
+    } else {
+      const long Kpv = bf16x3_kpad(v), Kpo = bf16x3_kpad(o), Rv = bf16x3_rpad(v), Rvv = bf16x3_rpad(vv) + 256;
+      CCGPU_REQUIRE(o <= 255 && v <= 65535, "pt: occupied or virtual space too large for the packed slices");
+      const size_t n0 = (size_t)o * o * Rv * Kpv, n1 = (size_t)o * o * Rv * Kpo, n2 = (size_t)o * Rvv * Kpo;
+      const size_t nn[4] = {n0, n1, n2, n2};
+      for (int q = 0; q < 4; ++q) {
+        ppH_[q].alloc(nn[q]);
+        ppL_[q].alloc(nn[q]);
+      }
+      // slot tails past v*v rows of the swapped -Tvvo are read by edge tiles only: keep them finite
+      CCGPU_HIPX(hipMemsetAsync(ppH_[3].p, 0, ppH_[3].bytes(), st_));
+      CCGPU_HIPX(hipMemsetAsync(ppL_[3].p, 0, ppL_[3].bytes(), st_));
+      for (int p = 0; p < o; ++p) {
+        // T(p,q)[x][d] = t_pq^{xd} for every q: [(p,q)][x][d]
+        const size_t ot = (size_t)p * o * Rv * Kpv;
+        bf16x3_pack(T_.p + (long)p * v * nov, v, v, nov, 1, v, o, Rv, Kpv, 1.0f, ppH_[0].p + ot, ppL_[0].p + ot,
+                    st_);
+        // [x][c][l] = -Tvvo_p[c][x][l]
+        const size_t os = (size_t)p * Rvv * Kpo;
+        bf16x3_pack(Tvvo_.p + (long)p * vv * o, v, o, (long)v * o, 1, o, v, v, Kpo, -1.0f, ppH_[3].p + os,
+                    ppL_[3].p + os, st_);
+      }
+      // Q_rq[z][l] = (zr|lq) for every (r,q)
+      bf16x3_pack(Q_.p, v, o, 1, v, (long)o * v, o * o, Rv, Kpo, 1.0f, ppH_[1].p, ppL_[1].p, st_);
+      // [(x,y)][l] = -Tvvo_p[x][y][l]
+      bf16x3_pack(Tvvo_.p, vv, o, o, 1, vv * o, o, Rvv, Kpo, -1.0f, ppH_[2].p, ppL_[2].p, st_);
+    }
+    // canonical 8x8x8 blocks A <= B <= C of the virtual cube, A | B << 10 | C << 20
+    const long nb = cdiv(v, 8);
+    CCGPU_REQUIRE(nb <= 1023, "pt: virtual space too large for the block table");
+    std::vector<int> ct;
+    for (long a = 0; a < nb; ++a)
+      for (long b = a; b < nb; ++b)
+        for (long c = b; c < nb; ++c) ct.push_back((int)(a | (b << 10) | (c << 20)));
+    ptctab_.alloc(ct.size());
+    CCGPU_HIPX(hipMemcpyAsync(ptctab_.p, ct.data(), ct.size() * sizeof(int), hipMemcpyHostToDevice, st_));
+    CCGPU_HIPX(hipStreamSynchronize(st_));
+  }
   tri_i_.clear();
   tri_j_.clear();
   tri_k_.clear();
@@ -258,6 +321,7 @@ void CCSD::pt_prepare(GemmPrec mode) {
         tri_k_.push_back(k);
       }
   pt_mode_ = mode;
+  pt_persist_ = persistent;
   pt_ready_ = true;
 }
 
@@ -323,15 +387,30 @@ void CCSD::pt_triples_mixed(long t, long, long) {
   bf16x3_multi(g3, 4, vv, v, 1, W, true, v, 1, 0, 1.0, 1.0, 2 * fv + 2 * fo, st_);
 }
 
+// (T) runs as per-triple launches; the persistent kernel only with
+// CCGPU_PT_PERSISTENT=1.  It is switched off: on an MI355X its first run
+// (H2O, o=5 v=19, FP64) did not finish, with or without the bounded waits,
+// and the cause is not yet known.
+static bool pt_use_persistent() {
+  const char* e = std::getenv("CCGPU_PT_PERSISTENT");
+  return e && e[0] == '1';
+}
+
 void CCSD::pt_energy(long first, long count, long stride, double* e4, double* e5, double* flops) {
   const bool mixed = bulk_ == GemmPrec::BF16X3;
-  pt_prepare(mixed ? GemmPrec::BF16X3 : GemmPrec::F64);
+  const bool persistent = pt_use_persistent();
+  pt_prepare(mixed ? GemmPrec::BF16X3 : GemmPrec::F64, persistent);
   const int o = dm_.o, v = dm_.v;
   const long nov = dm_.nov, vv = (long)v * v, vvv = vv * v, vo = (long)v * o;
   const long ntr = pt_ntriples();
   CCGPU_REQUIRE(first >= 0 && stride >= 1 && count >= 0, "pt: bad triple range");
   long n = 0;
   for (long t = first; t < ntr && n < count; t += stride) ++n;
+  if (flops) *flops = (double)n * (12.0 * (double)vv * vv + 12.0 * (double)o * vvv);
+  if (persistent) {
+    pt_energy_persistent(first, n, stride, e4, e5);
+    return;
+  }
   etrip_.ensure(2 * (size_t)std::max<long>(n, 1));
   const double* T = T_.p;
   double* W = Wt_.p;
@@ -395,7 +474,6 @@ void CCSD::pt_energy(long first, long count, long stride, double* e4, double* e5
   }
   if (e4) *e4 = s4;
   if (e5) *e5 = s5;
-  if (flops) *flops = (double)n * (12.0 * (double)vv * vv + 12.0 * (double)o * vvv);
 }
 
 }  // namespace ccgpu

@@ -1,0 +1,780 @@
+// (T) energy as one persistent kernel over a range of triples i >= j >= k
+// (mlcc/mlccsdpt_e_calc.F90:71, loop :326-411; W as in w_calc :966, energy
+// as in w_prod :1547-1640 and v_calc :1345, :1483-1527).
+//
+// Work items, claimed in order from one device counter by resident blocks:
+// per triple q, three GEMM passes that build W_q (G1: W[a][(b,c)] with
+// beta = 0; G2: batched over a, W[a][b][c] += ...; G3: W[(a,b)][c] += ...,
+// four of the twelve terms each, one 128 x 128 output tile per item) and an
+// energy pass (E, one item per group of canonical 8x8x8 virtual blocks).
+// The claim order is a software pipeline: step s holds
+// E(s-3) | G3(s-2) | G2(s-1) | G1(s), and triple q owns W slot q mod 4.
+// Every item waits only on items claimed before it (G2/G3/E on the whole
+// previous pass of the same triple, G1 on the end of triple q-4 that used
+// the same slot), so the items in flight always include one whose
+// dependencies are complete and the grid cannot deadlock; by the time an
+// item is claimed its dependencies have normally finished, which hides the
+// tails of the passes.  Each wait is bounded: past its time limit it raises
+// an error flag, every block leaves, and the host reports the error.
+//
+// The energy pass reads W once: for canonical blocks A <= B <= C it stages
+// the six block permutations of W, the eighteen 8x8 blocks of the (ia|jb)
+// integrals and the t1 slices it needs in LDS and evaluates the
+// permutation-symmetrised functional per orbit {a <= b <= c}.  The last
+// energy item of a triple sums the partials in a fixed order.
+#include <algorithm>
+#include <cstdlib>
+#include <cstring>
+#include <type_traits>
+
+#include "bf16x3_tile.h"
+#include "ccsd.h"
+#include "kernels.h"
+#include "ops.h"
+
+namespace ccgpu {
+
+namespace {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+typedef double d2 __attribute__((ext_vector_type(2)));
+
+// ------------------------------------------------------------ FP64 tiles
+// One term of a multi-term FP64 product: A(m,k) of batch z at
+// A + m a_m + k a_k + z a_b, B(n,k) at B + n b_n + k b_k + z b_b.
+struct DTerm {
+  const double* A;
+  const double* B;
+  long a_m, a_k, a_b;
+  long b_n, b_k, b_b;
+  long K;
+  double alpha;
+  int akf;  // 1: A is read along k and B along n; 0: A along m and B along k
+};
+// C(m,n) of batch z (at C + m c_m + n c_n + z c_b) = sum_t alpha_t A_t B_t^T + beta C
+struct DMulti {
+  DTerm t[4];
+  int nterm;
+  long M, N;
+  double beta;
+  double* C;
+  long c_m, c_n, c_b;
+};
+
+namespace dt {
+// Tile 128 x 128, K step 16, 256 threads as 2 x 2 waves of 64 x 64
+// (4 x 4 v_mfma_f64_16x16x4f64 tiles each); operands staged
+// global -> registers -> LDS, the next step's loads issued before the MFMAs.
+// Each step derives its eight load addresses from one pointer (an empty asm
+// keeps the compiler from hoisting eight 64-bit row offsets out of the K
+// loop, which would not fit beside the accumulators).
+// An operand read along k is staged [row][k] (row stride TK + 1), one read
+// along its rows [k][row] (stride T + 16): both conflict-free.
+constexpr int T = 128, TK = 16, NL = 8, IW = 4, LDR = T + 16;
+struct __attribute__((aligned(16))) Smem {
+  double A[TK * LDR];
+  double B[TK * LDR];
+};
+template <bool KF>
+__device__ __forceinline__ double& at(double* s, int r, int k) {
+  return KF ? s[r * (TK + 1) + k] : s[k * LDR + r];
+}
+
+template <bool AKF>
+__device__ __forceinline__ void term(const DTerm& t, long M, long N, long m0, long n0, long z, d4 (&acc)[IW][IW],
+                                     Smem& S) {
+  constexpr bool BKF = !AKF;
+  int tid = threadIdx.x;
+  asm volatile("" : "+v"(tid));  // thread-index arithmetic stays inside the item (not hoisted into the claim loop)
+  const int lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  const CCGPU_GLOBAL double* A = gptr(t.A + z * t.a_b);
+  const CCGPU_GLOBAL double* B = gptr(t.B + z * t.b_b);
+  const long K = t.K, a_m = t.a_m, a_k = t.a_k, b_n = t.b_n, b_k = t.b_k;
+  const double al = t.alpha;
+  double ra[NL], rb[NL];
+  auto load = [&](long k0) {
+    if (AKF) {  // column k0 + tid % 16, rows tid / 16 + 16 i
+      const long k = k0 + (tid & 15);
+      const bool kok = k < K;
+      const long mb = m0 + (tid >> 4);
+      const CCGPU_GLOBAL double* pa = A + (mb * a_m + k * a_k);
+      asm volatile("" : "+v"(pa));
+#pragma unroll
+      for (int i = 0; i < NL; ++i) {
+        ra[i] = (kok && mb + 16 * i < M) ? al * *pa : 0.0;
+        pa += 16 * a_m;
+      }
+    } else {  // row tid % 128, columns tid / 128 + 2 i
+      const long m = m0 + (tid & (T - 1));
+      const bool mok = m < M;
+      const long kb = k0 + (tid >> 7);
+      const CCGPU_GLOBAL double* pa = A + ((mok ? m : 0) * a_m + kb * a_k);
+      asm volatile("" : "+v"(pa));
+#pragma unroll
+      for (int i = 0; i < NL; ++i) {
+        ra[i] = (mok && kb + 2 * i < K) ? al * *pa : 0.0;
+        pa += 2 * a_k;
+      }
+    }
+    if (BKF) {
+      const long k = k0 + (tid & 15);
+      const bool kok = k < K;
+      const long nb = n0 + (tid >> 4);
+      const CCGPU_GLOBAL double* pb = B + (nb * b_n + k * b_k);
+      asm volatile("" : "+v"(pb));
+#pragma unroll
+      for (int i = 0; i < NL; ++i) {
+        rb[i] = (kok && nb + 16 * i < N) ? *pb : 0.0;
+        pb += 16 * b_n;
+      }
+    } else {
+      const long n = n0 + (tid & (T - 1));
+      const bool nok = n < N;
+      const long kb = k0 + (tid >> 7);
+      const CCGPU_GLOBAL double* pb = B + ((nok ? n : 0) * b_n + kb * b_k);
+      asm volatile("" : "+v"(pb));
+#pragma unroll
+      for (int i = 0; i < NL; ++i) {
+        rb[i] = (nok && kb + 2 * i < K) ? *pb : 0.0;
+        pb += 2 * b_k;
+      }
+    }
+  };
+  auto store = [&]() {
+#pragma unroll
+    for (int i = 0; i < NL; ++i) {
+      const int e = tid + 256 * i;
+      if (AKF)
+        at<true>(S.A, e >> 4, e & 15) = ra[i];
+      else
+        at<false>(S.A, e & (T - 1), e >> 7) = ra[i];
+      if (BKF)
+        at<true>(S.B, e >> 4, e & 15) = rb[i];
+      else
+        at<false>(S.B, e & (T - 1), e >> 7) = rb[i];
+    }
+  };
+  const int lr = lane & 15, lk = lane >> 4;
+  load(0);
+  for (long k0 = 0; k0 < K; k0 += TK) {
+    __syncthreads();
+    store();
+    __syncthreads();
+    if (k0 + TK < K) load(k0 + TK);
+#pragma unroll
+    for (int kk = 0; kk < TK / 4; ++kk) {
+      double a[IW], b[IW];
+#pragma unroll
+      for (int u = 0; u < IW; ++u) {
+        a[u] = at<AKF>(S.A, wm * 64 + u * 16 + lr, kk * 4 + lk);
+        b[u] = at<BKF>(S.B, wn * 64 + u * 16 + lr, kk * 4 + lk);
+      }
+#pragma unroll
+      for (int i = 0; i < IW; ++i)
+#pragma unroll
+        for (int j = 0; j < IW; ++j) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
+    }
+  }
+}
+
+// Four terms, the first two read with A along k (X) or along m (!X), the
+// other two the other way; the sequence is static so the accumulators stay
+// in registers from term to term.
+template <bool X>
+__device__ __forceinline__ void tile4(const DMulti& p, long m0, long n0, long z, Smem& S) {
+  d4 acc[IW][IW];
+#pragma unroll
+  for (int i = 0; i < IW; ++i)
+#pragma unroll
+    for (int j = 0; j < IW; ++j) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
+  term<X>(p.t[0], p.M, p.N, m0, n0, z, acc, S);
+  term<X>(p.t[1], p.M, p.N, m0, n0, z, acc, S);
+  term<!X>(p.t[2], p.M, p.N, m0, n0, z, acc, S);
+  term<!X>(p.t[3], p.M, p.N, m0, n0, z, acc, S);
+  __syncthreads();  // the next item reuses the LDS
+  // D[i][j] of a 16x16 f64 MFMA tile: lane = j + 16*(i%4), element i/4.
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wm = wave >> 1, wn = wave & 1, lr = lane & 15, lk = lane >> 4;
+  CCGPU_GLOBAL double* C = gptr(p.C + z * p.c_b);
+#pragma unroll
+  for (int i = 0; i < IW; ++i)
+#pragma unroll
+    for (int j = 0; j < IW; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const long m = m0 + wm * 64 + i * 16 + r * 4 + lk;
+        const long n = n0 + wn * 64 + j * 16 + lr;
+        if (m < p.M && n < p.N) {
+          CCGPU_GLOBAL double* q = C + m * p.c_m + n * p.c_n;
+          double v = acc[i][j][r];
+          if (p.beta != 0.0) v += p.beta * *q;
+          *q = v;
+        }
+      }
+}
+}  // namespace dt
+
+// ------------------------------------------------------------ energy items
+template <typename WT>
+struct __attribute__((aligned(16))) ESmem {
+  WT w[6][8 * 72];     // block permutation p of W: [x][y][z] at x*72 + y*9 + z
+  double jb[18][64];   // (ia|jb) blocks: [m*6 + pair][r*8 + c]
+  double t1s[3][3][8];  // t1 of i, j, k over the blocks A, B, C
+  double red[2][4];
+};
+
+// Positions (0 = A, 1 = B, 2 = C) of x, y, z in slot s: abc, acb, bac, bca, cab, cba.
+__device__ constexpr int PX[6] = {0, 0, 1, 1, 2, 2};
+__device__ constexpr int PY[6] = {1, 2, 0, 2, 0, 1};
+__device__ constexpr int PZ[6] = {2, 1, 2, 0, 1, 0};
+// index of the ordered position pair (u, w), u != w
+__host__ __device__ constexpr int pair_ix(int u, int w) { return u * 2 + (w > u ? w - 1 : w); }
+
+struct PtRun {
+  const void* desc;  // three descriptors per triple (G1, G2, G3): BMulti or DMulti
+  void* W;           // four slots of v^3 (float or double)
+  long wslot;
+  double* part;  // four slots of 2 nE energy partials
+  double* etrip;  // (E4, E5) per triple of the launch
+  int* ctr;       // [0,1] claim counter (64 bit), [2] error flag, [4 + 5q + phase] completions
+  const int* tri;  // (i, j, k) per triple
+  const int* ctab;  // canonical blocks A | B << 10 | C << 20
+  const double* J;
+  const double* t1;
+  const double* eps;
+  int o, v;
+  long nq, nE, nct;
+  long ng[3];        // items of G1, G2, G3
+  long tm[3], tn[3];  // tiles per batch of G1, G2, G3 (m, n)
+  long spin_limit;  // wall-clock ticks a dependency wait may take before the error flag is raised
+};
+
+// Tile (tm, tn) of a linear index: groups of 8 m-tiles walked n-major, so
+// consecutive claims share B panels.
+__device__ __forceinline__ void tile_grouped(long t, long tiles_m, long tiles_n, long& tm, long& tn) {
+  constexpr long GM = 8;
+  const long grp = t / (GM * tiles_n);
+  const long fm = grp * GM;
+  const long gsz = min(tiles_m - fm, GM);
+  const long r = t - grp * GM * tiles_n;
+  tm = fm + r % gsz;
+  tn = r / gsz;
+}
+
+// Thread 0: wait until *p >= target, polling with relaxed loads (an
+// acquire per poll would invalidate the caches on every poll, and hundreds
+// of polling blocks would keep the L2s of the producers cold); the caller
+// issues one acquire after the wait.  Returns 0 if the error flag is (or
+// gets) set: after limit ticks of the constant-rate wall clock at the latest.
+__device__ __forceinline__ int wait_ge(int* p, int target, int* err, long limit) {
+  const unsigned long long t0 = wall_clock64();
+  for (;;) {
+    if (__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= target) return 1;
+    if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return 0;
+    if ((long)(wall_clock64() - t0) > limit) {
+      __hip_atomic_store(err, 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      return 0;
+    }
+    __builtin_amdgcn_s_sleep(16);
+  }
+}
+
+// A block-uniform value read back from LDS, marked uniform so that what is
+// derived from it (item, triple, descriptor address) stays in scalar registers.
+__device__ __forceinline__ long uniform64(long x) {
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)(unsigned long)x);
+  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)((unsigned long)x >> 32));
+  return (long)(((unsigned long)hi << 32) | lo);
+}
+
+// Count one completion of this block's item.  Every wave first waits for its
+// own global stores to complete (a barrier alone only orders LDS), then thread
+// 0 increments with release semantics (one L2 write-back per item).
+__device__ __forceinline__ int signal_done(int* p) {
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  int old = 0;
+  if (threadIdx.x == 0) old = __hip_atomic_fetch_add(p, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  return old;
+}
+
+// Energy item e of triple (i, j, k): canonical blocks e, e + nE, ...; writes
+// its scaled (E4, E5) partial to part[2e], part[2e + 1].
+template <typename WT>
+__device__ __forceinline__ void energy_item(const PtRun& r, const WT* W_, int i, int j, int k, long e, double* part,
+                                            ESmem<WT>& S) {
+  const int o = r.o, v = r.v;
+  const CCGPU_GLOBAL WT* W = gptr(W_);
+  const CCGPU_GLOBAL double* J = gptr(r.J);
+  const CCGPU_GLOBAL double* t1 = gptr(r.t1);
+  const CCGPU_GLOBAL double* eps = gptr(r.eps);
+  const CCGPU_GLOBAL int* ctab = gptr(r.ctab);
+  const long nov = (long)o * v, vv = (long)v * v;
+  int tid = threadIdx.x;
+  asm volatile("" : "+v"(tid));  // thread-index arithmetic stays inside the item
+  const double eijk = eps[i] + eps[j] + eps[k];
+  const bool vec = (v % 4) == 0;
+  double s4 = 0.0, s5 = 0.0;
+  for (long ct = e; ct < r.nct; ct += r.nE) {
+    const int code = ctab[ct];
+    const int b0 = code & 1023, b1 = (code >> 10) & 1023, b2 = (code >> 20) & 1023;
+    auto blk = [&](int u) { return u == 0 ? b0 : (u == 1 ? b1 : b2); };  // selects, no indexed local array
+    __syncthreads();  // the previous blocks' LDS reads are done
+    // six W block permutations: 384 rows of 8
+    for (int row = tid; row < 384; row += 256) {
+      const int p = row >> 6, x8 = (row >> 3) & 7, y8 = row & 7;
+      const int px = p >> 1, lo = px == 0 ? 1 : 0, hi = px == 2 ? 1 : 2;
+      const int py = (p & 1) ? hi : lo, pz = (p & 1) ? lo : hi;
+      const long gx = blk(px) * 8 + x8, gy = blk(py) * 8 + y8, z0 = blk(pz) * 8;
+      WT* dst = &S.w[p][x8 * 72 + y8 * 9];
+      if (gx < v && gy < v) {
+        const CCGPU_GLOBAL WT* src = W + gx * vv + gy * v + z0;
+        if (vec && z0 + 8 <= v) {
+          if (sizeof(WT) == 4) {
+            const f4 a = *reinterpret_cast<const CCGPU_GLOBAL f4*>(src);
+            const f4 b = *reinterpret_cast<const CCGPU_GLOBAL f4*>(src + 4);
+#pragma unroll
+            for (int h = 0; h < 4; ++h) {
+              dst[h] = (WT)a[h];
+              dst[4 + h] = (WT)b[h];
+            }
+          } else {
+#pragma unroll
+            for (int h = 0; h < 4; ++h) {
+              const d2 a = *reinterpret_cast<const CCGPU_GLOBAL d2*>(src + 2 * h);
+              dst[2 * h] = (WT)a[0];
+              dst[2 * h + 1] = (WT)a[1];
+            }
+          }
+        } else {
+#pragma unroll
+          for (int z = 0; z < 8; ++z) dst[z] = (z0 + z < v) ? src[z] : (WT)0;
+        }
+      } else {
+#pragma unroll
+        for (int z = 0; z < 8; ++z) dst[z] = (WT)0;
+      }
+    }
+    // (ia|jb) blocks: m = 0 J_jk[y][z], 1 J_ik[x][z], 2 J_ij[x][y]; pair (u, w) of positions
+    for (int q = tid; q < 18 * 64; q += 256) {
+      const int b = q >> 6, m = b / 6, pi = b - m * 6;
+      const int u = pi >> 1, wi = pi & 1, w = wi + (wi >= u ? 1 : 0);
+      const int rr = (q >> 3) & 7, cc = q & 7;
+      const int p1 = m == 0 ? j : i, p2 = m == 2 ? j : k;
+      const long row = blk(u) * 8 + rr, col = blk(w) * 8 + cc;
+      S.jb[b][rr * 8 + cc] = (row < v && col < v) ? J[((long)p1 * v + row) * nov + (long)p2 * v + col] : 0.0;
+    }
+    if (tid < 72) {
+      const int m = tid / 24, u = (tid / 8) % 3, x8 = tid & 7;
+      const long x = blk(u) * 8 + x8;
+      const int om = m == 0 ? i : (m == 1 ? j : k);
+      S.t1s[m][u][x8] = x < v ? t1[(long)om * v + x] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int el = tid + 256 * h;
+      const int l0 = el >> 6, l1 = (el >> 3) & 7, l2 = el & 7;
+      auto loc = [&](int u) { return u == 0 ? l0 : (u == 1 ? l1 : l2); };
+      const int a = b0 * 8 + l0, b = b1 * 8 + l1, c = b2 * 8 + l2;
+      if (!(a <= b && b <= c && c < v)) continue;
+      double w[6], y[6];
+#pragma unroll
+      for (int s = 0; s < 6; ++s) {
+        const int xl = loc(PX[s]), yl = loc(PY[s]), zl = loc(PZ[s]);
+        w[s] = (double)S.w[s][xl * 72 + yl * 9 + zl];
+        y[s] = S.t1s[0][PX[s]][xl] * S.jb[pair_ix(PY[s], PZ[s])][yl * 8 + zl] +
+               S.t1s[1][PY[s]][yl] * S.jb[6 + pair_ix(PX[s], PZ[s])][xl * 8 + zl] +
+               S.t1s[2][PZ[s]][zl] * S.jb[12 + pair_ix(PX[s], PY[s])][xl * 8 + yl];
+      }
+      // slots 0 abc, 1 acb, 2 bac, 3 bca, 4 cab, 5 cba; even class {0,3,4}, odd {1,2,5}
+      const double wE = w[0] + w[3] + w[4], wO = w[1] + w[2] + w[5];
+      const double yE = y[0] + y[3] + y[4], yO = y[1] + y[2] + y[5];
+      double ww = 0.0, wy = 0.0;
+#pragma unroll
+      for (int s = 0; s < 6; ++s) {
+        ww += w[s] * w[s];
+        wy += w[s] * y[s];
+      }
+      // sum over the 6 slots of W_p (4X_p + X_cyc1 + X_cyc2 - 2 X_swaps)
+      const double f4 = 3.0 * ww + 3.0 * (wE * wE + wO * wO) - 2.0 * (wE + wO) * (wE + wO);
+      const double f5 = 3.0 * wy + 3.0 * (wE * yE + wO * yO) - 2.0 * (wE + wO) * (yE + yO);
+      // each distinct tuple once: slots alias when indices coincide
+      const double wt = (a == b && b == c) ? (1.0 / 6.0) : ((a == b || b == c) ? 0.5 : 1.0);
+      const double D = eijk - eps[o + a] - eps[o + b] - eps[o + c];
+      s4 += wt * f4 / D;
+      s5 += wt * f5 / D;
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    s4 += __shfl_down(s4, off, 64);
+    s5 += __shfl_down(s5, off, 64);
+  }
+  const int lane = tid & 63, wv = tid >> 6;
+  __syncthreads();
+  if (lane == 0) {
+    S.red[0][wv] = s4;
+    S.red[1][wv] = s5;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    // (i == j or j == k) counts the triple 3 times, otherwise 6: scale mult / 3
+    const double scale = (i == j || j == k) ? 1.0 : 2.0;
+    part[2 * e] = scale * (((S.red[0][0] + S.red[0][1]) + S.red[0][2]) + S.red[0][3]);
+    part[2 * e + 1] = scale * (((S.red[1][0] + S.red[1][1]) + S.red[1][2]) + S.red[1][3]);
+  }
+}
+
+// (E4, E5) of one triple = sum of its nE partials in a fixed order.
+__device__ __forceinline__ void reduce_partials(const double* part, long nE, double* out, double (&red)[2][4]) {
+  const int tid = threadIdx.x;
+  double s4 = 0.0, s5 = 0.0;
+  for (long q = tid; q < nE; q += 256) {
+    s4 += part[2 * q];
+    s5 += part[2 * q + 1];
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    s4 += __shfl_down(s4, off, 64);
+    s5 += __shfl_down(s5, off, 64);
+  }
+  const int lane = tid & 63, wv = tid >> 6;
+  __syncthreads();
+  if (lane == 0) {
+    red[0][wv] = s4;
+    red[1][wv] = s5;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    out[0] = ((red[0][0] + red[0][1]) + red[0][2]) + red[0][3];
+    out[1] = ((red[1][0] + red[1][1]) + red[1][2]) + red[1][3];
+  }
+}
+
+template <bool MIXED>
+// The launch block is read through a pointer: its fields are re-read per
+// item instead of being held in registers across the GEMM loops.
+__global__ __launch_bounds__(256, 2) void k_pt_persistent(const PtRun* rp) {
+  const PtRun& r = *rp;
+  using WT = typename std::conditional<MIXED, float, double>::type;
+  constexpr size_t SG = MIXED ? sizeof(bf3::Smem<2>) : sizeof(dt::Smem);
+  constexpr size_t SE = sizeof(ESmem<WT>);
+  __shared__ __attribute__((aligned(16))) unsigned char lds[SG > SE ? SG : SE];
+  __shared__ long s_item;
+  __shared__ int s_flag;
+  ESmem<WT>& ES = *reinterpret_cast<ESmem<WT>*>(lds);
+  const long per_step = r.nE + r.ng[2] + r.ng[1] + r.ng[0];
+  const long total = (r.nq + 3) * per_step;
+  unsigned long long* claim = reinterpret_cast<unsigned long long*>(r.ctr);
+  int* err = r.ctr + 2;
+  int* done = r.ctr + 4;
+  const int tid = threadIdx.x;
+  for (;;) {
+    if (tid == 0) {
+      long it = -1;
+      if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0)
+        it = (long)__hip_atomic_fetch_add(claim, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      s_item = (it >= 0 && it < total) ? it : -1;
+    }
+    __syncthreads();
+    const long it = uniform64(s_item);
+    __syncthreads();
+    if (it < 0) break;
+    // step s: E(s-3) | G3(s-2) | G2(s-1) | G1(s)
+    const long step = it / per_step;
+    long rr = it - step * per_step;
+    int ph;
+    long q;
+    if (rr < r.nE) {
+      ph = 3;
+      q = step - 3;
+    } else if ((rr -= r.nE) < r.ng[2]) {
+      ph = 2;
+      q = step - 2;
+    } else if ((rr -= r.ng[2]) < r.ng[1]) {
+      ph = 1;
+      q = step - 1;
+    } else {
+      rr -= r.ng[1];
+      ph = 0;
+      q = step;
+    }
+    if (q < 0 || q >= r.nq) continue;
+    if (tid == 0) {
+      int ok = 1;
+      if (ph == 0) {
+        if (q >= 4) ok = wait_ge(&done[(q - 4) * 5 + 4], 1, err, r.spin_limit);
+      } else {
+        ok = wait_ge(&done[q * 5 + ph - 1], (int)r.ng[ph - 1], err, r.spin_limit);
+      }
+      // one acquire for the block: its waves share this CU's L1 and the XCD's L2
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+      s_flag = ok;
+    }
+    __syncthreads();
+    if (!__builtin_amdgcn_readfirstlane(s_flag)) break;
+    const int slot = (int)(q & 3);
+    WT* W = static_cast<WT*>(r.W) + slot * r.wslot;
+    if (ph < 3) {
+      const long tmn = r.tm[ph] * r.tn[ph];
+      const long z = rr / tmn;
+      long tmi, tni;
+      tile_grouped(rr - z * tmn, r.tm[ph], r.tn[ph], tmi, tni);
+      if constexpr (MIXED) {
+        const BMulti& d = static_cast<const BMulti*>(r.desc)[q * 3 + ph];
+        bf3::tile<float, 2>(d, tmi * 128, tni * 128, z, *reinterpret_cast<bf3::Smem<2>*>(lds));
+      } else {
+        const DMulti& d = static_cast<const DMulti*>(r.desc)[q * 3 + ph];
+        dt::Smem& GS = *reinterpret_cast<dt::Smem*>(lds);
+        if (ph < 2)
+          dt::tile4<true>(d, tmi * 128, tni * 128, z, GS);
+        else
+          dt::tile4<false>(d, tmi * 128, tni * 128, z, GS);
+      }
+      signal_done(&done[q * 5 + ph]);
+    } else {
+      const int* tq = r.tri + 3 * q;
+      double* part = r.part + slot * 2 * r.nE;
+      energy_item<WT>(r, W, tq[0], tq[1], tq[2], rr, part, ES);
+      const int old = signal_done(&done[q * 5 + 3]);
+      if (tid == 0) {
+        s_flag = old == (int)r.nE - 1;
+        if (s_flag) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+      }
+      __syncthreads();
+      if (__builtin_amdgcn_readfirstlane(s_flag)) {  // the last energy item of the triple
+        reduce_partials(part, r.nE, r.etrip + 2 * q, ES.red);
+        signal_done(&done[q * 5 + 4]);
+      }
+    }
+  }
+}
+
+__global__ void k_zero_int(int* p, long n) {
+  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) p[e] = 0;
+}
+
+// FP64 descriptor with every operand span checked.
+DMulti d_desc(std::initializer_list<DTerm> terms, long M, long N, int batch, double* C, long c_m, long c_n,
+              long c_b, double beta) {
+  DMulti p{};
+  CCGPU_REQUIRE(terms.size() >= 1 && terms.size() <= 4, "pt: 1..4 terms");
+  for (const DTerm& t : terms) {
+    check_span(t.A, 0, (M - 1) * t.a_m + (t.K - 1) * t.a_k + (batch - 1) * t.a_b, sizeof(double), "pt A");
+    check_span(t.B, 0, (N - 1) * t.b_n + (t.K - 1) * t.b_k + (batch - 1) * t.b_b, sizeof(double), "pt B");
+    p.t[p.nterm++] = t;
+  }
+  CCGPU_REQUIRE(p.nterm == 4 && p.t[0].akf == p.t[1].akf && p.t[2].akf == p.t[3].akf && p.t[0].akf != p.t[2].akf,
+                "pt: FP64 tiles take four terms in two read orders");
+  p.M = M;
+  p.N = N;
+  p.beta = beta;
+  p.C = C;
+  p.c_m = c_m;
+  p.c_n = c_n;
+  p.c_b = c_b;
+  check_span(C, 0, (M - 1) * c_m + (N - 1) * c_n + (batch - 1) * c_b, sizeof(double), "pt C");
+  return p;
+}
+
+struct Resident {
+  int blocks = 0;
+};
+
+template <bool MIXED>
+int resident_blocks() {
+  static Resident r;
+  if (r.blocks == 0) {
+    if (dryrun()) return 512;
+    int dev = 0, per = 0;
+    CCGPU_HIP(hipGetDevice(&dev));
+    hipDeviceProp_t pr;
+    CCGPU_HIP(hipGetDeviceProperties(&pr, dev));
+    CCGPU_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_pt_persistent<MIXED>, 256, 0));
+    r.blocks = std::max(1, per) * pr.multiProcessorCount;
+  }
+  return r.blocks;
+}
+
+// Ticks per millisecond of the device's constant-rate wall clock.
+long wall_clock_khz() {
+  static int khz = 0;
+  if (khz == 0) {
+    if (dryrun()) return 100000;
+    int dev = 0;
+    CCGPU_HIP(hipGetDevice(&dev));
+    CCGPU_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+    if (khz <= 0) khz = 100000;
+  }
+  return khz;
+}
+
+// Bytes reserved for the launch block at the head of the descriptor buffer.
+constexpr size_t kRunBytes = (sizeof(PtRun) + 255) / 256 * 256;
+
+long env_long(const char* name, long dflt) {
+  const char* e = std::getenv(name);
+  return (e && *e) ? std::atol(e) : dflt;
+}
+
+}  // namespace
+
+// Host side of the persistent (T): descriptors per triple, launches of at
+// most CCGPU_PT_CHUNK triples (default: about one second of work each).
+void CCSD::pt_energy_persistent(long first, long n, long stride, double* e4, double* e5) {
+  const bool mixed = pt_mode_ == GemmPrec::BF16X3;
+  const int o = dm_.o, v = dm_.v;
+  const long nov = dm_.nov, vv = (long)v * v, vvv = vv * v;
+  const long nb = cdiv(v, 8), nct = nb * (nb + 1) * (nb + 2) / 6;
+  const long nE = std::min<long>(nct, 2048);
+  PtRun r{};
+  r.W = mixed ? (void*)W32_.p : (void*)Wt_.p;
+  r.wslot = vvv;
+  r.part = ptpart_.p;
+  r.ctab = ptctab_.p;
+  r.J = J_.p;
+  r.t1 = t1_.p;
+  r.eps = eps_.p;
+  r.o = o;
+  r.v = v;
+  r.nE = nE;
+  r.nct = nct;
+  r.spin_limit = env_long("CCGPU_PT_WAIT_MS", 5000) * wall_clock_khz();
+  check_span(r.W, 0, 4 * vvv - 1, mixed ? sizeof(float) : sizeof(double), "pt W slots");
+  check_span(r.part, 0, 8 * nE - 1, sizeof(double), "pt partials");
+  check_span(r.ctab, 0, nct - 1, sizeof(int), "pt block table");
+  check_span(r.J, 0, dm_.nov2 - 1, sizeof(double), "pt J");
+  // tiles per pass: G1 v x v^2, G2 v batches of v x v, G3 v^2 x v
+  const long Ms[3] = {v, v, vv}, Ns[3] = {vv, v, v}, Bs[3] = {1, v, 1};
+  for (int g = 0; g < 3; ++g) {
+    r.tm[g] = mixed ? bf16x3_rpad(Ms[g]) / 128 : cdiv(Ms[g], 128);
+    r.tn[g] = mixed ? bf16x3_rpad(Ns[g]) / 128 : cdiv(Ns[g], 128);
+    r.ng[g] = r.tm[g] * r.tn[g] * Bs[g];
+  }
+  const double ftri = 12.0 * (double)vv * vv + 12.0 * (double)o * vvv;
+  long chunk = env_long("CCGPU_PT_CHUNK", 0);
+  if (chunk <= 0) chunk = std::min<long>(4096, std::max<long>(8, (long)((mixed ? 150e12 : 35e12) / ftri)));
+  CCGPU_REQUIRE((chunk + 3) * (r.nE + r.ng[0] + r.ng[1] + r.ng[2]) < (1L << 40), "pt: launch too large");
+  etrip_.ensure(2 * (size_t)std::max<long>(n, 1));
+  const size_t dsz = mixed ? sizeof(BMulti) : sizeof(DMulti);
+  const long cq = std::min(chunk, std::max<long>(n, 1));
+  ptdesc_.ensure(kRunBytes + 3 * cq * dsz);
+  pttri_.ensure(3 * (size_t)cq);
+  ptctr_.ensure(4 + 5 * (size_t)cq);
+  std::vector<unsigned char> hd(kRunBytes + 3 * cq * dsz);
+  std::vector<int> ht(3 * cq);
+  // mixed operands
+  const long Kpv = bf16x3_kpad(v), Kpo = bf16x3_kpad(o), Rv = bf16x3_rpad(v), Rvv = bf16x3_rpad(vv) + 256;
+  const long gk = vv * Kpv;
+  auto Tm = [&](int p, int q, bool hi) { return (hi ? ppH_[0].p : ppL_[0].p) + ((long)p * o + q) * Rv * Kpv; };
+  auto Qm = [&](int p, int q, bool hi) { return (hi ? ppH_[1].p : ppL_[1].p) + ((long)p * o + q) * Rv * Kpo; };
+  auto Pm = [&](int p, bool hi) { return (hi ? ppH_[2].p : ppL_[2].p) + (long)p * Rvv * Kpo; };
+  auto Sm = [&](int p, bool hi) { return (hi ? ppH_[3].p : ppL_[3].p) + (long)p * Rvv * Kpo; };
+  auto TT = [&](int p, int q, const unsigned short* Bh, const unsigned short* Bl, long bbs) {  // T(p,q) x G slice
+    return BTerm{Tm(p, q, true), Tm(p, q, false), Bh, Bl, Kpv, 0, bbs};
+  };
+  auto QP = [&](int p, int q, const unsigned short* Bh, const unsigned short* Bl, long bbs) {  // Q_pq x Tvvo
+    return BTerm{Qm(p, q, true), Qm(p, q, false), Bh, Bl, Kpo, 0, bbs};
+  };
+  // FP64 operands
+  auto Td = [&](int p, int q) { return T_.p + (long)p * v * nov + (long)q * v; };  // [x][d], row stride nov
+  auto Qd = [&](int p, int q) { return Q_.p + ((long)p * o + q) * o * v; };       // [l][z]
+  auto Pd = [&](int p) { return Tvvo_.p + (long)p * vv * o; };                    // [x][y][l]
+  auto Sd = [&](int p) { return TvvoS_.p + (long)p * vv * o; };                   // [x][y][l] = Tvvo_p[y][x][l]
+  double s4 = 0.0, s5 = 0.0;
+  for (long c0 = 0; c0 < n; c0 += chunk) {
+    const long nq = std::min(chunk, n - c0);
+    for (long q = 0; q < nq; ++q) {
+      const long t = first + (c0 + q) * stride;
+      const int i = tri_i_[t], j = tri_j_[t], k = tri_k_[t];
+      ht[3 * q] = i;
+      ht[3 * q + 1] = j;
+      ht[3 * q + 2] = k;
+      const int slot = (int)(q & 3);
+      if (mixed) {
+        float* Wq = W32_.p + slot * vvv;
+        const unsigned short *GTk = GTh_.p + k * gk, *GTkl = GTl_.p + k * gk;
+        BMulti* d = reinterpret_cast<BMulti*>(hd.data() + kRunBytes) + 3 * q;
+        // G1: W[a][(b,c)] = t_ij^{ad} (bd|ck) + t_ik^{ad} (cd|bj) - (ai|lk) t_jl^{bc} - (ai|lj) t_kl^{cb}
+        const BTerm g1[4] = {TT(i, j, GTk, GTkl, 0), TT(i, k, GpTh_.p + j * gk, GpTl_.p + j * gk, 0),
+                             QP(i, k, Pm(j, true), Pm(j, false), 0), QP(i, j, Sm(k, true), Sm(k, false), 0)};
+        d[0] = bf16x3_desc(g1, 4, v, vv, 1, Wq, true, vv, 1, 0, 1.0, 0.0);
+        // G2 (batch a): W[a][b][c] += t_ji^{bd} (ad|ck) + t_jk^{bd} (cd|ai) - t_il^{ac} (bj|lk) - t_kl^{ca} (bj|li)
+        const BTerm g2[4] = {TT(j, i, GTk, GTkl, v * Kpv), TT(j, k, GpTh_.p + i * gk, GpTl_.p + i * gk, v * Kpv),
+                             QP(j, k, Pm(i, true), Pm(i, false), v * Kpo),
+                             QP(j, i, Sm(k, true), Sm(k, false), v * Kpo)};
+        d[1] = bf16x3_desc(g2, 4, v, v, v, Wq, true, v, 1, vv, 1.0, 1.0);
+        // G3: W[(a,b)][c] += (ad|bj) t_ki^{cd} + (bd|ai) t_kj^{cd} - t_il^{ab} (ck|lj) - t_jl^{ba} (ck|li)
+        const BTerm g3[4] = {
+            BTerm{GTh_.p + j * gk, GTl_.p + j * gk, Tm(k, i, true), Tm(k, i, false), Kpv, 0, 0},
+            BTerm{GpTh_.p + i * gk, GpTl_.p + i * gk, Tm(k, j, true), Tm(k, j, false), Kpv, 0, 0},
+            BTerm{Pm(i, true), Pm(i, false), Qm(k, j, true), Qm(k, j, false), Kpo, 0, 0},
+            BTerm{Sm(j, true), Sm(j, false), Qm(k, i, true), Qm(k, i, false), Kpo, 0, 0}};
+        d[2] = bf16x3_desc(g3, 4, vv, v, 1, Wq, true, v, 1, 0, 1.0, 1.0);
+      } else {
+        double* Wq = Wt_.p + slot * vvv;
+        const double* Gk = G_.p + (long)k * vvv;
+        DMulti* d = reinterpret_cast<DMulti*>(hd.data() + kRunBytes) + 3 * q;
+        // G1 (A along k, B along n for the G terms; A along m, B along k for the occupied ones)
+        d[0] = d_desc({DTerm{Td(i, j), Gk, nov, 1, 0, 1, vv, 0, v, 1.0, 1},
+                       DTerm{Td(i, k), Gp_.p + (long)j * vvv, nov, 1, 0, 1, vv, 0, v, 1.0, 1},
+                       DTerm{Qd(i, k), Pd(j), 1, v, 0, o, 1, 0, o, -1.0, 0},
+                       DTerm{Qd(i, j), Sd(k), 1, v, 0, o, 1, 0, o, -1.0, 0}},
+                      v, vv, 1, Wq, vv, 1, 0, 0.0);
+        // G2 (batch a)
+        d[1] = d_desc({DTerm{Td(j, i), Gk, nov, 1, 0, 1, vv, v, v, 1.0, 1},
+                       DTerm{Td(j, k), Gp_.p + (long)i * vvv, nov, 1, 0, 1, vv, v, v, 1.0, 1},
+                       DTerm{Qd(j, k), Pd(i), 1, v, 0, o, 1, (long)v * o, o, -1.0, 0},
+                       DTerm{Qd(j, i), Sd(k), 1, v, 0, o, 1, (long)v * o, o, -1.0, 0}},
+                      v, v, v, Wq, v, 1, vv, 1.0);
+        // G3
+        d[2] = d_desc({DTerm{G_.p + (long)j * vvv, Td(k, i), 1, vv, 0, nov, 1, 0, v, 1.0, 0},
+                       DTerm{Gp_.p + (long)i * vvv, Td(k, j), 1, vv, 0, nov, 1, 0, v, 1.0, 0},
+                       DTerm{Pd(i), Qd(k, j), o, 1, 0, 1, v, 0, o, -1.0, 1},
+                       DTerm{Sd(j), Qd(k, i), o, 1, 0, 1, v, 0, o, -1.0, 1}},
+                      vv, v, 1, Wq, v, 1, 0, 1.0);
+      }
+    }
+    r.nq = nq;
+    r.desc = ptdesc_.p + kRunBytes;
+    r.tri = pttri_.p;
+    r.ctr = ptctr_.p;
+    r.etrip = etrip_.p + 2 * c0;
+    check_span(r.etrip, 0, 2 * nq - 1, sizeof(double), "pt energies");
+    check_span(r.ctr, 0, 4 + 5 * nq - 1, sizeof(int), "pt counters");
+    std::memcpy(hd.data(), &r, sizeof(PtRun));
+    CCGPU_HIPX(hipMemcpyAsync(ptdesc_.p, hd.data(), kRunBytes + 3 * nq * dsz, hipMemcpyHostToDevice, st_));
+    CCGPU_HIPX(hipMemcpyAsync(pttri_.p, ht.data(), 3 * nq * sizeof(int), hipMemcpyHostToDevice, st_));
+    CCGPU_LAUNCH(k_zero_int, dim3(cdiv(4 + 5 * nq, 256)), dim3(256), 0, st_, ptctr_.p, 4 + 5 * nq);
+    const PtRun* run = reinterpret_cast<const PtRun*>(ptdesc_.p);
+    const long total = (nq + 3) * (r.nE + r.ng[0] + r.ng[1] + r.ng[2]);
+    // at most one pipeline step of items in flight: fewer blocks polling on small systems
+    const long grid = std::min<long>(std::min<long>(total, r.nE + r.ng[0] + r.ng[1] + r.ng[2]),
+                                     mixed ? resident_blocks<true>() : resident_blocks<false>());
+    const int id = prof_begin(mixed ? "k_pt_persistent<true>" : "k_pt_persistent<false>", (double)nq * ftri, st_);
+    if (mixed)
+      CCGPU_LAUNCH(k_pt_persistent<true>, dim3((unsigned)grid), dim3(256), 0, st_, run);
+    else
+      CCGPU_LAUNCH(k_pt_persistent<false>, dim3((unsigned)grid), dim3(256), 0, st_, run);
+    CCGPU_HIPX(hipGetLastError());
+    prof_end(id, st_);
+    int flag = 0;
+    CCGPU_HIPX(hipMemcpyAsync(&flag, ptctr_.p + 2, sizeof(int), hipMemcpyDeviceToHost, st_));
+    CCGPU_HIPX(hipStreamSynchronize(st_));
+    if (flag != 0)
+      throw Error(2, "pt: a dependency wait of the persistent (T) kernel exceeded its time limit");
+  }
+  if (n > 0) {
+    std::vector<double> et(2 * n, 0.0);
+    CCGPU_HIPX(hipMemcpyAsync(et.data(), etrip_.p, 2 * n * sizeof(double), hipMemcpyDeviceToHost, st_));
+    CCGPU_HIPX(hipStreamSynchronize(st_));
+    for (long q = 0; q < n; ++q) {
+      s4 += et[2 * q];
+      s5 += et[2 * q + 1];
+    }
+  }
+  if (e4) *e4 = s4;
+  if (e5) *e5 = s5;
+}
+
+}  // namespace ccgpu

@@ -39,6 +39,8 @@ for (o, v, naux) in {shapes!r}:
     assert rc == 0, L.ccgpu_last_error()
     rc = L.ccgpu_pt_partial(h, 0, max(0, nt.value - 2), 2, 1, C.byref(e), C.byref(n), C.byref(fl), C.byref(nt))
     assert rc == 0, L.ccgpu_last_error()
+    rc = L.ccgpu_pt_partial(h, 0, 1, 9, 2, C.byref(e), C.byref(n), C.byref(fl), C.byref(nt))  # W slots wrap
+    assert rc == 0, L.ccgpu_last_error()
     rc = L.ccgpu_ccsd_iterate(h, 0, C.byref(e), C.byref(n))  # workspace comes back after (T)
     assert rc == 0, L.ccgpu_last_error()
     rc = L.ccgpu_ccsd_iterate(h, 1, C.byref(e), C.byref(n))  # mixed-precision iteration
@@ -56,14 +58,16 @@ for (o, v, naux) in {shapes!r}:
 """
 
 
+@pytest.mark.parametrize("persistent", ["1", "0"])
 @pytest.mark.parametrize("shapes", [
     [(5, 19, 280), (5, 14, 172), (1, 1, 1), (2, 1, 3), (6, 40, 90), (3, 200, 10), (1, 6, 4), (4, 2, 7),
      (3, 5, 2)],
     [(32, 256, 1024)],
     [(64, 512, 2048)],
 ])
-def test_dryrun_bounds(shapes):
-    env = dict(os.environ, CCGPU_DRYRUN="1")
+def test_dryrun_bounds(shapes, persistent):
+    """Both (T) drivers: the persistent kernel and the per-triple launches."""
+    env = dict(os.environ, CCGPU_DRYRUN="1", CCGPU_PT_PERSISTENT=persistent)
     r = subprocess.run([sys.executable, "-c", SCRIPT.format(root=ROOT, shapes=shapes)],
                        env=env, capture_output=True, text=True, timeout=600)
     assert r.returncode == 0, r.stdout + r.stderr

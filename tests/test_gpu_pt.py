@@ -49,3 +49,35 @@ def test_pt_matches_oracle_random_amplitudes():
     assert nt == o * (o + 1) * (o + 2) // 6 - o
     assert abs(a4 + b4 - e4) < 1e-12 * max(1.0, abs(e4)) and abs(a5 + b5 - e5) < 1e-12 * max(1.0, abs(e5))
     assert fa + fb == pytest.approx(nt * (12.0 * v**4 + 12.0 * o * v**3))
+
+
+@pytest.mark.skip(reason="persistent (T) kernel switched off: its first GPU run (H2O, FP64) did not finish; "
+                         "cause not yet found, so it is not run on the GPU")
+@pytest.mark.parametrize("precision", [0, 1])
+def test_persistent_kernel_matches_per_triple_launches(precision, monkeypatch):
+    """The persistent (T) kernel (work queue over the triples, blocked energy
+    reads) against the per-triple launches, in launches of 5 triples (so the
+    four W slots wrap and several launches add up), with v = 136: edge tiles
+    in every pass and a partial last 8-block."""
+    import dalton_amd as d
+    from dalton_amd.synthetic import make_system
+
+    o, v, naux = 7, 136, 200
+    eps, B = make_system(o, v, naux, seed=8)
+    rng = np.random.default_rng(3)
+    t1 = 0.03 * rng.standard_normal((v, o))
+    t2 = 0.03 * rng.standard_normal((v, o, v, o))
+    t2 = t2 + t2.transpose(2, 3, 0, 1)
+    res = {}
+    for persistent in ("0", "1"):
+        monkeypatch.setenv("CCGPU_PT_PERSISTENT", persistent)
+        monkeypatch.setenv("CCGPU_PT_CHUNK", "5")
+        print(f"(T) precision {precision} persistent {persistent} o={o} v={v}", flush=True)
+        with d.CCContext(eps, B, o) as c:
+            c.set_amplitudes(occ.pack_t1(t1), occ.pack_t2(t2))
+            full = c.pt(precision=precision)
+            part = c.pt_partial(3, 11, 4, precision=precision)[:2]
+        res[persistent] = (full, part)
+    for a, b in zip(res["0"], res["1"]):
+        for x, y in zip(a, b):
+            assert abs(x - y) < 1e-11 * max(1.0, abs(x)), (x, y)
