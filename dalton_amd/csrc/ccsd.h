@@ -87,9 +87,16 @@ class CCSD {
   void ladder_mixed(double* Om);
   void mark(const char* name);
   void ensure_work();
-  void pt_prepare(GemmPrec mode, bool persistent);
+  // (T) drivers: per-triple launches of the pre-packed three-pass W build and
+  // blocked energy (default); the same as one persistent kernel; the first
+  // per-triple driver (twelve GEMMs, per-triple packing, transposed W copies)
+  enum class PtPath { Launch = 0, Persistent = 1, Legacy = 2 };
+  void pt_prepare(GemmPrec mode, PtPath path);
   void pt_triples_mixed(long first, long n, long stride);
   void pt_energy_persistent(long first, long n, long stride, double* e4, double* e5);
+  void pt_energy_launches(long first, long n, long stride, double* e4, double* e5);
+  void pt_descs(int i, int j, int k, int slot, unsigned char* out);
+  size_t pt_desc_bytes() const;
   void ensure_diis();
   void reset_cycle();
   void ensure_lhtr();
@@ -136,7 +143,7 @@ class CCSD {
   // (T), persistent kernel: bf16 hi/lo of every T(p,q) [x][d] and Q_rq [z][l]
   // slice and of -Tvvo_p in both (x,y) orders (mixed), Tvvo with (x,y)
   // swapped (FP64); four W slots (in W32_ / Wt_); launch tables and counters
-  bool pt_persist_ = true;
+  PtPath pt_path_ = PtPath::Launch;
   DBuf<unsigned short> ppH_[4], ppL_[4];
   DBuf<double> TvvoS_;
   DBuf<int> ptctr_, pttri_, ptctab_;

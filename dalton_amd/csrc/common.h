@@ -176,6 +176,8 @@ void bf16x3_multi(const BTerm* terms, int nterm, long M, long N, int batch, void
 // of 128 over Mp = bf16x3_rpad(M) rows of A and Np rows of B per batch).
 BMulti bf16x3_desc(const BTerm* terms, int nterm, long M, long N, int batch, void* C, bool c_float, long c_m,
                    long c_n, long c_b, double alpha, double beta);
+// Launch a product described by bf16x3_desc (batch as given there).
+void bf16x3_run(const BMulti& p, int batch, bool c_float, double flops, hipStream_t s);
 
 // V+/- of one particle-ladder a-batch from an FP32 Y block straight into the
 // split-bf16 operand layout hi/lo[2][Rp][Kp] (V+ then V-), zero padded.

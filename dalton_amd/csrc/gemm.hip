@@ -349,7 +349,7 @@ void run_f64(const GemmDesc& d, const F64Cfg& cfg, int am, int bm, hipStream_t s
     }
   }
   CCGPU_REQUIRE(nz <= 65535, "gemm: too many batches");
-  CCGPU_REQUIRE(tiles < (1L << 31), "gemm: too many tiles");
+  CCGPU_REQUIRE(tiles < (1L << 24), "gemm: too many tiles");
   dim3 grid((unsigned)tiles, 1, nz);
 #define CCGPU_F64_LAUNCH(TT)                                                   \
   if (am == KFAST && bm == KFAST) launch_f64<KFAST, KFAST, TT>(d, grid, kchunk, ws, s); \

@@ -369,19 +369,19 @@ void launch_multi(BMulti p, int batch, hipStream_t s) {
   CCGPU_REQUIRE(batch < 65536, "bf16x3 gemm: grid too large");
   if (var == 3 && p.Mp % DT == 0 && p.Np % DT == 0) {
     const long ntiles = (p.Mp / DT) * (p.Np / DT);
-    CCGPU_REQUIRE(ntiles < (1L << 31), "bf16x3 gemm: grid too large");
+    CCGPU_REQUIRE(ntiles < (1L << 23), "bf16x3 gemm: grid too large");
     CCGPU_LAUNCH(k_gemm_bf16x3_dl<OutT>, dim3((unsigned)ntiles, 1, (unsigned)batch), dim3(256), 0, s, p);
   } else if (var == 2 && p.Mp % BT == 0 && p.Np % BT == 0) {
     const long ntiles = (p.Mp / BT) * (p.Np / BT);
-    CCGPU_REQUIRE(ntiles < (1L << 31), "bf16x3 gemm: grid too large");
+    CCGPU_REQUIRE(ntiles < (1L << 23), "bf16x3 gemm: grid too large");
     CCGPU_LAUNCH(k_gemm_bf16x3_256<OutT>, dim3((unsigned)ntiles, 1, (unsigned)batch), dim3(256), 0, s, p);
   } else if (var == 1 && p.Mp % 256 == 0) {
     const long ntiles = (p.Mp / 256) * (p.Np / TN);
-    CCGPU_REQUIRE(ntiles < (1L << 31), "bf16x3 gemm: grid too large");
+    CCGPU_REQUIRE(ntiles < (1L << 23), "bf16x3 gemm: grid too large");
     CCGPU_LAUNCH((k_gemm_bf16x3<OutT, 4>), dim3((unsigned)ntiles, 1, (unsigned)batch), dim3(512), 0, s, p);
   } else {
     const long ntiles = (p.Mp / TM) * (p.Np / TN);
-    CCGPU_REQUIRE(ntiles < (1L << 31), "bf16x3 gemm: grid too large");
+    CCGPU_REQUIRE(ntiles < (1L << 23), "bf16x3 gemm: grid too large");
     CCGPU_LAUNCH((k_gemm_bf16x3<OutT, 2>), dim3((unsigned)ntiles, 1, (unsigned)batch), dim3(256), 0, s, p);
   }
   CCGPU_HIPX(hipGetLastError());
@@ -406,7 +406,7 @@ void bf16x3_pack(const double* src, long rows, long K, long s_r, long s_k, long 
   check_span(hi, 0, (long)batch * Rp * Kp - 1, 2, "bf16x3 pack hi");
   check_span(lo, 0, (long)batch * Rp * Kp - 1, 2, "bf16x3 pack lo");
   const long rt = (Rp + 31) / 32, kt = (Kp + 31) / 32;
-  CCGPU_REQUIRE(kt < 65536 && batch < 65536 && rt < (1L << 31), "bf16x3 pack: grid too large");
+  CCGPU_REQUIRE(kt < 65536 && batch < 65536 && rt < (1L << 27), "bf16x3 pack: grid too large");
   dim3 grid((unsigned)rt, (unsigned)kt, (unsigned)batch);
   const bool rfast = s_r == 1 && s_k != 1;
   if (rfast)
@@ -445,6 +445,10 @@ BMulti bf16x3_desc(const BTerm* terms, int nterm, long M, long N, int batch, voi
 void bf16x3_multi(const BTerm* terms, int nterm, long M, long N, int batch, void* C, bool c_float, long c_m,
                   long c_n, long c_b, double alpha, double beta, double flops, hipStream_t s) {
   const BMulti p = bf16x3_desc(terms, nterm, M, N, batch, C, c_float, c_m, c_n, c_b, alpha, beta);
+  bf16x3_run(p, batch, c_float, flops, s);
+}
+
+void bf16x3_run(const BMulti& p, int batch, bool c_float, double flops, hipStream_t s) {
   const int id = prof_begin("k_gemm_bf16x3", flops, s);
   if (c_float)
     launch_multi<float>(p, batch, s);

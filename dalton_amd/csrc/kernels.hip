@@ -1,4 +1,5 @@
 // Bandwidth-bound kernels of the CC path (see kernels.h).
+#include <algorithm>
 #include <cmath>
 
 #include "kernels.h"
@@ -12,57 +13,67 @@ struct P4 {
   long os[4];  // output strides (row-major)
   int q, r1, r2;
   long nt3, ntq;
+  long nb;  // logical blocks; launches cap the grid and stride over them
 };
+
+// At most this many blocks per launch: the work-items of one dimension must
+// stay below 2^32, so large permutations stride over their logical blocks.
+constexpr long kPermGrid = 1L << 20;
 
 __global__ void k_perm_direct(double* __restrict__ out, const double* __restrict__ in, P4 p,
                               double alpha, double beta) {
-  long bx = blockIdx.x;
-  const long t3 = bx % p.nt3;
-  bx /= p.nt3;
-  const long j2 = bx % p.n[2];
-  bx /= p.n[2];
-  const long j1 = bx % p.n[1];
-  const long j0 = bx / p.n[1];
-  const long j3 = t3 * 256 + threadIdx.x;
-  if (j3 >= p.n[3]) return;
-  const double v = in[j0 * p.s[0] + j1 * p.s[1] + j2 * p.s[2] + j3 * p.s[3]];
-  double* o = out + ((j0 * p.n[1] + j1) * p.n[2] + j2) * p.n[3] + j3;
-  double r = alpha * v;
-  if (beta != 0.0) r += beta * *o;
-  *o = r;
+  for (long b0 = blockIdx.x; b0 < p.nb; b0 += gridDim.x) {
+    long bx = b0;
+    const long t3 = bx % p.nt3;
+    bx /= p.nt3;
+    const long j2 = bx % p.n[2];
+    bx /= p.n[2];
+    const long j1 = bx % p.n[1];
+    const long j0 = bx / p.n[1];
+    const long j3 = t3 * 256 + threadIdx.x;
+    if (j3 >= p.n[3]) continue;
+    const double v = in[j0 * p.s[0] + j1 * p.s[1] + j2 * p.s[2] + j3 * p.s[3]];
+    double* o = out + ((j0 * p.n[1] + j1) * p.n[2] + j2) * p.n[3] + j3;
+    double r = alpha * v;
+    if (beta != 0.0) r += beta * *o;
+    *o = r;
+  }
 }
 
 // 32x32 LDS transpose over (output-fastest dim 3, input-stride-1 dim q).
 __global__ void k_perm_tiled(double* __restrict__ out, const double* __restrict__ in, P4 p,
                              double alpha, double beta) {
   __shared__ double tile[32][33];
-  long bx = blockIdx.x;
-  const long t3 = bx % p.nt3;
-  bx /= p.nt3;
-  const long tq = bx % p.ntq;
-  bx /= p.ntq;
-  const long jr2 = bx % p.n[p.r2];
-  const long jr1 = bx / p.n[p.r2];
-  const long ib = jr1 * p.s[p.r1] + jr2 * p.s[p.r2];
-  const long ob = jr1 * p.os[p.r1] + jr2 * p.os[p.r2];
   const int tx = threadIdx.x, ty = threadIdx.y;
-  const long j30 = t3 * 32, jq0 = tq * 32;
   const long nq = p.n[p.q], n3 = p.n[3];
+  for (long b0 = blockIdx.x; b0 < p.nb; b0 += gridDim.x) {
+    long bx = b0;
+    const long t3 = bx % p.nt3;
+    bx /= p.nt3;
+    const long tq = bx % p.ntq;
+    bx /= p.ntq;
+    const long jr2 = bx % p.n[p.r2];
+    const long jr1 = bx / p.n[p.r2];
+    const long ib = jr1 * p.s[p.r1] + jr2 * p.s[p.r2];
+    const long ob = jr1 * p.os[p.r1] + jr2 * p.os[p.r2];
+    const long j30 = t3 * 32, jq0 = tq * 32;
 #pragma unroll
-  for (int k = 0; k < 32; k += 8) {
-    const long jq = jq0 + tx, j3 = j30 + ty + k;
-    if (jq < nq && j3 < n3) tile[ty + k][tx] = in[ib + jq + j3 * p.s[3]];
-  }
-  __syncthreads();
-#pragma unroll
-  for (int k = 0; k < 32; k += 8) {
-    const long j3 = j30 + tx, jq = jq0 + ty + k;
-    if (jq < nq && j3 < n3) {
-      double* o = out + ob + jq * p.os[p.q] + j3;
-      double r = alpha * tile[tx][ty + k];
-      if (beta != 0.0) r += beta * *o;
-      *o = r;
+    for (int k = 0; k < 32; k += 8) {
+      const long jq = jq0 + tx, j3 = j30 + ty + k;
+      if (jq < nq && j3 < n3) tile[ty + k][tx] = in[ib + jq + j3 * p.s[3]];
     }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 32; k += 8) {
+      const long j3 = j30 + tx, jq = jq0 + ty + k;
+      if (jq < nq && j3 < n3) {
+        double* o = out + ob + jq * p.os[p.q] + j3;
+        double r = alpha * tile[tx][ty + k];
+        if (beta != 0.0) r += beta * *o;
+        *o = r;
+      }
+    }
+    __syncthreads();  // the tile is refilled for the next logical block
   }
 }
 
@@ -95,9 +106,9 @@ void permute4(double* out, const double* in, const long n[4], const long s[4], d
       if (s[i] == 1 && n[i] > 1) { q = i; break; }
   if (q < 0) {
     p.nt3 = (n[3] + 255) / 256;
-    const long nb = p.nt3 * n[0] * n[1] * n[2];
-    CCGPU_REQUIRE(nb < (1L << 31), "permute4: grid too large");
-    CCGPU_LAUNCH(k_perm_direct, dim3((unsigned)nb), dim3(256), 0, st, out, in, p, alpha, beta);
+    p.nb = p.nt3 * n[0] * n[1] * n[2];
+    CCGPU_LAUNCH(k_perm_direct, dim3((unsigned)std::min(p.nb, kPermGrid)), dim3(256), 0, st, out, in, p, alpha,
+                 beta);
   } else {
     p.q = q;
     int others[2], k = 0;
@@ -107,9 +118,9 @@ void permute4(double* out, const double* in, const long n[4], const long s[4], d
     p.r2 = others[1];
     p.nt3 = (n[3] + 31) / 32;
     p.ntq = (n[q] + 31) / 32;
-    const long nb = p.nt3 * p.ntq * n[p.r1] * n[p.r2];
-    CCGPU_REQUIRE(nb < (1L << 31), "permute4: grid too large");
-    CCGPU_LAUNCH(k_perm_tiled, dim3((unsigned)nb), dim3(32, 8), 0, st, out, in, p, alpha, beta);
+    p.nb = p.nt3 * p.ntq * n[p.r1] * n[p.r2];
+    CCGPU_LAUNCH(k_perm_tiled, dim3((unsigned)std::min(p.nb, kPermGrid)), dim3(32, 8), 0, st, out, in, p, alpha,
+                 beta);
   }
   CCGPU_HIPX(hipGetLastError());
 }

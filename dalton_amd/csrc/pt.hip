@@ -17,6 +17,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
+#include <string>
 
 #include "ccsd.h"
 #include "kernels.h"
@@ -174,6 +175,11 @@ __global__ void k_pt_reduce(const double* __restrict__ part, long n, double* out
   }
 }
 
+// p[0..n) = 0 (grid-stride; sizes past 2^31 bytes)
+__global__ void k_zero_u16(unsigned short* p, long n) {
+  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) p[e] = 0;
+}
+
 }  // namespace
 
 long CCSD::pt_ntriples() const {
@@ -201,8 +207,9 @@ void CCSD::pt_release() {
 // persistent kernel takes every per-triple operand pre-packed and four W
 // slots; the per-triple launches pack per triple and keep one W with its two
 // re-ordered copies.
-void CCSD::pt_prepare(GemmPrec mode, bool persistent) {
-  if (pt_ready_ && pt_mode_ == mode && pt_persist_ == persistent) return;
+void CCSD::pt_prepare(GemmPrec mode, PtPath path) {
+  if (pt_ready_ && pt_mode_ == mode && pt_path_ == path) return;
+  const bool persistent = path != PtPath::Legacy;  // pre-packed operands, W slots
   pt_release();
   release_work();
   const int o = dm_.o, v = dm_.v, naux = dm_.naux;
@@ -281,8 +288,8 @@ void CCSD::pt_prepare(GemmPrec mode, bool persistent) {
         ppL_[q].alloc(nn[q]);
       }
       // slot tails past v*v rows of the swapped -Tvvo are read by edge tiles only: keep them finite
-      CCGPU_HIPX(hipMemsetAsync(ppH_[3].p, 0, ppH_[3].bytes(), st_));
-      CCGPU_HIPX(hipMemsetAsync(ppL_[3].p, 0, ppL_[3].bytes(), st_));
+      for (DBuf<unsigned short>* b : {&ppH_[3], &ppL_[3]})
+        CCGPU_LAUNCH(k_zero_u16, dim3(4096), dim3(256), 0, st_, b->p, (long)b->n);
       for (int p = 0; p < o; ++p) {
         // T(p,q)[x][d] = t_pq^{xd} for every q: [(p,q)][x][d]
         const size_t ot = (size_t)p * o * Rv * Kpv;
@@ -321,7 +328,7 @@ void CCSD::pt_prepare(GemmPrec mode, bool persistent) {
         tri_k_.push_back(k);
       }
   pt_mode_ = mode;
-  pt_persist_ = persistent;
+  pt_path_ = path;
   pt_ready_ = true;
 }
 
@@ -387,19 +394,21 @@ void CCSD::pt_triples_mixed(long t, long, long) {
   bf16x3_multi(g3, 4, vv, v, 1, W, true, v, 1, 0, 1.0, 1.0, 2 * fv + 2 * fo, st_);
 }
 
-// (T) runs as per-triple launches; the persistent kernel only with
-// CCGPU_PT_PERSISTENT=1.  It is switched off: on an MI355X its first run
-// (H2O, o=5 v=19, FP64) did not finish, with or without the bounded waits,
-// and the cause is not yet known.
-static bool pt_use_persistent() {
-  const char* e = std::getenv("CCGPU_PT_PERSISTENT");
-  return e && e[0] == '1';
+// (T) driver from CCGPU_PT_PATH: "launch" (default), "legacy", or
+// "persistent".  The persistent kernel is switched off unless asked for by
+// name: on an MI355X its first run (H2O, o=5 v=19, FP64) did not finish,
+// with or without the bounded waits, and the cause is not yet known.
+static int pt_path_from_env() {
+  const char* e = std::getenv("CCGPU_PT_PATH");
+  if (e && std::string(e) == "persistent") return 1;
+  if (e && std::string(e) == "legacy") return 2;
+  return 0;
 }
 
 void CCSD::pt_energy(long first, long count, long stride, double* e4, double* e5, double* flops) {
   const bool mixed = bulk_ == GemmPrec::BF16X3;
-  const bool persistent = pt_use_persistent();
-  pt_prepare(mixed ? GemmPrec::BF16X3 : GemmPrec::F64, persistent);
+  const PtPath path = static_cast<PtPath>(pt_path_from_env());
+  pt_prepare(mixed ? GemmPrec::BF16X3 : GemmPrec::F64, path);
   const int o = dm_.o, v = dm_.v;
   const long nov = dm_.nov, vv = (long)v * v, vvv = vv * v, vo = (long)v * o;
   const long ntr = pt_ntriples();
@@ -407,8 +416,12 @@ void CCSD::pt_energy(long first, long count, long stride, double* e4, double* e5
   long n = 0;
   for (long t = first; t < ntr && n < count; t += stride) ++n;
   if (flops) *flops = (double)n * (12.0 * (double)vv * vv + 12.0 * (double)o * vvv);
-  if (persistent) {
+  if (path == PtPath::Persistent) {
     pt_energy_persistent(first, n, stride, e4, e5);
+    return;
+  }
+  if (path == PtPath::Launch) {
+    pt_energy_launches(first, n, stride, e4, e5);
     return;
   }
   etrip_.ensure(2 * (size_t)std::max<long>(n, 1));

@@ -216,6 +216,16 @@ __device__ __forceinline__ void tile4(const DMulti& p, long m0, long n0, long z,
 }  // namespace dt
 
 // ------------------------------------------------------------ energy items
+// Inputs of the energy evaluation of one triple.
+struct EArgs {
+  const int* ctab;  // canonical blocks A | B << 10 | C << 20
+  const double* J;   // (ia|jb) as [(i,a)][(j,b)]
+  const double* t1;  // [i][a]
+  const double* eps;  // occupied first
+  int o, v;
+  long nE, nct;  // energy items (blocks) and canonical blocks
+};
+
 template <typename WT>
 struct __attribute__((aligned(16))) ESmem {
   WT w[6][8 * 72];     // block permutation p of W: [x][y][z] at x*72 + y*9 + z
@@ -239,12 +249,8 @@ struct PtRun {
   double* etrip;  // (E4, E5) per triple of the launch
   int* ctr;       // [0,1] claim counter (64 bit), [2] error flag, [4 + 5q + phase] completions
   const int* tri;  // (i, j, k) per triple
-  const int* ctab;  // canonical blocks A | B << 10 | C << 20
-  const double* J;
-  const double* t1;
-  const double* eps;
-  int o, v;
-  long nq, nE, nct;
+  EArgs e;
+  long nq;
   long ng[3];        // items of G1, G2, G3
   long tm[3], tn[3];  // tiles per batch of G1, G2, G3 (m, n)
   long spin_limit;  // wall-clock ticks a dependency wait may take before the error flag is raised
@@ -302,7 +308,7 @@ __device__ __forceinline__ int signal_done(int* p) {
 // Energy item e of triple (i, j, k): canonical blocks e, e + nE, ...; writes
 // its scaled (E4, E5) partial to part[2e], part[2e + 1].
 template <typename WT>
-__device__ __forceinline__ void energy_item(const PtRun& r, const WT* W_, int i, int j, int k, long e, double* part,
+__device__ __forceinline__ void energy_item(const EArgs& r, const WT* W_, int i, int j, int k, long e, double* part,
                                             ESmem<WT>& S) {
   const int o = r.o, v = r.v;
   const CCGPU_GLOBAL WT* W = gptr(W_);
@@ -463,7 +469,8 @@ __global__ __launch_bounds__(256, 2) void k_pt_persistent(const PtRun* rp) {
   __shared__ long s_item;
   __shared__ int s_flag;
   ESmem<WT>& ES = *reinterpret_cast<ESmem<WT>*>(lds);
-  const long per_step = r.nE + r.ng[2] + r.ng[1] + r.ng[0];
+  const long nE = r.e.nE;
+  const long per_step = nE + r.ng[2] + r.ng[1] + r.ng[0];
   const long total = (r.nq + 3) * per_step;
   unsigned long long* claim = reinterpret_cast<unsigned long long*>(r.ctr);
   int* err = r.ctr + 2;
@@ -485,10 +492,10 @@ __global__ __launch_bounds__(256, 2) void k_pt_persistent(const PtRun* rp) {
     long rr = it - step * per_step;
     int ph;
     long q;
-    if (rr < r.nE) {
+    if (rr < nE) {
       ph = 3;
       q = step - 3;
-    } else if ((rr -= r.nE) < r.ng[2]) {
+    } else if ((rr -= nE) < r.ng[2]) {
       ph = 2;
       q = step - 2;
     } else if ((rr -= r.ng[2]) < r.ng[1]) {
@@ -534,20 +541,52 @@ __global__ __launch_bounds__(256, 2) void k_pt_persistent(const PtRun* rp) {
       signal_done(&done[q * 5 + ph]);
     } else {
       const int* tq = r.tri + 3 * q;
-      double* part = r.part + slot * 2 * r.nE;
-      energy_item<WT>(r, W, tq[0], tq[1], tq[2], rr, part, ES);
+      double* part = r.part + slot * 2 * nE;
+      energy_item<WT>(r.e, W, tq[0], tq[1], tq[2], rr, part, ES);
       const int old = signal_done(&done[q * 5 + 3]);
       if (tid == 0) {
-        s_flag = old == (int)r.nE - 1;
+        s_flag = old == (int)nE - 1;
         if (s_flag) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
       }
       __syncthreads();
       if (__builtin_amdgcn_readfirstlane(s_flag)) {  // the last energy item of the triple
-        reduce_partials(part, r.nE, r.etrip + 2 * q, ES.red);
+        reduce_partials(part, nE, r.etrip + 2 * q, ES.red);
         signal_done(&done[q * 5 + 4]);
       }
     }
   }
+}
+
+// ------------------------------------------------ per-triple launches
+// The same W passes and energy evaluation as plain launches (stream order
+// instead of in-kernel dependencies).
+
+// Tile of block b: each XCD (block b runs on XCD b % 8) gets a contiguous
+// range of tiles, walked in groups of 8 m-tiles.
+__device__ __forceinline__ void tile_xcd(long b, long tiles_m, long tiles_n, long& tm, long& tn) {
+  const long nt = tiles_m * tiles_n;
+  const long per = nt / 8, full = per * 8;
+  tile_grouped((b < full) ? (b % 8) * per + b / 8 : b, tiles_m, tiles_n, tm, tn);
+}
+
+template <bool X>
+__global__ __launch_bounds__(256, 2) void k_dtile4(DMulti p, long tiles_m, long tiles_n) {
+  __shared__ dt::Smem S;
+  long tm, tn;
+  tile_xcd(blockIdx.x, tiles_m, tiles_n, tm, tn);
+  dt::tile4<X>(p, tm * 128, tn * 128, blockIdx.z, S);
+}
+
+template <typename WT>
+__global__ __launch_bounds__(256) void k_pt_energy_blocked(EArgs a, const WT* W, int i, int j, int k,
+                                                           double* part) {
+  __shared__ ESmem<WT> S;
+  energy_item<WT>(a, W, i, j, k, blockIdx.x, part, S);
+}
+
+__global__ __launch_bounds__(256) void k_reduce_pairs(const double* part, long n, double* out) {
+  __shared__ double red[2][4];
+  reduce_partials(part, n, out, red);
 }
 
 __global__ void k_zero_int(int* p, long n) {
@@ -617,70 +656,195 @@ long env_long(const char* name, long dflt) {
   return (e && *e) ? std::atol(e) : dflt;
 }
 
+// Tiles of the three W passes: G1 v x v^2, G2 v batches of v x v, G3 v^2 x v.
+struct PassShape {
+  long tm[3], tn[3], batch[3], ng[3];
+};
+PassShape pass_shape(bool mixed, int v) {
+  PassShape ps{};
+  const long vv = (long)v * v;
+  const long Ms[3] = {v, v, vv}, Ns[3] = {vv, v, v}, Bs[3] = {1, v, 1};
+  for (int g = 0; g < 3; ++g) {
+    ps.tm[g] = mixed ? bf16x3_rpad(Ms[g]) / 128 : cdiv(Ms[g], 128);
+    ps.tn[g] = mixed ? bf16x3_rpad(Ns[g]) / 128 : cdiv(Ns[g], 128);
+    ps.batch[g] = Bs[g];
+    ps.ng[g] = ps.tm[g] * ps.tn[g] * Bs[g];
+  }
+  return ps;
+}
+
 }  // namespace
+
+size_t CCSD::pt_desc_bytes() const { return pt_mode_ == GemmPrec::BF16X3 ? sizeof(BMulti) : sizeof(DMulti); }
+
+// Descriptors of the three W passes of triple (i, j, k) into W slot `slot`
+// (BMulti for the mixed path, DMulti for FP64), every operand span checked.
+void CCSD::pt_descs(int i, int j, int k, int slot, unsigned char* out) {
+  const bool mixed = pt_mode_ == GemmPrec::BF16X3;
+  const int o = dm_.o, v = dm_.v;
+  const long nov = dm_.nov, vv = (long)v * v, vvv = vv * v;
+  if (mixed) {
+    const long Kpv = bf16x3_kpad(v), Kpo = bf16x3_kpad(o), Rv = bf16x3_rpad(v), Rvv = bf16x3_rpad(vv) + 256;
+    const long gk = vv * Kpv;
+    auto Tm = [&](int p, int q, bool hi) { return (hi ? ppH_[0].p : ppL_[0].p) + ((long)p * o + q) * Rv * Kpv; };
+    auto Qm = [&](int p, int q, bool hi) { return (hi ? ppH_[1].p : ppL_[1].p) + ((long)p * o + q) * Rv * Kpo; };
+    auto Pm = [&](int p, bool hi) { return (hi ? ppH_[2].p : ppL_[2].p) + (long)p * Rvv * Kpo; };
+    auto Sm = [&](int p, bool hi) { return (hi ? ppH_[3].p : ppL_[3].p) + (long)p * Rvv * Kpo; };
+    auto TT = [&](int p, int q, const unsigned short* Bh, const unsigned short* Bl, long bbs) {  // T(p,q) x G
+      return BTerm{Tm(p, q, true), Tm(p, q, false), Bh, Bl, Kpv, 0, bbs};
+    };
+    auto QP = [&](int p, int q, const unsigned short* Bh, const unsigned short* Bl, long bbs) {  // Q_pq x Tvvo
+      return BTerm{Qm(p, q, true), Qm(p, q, false), Bh, Bl, Kpo, 0, bbs};
+    };
+    float* Wq = W32_.p + slot * vvv;
+    const unsigned short *GTk = GTh_.p + k * gk, *GTkl = GTl_.p + k * gk;
+    BMulti* d = reinterpret_cast<BMulti*>(out);
+    // G1: W[a][(b,c)] = t_ij^{ad} (bd|ck) + t_ik^{ad} (cd|bj) - (ai|lk) t_jl^{bc} - (ai|lj) t_kl^{cb}
+    const BTerm g1[4] = {TT(i, j, GTk, GTkl, 0), TT(i, k, GpTh_.p + j * gk, GpTl_.p + j * gk, 0),
+                         QP(i, k, Pm(j, true), Pm(j, false), 0), QP(i, j, Sm(k, true), Sm(k, false), 0)};
+    d[0] = bf16x3_desc(g1, 4, v, vv, 1, Wq, true, vv, 1, 0, 1.0, 0.0);
+    // G2 (batch a): W[a][b][c] += t_ji^{bd} (ad|ck) + t_jk^{bd} (cd|ai) - t_il^{ac} (bj|lk) - t_kl^{ca} (bj|li)
+    const BTerm g2[4] = {TT(j, i, GTk, GTkl, v * Kpv), TT(j, k, GpTh_.p + i * gk, GpTl_.p + i * gk, v * Kpv),
+                         QP(j, k, Pm(i, true), Pm(i, false), v * Kpo), QP(j, i, Sm(k, true), Sm(k, false), v * Kpo)};
+    d[1] = bf16x3_desc(g2, 4, v, v, v, Wq, true, v, 1, vv, 1.0, 1.0);
+    // G3: W[(a,b)][c] += (ad|bj) t_ki^{cd} + (bd|ai) t_kj^{cd} - t_il^{ab} (ck|lj) - t_jl^{ba} (ck|li)
+    const BTerm g3[4] = {BTerm{GTh_.p + j * gk, GTl_.p + j * gk, Tm(k, i, true), Tm(k, i, false), Kpv, 0, 0},
+                         BTerm{GpTh_.p + i * gk, GpTl_.p + i * gk, Tm(k, j, true), Tm(k, j, false), Kpv, 0, 0},
+                         BTerm{Pm(i, true), Pm(i, false), Qm(k, j, true), Qm(k, j, false), Kpo, 0, 0},
+                         BTerm{Sm(j, true), Sm(j, false), Qm(k, i, true), Qm(k, i, false), Kpo, 0, 0}};
+    d[2] = bf16x3_desc(g3, 4, vv, v, 1, Wq, true, v, 1, 0, 1.0, 1.0);
+  } else {
+    auto Td = [&](int p, int q) { return T_.p + (long)p * v * nov + (long)q * v; };  // [x][d], row stride nov
+    auto Qd = [&](int p, int q) { return Q_.p + ((long)p * o + q) * o * v; };       // [l][z]
+    auto Pd = [&](int p) { return Tvvo_.p + (long)p * vv * o; };                    // [x][y][l]
+    auto Sd = [&](int p) { return TvvoS_.p + (long)p * vv * o; };                   // [x][y][l] = Tvvo_p[y][x][l]
+    double* Wq = Wt_.p + slot * vvv;
+    const double* Gk = G_.p + (long)k * vvv;
+    DMulti* d = reinterpret_cast<DMulti*>(out);
+    // the same twelve terms; A read along k and B along n for the G terms of G1 and G2, the other way round
+    // for their occupied terms; the reverse in G3
+    d[0] = d_desc({DTerm{Td(i, j), Gk, nov, 1, 0, 1, vv, 0, v, 1.0, 1},
+                   DTerm{Td(i, k), Gp_.p + (long)j * vvv, nov, 1, 0, 1, vv, 0, v, 1.0, 1},
+                   DTerm{Qd(i, k), Pd(j), 1, v, 0, o, 1, 0, o, -1.0, 0},
+                   DTerm{Qd(i, j), Sd(k), 1, v, 0, o, 1, 0, o, -1.0, 0}},
+                  v, vv, 1, Wq, vv, 1, 0, 0.0);
+    d[1] = d_desc({DTerm{Td(j, i), Gk, nov, 1, 0, 1, vv, v, v, 1.0, 1},
+                   DTerm{Td(j, k), Gp_.p + (long)i * vvv, nov, 1, 0, 1, vv, v, v, 1.0, 1},
+                   DTerm{Qd(j, k), Pd(i), 1, v, 0, o, 1, (long)v * o, o, -1.0, 0},
+                   DTerm{Qd(j, i), Sd(k), 1, v, 0, o, 1, (long)v * o, o, -1.0, 0}},
+                  v, v, v, Wq, v, 1, vv, 1.0);
+    d[2] = d_desc({DTerm{G_.p + (long)j * vvv, Td(k, i), 1, vv, 0, nov, 1, 0, v, 1.0, 0},
+                   DTerm{Gp_.p + (long)i * vvv, Td(k, j), 1, vv, 0, nov, 1, 0, v, 1.0, 0},
+                   DTerm{Pd(i), Qd(k, j), o, 1, 0, 1, v, 0, o, -1.0, 1},
+                   DTerm{Sd(j), Qd(k, i), o, 1, 0, 1, v, 0, o, -1.0, 1}},
+                  vv, v, 1, Wq, v, 1, 0, 1.0);
+  }
+}
+
+namespace {
+EArgs energy_args(const int* ctab, const double* J, const double* t1, const double* eps, int o, int v) {
+  const long nb = cdiv(v, 8), nct = nb * (nb + 1) * (nb + 2) / 6;
+  return EArgs{ctab, J, t1, eps, o, v, std::min<long>(nct, 2048), nct};
+}
+}  // namespace
+
+// (T) as launches per triple: three W passes, the blocked energy, the sum.
+void CCSD::pt_energy_launches(long first, long n, long stride, double* e4, double* e5) {
+  const bool mixed = pt_mode_ == GemmPrec::BF16X3;
+  const int o = dm_.o, v = dm_.v;
+  const long vv = (long)v * v, vvv = vv * v;
+  const EArgs ea = energy_args(ptctab_.p, J_.p, t1_.p, eps_.p, o, v);
+  const PassShape ps = pass_shape(mixed, v);
+  check_span(ptpart_.p, 0, 2 * ea.nE - 1, sizeof(double), "pt partials");
+  check_span(ea.ctab, 0, ea.nct - 1, sizeof(int), "pt block table");
+  check_span(ea.J, 0, dm_.nov2 - 1, sizeof(double), "pt J");
+  check_span(mixed ? (const void*)W32_.p : (const void*)Wt_.p, 0, vvv - 1, mixed ? sizeof(float) : sizeof(double),
+             "pt W");
+  etrip_.ensure(2 * (size_t)std::max<long>(n, 1));
+  // flops of the passes: two K = v and two K = o terms each
+  const double fg = 2.0 * (2.0 * (double)vv * vv + 2.0 * (double)vv * v * o);
+  std::vector<unsigned char> hd(3 * pt_desc_bytes());
+  long idx = 0;
+  for (long t = first; idx < n; t += stride, ++idx) {
+    const int i = tri_i_[t], j = tri_j_[t], k = tri_k_[t];
+    pt_descs(i, j, k, 0, hd.data());
+    for (int g = 0; g < 3; ++g) {
+      const long ntiles = ps.tm[g] * ps.tn[g];
+      CCGPU_REQUIRE(ntiles < (1L << 24) && ps.batch[g] < 65536, "pt: grid too large");
+      if (mixed) {
+        bf16x3_run(reinterpret_cast<const BMulti*>(hd.data())[g], (int)ps.batch[g], true, fg, st_);
+      } else {
+        const DMulti& d = reinterpret_cast<const DMulti*>(hd.data())[g];
+        const dim3 grid((unsigned)ntiles, 1, (unsigned)ps.batch[g]);
+        const int id = prof_begin(g < 2 ? "k_dtile4<true>" : "k_dtile4<false>", fg, st_);
+        if (g < 2)
+          CCGPU_LAUNCH(k_dtile4<true>, grid, dim3(256), 0, st_, d, ps.tm[g], ps.tn[g]);
+        else
+          CCGPU_LAUNCH(k_dtile4<false>, grid, dim3(256), 0, st_, d, ps.tm[g], ps.tn[g]);
+        prof_end(id, st_);
+      }
+    }
+    const int id = prof_begin(mixed ? "k_pt_energy_blocked<float>" : "k_pt_energy_blocked<double>", 0.0, st_);
+    if (mixed)
+      CCGPU_LAUNCH(k_pt_energy_blocked<float>, dim3((unsigned)ea.nE), dim3(256), 0, st_, ea, W32_.p, i, j, k,
+                   ptpart_.p);
+    else
+      CCGPU_LAUNCH(k_pt_energy_blocked<double>, dim3((unsigned)ea.nE), dim3(256), 0, st_, ea, Wt_.p, i, j, k,
+                   ptpart_.p);
+    prof_end(id, st_);
+    CCGPU_LAUNCH(k_reduce_pairs, dim3(1), dim3(256), 0, st_, ptpart_.p, ea.nE, etrip_.p + 2 * idx);
+    CCGPU_HIPX(hipGetLastError());
+  }
+  double s4 = 0.0, s5 = 0.0;
+  if (n > 0) {
+    std::vector<double> et(2 * n, 0.0);
+    CCGPU_HIPX(hipMemcpyAsync(et.data(), etrip_.p, 2 * n * sizeof(double), hipMemcpyDeviceToHost, st_));
+    CCGPU_HIPX(hipStreamSynchronize(st_));
+    for (long q = 0; q < n; ++q) {
+      s4 += et[2 * q];
+      s5 += et[2 * q + 1];
+    }
+  }
+  if (e4) *e4 = s4;
+  if (e5) *e5 = s5;
+}
 
 // Host side of the persistent (T): descriptors per triple, launches of at
 // most CCGPU_PT_CHUNK triples (default: about one second of work each).
 void CCSD::pt_energy_persistent(long first, long n, long stride, double* e4, double* e5) {
   const bool mixed = pt_mode_ == GemmPrec::BF16X3;
   const int o = dm_.o, v = dm_.v;
-  const long nov = dm_.nov, vv = (long)v * v, vvv = vv * v;
-  const long nb = cdiv(v, 8), nct = nb * (nb + 1) * (nb + 2) / 6;
-  const long nE = std::min<long>(nct, 2048);
+  const long vv = (long)v * v, vvv = vv * v;
   PtRun r{};
   r.W = mixed ? (void*)W32_.p : (void*)Wt_.p;
   r.wslot = vvv;
   r.part = ptpart_.p;
-  r.ctab = ptctab_.p;
-  r.J = J_.p;
-  r.t1 = t1_.p;
-  r.eps = eps_.p;
-  r.o = o;
-  r.v = v;
-  r.nE = nE;
-  r.nct = nct;
+  r.e = energy_args(ptctab_.p, J_.p, t1_.p, eps_.p, o, v);
+  const long nE = r.e.nE;
   r.spin_limit = env_long("CCGPU_PT_WAIT_MS", 5000) * wall_clock_khz();
   check_span(r.W, 0, 4 * vvv - 1, mixed ? sizeof(float) : sizeof(double), "pt W slots");
   check_span(r.part, 0, 8 * nE - 1, sizeof(double), "pt partials");
-  check_span(r.ctab, 0, nct - 1, sizeof(int), "pt block table");
-  check_span(r.J, 0, dm_.nov2 - 1, sizeof(double), "pt J");
-  // tiles per pass: G1 v x v^2, G2 v batches of v x v, G3 v^2 x v
-  const long Ms[3] = {v, v, vv}, Ns[3] = {vv, v, v}, Bs[3] = {1, v, 1};
+  check_span(r.e.ctab, 0, r.e.nct - 1, sizeof(int), "pt block table");
+  check_span(r.e.J, 0, dm_.nov2 - 1, sizeof(double), "pt J");
+  const PassShape ps = pass_shape(mixed, v);
   for (int g = 0; g < 3; ++g) {
-    r.tm[g] = mixed ? bf16x3_rpad(Ms[g]) / 128 : cdiv(Ms[g], 128);
-    r.tn[g] = mixed ? bf16x3_rpad(Ns[g]) / 128 : cdiv(Ns[g], 128);
-    r.ng[g] = r.tm[g] * r.tn[g] * Bs[g];
+    r.tm[g] = ps.tm[g];
+    r.tn[g] = ps.tn[g];
+    r.ng[g] = ps.ng[g];
   }
+  const long per_step = nE + r.ng[0] + r.ng[1] + r.ng[2];
   const double ftri = 12.0 * (double)vv * vv + 12.0 * (double)o * vvv;
   long chunk = env_long("CCGPU_PT_CHUNK", 0);
   if (chunk <= 0) chunk = std::min<long>(4096, std::max<long>(8, (long)((mixed ? 150e12 : 35e12) / ftri)));
-  CCGPU_REQUIRE((chunk + 3) * (r.nE + r.ng[0] + r.ng[1] + r.ng[2]) < (1L << 40), "pt: launch too large");
+  CCGPU_REQUIRE((chunk + 3) * per_step < (1L << 40), "pt: launch too large");
   etrip_.ensure(2 * (size_t)std::max<long>(n, 1));
-  const size_t dsz = mixed ? sizeof(BMulti) : sizeof(DMulti);
+  const size_t dsz = pt_desc_bytes();
   const long cq = std::min(chunk, std::max<long>(n, 1));
   ptdesc_.ensure(kRunBytes + 3 * cq * dsz);
   pttri_.ensure(3 * (size_t)cq);
   ptctr_.ensure(4 + 5 * (size_t)cq);
   std::vector<unsigned char> hd(kRunBytes + 3 * cq * dsz);
   std::vector<int> ht(3 * cq);
-  // mixed operands
-  const long Kpv = bf16x3_kpad(v), Kpo = bf16x3_kpad(o), Rv = bf16x3_rpad(v), Rvv = bf16x3_rpad(vv) + 256;
-  const long gk = vv * Kpv;
-  auto Tm = [&](int p, int q, bool hi) { return (hi ? ppH_[0].p : ppL_[0].p) + ((long)p * o + q) * Rv * Kpv; };
-  auto Qm = [&](int p, int q, bool hi) { return (hi ? ppH_[1].p : ppL_[1].p) + ((long)p * o + q) * Rv * Kpo; };
-  auto Pm = [&](int p, bool hi) { return (hi ? ppH_[2].p : ppL_[2].p) + (long)p * Rvv * Kpo; };
-  auto Sm = [&](int p, bool hi) { return (hi ? ppH_[3].p : ppL_[3].p) + (long)p * Rvv * Kpo; };
-  auto TT = [&](int p, int q, const unsigned short* Bh, const unsigned short* Bl, long bbs) {  // T(p,q) x G slice
-    return BTerm{Tm(p, q, true), Tm(p, q, false), Bh, Bl, Kpv, 0, bbs};
-  };
-  auto QP = [&](int p, int q, const unsigned short* Bh, const unsigned short* Bl, long bbs) {  // Q_pq x Tvvo
-    return BTerm{Qm(p, q, true), Qm(p, q, false), Bh, Bl, Kpo, 0, bbs};
-  };
-  // FP64 operands
-  auto Td = [&](int p, int q) { return T_.p + (long)p * v * nov + (long)q * v; };  // [x][d], row stride nov
-  auto Qd = [&](int p, int q) { return Q_.p + ((long)p * o + q) * o * v; };       // [l][z]
-  auto Pd = [&](int p) { return Tvvo_.p + (long)p * vv * o; };                    // [x][y][l]
-  auto Sd = [&](int p) { return TvvoS_.p + (long)p * vv * o; };                   // [x][y][l] = Tvvo_p[y][x][l]
   double s4 = 0.0, s5 = 0.0;
   for (long c0 = 0; c0 < n; c0 += chunk) {
     const long nq = std::min(chunk, n - c0);
@@ -690,50 +854,7 @@ void CCSD::pt_energy_persistent(long first, long n, long stride, double* e4, dou
       ht[3 * q] = i;
       ht[3 * q + 1] = j;
       ht[3 * q + 2] = k;
-      const int slot = (int)(q & 3);
-      if (mixed) {
-        float* Wq = W32_.p + slot * vvv;
-        const unsigned short *GTk = GTh_.p + k * gk, *GTkl = GTl_.p + k * gk;
-        BMulti* d = reinterpret_cast<BMulti*>(hd.data() + kRunBytes) + 3 * q;
-        // G1: W[a][(b,c)] = t_ij^{ad} (bd|ck) + t_ik^{ad} (cd|bj) - (ai|lk) t_jl^{bc} - (ai|lj) t_kl^{cb}
-        const BTerm g1[4] = {TT(i, j, GTk, GTkl, 0), TT(i, k, GpTh_.p + j * gk, GpTl_.p + j * gk, 0),
-                             QP(i, k, Pm(j, true), Pm(j, false), 0), QP(i, j, Sm(k, true), Sm(k, false), 0)};
-        d[0] = bf16x3_desc(g1, 4, v, vv, 1, Wq, true, vv, 1, 0, 1.0, 0.0);
-        // G2 (batch a): W[a][b][c] += t_ji^{bd} (ad|ck) + t_jk^{bd} (cd|ai) - t_il^{ac} (bj|lk) - t_kl^{ca} (bj|li)
-        const BTerm g2[4] = {TT(j, i, GTk, GTkl, v * Kpv), TT(j, k, GpTh_.p + i * gk, GpTl_.p + i * gk, v * Kpv),
-                             QP(j, k, Pm(i, true), Pm(i, false), v * Kpo),
-                             QP(j, i, Sm(k, true), Sm(k, false), v * Kpo)};
-        d[1] = bf16x3_desc(g2, 4, v, v, v, Wq, true, v, 1, vv, 1.0, 1.0);
-        // G3: W[(a,b)][c] += (ad|bj) t_ki^{cd} + (bd|ai) t_kj^{cd} - t_il^{ab} (ck|lj) - t_jl^{ba} (ck|li)
-        const BTerm g3[4] = {
-            BTerm{GTh_.p + j * gk, GTl_.p + j * gk, Tm(k, i, true), Tm(k, i, false), Kpv, 0, 0},
-            BTerm{GpTh_.p + i * gk, GpTl_.p + i * gk, Tm(k, j, true), Tm(k, j, false), Kpv, 0, 0},
-            BTerm{Pm(i, true), Pm(i, false), Qm(k, j, true), Qm(k, j, false), Kpo, 0, 0},
-            BTerm{Sm(j, true), Sm(j, false), Qm(k, i, true), Qm(k, i, false), Kpo, 0, 0}};
-        d[2] = bf16x3_desc(g3, 4, vv, v, 1, Wq, true, v, 1, 0, 1.0, 1.0);
-      } else {
-        double* Wq = Wt_.p + slot * vvv;
-        const double* Gk = G_.p + (long)k * vvv;
-        DMulti* d = reinterpret_cast<DMulti*>(hd.data() + kRunBytes) + 3 * q;
-        // G1 (A along k, B along n for the G terms; A along m, B along k for the occupied ones)
-        d[0] = d_desc({DTerm{Td(i, j), Gk, nov, 1, 0, 1, vv, 0, v, 1.0, 1},
-                       DTerm{Td(i, k), Gp_.p + (long)j * vvv, nov, 1, 0, 1, vv, 0, v, 1.0, 1},
-                       DTerm{Qd(i, k), Pd(j), 1, v, 0, o, 1, 0, o, -1.0, 0},
-                       DTerm{Qd(i, j), Sd(k), 1, v, 0, o, 1, 0, o, -1.0, 0}},
-                      v, vv, 1, Wq, vv, 1, 0, 0.0);
-        // G2 (batch a)
-        d[1] = d_desc({DTerm{Td(j, i), Gk, nov, 1, 0, 1, vv, v, v, 1.0, 1},
-                       DTerm{Td(j, k), Gp_.p + (long)i * vvv, nov, 1, 0, 1, vv, v, v, 1.0, 1},
-                       DTerm{Qd(j, k), Pd(i), 1, v, 0, o, 1, (long)v * o, o, -1.0, 0},
-                       DTerm{Qd(j, i), Sd(k), 1, v, 0, o, 1, (long)v * o, o, -1.0, 0}},
-                      v, v, v, Wq, v, 1, vv, 1.0);
-        // G3
-        d[2] = d_desc({DTerm{G_.p + (long)j * vvv, Td(k, i), 1, vv, 0, nov, 1, 0, v, 1.0, 0},
-                       DTerm{Gp_.p + (long)i * vvv, Td(k, j), 1, vv, 0, nov, 1, 0, v, 1.0, 0},
-                       DTerm{Pd(i), Qd(k, j), o, 1, 0, 1, v, 0, o, -1.0, 1},
-                       DTerm{Sd(j), Qd(k, i), o, 1, 0, 1, v, 0, o, -1.0, 1}},
-                      vv, v, 1, Wq, v, 1, 0, 1.0);
-      }
+      pt_descs(i, j, k, (int)(q & 3), hd.data() + kRunBytes + 3 * q * dsz);
     }
     r.nq = nq;
     r.desc = ptdesc_.p + kRunBytes;
@@ -747,9 +868,9 @@ void CCSD::pt_energy_persistent(long first, long n, long stride, double* e4, dou
     CCGPU_HIPX(hipMemcpyAsync(pttri_.p, ht.data(), 3 * nq * sizeof(int), hipMemcpyHostToDevice, st_));
     CCGPU_LAUNCH(k_zero_int, dim3(cdiv(4 + 5 * nq, 256)), dim3(256), 0, st_, ptctr_.p, 4 + 5 * nq);
     const PtRun* run = reinterpret_cast<const PtRun*>(ptdesc_.p);
-    const long total = (nq + 3) * (r.nE + r.ng[0] + r.ng[1] + r.ng[2]);
+    const long total = (nq + 3) * per_step;
     // at most one pipeline step of items in flight: fewer blocks polling on small systems
-    const long grid = std::min<long>(std::min<long>(total, r.nE + r.ng[0] + r.ng[1] + r.ng[2]),
+    const long grid = std::min<long>(std::min<long>(total, per_step),
                                      mixed ? resident_blocks<true>() : resident_blocks<false>());
     const int id = prof_begin(mixed ? "k_pt_persistent<true>" : "k_pt_persistent<false>", (double)nq * ftri, st_);
     if (mixed)

@@ -58,16 +58,16 @@ for (o, v, naux) in {shapes!r}:
 """
 
 
-@pytest.mark.parametrize("persistent", ["1", "0"])
+@pytest.mark.parametrize("path", ["launch", "legacy", "persistent"])
 @pytest.mark.parametrize("shapes", [
     [(5, 19, 280), (5, 14, 172), (1, 1, 1), (2, 1, 3), (6, 40, 90), (3, 200, 10), (1, 6, 4), (4, 2, 7),
      (3, 5, 2)],
     [(32, 256, 1024)],
     [(64, 512, 2048)],
 ])
-def test_dryrun_bounds(shapes, persistent):
-    """Both (T) drivers: the persistent kernel and the per-triple launches."""
-    env = dict(os.environ, CCGPU_DRYRUN="1", CCGPU_PT_PERSISTENT=persistent)
+def test_dryrun_bounds(shapes, path):
+    """Every (T) driver: per-triple launches, the first launch sequence, the persistent kernel."""
+    env = dict(os.environ, CCGPU_DRYRUN="1", CCGPU_PT_PATH=path)
     r = subprocess.run([sys.executable, "-c", SCRIPT.format(root=ROOT, shapes=shapes)],
                        env=env, capture_output=True, text=True, timeout=600)
     assert r.returncode == 0, r.stdout + r.stderr
