@@ -97,6 +97,7 @@ class CCSD {
   void pt_energy_launches(long first, long n, long stride, double* e4, double* e5);
   void pt_descs(int i, int j, int k, int slot, unsigned char* out);
   size_t pt_desc_bytes() const;
+  void side_stream();
   void ensure_diis();
   void reset_cycle();
   void ensure_lhtr();

@@ -181,6 +181,17 @@ void CCSD::release_work() {
   diis_off_ = iter_ > 0 ? iter_ - 1 : 0;  // the DIIS history is gone: restart the cycle
 }
 
+// Second stream (default priority) with two pairs of events, made once:
+// staging copies of the left transformation, (T) energies beside the W passes.
+void CCSD::side_stream() {
+  if (cst_ || dryrun()) return;
+  CCGPU_HIP(hipStreamCreateWithFlags(&cst_, hipStreamNonBlocking));
+  for (int q = 0; q < 2; ++q) {
+    CCGPU_HIP(hipEventCreateWithFlags(&ev_in_[q], hipEventDisableTiming));
+    CCGPU_HIP(hipEventCreateWithFlags(&ev_done_[q], hipEventDisableTiming));
+  }
+}
+
 CCSD::~CCSD() {
   if (dryrun()) return;
   if (cst_) {

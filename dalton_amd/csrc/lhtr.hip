@@ -221,16 +221,8 @@ void CCSD::lhtr_host(int nvec, const double* l1, const double* l2p, double* rho1
   if (nvec == 0) return;
   const long nov = dm_.nov, np2 = dm_.ntamp - nov;
   lhtr_prepare();
-  if (!lst_.p) {
-    lst_.alloc(4 * (size_t)(np2 + nov));  // [in slot 0, in slot 1, out slot 0, out slot 1] x (l1/rho1, l2/rho2)
-    if (!dryrun()) {
-      CCGPU_HIP(hipStreamCreateWithFlags(&cst_, hipStreamNonBlocking));
-      for (int q = 0; q < 2; ++q) {
-        CCGPU_HIP(hipEventCreateWithFlags(&ev_in_[q], hipEventDisableTiming));
-        CCGPU_HIP(hipEventCreateWithFlags(&ev_done_[q], hipEventDisableTiming));
-      }
-    }
-  }
+  if (!lst_.p) lst_.alloc(4 * (size_t)(np2 + nov));  // [in 0, in 1, out 0, out 1] x (l1/rho1, l2/rho2)
+  side_stream();
   const long slot = np2 + nov;
   auto in1 = [&](int q) { return lst_.p + q * slot; };
   auto in2 = [&](int q) { return lst_.p + q * slot + nov; };
