@@ -152,6 +152,52 @@ def _log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def pt_leg(ctx, ntrip, o, v, log):
+    """(T) over ntrip triples spread evenly over the list, mixed then FP64,
+    from the context's current amplitudes; per-kernel times from the
+    library's own event profiler."""
+    import dalton_amd as d
+    from dalton_amd.cc import profile_start, profile_stop
+
+    pt = {}
+    for label, prec in (("mixed", d.MIXED), ("fp64", d.FP64)):
+        _, _, _, ntr = ctx.pt_partial(0, 0, 1, precision=prec)  # integral layouts of this precision (untimed)
+        stride = max(1, ntr // ntrip)
+        sync()
+        profile_start()
+        t0 = time.perf_counter()
+        e4, e5, fl, _ = ctx.pt_partial(0, ntrip, stride, precision=prec)
+        sync()
+        dtp = time.perf_counter() - t0
+        pp = profile_stop()
+        pt[label] = {"seconds": round(dtp, 4), "tflops": round(fl / dtp / 1e12, 3),
+                     "est_full_seconds": round(dtp / ntrip * ntr, 1), "e4": e4, "e5": e5,
+                     "kernels": kern_table_of(pp)}
+        log(f"(T) {label}: {ntrip} triples in {dtp:.3f} s, {fl / dtp / 1e12:.2f} TFLOP/s")
+    pt["triples_timed"] = ntrip
+    pt["triples_total"] = ntr
+    pt["stride"] = stride
+    pt["flops_per_triple"] = 12.0 * v**4 + 12.0 * o * v**3
+    for e in ("e4", "e5"):  # relative, or absolute where the FP64 value is zero (E5 at t1 = 0)
+        dev = abs(pt["mixed"][e] - pt["fp64"][e])
+        pt[f"rel_err_{e}_mixed_vs_fp64"] = dev / abs(pt["fp64"][e]) if pt["fp64"][e] != 0.0 else dev
+    pt["speedup_mixed_vs_fp64"] = round(pt["fp64"]["seconds"] / pt["mixed"]["seconds"], 3)
+    return pt
+
+
+def pt_workload(o, v, naux, seed, ntrip):
+    """The (T) leg alone (MP2 amplitudes), for profiling the (T) kernels."""
+    import dalton_amd as d
+    from dalton_amd.synthetic import make_system
+
+    eps, B = make_system(o, v, naux, seed)
+    with d.CCContext(eps, B, o) as c:
+        c.guess()
+        pt = pt_leg(c, ntrip, o, v, _log)
+    return {"metric": f"(T) TFLOP/s at o{o}/v{v}, {ntrip} triples, MP2 amplitudes", "unit": "TFLOP/s",
+            "workload": f"synthetic o={o} v={v} naux={naux} seed {seed}", "pt": pt}
+
+
 def pt48_workload(o=48, v=384, naux=1536, seed=5):
     """(T) over every i>=j>=k triple at o=48 v=384 naux=1536, FP64 and mixed,
     after converging CCSD in the same precision.  The integral scale is the
@@ -172,10 +218,17 @@ def pt48_workload(o=48, v=384, naux=1536, seed=5):
             sync()
             tc = time.perf_counter() - t0
             _log(f"{label}: CCSD {r['niter']} it in {tc:.1f} s, E = {r['e_ccsd']:.12f}")
-            c.pt_partial(0, 0, 1, precision=prec)  # (T) integral layouts (untimed)
+            _, _, _, ntr = c.pt_partial(0, 0, 1, precision=prec)  # (T) integral layouts (untimed)
             sync()
             t0 = time.perf_counter()
-            e4, e5 = c.pt(prec)
+            e4 = e5 = 0.0
+            nchunk = 8  # the full triple list in contiguous ranges, one progress line each
+            for q in range(nchunk):
+                a, b = q * ntr // nchunk, (q + 1) * ntr // nchunk
+                p4, p5, _, _ = c.pt_partial(a, b - a, 1, precision=prec)
+                e4 += p4
+                e5 += p5
+                _log(f"{label}: (T) triples {a}..{b - 1} of {ntr} done at {time.perf_counter() - t0:.1f} s")
             sync()
             tp = time.perf_counter() - t0
             _log(f"{label}: (T) {tp:.1f} s, E4 = {e4:.12f} E5 = {e5:.12f}")
@@ -306,8 +359,9 @@ def main():
                     help="(T) triples timed, spread evenly over the i>=j>=k list (0: skip)")
     ap.add_argument("--lhtr-vectors", type=int, default=16,
                     help="left-transformation trial vectors timed at o=32 v=256 naux=1024 (0: skip)")
-    ap.add_argument("--workload", choices=["ccsd", "pt48", "batch", "copy", "blas"], default="ccsd",
-                    help="ccsd: the headline line (default); pt48: full (T) at o=48 v=384 in both "
+    ap.add_argument("--workload", choices=["ccsd", "pt", "pt48", "batch", "copy", "blas"], default="ccsd",
+                    help="ccsd: the headline line (default); pt: the (T) leg alone at --o/--v; "
+                         "pt48: full (T) at o=48 v=384 in both "
                          "precisions on a convergent system; batch: 32 uneven systems converged in "
                          "one graph replay per iteration")
     args = ap.parse_args()
@@ -315,6 +369,9 @@ def main():
         print("bench.py: this build runs on one GPU only; --gpus must be 1 "
               "(how the path would shard is described in DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "pt":
+        print(json.dumps(pt_workload(args.o, args.v, args.naux, args.seed, args.pt_triples)), flush=True)
+        return
     if args.workload == "pt48":
         print(json.dumps(pt48_workload()), flush=True)
         return
@@ -371,9 +428,7 @@ def main():
     if fp64_leg:
         spi_d, hist_d, prof_d = timed_steps(d.FP64, args.fp64_steps, "fp64")
 
-    def kern_table(prof):
-        return {k: {"count": v_["count"], "ms": round(v_["ms"], 3),
-                    "tflops": round(v_["flops"] / max(v_["ms"], 1e-9) / 1e9, 2)} for k, v_ in prof["kernels"].items()}
+    kern_table = kern_table_of
 
     def roofline(prof, dt_total, peak, peak_note, attainable=None):
         kern = prof["kernels"]
@@ -390,29 +445,7 @@ def main():
                 "frac_attainable": round(ach / attainable, 4) if attainable else None}
 
     F = f_ccsd(o, v, naux)
-    pt = {}
-    if args.pt_triples > 0:
-        for label, prec in (("mixed", d.MIXED), ("fp64", d.FP64)):
-            _, _, _, ntr = ctx.pt_partial(0, 0, 1, precision=prec)  # integral layouts of this precision (untimed)
-            stride = max(1, ntr // args.pt_triples)
-            sync()
-            profile_start()
-            t0 = time.perf_counter()
-            e4, e5, fl, _ = ctx.pt_partial(0, args.pt_triples, stride, precision=prec)
-            sync()
-            dtp = time.perf_counter() - t0
-            pp = profile_stop()
-            pt[label] = {"seconds": round(dtp, 4), "tflops": round(fl / dtp / 1e12, 3),
-                         "est_full_seconds": round(dtp / args.pt_triples * ntr, 1), "e4": e4, "e5": e5,
-                         "kernels": kern_table(pp)}
-            log(f"(T) {label}: {args.pt_triples} triples in {dtp:.3f} s, {fl / dtp / 1e12:.2f} TFLOP/s")
-        pt["triples_timed"] = args.pt_triples
-        pt["triples_total"] = ntr
-        pt["stride"] = stride
-        pt["flops_per_triple"] = 12.0 * v**4 + 12.0 * o * v**3
-        pt["rel_err_e4_mixed_vs_fp64"] = abs(pt["mixed"]["e4"] - pt["fp64"]["e4"]) / abs(pt["fp64"]["e4"])
-        pt["rel_err_e5_mixed_vs_fp64"] = abs(pt["mixed"]["e5"] - pt["fp64"]["e5"]) / abs(pt["fp64"]["e5"])
-        pt["speedup_mixed_vs_fp64"] = round(pt["fp64"]["seconds"] / pt["mixed"]["seconds"], 3)
+    pt = pt_leg(ctx, args.pt_triples, o, v, log) if args.pt_triples > 0 else {}
     ctx.close()
     del ctx
 
