@@ -71,17 +71,20 @@ __device__ __forceinline__ void tile(const BMulti& p, long m0, long n0, long z, 
     const int c = tid + q * NT;
     loffb[q] = (c / CPR) * LDK + (c % CPR) * 8;
   }
-  for (int t = 0; t < p.nterm; ++t) {
+  // thread t owns 16-byte chunks c = t + NT q: row c / (TK/8), k-chunk c % (TK/8).
+  // Plain register arrays with compile-time indices (kept out of scratch).
+  u32x4 rah[NCHA], ral[NCHA], rbh[NCHB], rbl[NCHB];
+  long goffa[NCHA], goffb[NCHB];
+  const CCGPU_GLOBAL unsigned short *pAh, *pAl, *pBh, *pBl;
+  long Kp = 0;
+  // point the loaders at term t
+  auto select = [&](int t) {
     const BTerm& T = p.t[t];
-    const long Kp = T.Kp;
-    const CCGPU_GLOBAL unsigned short* pAh = gptr(T.Ah + z * T.a_bs + m0 * Kp);
-    const CCGPU_GLOBAL unsigned short* pAl = gptr(T.Al + z * T.a_bs + m0 * Kp);
-    const CCGPU_GLOBAL unsigned short* pBh = gptr(T.Bh + z * T.b_bs + n0 * Kp);
-    const CCGPU_GLOBAL unsigned short* pBl = gptr(T.Bl + z * T.b_bs + n0 * Kp);
-    // thread t owns 16-byte chunks c = t + NT q: row c / (TK/8), k-chunk c % (TK/8).
-    // Plain register arrays with compile-time indices (kept out of scratch).
-    u32x4 rah[NCHA], ral[NCHA], rbh[NCHB], rbl[NCHB];
-    long goffa[NCHA], goffb[NCHB];
+    Kp = T.Kp;
+    pAh = gptr(T.Ah + z * T.a_bs + m0 * Kp);
+    pAl = gptr(T.Al + z * T.a_bs + m0 * Kp);
+    pBh = gptr(T.Bh + z * T.b_bs + n0 * Kp);
+    pBl = gptr(T.Bl + z * T.b_bs + n0 * Kp);
 #pragma unroll
     for (int q = 0; q < NCHA; ++q) {
       const int c = tid + q * NT;
@@ -92,6 +95,7 @@ __device__ __forceinline__ void tile(const BMulti& p, long m0, long n0, long z, 
       const int c = tid + q * NT;
       goffb[q] = (long)(c / CPR) * Kp + (c % CPR) * 8;
     }
+  };
 #define CCGPU_BF3_LOAD(k0)                                                       \
   _Pragma("unroll") for (int q = 0; q < NCHA; ++q) {                             \
     rah[q] = *reinterpret_cast<const CCGPU_GLOBAL u32x4*>(pAh + goffa[q] + (k0)); \
@@ -110,13 +114,21 @@ __device__ __forceinline__ void tile(const BMulti& p, long m0, long n0, long z, 
     *reinterpret_cast<u32x4*>(&sB[0][0][0] + loffb[q]) = rbh[q];                 \
     *reinterpret_cast<u32x4*>(&sB[1][0][0] + loffb[q]) = rbl[q];                 \
   }
-    CCGPU_BF3_LOAD(0)
-    for (long k0 = 0; k0 < Kp; k0 += TK) {
+  // One pipeline over all terms: the last K step of a term issues the first
+  // loads of the next, so a term boundary costs no exposed load latency.
+  select(0);
+  CCGPU_BF3_LOAD(0)
+  for (int t = 0; t < p.nterm; ++t) {
+    const long Kt = Kp;  // this term's K (the loaders move on to the next term at its last step)
+    for (long k0 = 0; k0 < Kt; k0 += TK) {
       __syncthreads();
       CCGPU_BF3_STORE()
       __syncthreads();
-      if (k0 + TK < Kp) {
+      if (k0 + TK < Kt) {
         CCGPU_BF3_LOAD(k0 + TK)
+      } else if (t + 1 < p.nterm) {
+        select(t + 1);
+        CCGPU_BF3_LOAD(0)
       }
 #pragma unroll
       for (int ks = 0; ks < TK; ks += 32) {
@@ -143,10 +155,10 @@ __device__ __forceinline__ void tile(const BMulti& p, long m0, long n0, long z, 
           for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
       }
     }
+  }
 #undef CCGPU_BF3_LOAD
 #undef CCGPU_BF3_STORE
-    __syncthreads();
-  }
+  __syncthreads();  // the caller's next tile reuses the LDS
   // D[i][j] of a 16x16 f32 MFMA tile: lane = j + 16*(i/4), element i%4.
   CCGPU_GLOBAL OutT* C = gptr(static_cast<OutT*>(p.C) + z * p.c_b);
 #pragma unroll
