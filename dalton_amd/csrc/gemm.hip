@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <map>
 #include <mutex>
+#include <type_traits>
 
 #include "common.h"
 
@@ -91,130 +92,140 @@ __global__ __launch_bounds__(256, 2) void k_dgemm(GemmDesc d, long kchunk, doubl
     B += (long)blockIdx.z * d.b_b;
   }
 
-  double ra[NL], rb[NL];
-  // k-fastest: thread owns column c = tid % 16 and rows tid/16 + 16 i;
-  // row-fastest: thread owns row tid % T and columns tid/T + RS i.
-  auto load = [&](long k0) {
-    if (AMODE == KFAST) {
-      const long k = k0 + (tid % TK);
-      const bool kok = k < kend;
-      const long ko = kok ? koff<K2>(k, d.a_k, d.a_ko, d.Kin) : 0;
-      const long mb = m0 + tid / TK;
-#pragma unroll
-      for (int i = 0; i < NL; ++i) {
-        const long m = mb + 16 * i;
-        ra[i] = (kok && m < d.M) ? A[m * d.a_m + ko] : 0.0;
+  // Interior tiles (inside M x N, whole K steps) load without bounds checks:
+  // the checks would otherwise branch around every load of the K loop.
+  const bool inside = m0 + TM <= d.M && n0 + TN <= d.N && (kend - kbeg) % TK == 0;
+  auto run = [&](auto guarded) {
+    constexpr bool G = decltype(guarded)::value;
+    double ra[NL], rb[NL];
+    // k-fastest: thread owns column c = tid % 16 and rows tid/16 + 16 i;
+    // row-fastest: thread owns row tid % T and columns tid/T + RS i.
+    auto load = [&](long k0) {
+      if (AMODE == KFAST) {
+        const long k = k0 + (tid % TK);
+        const bool kok = k < kend;
+        const long ko = (!G || kok) ? koff<K2>(k, d.a_k, d.a_ko, d.Kin) : 0;
+        const long mb = m0 + tid / TK;
+  #pragma unroll
+        for (int i = 0; i < NL; ++i) {
+          const long m = mb + 16 * i;
+          ra[i] = (!G || (kok && m < d.M)) ? A[m * d.a_m + ko] : 0.0;
+        }
+      } else {
+        const long m = m0 + (tid % TM);
+        const bool mok = m < d.M;
+        const double* Ar = A + (mok ? m : 0) * d.a_m;
+        const long kb = k0 + tid / TM;
+  #pragma unroll
+        for (int i = 0; i < NL; ++i) {
+          const long k = kb + RS * i;
+          ra[i] = (!G || (mok && k < kend)) ? Ar[koff<K2>(k, d.a_k, d.a_ko, d.Kin)] : 0.0;
+        }
       }
-    } else {
-      const long m = m0 + (tid % TM);
-      const bool mok = m < d.M;
-      const double* Ar = A + (mok ? m : 0) * d.a_m;
-      const long kb = k0 + tid / TM;
-#pragma unroll
-      for (int i = 0; i < NL; ++i) {
-        const long k = kb + RS * i;
-        ra[i] = (mok && k < kend) ? Ar[koff<K2>(k, d.a_k, d.a_ko, d.Kin)] : 0.0;
+      if (BMODE == KFAST) {
+        const long k = k0 + (tid % TK);
+        const bool kok = k < kend;
+        const long ko = (!G || kok) ? koff<K2>(k, d.b_k, d.b_ko, d.Kin) : 0;
+        const long nb = n0 + tid / TK;
+  #pragma unroll
+        for (int i = 0; i < NL; ++i) {
+          const long n = nb + 16 * i;
+          rb[i] = (!G || (kok && n < d.N)) ? B[n * d.b_n + ko] : 0.0;
+        }
+      } else {
+        const long n = n0 + (tid % TN);
+        const bool nok = n < d.N;
+        const double* Br = B + (nok ? n : 0) * d.b_n;
+        const long kb = k0 + tid / TN;
+  #pragma unroll
+        for (int i = 0; i < NL; ++i) {
+          const long k = kb + RS * i;
+          rb[i] = (!G || (nok && k < kend)) ? Br[koff<K2>(k, d.b_k, d.b_ko, d.Kin)] : 0.0;
+        }
       }
-    }
-    if (BMODE == KFAST) {
-      const long k = k0 + (tid % TK);
-      const bool kok = k < kend;
-      const long ko = kok ? koff<K2>(k, d.b_k, d.b_ko, d.Kin) : 0;
-      const long nb = n0 + tid / TK;
-#pragma unroll
+    };
+    auto store = [&]() {
+  #pragma unroll
       for (int i = 0; i < NL; ++i) {
-        const long n = nb + 16 * i;
-        rb[i] = (kok && n < d.N) ? B[n * d.b_n + ko] : 0.0;
+        const int e = tid + i * 256;
+        int r, c;
+        if (AMODE == KFAST) { r = e / TK; c = e % TK; } else { r = e % TM; c = e / TM; }
+        As.at(r, c) = ra[i];
       }
-    } else {
-      const long n = n0 + (tid % TN);
-      const bool nok = n < d.N;
-      const double* Br = B + (nok ? n : 0) * d.b_n;
-      const long kb = k0 + tid / TN;
-#pragma unroll
+  #pragma unroll
       for (int i = 0; i < NL; ++i) {
-        const long k = kb + RS * i;
-        rb[i] = (nok && k < kend) ? Br[koff<K2>(k, d.b_k, d.b_ko, d.Kin)] : 0.0;
+        const int e = tid + i * 256;
+        int r, c;
+        if (BMODE == KFAST) { r = e / TK; c = e % TK; } else { r = e % TN; c = e / TN; }
+        Bs.at(r, c) = rb[i];
       }
-    }
-  };
-  auto store = [&]() {
-#pragma unroll
-    for (int i = 0; i < NL; ++i) {
-      const int e = tid + i * 256;
-      int r, c;
-      if (AMODE == KFAST) { r = e / TK; c = e % TK; } else { r = e % TM; c = e / TM; }
-      As.at(r, c) = ra[i];
-    }
-#pragma unroll
-    for (int i = 0; i < NL; ++i) {
-      const int e = tid + i * 256;
-      int r, c;
-      if (BMODE == KFAST) { r = e / TK; c = e % TK; } else { r = e % TN; c = e / TN; }
-      Bs.at(r, c) = rb[i];
-    }
-  };
+    };
 
-  d4 acc[IW][IW];
-#pragma unroll
-  for (int i = 0; i < IW; ++i)
-#pragma unroll
-    for (int j = 0; j < IW; ++j) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
-
-  const int lr = lane & 15, lk = lane >> 4;
-  load(kbeg);
-  for (long k0 = kbeg; k0 < kend; k0 += TK) {
-    __syncthreads();
-    store();
-    __syncthreads();
-    if (k0 + TK < kend) load(k0 + TK);
-#pragma unroll
-    for (int kk = 0; kk < TK / 4; ++kk) {
-      double a[IW], b[IW];
-#pragma unroll
-      for (int t = 0; t < IW; ++t) a[t] = As.at(wm * (T / 2) + t * 16 + lr, kk * 4 + lk);
-#pragma unroll
-      for (int t = 0; t < IW; ++t) b[t] = Bs.at(wn * (T / 2) + t * 16 + lr, kk * 4 + lk);
-#pragma unroll
-      for (int i = 0; i < IW; ++i)
-#pragma unroll
-        for (int j = 0; j < IW; ++j)
-          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
-    }
-  }
-
-  // D[i][j] of a 16x16 f64 MFMA tile: lane = j + 16*(i%4), element i/4.
-  if (kchunk > 0) {
-    // split-K partial: ws[z][m][n] (dense M x N)
-    double* W = ws + (long)blockIdx.z * d.M * d.N;
-#pragma unroll
+    d4 acc[IW][IW];
+  #pragma unroll
     for (int i = 0; i < IW; ++i)
-#pragma unroll
+  #pragma unroll
+      for (int j = 0; j < IW; ++j) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
+
+    const int lr = lane & 15, lk = lane >> 4;
+    load(kbeg);
+    for (long k0 = kbeg; k0 < kend; k0 += TK) {
+      __syncthreads();
+      store();
+      __syncthreads();
+      if (k0 + TK < kend) load(k0 + TK);
+  #pragma unroll
+      for (int kk = 0; kk < TK / 4; ++kk) {
+        double a[IW], b[IW];
+  #pragma unroll
+        for (int t = 0; t < IW; ++t) a[t] = As.at(wm * (T / 2) + t * 16 + lr, kk * 4 + lk);
+  #pragma unroll
+        for (int t = 0; t < IW; ++t) b[t] = Bs.at(wn * (T / 2) + t * 16 + lr, kk * 4 + lk);
+  #pragma unroll
+        for (int i = 0; i < IW; ++i)
+  #pragma unroll
+          for (int j = 0; j < IW; ++j)
+            acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
+      }
+    }
+
+    // D[i][j] of a 16x16 f64 MFMA tile: lane = j + 16*(i%4), element i/4.
+    if (kchunk > 0) {
+      // split-K partial: ws[z][m][n] (dense M x N)
+      double* W = ws + (long)blockIdx.z * d.M * d.N;
+  #pragma unroll
+      for (int i = 0; i < IW; ++i)
+  #pragma unroll
+        for (int j = 0; j < IW; ++j)
+  #pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const long m = m0 + wm * (T / 2) + i * 16 + r * 4 + lk;
+            const long n = n0 + wn * (T / 2) + j * 16 + lr;
+            if (m < d.M && n < d.N) W[m * d.N + n] = acc[i][j][r];
+          }
+      return;
+    }
+    double* C = static_cast<double*>(d.C) + (long)blockIdx.z * d.c_b;
+  #pragma unroll
+    for (int i = 0; i < IW; ++i)
+  #pragma unroll
       for (int j = 0; j < IW; ++j)
-#pragma unroll
+  #pragma unroll
         for (int r = 0; r < 4; ++r) {
           const long m = m0 + wm * (T / 2) + i * 16 + r * 4 + lk;
           const long n = n0 + wn * (T / 2) + j * 16 + lr;
-          if (m < d.M && n < d.N) W[m * d.N + n] = acc[i][j][r];
+          if (m < d.M && n < d.N) {
+            double* p = C + m * d.c_m + n * d.c_n;
+            double v = d.alpha * acc[i][j][r];
+            if (d.beta != 0.0) v += d.beta * *p;
+            *p = v;
+          }
         }
-    return;
-  }
-  double* C = static_cast<double*>(d.C) + (long)blockIdx.z * d.c_b;
-#pragma unroll
-  for (int i = 0; i < IW; ++i)
-#pragma unroll
-    for (int j = 0; j < IW; ++j)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const long m = m0 + wm * (T / 2) + i * 16 + r * 4 + lk;
-        const long n = n0 + wn * (T / 2) + j * 16 + lr;
-        if (m < d.M && n < d.N) {
-          double* p = C + m * d.c_m + n * d.c_n;
-          double v = d.alpha * acc[i][j][r];
-          if (d.beta != 0.0) v += d.beta * *p;
-          *p = v;
-        }
-      }
+  };
+  if (inside)
+    run(std::false_type{});
+  else
+    run(std::true_type{});
 }
 
 // C = alpha * sum_z ws[z] + beta * C  (fixed summation order: deterministic)

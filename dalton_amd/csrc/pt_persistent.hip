@@ -80,7 +80,9 @@ __device__ __forceinline__ double& at(double* s, int r, int k) {
   return KF ? s[r * (TK + 1) + k] : s[k * LDR + r];
 }
 
-template <bool AKF>
+// G = false: the tile lies inside M x N and K is a multiple of 16, so the
+// loads need no bounds checks.
+template <bool AKF, bool G = true>
 __device__ __forceinline__ void term(const DTerm& t, long M, long N, long m0, long n0, long z, d4 (&acc)[IW][IW],
                                      Smem& S) {
   constexpr bool BKF = !AKF;
@@ -102,7 +104,7 @@ __device__ __forceinline__ void term(const DTerm& t, long M, long N, long m0, lo
       asm volatile("" : "+v"(pa));
 #pragma unroll
       for (int i = 0; i < NL; ++i) {
-        ra[i] = (kok && mb + 16 * i < M) ? al * *pa : 0.0;
+        ra[i] = (!G || (kok && mb + 16 * i < M)) ? al * *pa : 0.0;
         pa += 16 * a_m;
       }
     } else {  // row tid % 128, columns tid / 128 + 2 i
@@ -113,7 +115,7 @@ __device__ __forceinline__ void term(const DTerm& t, long M, long N, long m0, lo
       asm volatile("" : "+v"(pa));
 #pragma unroll
       for (int i = 0; i < NL; ++i) {
-        ra[i] = (mok && kb + 2 * i < K) ? al * *pa : 0.0;
+        ra[i] = (!G || (mok && kb + 2 * i < K)) ? al * *pa : 0.0;
         pa += 2 * a_k;
       }
     }
@@ -125,7 +127,7 @@ __device__ __forceinline__ void term(const DTerm& t, long M, long N, long m0, lo
       asm volatile("" : "+v"(pb));
 #pragma unroll
       for (int i = 0; i < NL; ++i) {
-        rb[i] = (kok && nb + 16 * i < N) ? *pb : 0.0;
+        rb[i] = (!G || (kok && nb + 16 * i < N)) ? *pb : 0.0;
         pb += 16 * b_n;
       }
     } else {
@@ -136,7 +138,7 @@ __device__ __forceinline__ void term(const DTerm& t, long M, long N, long m0, lo
       asm volatile("" : "+v"(pb));
 #pragma unroll
       for (int i = 0; i < NL; ++i) {
-        rb[i] = (nok && kb + 2 * i < K) ? *pb : 0.0;
+        rb[i] = (!G || (nok && kb + 2 * i < K)) ? *pb : 0.0;
         pb += 2 * b_k;
       }
     }
@@ -181,17 +183,17 @@ __device__ __forceinline__ void term(const DTerm& t, long M, long N, long m0, lo
 // Four terms, the first two read with A along k (X) or along m (!X), the
 // other two the other way; the sequence is static so the accumulators stay
 // in registers from term to term.
-template <bool X>
+template <bool X, bool G = true>
 __device__ __forceinline__ void tile4(const DMulti& p, long m0, long n0, long z, Smem& S) {
   d4 acc[IW][IW];
 #pragma unroll
   for (int i = 0; i < IW; ++i)
 #pragma unroll
     for (int j = 0; j < IW; ++j) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
-  term<X>(p.t[0], p.M, p.N, m0, n0, z, acc, S);
-  term<X>(p.t[1], p.M, p.N, m0, n0, z, acc, S);
-  term<!X>(p.t[2], p.M, p.N, m0, n0, z, acc, S);
-  term<!X>(p.t[3], p.M, p.N, m0, n0, z, acc, S);
+  term<X, G>(p.t[0], p.M, p.N, m0, n0, z, acc, S);
+  term<X, G>(p.t[1], p.M, p.N, m0, n0, z, acc, S);
+  term<!X, G>(p.t[2], p.M, p.N, m0, n0, z, acc, S);
+  term<!X, G>(p.t[3], p.M, p.N, m0, n0, z, acc, S);
   __syncthreads();  // the next item reuses the LDS
   // D[i][j] of a 16x16 f64 MFMA tile: lane = j + 16*(i%4), element i/4.
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -574,7 +576,13 @@ __global__ __launch_bounds__(256, 2) void k_dtile4(DMulti p, long tiles_m, long 
   __shared__ dt::Smem S;
   long tm, tn;
   tile_xcd(blockIdx.x, tiles_m, tiles_n, tm, tn);
-  dt::tile4<X>(p, tm * 128, tn * 128, blockIdx.z, S);
+  const long m0 = tm * 128, n0 = tn * 128;
+  bool inside = m0 + 128 <= p.M && n0 + 128 <= p.N;
+  for (int t = 0; t < 4; ++t) inside = inside && p.t[t].K % dt::TK == 0;
+  if (inside)
+    dt::tile4<X, false>(p, m0, n0, blockIdx.z, S);
+  else
+    dt::tile4<X, true>(p, m0, n0, blockIdx.z, S);
 }
 
 template <typename WT>
