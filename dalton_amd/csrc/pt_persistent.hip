@@ -410,9 +410,9 @@ __device__ __forceinline__ void energy_item(const EArgs& r, const WT* W_, int i,
       const double f5 = 3.0 * wy + 3.0 * (wE * yE + wO * yO) - 2.0 * (wE + wO) * (yE + yO);
       // each distinct tuple once: slots alias when indices coincide
       const double wt = (a == b && b == c) ? (1.0 / 6.0) : ((a == b || b == c) ? 0.5 : 1.0);
-      const double D = eijk - eps[o + a] - eps[o + b] - eps[o + c];
-      s4 += wt * f4 / D;
-      s5 += wt * f5 / D;
+      const double rD = wt / (eijk - eps[o + a] - eps[o + b] - eps[o + c]);
+      s4 += f4 * rD;
+      s5 += f5 * rD;
     }
   }
   for (int off = 32; off > 0; off >>= 1) {
