@@ -20,6 +20,9 @@ SIGNATURES = {
     "ccgpu_last_error": (C.c_char_p, []),
     "ccgpu_init": (C.c_int, [C.c_int, C.c_int, C.c_int, _dp, _dp, C.c_int, C.POINTER(C.c_void_p)]),
     "ccgpu_finalize": (C.c_int, [C.c_void_p]),
+    "ccgpu_df_factor_ao": (C.c_int, [C.c_int, C.c_int, _dp, _dp, C.c_double, C.c_long, _dp, C.POINTER(C.c_long)]),
+    "ccgpu_init_ao": (C.c_int, [C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, C.c_double, C.c_int,
+                                C.POINTER(C.c_void_p), C.POINTER(C.c_long)]),
     "ccgpu_rhs": (C.c_int, [C.c_void_p, _dp, _dp, _dp, _dp, C.c_int]),
     "ccgpu_lhtr": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _dp, _dp, _dp, _dp]),
     "ccgpu_ccsd_guess": (C.c_int, [C.c_void_p, _dp]),

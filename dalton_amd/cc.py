@@ -41,6 +41,20 @@ def unpack_t2(p: np.ndarray, o: int, v: int) -> np.ndarray:
     return M.reshape(o, v, o, v).transpose(1, 0, 3, 2).copy()
 
 
+def df_factor_ao(cmo: np.ndarray, eri_pairs: np.ndarray, nbas: int, thr: float = 1e-12, maxvec: int = 0):
+    """MO Cholesky factor B[P,p,q] of AO integrals (ccgpu_df_factor_ao); maxvec
+    0 allows one vector per AO pair."""
+    cmo = np.ascontiguousarray(cmo, dtype=np.float64).reshape(-1)
+    eri_pairs = np.ascontiguousarray(eri_pairs, dtype=np.float64).reshape(-1)
+    nmo = cmo.size // nbas
+    npair = nbas * (nbas + 1) // 2
+    cap = npair if maxvec <= 0 else min(maxvec, npair)
+    out = np.empty(cap * nmo * nmo)
+    naux = C.c_long()
+    check(lib().ccgpu_df_factor_ao(nbas, nmo, ptr(cmo), ptr(eri_pairs), thr, cap, ptr(out), C.byref(naux)))
+    return out[: naux.value * nmo * nmo].reshape(naux.value, nmo, nmo)
+
+
 class CCContext:
     def __init__(self, eps: np.ndarray, B: np.ndarray, nocc: int, mxdiis: int = 8):
         eps = np.ascontiguousarray(eps, dtype=np.float64)
@@ -52,6 +66,28 @@ class CCContext:
         h = C.c_void_p()
         check(lib().ccgpu_init(self.o, self.v, self.naux, ptr(eps), ptr(B), mxdiis, C.byref(h)))
         self._h = h
+
+    @classmethod
+    def from_ao(cls, eps: np.ndarray, cmo: np.ndarray, eri_pairs: np.ndarray, nocc: int, thr: float = 1e-12,
+                mxdiis: int = 8):
+        """Context from AO integrals (ccgpu_init_ao): eri_pairs is the packed lower
+        triangle of the AO pair supermatrix, cmo the MO coefficients column by
+        column (cmo[p*nbas + a] = C[a,p]); the Cholesky factor is formed on the GPU."""
+        eps = np.ascontiguousarray(eps, dtype=np.float64)
+        cmo = np.ascontiguousarray(cmo, dtype=np.float64).reshape(-1)
+        eri_pairs = np.ascontiguousarray(eri_pairs, dtype=np.float64).reshape(-1)
+        N = eps.shape[0]
+        nbas = cmo.size // N
+        if nbas * N != cmo.size or eri_pairs.size != (nbas * (nbas + 1) // 2) * (nbas * (nbas + 1) // 2 + 1) // 2:
+            raise ValueError("cmo must hold nbas x len(eps) values and eri_pairs the packed pair supermatrix")
+        self = cls.__new__(cls)
+        self.o, self.v = nocc, N - nocc
+        h, naux = C.c_void_p(), C.c_long()
+        check(lib().ccgpu_init_ao(self.o, self.v, nbas, ptr(eps), ptr(cmo), ptr(eri_pairs), thr, mxdiis,
+                                  C.byref(h), C.byref(naux)))
+        self._h = h
+        self.naux = naux.value
+        return self
 
     def close(self):
         if getattr(self, "_h", None):
@@ -245,6 +281,28 @@ class CCBatch:
         c, r = C.c_int(), C.c_int()
         check(lib().ccgpu_batch_stats(self._h, C.byref(c), C.byref(r)))
         return {"captures": c.value, "replays": r.value}
+
+    @classmethod
+    def from_ao(cls, eps: np.ndarray, cmo: np.ndarray, eri_pairs: np.ndarray, nocc: int, thr: float = 1e-12,
+                mxdiis: int = 8):
+        """Context from AO integrals (ccgpu_init_ao): eri_pairs is the packed lower
+        triangle of the AO pair supermatrix, cmo the MO coefficients column by
+        column (cmo[p*nbas + a] = C[a,p]); the Cholesky factor is formed on the GPU."""
+        eps = np.ascontiguousarray(eps, dtype=np.float64)
+        cmo = np.ascontiguousarray(cmo, dtype=np.float64).reshape(-1)
+        eri_pairs = np.ascontiguousarray(eri_pairs, dtype=np.float64).reshape(-1)
+        N = eps.shape[0]
+        nbas = cmo.size // N
+        if nbas * N != cmo.size or eri_pairs.size != (nbas * (nbas + 1) // 2) * (nbas * (nbas + 1) // 2 + 1) // 2:
+            raise ValueError("cmo must hold nbas x len(eps) values and eri_pairs the packed pair supermatrix")
+        self = cls.__new__(cls)
+        self.o, self.v = nocc, N - nocc
+        h, naux = C.c_void_p(), C.c_long()
+        check(lib().ccgpu_init_ao(self.o, self.v, nbas, ptr(eps), ptr(cmo), ptr(eri_pairs), thr, mxdiis,
+                                  C.byref(h), C.byref(naux)))
+        self._h = h
+        self.naux = naux.value
+        return self
 
     def close(self):
         if getattr(self, "_h", None):

@@ -38,6 +38,20 @@ ccgpu::GemmPrec prec_of(int precision) {
   CCGPU_REQUIRE(precision == CCGPU_FP64 || precision == CCGPU_MIXED, "unknown precision selector");
   return precision == CCGPU_MIXED ? ccgpu::GemmPrec::BF16X3 : ccgpu::GemmPrec::F64;
 }
+// A non-blocking stream for work outside a context, with its scratch released at the end.
+struct SideStream {
+  hipStream_t s = nullptr;
+  SideStream() {
+    if (!ccgpu::dryrun()) CCGPU_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  }
+  ~SideStream() {
+    if (ccgpu::dryrun() || !s) return;
+    (void)hipStreamSynchronize(s);
+    ccgpu::release_scratch(s);
+    (void)hipStreamDestroy(s);
+  }
+};
+
 // CCGPU_DRYRUN=1 (read once, at the first context or batch creation)
 bool dryrun_env() {
   static const bool dry = [] {
@@ -63,6 +77,41 @@ int ccgpu_init(int nocc, int nvir, int naux, const double* eps, const double* B,
     *ctx = nullptr;
     auto c = std::make_unique<ccgpu_ctx>();
     c->cc = std::make_unique<ccgpu::CCSD>(nocc, nvir, naux, eps, B, mxdiis);
+    *ctx = c.release();
+  });
+}
+
+int ccgpu_df_factor_ao(int nbas, int nmo, const double* cmo, const double* eri_ao, double thr, long maxvec,
+                       double* B, long* naux) {
+  return guarded([&] {
+    const bool dry = dryrun_env();
+    CCGPU_REQUIRE(naux != nullptr && maxvec > 0 && (dry || (cmo != nullptr && eri_ao != nullptr)),
+                  "ccgpu_df_factor_ao: null argument or no vector room");
+    SideStream ss;
+    bool conv = false;
+    const std::vector<double> f = ccgpu::df_factor_ao(nbas, nmo, cmo, eri_ao, thr, maxvec, naux, &conv, ss.s);
+    if (B != nullptr && !dry) std::memcpy(B, f.data(), f.size() * sizeof(double));
+    if (!conv) throw ccgpu::Error(4, "ccgpu_df_factor_ao: threshold not reached within maxvec vectors");
+  });
+}
+
+int ccgpu_init_ao(int nocc, int nvir, int nbas, const double* eps, const double* cmo, const double* eri_ao,
+                  double thr, int mxdiis, ccgpu_ctx** ctx, long* naux) {
+  return guarded([&] {
+    const bool dry = dryrun_env();
+    CCGPU_REQUIRE(ctx != nullptr && naux != nullptr &&
+                      (dry || (eps != nullptr && cmo != nullptr && eri_ao != nullptr)),
+                  "ccgpu_init_ao: null argument");
+    *ctx = nullptr;
+    std::vector<double> f;
+    {
+      SideStream ss;
+      bool conv = false;
+      f = ccgpu::df_factor_ao(nbas, nocc + nvir, cmo, eri_ao, thr, 0, naux, &conv, ss.s);
+    }
+    CCGPU_REQUIRE(*naux > 0 && *naux < (1L << 31), "ccgpu_init_ao: no Cholesky vectors above the threshold");
+    auto c = std::make_unique<ccgpu_ctx>();
+    c->cc = std::make_unique<ccgpu::CCSD>(nocc, nvir, (int)*naux, eps, f.data(), mxdiis);
     *ctx = c.release();
   });
 }

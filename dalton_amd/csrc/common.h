@@ -179,6 +179,11 @@ BMulti bf16x3_desc(const BTerm* terms, int nterm, long M, long N, int batch, voi
 // Launch a product described by bf16x3_desc (batch as given there).
 void bf16x3_run(const BMulti& p, int batch, bool c_float, double flops, hipStream_t s);
 
+// MO density-fitting factor of AO integrals by pivoted Cholesky (cholesky.hip):
+// returns naux x nmo x nmo values; maxvec <= 0 allows one vector per AO pair.
+std::vector<double> df_factor_ao(int nbas, int nmo, const double* cmo, const double* eri_ao, double thr, long maxvec,
+                                 long* naux, bool* converged, hipStream_t st);
+
 // V+/- of one particle-ladder a-batch from an FP32 Y block straight into the
 // split-bf16 operand layout hi/lo[2][Rp][Kp] (V+ then V-), zero padded.
 void ladder_vpm_pack(unsigned short* hi, unsigned short* lo, const float* Y, long ldY, int a0, int a1, int v,

@@ -68,6 +68,28 @@ int ccgpu_rhs(ccgpu_ctx* ctx, const double* t1am, const double* t2am, double* om
 int ccgpu_lhtr(ccgpu_ctx* ctx, int precision, int nsimtr, const double* ctr1, const double* ctr2, double* rho1,
                double* rho2);
 
+/* The density-fitting factor from atomic-orbital integrals, the first step of
+ * the in-Dalton drop-in: Dalton's CC code reads AO integral distributions
+ * (CCRDAO, cc/ccsd_rhs.F:5652) and MO coefficients (SIRIFC, CMO), and its
+ * Cholesky module decomposes the AO integrals (CC_CHOLESKY,
+ * choles/cc_cholesky.F:2).  Here: pivoted Cholesky decomposition on the GPU
+ * of the AO pair supermatrix V[(ab)][(cd)] = (ab|cd) until the largest
+ * remaining diagonal is below thr, then B[P][p][q] = sum C[a][p] L_P[ab] C[b][q].
+ * eri_ao: the packed lower triangle of V over pairs (ab) = a(a+1)/2 + b,
+ * a >= b, element (P, Q), P >= Q, at P(P+1)/2 + Q (nbas(nbas+1)/2 pairs);
+ * cmo: the MO coefficients column by column, C[a][p] at cmo[p*nbas + a]
+ * (Dalton's CMO array for one symmetry block), nmo <= nbas columns.
+ * B (may be NULL) receives naux x nmo x nmo values; at most maxvec vectors
+ * are made (returns 4 if the threshold is not reached within them). */
+int ccgpu_df_factor_ao(int nbas, int nmo, const double* cmo, const double* eri_ao, double thr, long maxvec,
+                       double* B, long* naux);
+
+/* ccgpu_init from AO integrals: the factor of ccgpu_df_factor_ao (no vector
+ * limit) with nmo = nocc + nvir, then the context as ccgpu_init; *naux
+ * receives the number of Cholesky vectors. */
+int ccgpu_init_ao(int nocc, int nvir, int nbas, const double* eps, const double* cmo, const double* eri_ao,
+                  double thr, int mxdiis, ccgpu_ctx** ctx, long* naux);
+
 /* MP2 starting guess and its energy: CCSD_GUESS (cc/ccsd_energy.F:1654) followed
  * by CCSD_ECCSD (cc/ccsd_energy.F:1964).  e_mp2 is the correlation energy. */
 int ccgpu_ccsd_guess(ccgpu_ctx* ctx, double* e_mp2);

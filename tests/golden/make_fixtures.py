@@ -20,6 +20,7 @@ Molecules (geometries from the reference's test inputs):
                     output exists for it)
 
 Usage:  python tests/golden/make_fixtures.py [/path/to/reference]
+        python tests/golden/make_fixtures.py /path/to/reference --ao   (AO fixture of h2o_ccpvdz_test)
 """
 from __future__ import annotations
 
@@ -71,7 +72,31 @@ def build(name: str, spec: dict) -> dict:
     return dict(eps=eps, Bpk=B[:, iu[0], iu[1]], escf=E, enuc=enuc, nocc=spec["nocc"])
 
 
+def build_ao(name: str, spec: dict) -> dict:
+    """AO-side fixture of one molecule for the AO -> density-fitting front end:
+    the AO two-electron integrals as the packed lower triangle of the pair
+    supermatrix (pairs (a,b), a >= b, at a(a+1)/2 + b; element (P,Q), P >= Q,
+    at P(P+1)/2 + Q), the canonical MO coefficients column by column
+    (cmo[p*nbas + a] = C[a,p]) and the orbital energies."""
+    bas = gto.Basis(spec["atoms"], os.path.join(REF, "basis", spec["basis"]))
+    S, T, V = gto.one_electron(bas)
+    G = gto.two_electron(bas)
+    enuc = bas.nuclear_repulsion()
+    E, eps, C = gto.rhf(S, T + V, G, spec["nocc"], enuc)
+    n = bas.nbf
+    il = np.tril_indices(n)
+    Vs = G[il[0], il[1]][:, il[0], il[1]]
+    ip = np.tril_indices(Vs.shape[0])
+    print(f"{name} (AO): nbf={n} pairs={Vs.shape[0]} E_SCF={E:.12f}")
+    return dict(eri_pairs=Vs[ip[0], ip[1]], cmo=np.ascontiguousarray(C.T).reshape(-1), eps=eps, escf=E,
+                enuc=enuc, nocc=spec["nocc"], nbas=n)
+
+
 if __name__ == "__main__":
+    if "--ao" in sys.argv:
+        d = build_ao("h2o_ccpvdz_test", MOLECULES["h2o_ccpvdz_test"])
+        np.savez_compressed(os.path.join(HERE, "h2o_ccpvdz_test_ao.npz"), **d)
+        sys.exit(0)
     for name, spec in MOLECULES.items():
         d = build(name, spec)
         np.savez_compressed(os.path.join(HERE, name + ".npz"), **d)

@@ -106,3 +106,31 @@ def test_dryrun_batch_bounds():
                        env=env, capture_output=True, text=True, timeout=600)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "ok batch" in r.stdout
+
+
+AO_SCRIPT = r"""
+import ctypes as C, sys
+sys.path.insert(0, {root!r})
+from dalton_amd import _lib
+L = _lib.lib()
+for nbas, nocc, nvir in ((24, 5, 19), (7, 2, 3), (130, 20, 110)):
+    naux = C.c_long()
+    rc = L.ccgpu_df_factor_ao(nbas, nocc + nvir, None, None, 1e-10, 50, None, C.byref(naux))
+    assert rc == 0, L.ccgpu_last_error()
+    h = C.c_void_p()
+    rc = L.ccgpu_init_ao(nocc, nvir, nbas, None, None, None, 1e-10, 8, C.byref(h), C.byref(naux))
+    assert rc == 0, L.ccgpu_last_error()
+    e = C.c_double()
+    assert L.ccgpu_ccsd_guess(h, C.byref(e)) == 0, L.ccgpu_last_error()
+    assert L.ccgpu_finalize(h) == 0
+    print("ok", nbas, naux.value, flush=True)
+"""
+
+
+def test_dryrun_ao_front_end():
+    """Spans of the AO Cholesky front end (a few vectors) and of the context it builds."""
+    env = dict(os.environ, CCGPU_DRYRUN="1")
+    r = subprocess.run([sys.executable, "-c", AO_SCRIPT.format(root=ROOT)], env=env, capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert r.stdout.count("ok") == 3
