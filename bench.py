@@ -198,6 +198,57 @@ def pt_workload(o, v, naux, seed, ntrip):
             "workload": f"synthetic o={o} v={v} naux={naux} seed {seed}", "pt": pt}
 
 
+def ao_workload(nbas=120, rank=1500, seed=6):
+    """AO -> density-fitting front end (ccgpu_df_factor_ao) on a synthetic
+    positive semidefinite pair supermatrix of known rank (V = X^T X over the
+    nbas(nbas+1)/2 AO pairs) and random orthonormal MO coefficients, against
+    the oracle's single-column pivoted Cholesky on the host cores."""
+    import dalton_amd as d
+    from oracle import cholesky as ch
+
+    rng = np.random.default_rng(seed)
+    M = nbas * (nbas + 1) // 2
+    X = rng.standard_normal((rank, M)) / np.sqrt(rank)
+    V = X.T @ X
+    il = np.tril_indices(M)
+    packed = V[il]
+    C = np.linalg.qr(rng.standard_normal((nbas, nbas)))[0]
+    cmo = np.ascontiguousarray(C.T).reshape(-1)
+    res = {"metric": "AO pivoted Cholesky + MO transform", "unit": "s",
+           "workload": f"synthetic PSD pair supermatrix, nbas={nbas} ({M} pairs), rank {rank}, seed {seed}",
+           "threshold": 1e-10}
+    sync()  # the first call imports torch: outside the timed region
+    d.df_factor_ao(np.eye(4).reshape(-1), np.eye(10)[np.tril_indices(10)], 4, 1e-10)  # code objects, allocator
+    t0 = time.perf_counter()
+    B = d.df_factor_ao(cmo, packed, nbas, 1e-10)  # synchronous: returns with B on the host
+    dt = time.perf_counter() - t0
+    K = B.shape[0]
+    res["gpu"] = {"seconds": round(dt, 4), "naux": K, "ms_per_vector": round(dt / max(K, 1) * 1e3, 4),
+                  "note": "host arrays in and out: includes the 0.2 GB upload of the "
+                  "packed supermatrix and the download of B"}
+    _log(f"ao: GPU {K} vectors in {dt:.3f} s")
+    t0 = time.perf_counter()
+    L, _, conv = ch.pivoted_cholesky(V, 1e-10)
+    dc = time.perf_counter() - t0
+    res["cpu_baseline"] = {"value": round(dc, 3), "unit": "s", "kind": "port", "cores": len(os.sched_getaffinity(0)),
+                           "sample": "oracle pivoted_cholesky on the same supermatrix (decomposition only)"}
+    _log(f"ao: oracle {L.shape[0]} vectors in {dc:.3f} s")
+    Bo = np.einsum("ap,kab,bq->kpq", C, _pairs_square(L, nbas), C, optimize=True)
+    g = np.einsum("kpq,krs->pqrs", B[:, :24, :24], B[:, :24, :24], optimize=True)
+    go = np.einsum("kpq,krs->pqrs", Bo[:, :24, :24], Bo[:, :24, :24], optimize=True)
+    res["naux_oracle"] = int(L.shape[0])
+    res["max_abs_dev_integrals_24_mos"] = float(np.abs(g - go).max())
+    return res
+
+
+def _pairs_square(L, nbas):
+    il = np.tril_indices(nbas)
+    S = np.zeros((L.shape[0], nbas, nbas))
+    S[:, il[0], il[1]] = L
+    S[:, il[1], il[0]] = L
+    return S
+
+
 def pt48_workload(o=48, v=384, naux=1536, seed=5):
     """(T) over every i>=j>=k triple at o=48 v=384 naux=1536, FP64 and mixed,
     after converging CCSD in the same precision.  The integral scale is the
@@ -359,9 +410,9 @@ def main():
                     help="(T) triples timed, spread evenly over the i>=j>=k list (0: skip)")
     ap.add_argument("--lhtr-vectors", type=int, default=16,
                     help="left-transformation trial vectors timed at o=32 v=256 naux=1024 (0: skip)")
-    ap.add_argument("--workload", choices=["ccsd", "pt", "pt48", "batch", "copy", "blas"], default="ccsd",
+    ap.add_argument("--workload", choices=["ccsd", "pt", "pt48", "batch", "ao", "copy", "blas"], default="ccsd",
                     help="ccsd: the headline line (default); pt: the (T) leg alone at --o/--v; "
-                         "pt48: full (T) at o=48 v=384 in both "
+                         "ao: the AO Cholesky front end; pt48: full (T) at o=48 v=384 in both "
                          "precisions on a convergent system; batch: 32 uneven systems converged in "
                          "one graph replay per iteration")
     args = ap.parse_args()
@@ -371,6 +422,9 @@ def main():
         sys.exit(2)
     if args.workload == "pt":
         print(json.dumps(pt_workload(args.o, args.v, args.naux, args.seed, args.pt_triples)), flush=True)
+        return
+    if args.workload == "ao":
+        print(json.dumps(ao_workload()), flush=True)
         return
     if args.workload == "pt48":
         print(json.dumps(pt48_workload()), flush=True)
