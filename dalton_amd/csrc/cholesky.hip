@@ -5,17 +5,19 @@
 // cc/ccsd_rhs.F:5652, and its CMO) start the CC path, whose integrals enter
 // as B[P][p][q] with (pq|rs) = sum_P B[P][p][q] B[P][r][s].
 //
-// Decomposition: the single-column pivoted Cholesky of Beebe and Linderberg
-// (1977) as used by Koch, Sanchez de Meras and Pedersen (2003), the method of
-// the reference's CC_CHOLESKY (choles/cc_cholesky.F:2): repeatedly take the
-// largest remaining diagonal element V[p][p] of the pair supermatrix, stop
-// when it falls below the threshold, otherwise
+// Decomposition: the pivoted Cholesky of Beebe and Linderberg (1977) as used
+// by Koch, Sanchez de Meras and Pedersen (2003), the method of the
+// reference's CC_CHOLESKY (choles/cc_cholesky.F:2): repeatedly take the
+// largest remaining diagonal element d_p of the pair supermatrix, stop when it
+// falls below the threshold, otherwise
 //   L_k[q] = (V[p][q] - sum_{j<k} L_j[p] L_j[q]) / sqrt(d_p),  d_q -= L_k[q]^2.
 // The reference computes integral columns shell quadruple by shell quadruple
 // and qualifies several diagonals per pass; here the supermatrix is resident
-// in HBM and one column is added per step.  Either way every element of
-// V - sum_k L_k L_k^T is bounded by the final largest diagonal, below the
-// threshold, since V is positive semidefinite.
+// in HBM and the vectors come in panels: within a panel each pivot is chosen
+// from the current diagonal exactly as above (the same vectors), the sum over
+// earlier panels is kept in the residual R = V - sum L L^T, updated by one
+// rank-64 GEMM per panel.  Every element of the final residual is bounded by
+// the final largest diagonal, below the threshold (V is positive semidefinite).
 #include <algorithm>
 #include <cmath>
 #include <vector>
@@ -73,18 +75,34 @@ __global__ __launch_bounds__(1024) void k_argmax(const double* __restrict__ d, l
   }
 }
 
-// Vector k for pivot p: L[k][q] = (V[p][q] - sum_{j<k} L[j][p] L[j][q]) / sqrt(d_p);
-// d[q] -= L[k][q]^2, and d[p] = 0 exactly.
-__global__ void k_chol_column(double* __restrict__ L, double* __restrict__ d, const double* __restrict__ V, long M,
-                              long k, long p, double dp) {
+// Step j of the panel that starts at vector K0: pivot p = argmax of the
+// remaining diagonal (piv[1], written on the device), then
+//   L[K0 + j][q] = (R[p][q] - sum_{i<j} L[K0+i][p] L[K0+i][q]) / sqrt(d_p),
+//   d[q] -= L[K0 + j][q]^2, d[p] = 0,
+// where R = V minus the vectors of earlier panels.  Stops (st[0] = 1,
+// st[1] = vectors made) once the largest diagonal is below thr.
+__global__ void k_chol_panel_col(double* __restrict__ L, double* __restrict__ d, const double* __restrict__ R,
+                                 const double* __restrict__ piv, long* __restrict__ st, long M, long K0, int j,
+                                 double thr) {
+  if (st[0] != 0) return;
+  const double dp = piv[0];
+  if (dp < thr) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      st[0] = 1;
+      st[1] = K0 + j;
+    }
+    return;
+  }
+  const long p = (long)piv[1];
   const double rs = 1.0 / sqrt(dp);
   for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < M; q += (long)gridDim.x * blockDim.x) {
-    double s = V[p * M + q];
-    for (long j = 0; j < k; ++j) s -= L[j * M + p] * L[j * M + q];
+    double s = R[p * M + q];
+    for (int i = 0; i < j; ++i) s -= L[(K0 + i) * M + p] * L[(K0 + i) * M + q];
     const double l = s * rs;
-    L[k * M + q] = l;
+    L[(K0 + j) * M + q] = l;
     d[q] = (q == p) ? 0.0 : d[q] - l * l;
   }
+  if (blockIdx.x == 0 && threadIdx.x == 0) st[1] = K0 + j + 1;
 }
 
 // S[k][a][b] = S[k][b][a] = L[k][a(a+1)/2 + b] (a >= b): one pair vector as a symmetric n x n matrix.
@@ -116,24 +134,44 @@ std::vector<double> df_factor_ao(int nbas, int nmo, const double* cmo, const dou
   CCGPU_LAUNCH(k_unpack_pairs, dim3(grid_for(M * M)), dim3(256), 0, st, V.p, Vp.p, M);
   Vp.release();
   CCGPU_LAUNCH(k_diag, dim3(grid_for(M)), dim3(256), 0, st, d.p, V.p, M);
+  // Panels of kNb vectors: pivots chosen on the device one by one (each
+  // corrected by its own panel only), then R -= L_panel^T L_panel as one
+  // FP64 MFMA GEMM; one host synchronisation per panel.
+  constexpr int kNb = 64;
+  DBuf<long> state(2);
+  CCGPU_HIPX(hipMemsetAsync(state.p, 0, state.bytes(), st));
   long K = 0;
   *converged = false;
-  for (;;) {
-    CCGPU_LAUNCH(k_argmax, dim3(1), dim3(1024), 0, st, d.p, M, piv.p);
-    double h[2] = {0.0, 0.0};
-    CCGPU_HIPX(hipMemcpyAsync(h, piv.p, sizeof(h), hipMemcpyDeviceToHost, st));
+  while (true) {
+    const int nb = (int)std::min<long>(kNb, Kmax - K);
+    if (nb > 0) check_span(L.p, 0, (K + nb) * M - 1, sizeof(double), "cholesky L");
+    for (int j = 0; j < nb; ++j) {
+      CCGPU_LAUNCH(k_argmax, dim3(1), dim3(1024), 0, st, d.p, M, piv.p);
+      CCGPU_LAUNCH(k_chol_panel_col, dim3(grid_for(M)), dim3(256), 0, st, L.p, d.p, V.p, piv.p, state.p, M, K, j,
+                   thr);
+    }
+    long h[2] = {0, K + nb};
+    CCGPU_HIPX(hipMemcpyAsync(h, state.p, sizeof(h), hipMemcpyDeviceToHost, st));
     CCGPU_HIPX(hipStreamSynchronize(st));
-    if (dryrun()) h[0] = (K < std::min<long>(Kmax, 3)) ? 1.0 : 0.0;  // a few vectors, to check the spans
-    if (h[0] < thr) {  // every remaining diagonal below the threshold (all zero after M vectors)
+    if (dryrun()) h[0] = 1, h[1] = std::min<long>(Kmax, 3);  // a few vectors, to check the spans
+    const long K1 = h[1];
+    CCGPU_REQUIRE(K1 >= K && K1 <= K + nb, "df_factor_ao: inconsistent vector count");
+    if (h[0] != 0) {  // the largest remaining diagonal fell below the threshold
+      K = K1;
       *converged = true;
       break;
     }
-    if (K == Kmax) break;  // no room for another vector
-    const long p = (long)h[1];
-    CCGPU_REQUIRE(p >= 0 && p < M, "df_factor_ao: pivot out of range");
-    check_span(L.p, 0, (K + 1) * M - 1, sizeof(double), "cholesky L");
-    CCGPU_LAUNCH(k_chol_column, dim3(grid_for(M)), dim3(256), 0, st, L.p, d.p, V.p, M, K, p, h[0]);
-    ++K;
+    K = K1;
+    if (K == Kmax) {  // no room for another vector: converged only if nothing above the threshold remains
+      CCGPU_LAUNCH(k_argmax, dim3(1), dim3(1024), 0, st, d.p, M, piv.p);
+      double hv[2] = {0.0, 0.0};
+      CCGPU_HIPX(hipMemcpyAsync(hv, piv.p, sizeof(hv), hipMemcpyDeviceToHost, st));
+      CCGPU_HIPX(hipStreamSynchronize(st));
+      *converged = hv[0] < thr;
+      break;
+    }
+    // R[m][n] -= sum_{i < nb} L[K - nb + i][m] L[K - nb + i][n]
+    g(M, M, nb, -1.0, L.p + (K - nb) * M, 1, M, L.p + (K - nb) * M, M, 1, 1.0, V.p, M, 1, st);
   }
   CCGPU_HIPX(hipGetLastError());
   V.release();
