@@ -33,11 +33,12 @@ def _T(x):
     return torch.as_tensor(x, dtype=torch.float64)
 
 
-def residual_torch(sys: DFSystem, t1, t2):
-    """Same equations as oracle.ccsd.residual, in torch (t1 [a,i], t2 [a,i,b,j])."""
+def residual_torch(sys: DFSystem, t1, t2, h=None):
+    """Same equations as oracle.ccsd.residual, in torch (t1 [a,i], t2 [a,i,b,j]);
+    h (N x N tensor) replaces sys.h when given."""
     o, v, N = sys.o, sys.v, sys.N
     B = _T(sys.B)
-    h = _T(sys.h)
+    h = _T(sys.h) if h is None else h
     X = torch.eye(N, dtype=torch.float64)
     Y = torch.eye(N, dtype=torch.float64)
     X = X.clone()

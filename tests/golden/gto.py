@@ -38,7 +38,7 @@ class Shell:
 def read_dalton_basis(path: str, Z: int) -> list[tuple[int, np.ndarray, np.ndarray]]:
     """Parse one element block ("a Z") of a Dalton basis-library file.
 
-    Format (Dalton basis/*): after "a Z", blocks of
+    Format (Dalton basis/*): after "a Z" (or "A Z"), blocks of
       nprim ncontr 0
       exp c_1 ... c_ncontr     (one line per primitive, may wrap)
     in increasing angular momentum, comment lines start with '$'.
@@ -47,7 +47,7 @@ def read_dalton_basis(path: str, Z: int) -> list[tuple[int, np.ndarray, np.ndarr
         lines = fh.read().splitlines()
     start = None
     for k, ln in enumerate(lines):
-        if re.match(r"^a\s+%d\s*$" % Z, ln):
+        if re.match(r"^[aA]\s+%d\s*$" % Z, ln):
             start = k + 1
             break
     if start is None:
@@ -57,7 +57,7 @@ def read_dalton_basis(path: str, Z: int) -> list[tuple[int, np.ndarray, np.ndarr
     k = start
     while k < len(lines):
         ln = lines[k].strip()
-        if ln.startswith("a "):
+        if ln.startswith(("a ", "A ")):
             break
         if not ln or ln.startswith("$"):
             k += 1
@@ -328,6 +328,38 @@ def one_electron(basis: Basis):
             T[ia0 : ia0 + blk.shape[0], ib0 : ib0 + blk.shape[1]] = _contract_pair(sa, sb, Tp)
             V[ia0 : ia0 + blk.shape[0], ib0 : ib0 + blk.shape[1]] = _contract_pair(sa, sb, Vp)
     return S, T, V
+
+
+def dipole(basis: Basis, origin=(0.0, 0.0, 0.0)) -> np.ndarray:
+    """Dipole-length integrals <mu| r_d - origin_d |nu>, shape (3, n, n).
+
+    Per dimension, (x - C) = (x - B) + (B - C): the first term raises the
+    x power of the ket primitive by one, so the moment is an overlap with
+    lb + 1 plus (B - C) times the plain overlap."""
+    n = basis.nbf
+    M = np.zeros((3, n, n))
+    Cc = np.asarray(origin, float)
+    for A, sa in enumerate(basis.shells):
+        for B, sb in enumerate(basis.shells):
+            a = sa.exps[:, None]
+            b = sb.exps[None, :]
+            p = a + b
+            Sx = [hermite_E(sa.l, sb.l + 1, a, b, sa.center[d], sb.center[d])[:, :, 0] for d in range(3)]
+            pref = (np.pi / p) ** 1.5
+            ca, cb = cart_powers(sa.l), cart_powers(sb.l)
+            Mp = np.zeros((3, len(sa.exps), len(sb.exps), len(ca), len(cb)))
+            for ia, pa in enumerate(ca):
+                for ib, pb in enumerate(cb):
+                    s1 = [Sx[d][pa[d], pb[d]] for d in range(3)]
+                    for d in range(3):
+                        m1 = Sx[d][pa[d], pb[d] + 1] + (sb.center[d] - Cc[d]) * s1[d]
+                        e, f = [s1[k] for k in range(3) if k != d]
+                        Mp[d, :, :, ia, ib] = pref * m1 * e * f
+            ia0, ib0 = basis.off[A], basis.off[B]
+            for d in range(3):
+                blk = _contract_pair(sa, sb, Mp[d])
+                M[d, ia0 : ia0 + blk.shape[0], ib0 : ib0 + blk.shape[1]] = blk
+    return M
 
 
 def two_electron(basis: Basis) -> np.ndarray:

@@ -18,9 +18,14 @@ Molecules (geometries from the reference's test inputs):
 * h2o_ccpvdz_base   the water geometry O (0,0,0), H (0,+-0.757,0.586) A
                     used for the timing/parity configuration (no Dalton
                     output exists for it)
+* h2o_sto3g_exgr    DALTON/test/cc_exgr_prop (STO-3G, bohr), whose output
+                    holds the unrelaxed CCSD dipole moment; this fixture also
+                    stores the MO dipole-length integrals (origin 0, as
+                    printed there), dip[d][p][q]
 
 Usage:  python tests/golden/make_fixtures.py [/path/to/reference]
         python tests/golden/make_fixtures.py /path/to/reference --ao   (AO fixture of h2o_ccpvdz_test)
+        python tests/golden/make_fixtures.py /path/to/reference --only NAME   (one molecule)
 """
 from __future__ import annotations
 
@@ -53,6 +58,12 @@ MOLECULES = {
                (1, np.array([0.0, 0.757, 0.586]) * A),
                (1, np.array([0.0, -0.757, 0.586]) * A)],
         nocc=5, adapt=False),
+    "h2o_sto3g_exgr": dict(
+        basis="STO-3G",
+        atoms=[(8, np.array([0.0, 0.0, -0.13594222])),
+               (1, np.array([0.0, 1.4476145, 1.07875057])),
+               (1, np.array([0.0, -1.4476145, 1.07875057]))],
+        nocc=5, adapt=False, dipole=True),
 }
 
 
@@ -69,7 +80,13 @@ def build(name: str, spec: dict) -> dict:
     err = np.abs(np.einsum("Ppq,Prs->pqrs", B, B, optimize=True) - Gmo).max()
     iu = np.triu_indices(bas.nbf)
     print(f"{name}: nbf={bas.nbf} naux={B.shape[0]} E_nuc={enuc:.12f} E_SCF={E:.12f} chol_err={err:.1e}")
-    return dict(eps=eps, Bpk=B[:, iu[0], iu[1]], escf=E, enuc=enuc, nocc=spec["nocc"])
+    out = dict(eps=eps, Bpk=B[:, iu[0], iu[1]], escf=E, enuc=enuc, nocc=spec["nocc"])
+    if spec.get("dipole"):
+        M = gto.dipole(bas)
+        out["dip"] = np.einsum("dab,ap,bq->dpq", M, C, C, optimize=True)
+        mu = -2.0 * np.einsum("dii->d", out["dip"][:, : spec["nocc"], : spec["nocc"]])
+        print(f"{name}: electronic SCF dipole {mu}")
+    return out
 
 
 def build_ao(name: str, spec: dict) -> dict:
@@ -97,6 +114,9 @@ if __name__ == "__main__":
         d = build_ao("h2o_ccpvdz_test", MOLECULES["h2o_ccpvdz_test"])
         np.savez_compressed(os.path.join(HERE, "h2o_ccpvdz_test_ao.npz"), **d)
         sys.exit(0)
+    only = sys.argv[sys.argv.index("--only") + 1] if "--only" in sys.argv else None
     for name, spec in MOLECULES.items():
+        if only is not None and name != only:
+            continue
         d = build(name, spec)
         np.savez_compressed(os.path.join(HERE, name + ".npz"), **d)

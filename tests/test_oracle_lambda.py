@@ -1,0 +1,120 @@
+"""The oracle of the CCSD multiplier (L0) equations, eta and the unrelaxed
+one-electron density, against finite differences, a dense solve and the
+reference's printed unrelaxed CCSD dipole moment of water (CPU only)."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+from oracle import ccsd, lambda_eq as le
+
+GOLD = os.path.join(ROOT, "tests", "golden")
+
+
+def _synthetic(o, v, naux, seed):
+    from dalton_amd.synthetic import make_system
+
+    eps, B = make_system(o, v, naux, seed)
+    s = ccsd.DFSystem(eps, B, o)
+    r = ccsd.solve(s, maxiter=200, threnr=1e-13, thrvec=1e-11)
+    return s, ccsd.pack_t1(r["t1"]), ccsd.pack_t2(r["t2"])
+
+
+def _water_sto3g():
+    f = np.load(os.path.join(GOLD, "h2o_sto3g_exgr.npz"))
+    N = f["eps"].size
+    iu = np.triu_indices(N)
+    B = np.zeros((f["Bpk"].shape[0], N, N))
+    B[:, iu[0], iu[1]] = f["Bpk"]
+    B[:, iu[1], iu[0]] = f["Bpk"]
+    return f, ccsd.DFSystem(f["eps"], B, int(f["nocc"]))
+
+
+def test_eta_matches_finite_differences_of_the_energy():
+    from dalton_amd.synthetic import make_system
+
+    eps, B = make_system(3, 6, 18, 4)
+    s = ccsd.DFSystem(eps, B, 3)
+    s.h = s.h + 0.01 * np.eye(s.N)[::-1]  # nonzero F_ia, so the singles term counts
+    rng = np.random.default_rng(2)
+    nt1 = 18
+    t1 = 0.05 * rng.standard_normal(nt1)
+    t2 = 0.05 * rng.standard_normal(nt1 * (nt1 + 1) // 2)
+    e1, e2 = le.eta(s, t1, t2)
+
+    def E(a, b):
+        import torch
+
+        from oracle import lhtr
+
+        return float(le.energy_torch(s, torch.as_tensor(s.h), torch.as_tensor(a).reshape(3, 6).T,
+                                     lhtr._unpack_t2_torch(torch.as_tensor(b), 3, 6)))
+
+    h = 1e-5
+    for _ in range(8):
+        d1 = rng.standard_normal(t1.size)
+        d2 = rng.standard_normal(t2.size)
+        fd = (E(t1 + h * d1, t2 + h * d2) - E(t1 - h * d1, t2 - h * d2)) / (2 * h)
+        an = float(np.dot(e1, d1) + np.dot(e2, d2))
+        assert abs(fd - an) < 1e-8 * max(1.0, abs(an)), (fd, an)
+    # closed form of the doubles part: 2 L_iajb off the packed diagonal, L_iaia on it
+    g = s.g_vovo()
+    L = 2.0 * g - g.transpose(0, 3, 2, 1)
+    Lm = L.transpose(1, 0, 3, 2).reshape(nt1, nt1)
+    r, c = np.tril_indices(nt1)
+    ref2 = np.where(r == c, 1.0, 2.0) * Lm[r, c]
+    assert np.abs(e2 - ref2).max() < 1e-13
+
+
+def test_reduced_space_solution_equals_dense_solve():
+    s, t1p, t2p = _synthetic(2, 4, 12, 7)
+    l1, l2, info = le.solve_lambda(s, t1p, t2p, thr=1e-12)
+    A = le.explicit_A(s, t1p, t2p)
+    e1, e2 = le.eta(s, t1p, t2p)
+    lam = np.linalg.solve(A.T, -np.concatenate([e1, e2]))
+    assert np.abs(lam - np.concatenate([l1, l2])).max() < 1e-10
+    assert info["resnorm"] < 1e-12
+
+
+def test_collapsed_space_reaches_the_same_solution():
+    s, t1p, t2p = _synthetic(3, 5, 15, 9)
+    l1, l2, _ = le.solve_lambda(s, t1p, t2p, thr=1e-11)
+    m1, m2, info = le.solve_lambda(s, t1p, t2p, thr=1e-11, maxred=3)
+    assert np.abs(l1 - m1).max() < 1e-9 and np.abs(l2 - m2).max() < 1e-9
+
+
+@pytest.mark.parametrize("seed", [3, 11])
+def test_density_is_the_derivative_of_the_ccsd_energy(seed):
+    """<V> from the multipliers equals dE/dx for h -> h + x V at fixed
+    orbitals, the amplitudes re-solved at each x (central differences)."""
+    s, t1p, t2p = _synthetic(3, 5, 15, seed)
+    l1, l2, _ = le.solve_lambda(s, t1p, t2p, thr=1e-12)
+    D = le.density(s, t1p, t2p, l1, l2)
+    rng = np.random.default_rng(seed)
+    V = rng.standard_normal((s.N, s.N))
+    V = 0.5 * (V + V.T)
+    x = 1e-4
+    fd = (le.total_energy_change(s, V, x, 1e-14) - le.total_energy_change(s, V, -x, 1e-14)) / (2 * x)
+    an = float(np.sum(D * V))
+    assert abs(fd - an) < 1e-7 * max(1.0, abs(an)), (fd, an)
+
+
+def test_water_sto3g_unrelaxed_ccsd_dipole_matches_dalton():
+    """DALTON/test/cc_exgr_prop: H2O/STO-3G, all electrons.  E_CCSD and the
+    electronic dipole moment of the unrelaxed CCSD density (origin 0)."""
+    gold = json.load(open(os.path.join(GOLD, "dalton_golden.json")))["h2o_sto3g_exgr"]
+    f, s = _water_sto3g()
+    assert abs(float(f["escf"]) - gold["e_scf"]) < 1e-11
+    mu_scf = -2.0 * np.einsum("dii->d", f["dip"][:, : s.o, : s.o])
+    assert np.abs(mu_scf - np.array(gold["dipole_el_ccs"])).max() < 1e-8  # reference SCF converged to 7e-9
+    r = ccsd.solve(s, maxiter=200, threnr=1e-13, thrvec=1e-11)
+    assert abs(float(f["escf"]) + r["e_ccsd"] - gold["e_ccsd"]) < 1e-9
+    t1p, t2p = ccsd.pack_t1(r["t1"]), ccsd.pack_t2(r["t2"])
+    l1, l2, _ = le.solve_lambda(s, t1p, t2p, thr=1e-11)
+    D = le.density(s, t1p, t2p, l1, l2)
+    mu = -np.einsum("pq,dpq->d", D, f["dip"])
+    assert np.abs(mu - np.array(gold["dipole_el_ccsd"])).max() < 1e-8, mu
+    # a plausible mistake moves it far outside: the reference part alone is the SCF value
+    assert abs(mu[2] - mu_scf[2]) > 0.05
