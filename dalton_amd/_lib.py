@@ -25,6 +25,10 @@ SIGNATURES = {
                                 C.POINTER(C.c_void_p), C.POINTER(C.c_long)]),
     "ccgpu_rhs": (C.c_int, [C.c_void_p, _dp, _dp, _dp, _dp, C.c_int]),
     "ccgpu_lhtr": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _dp, _dp, _dp, _dp]),
+    "ccgpu_eta": (C.c_int, [C.c_void_p, _dp, _dp]),
+    "ccgpu_lambda_solve": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_int, C.c_int, _dp, _dp,
+                                     C.POINTER(C.c_int), _dp]),
+    "ccgpu_onedens": (C.c_int, [C.c_void_p, C.c_int, _dp, _dp, _dp]),
     "ccgpu_ccsd_guess": (C.c_int, [C.c_void_p, _dp]),
     "ccgpu_ccsd_iterate": (C.c_int, [C.c_void_p, C.c_int, _dp, _dp]),
     "ccgpu_ccsd_energy": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_int, _dp, _dp,

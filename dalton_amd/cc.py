@@ -142,6 +142,35 @@ class CCContext:
             check(lib().ccgpu_lhtr(self._h, precision, n, ptr(c1), ptr(c2), ptr(r1), ptr(r2)))
         return (r1[0], r2[0]) if single else (r1, r2)
 
+    def eta(self):
+        """CC_ETA at the current amplitudes: eta = dE/dt, packed storage."""
+        e1 = np.empty(self.nt1)
+        e2 = np.empty(self.nt2)
+        check(lib().ccgpu_eta(self._h, ptr(e1), ptr(e2)))
+        return e1, e2
+
+    def lambda_solve(self, thr: float = 1e-10, maxit: int = 100, maxred: int = 20, precision: int = FP64):
+        """Ground-state multipliers: lambda A + eta = 0 (CCEQ_SOL for 'L0').
+        Returns (l1, l2, niter, resnorm)."""
+        l1 = np.empty(self.nt1)
+        l2 = np.empty(self.nt2)
+        it = C.c_int()
+        rn = C.c_double()
+        check(lib().ccgpu_lambda_solve(self._h, precision, thr, maxit, maxred, ptr(l1), ptr(l2), C.byref(it),
+                                       C.byref(rn)))
+        return l1, l2, it.value, rn.value
+
+    def onedens(self, l1: np.ndarray, l2: np.ndarray, precision: int = FP64) -> np.ndarray:
+        """Unrelaxed MO one-electron density D[p,q] = dL/dh_pq (CC_D1AO, MO basis)."""
+        l1 = np.ascontiguousarray(l1, dtype=np.float64)
+        l2 = np.ascontiguousarray(l2, dtype=np.float64)
+        if l1.size != self.nt1 or l2.size != self.nt2:
+            raise ValueError("multipliers must have NT1AMX and NT2AMX elements")
+        N = self.o + self.v
+        D = np.empty((N, N))
+        check(lib().ccgpu_onedens(self._h, precision, ptr(l1), ptr(l2), ptr(D)))
+        return D
+
     def guess(self) -> float:
         e = C.c_double()
         check(lib().ccgpu_ccsd_guess(self._h, C.byref(e)))

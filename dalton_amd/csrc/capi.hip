@@ -139,6 +139,30 @@ int ccgpu_lhtr(ccgpu_ctx* ctx, int precision, int nsimtr, const double* ctr1, co
   });
 }
 
+int ccgpu_eta(ccgpu_ctx* ctx, double* eta1, double* eta2) {
+  return guarded([&] {
+    CCGPU_REQUIRE(ctx && eta1 && eta2, "ccgpu_eta: null argument");
+    ctx->cc->eta_host(eta1, eta2);
+  });
+}
+
+int ccgpu_lambda_solve(ccgpu_ctx* ctx, int precision, double thr, int maxit, int maxred, double* l1, double* l2,
+                       int* niter, double* resnorm) {
+  return guarded([&] {
+    CCGPU_REQUIRE(ctx && l1 && l2, "ccgpu_lambda_solve: null argument");
+    ctx->cc->set_precision(prec_of(precision));
+    ctx->cc->lambda_solve(thr, maxit, maxred, l1, l2, resnorm, niter);
+  });
+}
+
+int ccgpu_onedens(ccgpu_ctx* ctx, int precision, const double* l1, const double* l2, double* dens) {
+  return guarded([&] {
+    CCGPU_REQUIRE(ctx && l1 && l2 && dens, "ccgpu_onedens: null argument");
+    ctx->cc->set_precision(prec_of(precision));
+    ctx->cc->onedens_host(l1, l2, dens);
+  });
+}
+
 int ccgpu_ccsd_guess(ccgpu_ctx* ctx, double* e_mp2) {
   return guarded([&] {
     CCGPU_REQUIRE(ctx, "ccgpu_ccsd_guess: null context");

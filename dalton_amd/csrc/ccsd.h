@@ -75,6 +75,16 @@ class CCSD {
   // and reused for every vector; the DIIS history is released first.
   void lhtr_host(int nvec, const double* l1, const double* l2p, double* rho1, double* rho2p);
 
+  // Ground-state multipliers (lambda.hip): eta = dE/dt (CC_ETA), the
+  // reduced-space solution of lambda A + eta = 0 (CCEQ_SOL from the left;
+  // returns the iteration count, also in *niter, and throws code 4 if not
+  // converged, lambda and *niter still set) and the
+  // unrelaxed MO one-electron density D[p][q] = dL/dh[p][q] (CC_D1AO).
+  // Host arrays in the layouts of lhtr_host.
+  void eta_host(double* e1, double* e2p);
+  int lambda_solve(double thr, int maxit, int maxred, double* l1, double* l2p, double* resnorm, int* niter);
+  void onedens_host(const double* l1, const double* l2p, double* D);
+
   const double* t1() const { return t1_.p; }
   const double* T() const { return T_.p; }
   const double* eps() const { return eps_.p; }
@@ -104,6 +114,8 @@ class CCSD {
   void lhtr_prepare();
   void lhtr_apply(const double* l1, double* Xb, double* rho1, double* rho2s);
   void lhtr_ladder(const double* Xb);
+  void lhtr_packed(const double* x, double* y);
+  void eta_device(double* e);
 
   Dims dm_;
   int mxdiis_;

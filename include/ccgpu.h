@@ -68,6 +68,39 @@ int ccgpu_rhs(ccgpu_ctx* ctx, const double* t1am, const double* t2am, double* om
 int ccgpu_lhtr(ccgpu_ctx* ctx, int precision, int nsimtr, const double* ctr1, const double* ctr2, double* rho1,
                double* rho2);
 
+/* Right-hand side of the ground-state multiplier equations at the context's
+ * current amplitudes: replaces CC_ETA (cc/cc_fop.F:2955) for MODEL = CCSD
+ * with canonical Hartree-Fock orbitals.  eta = dE/dt in the packed storage
+ * of ccgpu_lhtr (eta1 NT1AMX, eta2 NT2AMX values):
+ *   eta1[(a,i)] = 2 sum_bj L_iajb t_bj,
+ *   eta2[ai<=bj] = 2 L_iajb (ai < bj), L_iaia (ai = bj),  L_iajb = 2(ia|jb) - (ib|ja). */
+int ccgpu_eta(ccgpu_ctx* ctx, double* eta1, double* eta2);
+
+/* Solves the ground-state multiplier (L0) equations lambda A + eta = 0 at
+ * the context's current amplitudes: replaces the CCEQ_SOL call for the list
+ * 'L0' in CC_FOP (cc/cc_fop.F:274; cc/cceq_solv.F:20, ISIDE = -1) by a
+ * reduced-space solver whose trial vectors stay in device memory.  A is the
+ * Jacobian of ccgpu_lhtr, both sides packed, so lambda is the multiplier of
+ * L = E + sum_p lambda_p Omega_p with packed dot products.  Converged when
+ * the 2-norm of lambda A + eta is below thr (Dalton's THRLEQ); at most maxit
+ * transformations, maxred trial vectors (capped at 64 and by free device
+ * memory; a full space collapses onto the current solution).  precision 1
+ * (BF16x3 bulk) converges to about 1e-5 |eta| and then continues in FP64.
+ * l1 (NT1AMX) and l2 (NT2AMX) receive lambda; niter and resnorm (may be
+ * NULL) the transformation count and final residual norm.  Returns 4 if thr
+ * is not reached within maxit (the last lambda is still returned). */
+int ccgpu_lambda_solve(ccgpu_ctx* ctx, int precision, double thr, int maxit, int maxred, double* l1, double* l2,
+                       int* niter, double* resnorm);
+
+/* Unrelaxed one-electron density in the MO basis from the amplitudes and
+ * multipliers lambda (l1, l2 as returned by ccgpu_lambda_solve): CC_D1AO
+ * (cc/cc_d1ao.F:20) without its back-transformation to the AO basis.
+ * dens[p*N + q] = dL/dh_pq, N = nocc + nvir, reference part 2 delta_ij
+ * included, so that an unrelaxed first-order property (CC_FOP) of a
+ * one-electron operator X is sum_pq dens[p*N + q] X_pq (times the particle
+ * charge, -1 for electrons).  The density is not symmetrised. */
+int ccgpu_onedens(ccgpu_ctx* ctx, int precision, const double* l1, const double* l2, double* dens);
+
 /* The density-fitting factor from atomic-orbital integrals, the first step of
  * the in-Dalton drop-in: Dalton's CC code reads AO integral distributions
  * (CCRDAO, cc/ccsd_rhs.F:5652) and MO coefficients (SIRIFC, CMO), and its

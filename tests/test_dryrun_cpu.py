@@ -51,6 +51,13 @@ for (o, v, naux) in {shapes!r}:
     for prec, nvec in ((0, 2), (1, 1), (0, 1)):  # left transformation, both precisions
         rc = L.ccgpu_lhtr(h, prec, nvec, _lib.ptr(d), _lib.ptr(d), _lib.ptr(d), _lib.ptr(d))
         assert rc == 0, L.ccgpu_last_error()
+    # multiplier equations: eta, the reduced-space solve (both precisions), density
+    assert L.ccgpu_eta(h, _lib.ptr(d), _lib.ptr(d)) == 0, L.ccgpu_last_error()
+    it = C.c_int(); rn = C.c_double()
+    for prec in (0, 1):
+        rc = L.ccgpu_lambda_solve(h, prec, 1e-8, 5, 4, _lib.ptr(d), _lib.ptr(d), C.byref(it), C.byref(rn))
+        assert rc == 0, L.ccgpu_last_error()
+    assert L.ccgpu_onedens(h, 0, _lib.ptr(d), _lib.ptr(d), _lib.ptr(d)) == 0, L.ccgpu_last_error()
     rc = L.ccgpu_ccsd_iterate(h, 0, C.byref(e), C.byref(n))  # DIIS history comes back after lhtr
     assert rc == 0, L.ccgpu_last_error()
     assert L.ccgpu_finalize(h) == 0

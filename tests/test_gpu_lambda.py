@@ -1,0 +1,128 @@
+"""HIP ground-state multipliers: eta (CC_ETA), the reduced-space solution of
+lambda A + eta = 0 (CCEQ_SOL for 'L0') and the unrelaxed one-electron
+density (CC_D1AO), against the oracle and against the reference's printed
+unrelaxed CCSD dipole moment of water (STO-3G)."""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, load_molecule
+from oracle import ccsd as occ
+from oracle import lambda_eq as ole
+
+pytestmark = pytest.mark.gpu
+
+
+def _synthetic(o, v, naux, seed=11):
+    from dalton_amd.synthetic import make_system
+
+    eps, B = make_system(o, v, naux, seed)
+    return eps, B, o
+
+
+def _converged(d, eps, B, o):
+    c = d.CCContext(eps, B, o)
+    c.e_ccsd = c.ccsd_energy(maxit=200, threnr=1e-12, thrvec=1e-10)["e_ccsd"]
+    return c
+
+
+@pytest.mark.parametrize("name", ["h2o_ccpvdz_test", "synthetic"])
+def test_eta_matches_oracle(name):
+    import dalton_amd as d
+
+    eps, B, o = _synthetic(4, 23, 60) if name == "synthetic" else load_molecule(name)[:3]
+    print(f"eta vs oracle {name}", flush=True)
+    rng = np.random.default_rng(3)
+    with d.CCContext(eps, B, o) as c:
+        t1 = 0.03 * rng.standard_normal(c.nt1)
+        t2 = 0.03 * rng.standard_normal(c.nt2)
+        c.set_amplitudes(t1, t2)
+        e1, e2 = c.eta()
+    q1, q2 = ole.eta(occ.DFSystem(eps, B, o), t1, t2)
+    s = max(1.0, np.abs(q2).max())
+    assert np.abs(e1 - q1).max() < 1e-12 * s and np.abs(e2 - q2).max() < 1e-12 * s
+
+
+def test_onedens_matches_oracle_at_random_multipliers():
+    import dalton_amd as d
+
+    eps, B, o = _synthetic(4, 23, 60)
+    print("onedens vs oracle o4 v23", flush=True)
+    rng = np.random.default_rng(8)
+    with d.CCContext(eps, B, o) as c:
+        t1 = 0.03 * rng.standard_normal(c.nt1)
+        t2 = 0.03 * rng.standard_normal(c.nt2)
+        l1 = rng.standard_normal(c.nt1)
+        l2 = rng.standard_normal(c.nt2)
+        c.set_amplitudes(t1, t2)
+        D = c.onedens(l1, l2)
+    Q = ole.density(occ.DFSystem(eps, B, o), t1, t2, l1, l2)
+    assert np.abs(D - Q).max() < 1e-11 * max(1.0, np.abs(Q).max()), np.abs(D - Q).max()
+
+
+def test_lambda_matches_oracle_water_ccpvdz():
+    import dalton_amd as d
+
+    eps, B, o, _ = load_molecule("h2o_ccpvdz_test")
+    print("lambda solve vs oracle h2o/cc-pVDZ", flush=True)
+    with _converged(d, eps, B, o) as c:
+        t1, t2 = c.amplitudes()
+        l1, l2, it, rn = c.lambda_solve(thr=1e-10)
+        D = c.onedens(l1, l2)
+        e1, e2 = c.eta()
+        r1, r2 = c.lhtr(l1, l2)
+    assert rn < 1e-10 and 0 < it < 60
+    # the returned multipliers satisfy the equations through the separate lhtr entry point
+    assert np.sqrt(np.sum((r1 + e1) ** 2) + np.sum((r2 + e2) ** 2)) < 2e-10
+    s = occ.DFSystem(eps, B, o)
+    q1, q2, _ = ole.solve_lambda(s, t1, t2, thr=1e-11)
+    assert np.abs(l1 - q1).max() < 1e-8 and np.abs(l2 - q2).max() < 1e-8
+    Q = ole.density(s, t1, t2, q1, q2)
+    assert np.abs(D - Q).max() < 1e-8
+
+
+def test_unrelaxed_ccsd_dipole_matches_dalton(golden):
+    """DALTON/test/cc_exgr_prop (H2O/STO-3G): CCSD energy and the electronic
+    dipole moment of the unrelaxed CCSD density, all on the GPU."""
+    import dalton_amd as d
+
+    g = golden["h2o_sto3g_exgr"]
+    f = np.load(os.path.join(GOLDEN, "h2o_sto3g_exgr.npz"))
+    eps, B, o, escf = load_molecule("h2o_sto3g_exgr")
+    print("unrelaxed CCSD dipole h2o/STO-3G", flush=True)
+    with _converged(d, eps, B, o) as c:
+        e = c.e_ccsd
+        l1, l2, it, rn = c.lambda_solve(thr=1e-10)
+        D = c.onedens(l1, l2)
+    assert abs(escf + e - g["e_ccsd"]) < 1e-9
+    mu = -np.einsum("pq,dpq->d", D, f["dip"])
+    assert np.abs(mu - np.array(g["dipole_el_ccsd"])).max() < 1e-8, mu
+
+
+def test_mixed_solve_ends_in_fp64_at_the_same_multipliers():
+    import dalton_amd as d
+
+    eps, B, o, _ = load_molecule("h2o_ccpvdz_test")
+    print("lambda solve mixed vs FP64 h2o/cc-pVDZ", flush=True)
+    with _converged(d, eps, B, o) as c:
+        a1, a2, _, _ = c.lambda_solve(thr=1e-10)
+        m1, m2, it, rn = c.lambda_solve(thr=1e-10, precision=d.MIXED)
+    assert rn < 1e-10
+    assert np.abs(a1 - m1).max() < 1e-9 and np.abs(a2 - m2).max() < 1e-9
+
+
+def test_small_space_collapse_and_iteration_limit():
+    import dalton_amd as d
+    from dalton_amd._lib import CCGPUError
+
+    eps, B, o = _synthetic(4, 23, 60, seed=5)
+    print("lambda solve collapse / limit o4 v23", flush=True)
+    with _converged(d, eps, B, o) as c:
+        a1, a2, it_a, _ = c.lambda_solve(thr=1e-10, maxred=30)
+        b1, b2, it_b, _ = c.lambda_solve(thr=1e-10, maxred=3)
+        with pytest.raises(CCGPUError) as ei:
+            c.lambda_solve(thr=1e-10, maxit=2)
+    assert ei.value.code == 4
+    assert np.abs(a1 - b1).max() < 1e-9 and np.abs(a2 - b2).max() < 1e-9
+    assert it_b >= it_a
