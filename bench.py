@@ -294,6 +294,57 @@ def pt48_workload(o=48, v=384, naux=1536, seed=5):
     return res
 
 
+def lambda_workload(o=32, v=256, naux=1024, seed=4, thr=1e-8):
+    """Ground-state multipliers at o=32 v=256 naux=1024: CCSD converged in
+    FP64 (untimed), then the L0 equations lambda A + eta = 0 solved to
+    |r| < thr from the FP64 context and from the mixed one (BF16x3 phase,
+    then FP64), and the unrelaxed density of each; the integral scale is the
+    convergent one of dalton_amd.synthetic.batch_sigma."""
+    import dalton_amd as d
+    from dalton_amd.cc import profile_start, profile_stop
+    from dalton_amd.synthetic import batch_sigma, make_system
+
+    sigma = batch_sigma(v, naux)
+    eps, B = make_system(o, v, naux, seed, sigma=sigma)
+    res = {"metric": "L0 multiplier solve at o=32 v=256 naux=1024, FP64 vs mixed", "unit": "s",
+           "workload": f"synthetic o={o} v={v} naux={naux} seed {seed} sigma {sigma:.5f}, thr {thr:g}",
+           "flops_per_transform_nominal": 2.0 * f_ccsd(o, v, naux)}
+    rng = np.random.default_rng(seed)
+    V = rng.standard_normal((o + v, o + v))
+    V = 0.5 * (V + V.T)
+    with d.CCContext(eps, B, o) as c:
+        del B
+        t0 = time.perf_counter()
+        r = c.ccsd_energy(maxit=80, threnr=1e-10, precision=d.FP64)
+        sync()
+        _log(f"CCSD {r['niter']} iterations in {time.perf_counter() - t0:.1f} s")
+        res["ccsd_iterations"] = r["niter"]
+        c.lhtr(np.zeros(c.nt1), np.zeros(c.nt2), d.FP64)  # library warm-up (untimed)
+        out = {}
+        for label, prec in (("fp64", d.FP64), ("mixed", d.MIXED)):
+            sync()
+            profile_start()
+            t0 = time.perf_counter()
+            l1, l2, it, rn = c.lambda_solve(thr=thr, maxit=100, maxred=20, precision=prec)
+            sync()
+            dt = time.perf_counter() - t0
+            prof = profile_stop()
+            D = c.onedens(l1, l2)
+            gem = sum(k_["flops"] for k_ in prof["kernels"].values())
+            gms = sum(k_["ms"] for k_ in prof["kernels"].values())
+            res[label] = {"seconds": round(dt, 3), "transforms": it, "s_per_transform": round(dt / it, 4),
+                          "resnorm": rn, "gemm_tflops": round(gem / max(gms, 1e-9) / 1e9, 2),
+                          "property_V": float(np.sum(D * V)), "kernels": kern_table_of(prof)}
+            out[label] = (l1, l2)
+            _log(f"lambda {label}: {it} transforms in {dt:.2f} s, |r| = {rn:.2e}")
+    a1, a2 = out["fp64"]
+    b1, b2 = out["mixed"]
+    res["max_abs_dev_lambda_mixed_vs_fp64"] = float(max(np.abs(a1 - b1).max(), np.abs(a2 - b2).max()))
+    res["abs_dev_property_mixed_vs_fp64"] = abs(res["mixed"]["property_V"] - res["fp64"]["property_V"])
+    res["speedup_mixed_vs_fp64"] = round(res["fp64"]["seconds"] / res["mixed"]["seconds"], 3)
+    return res
+
+
 def batch_workload(nsys=32, seed=5):
     """32 uneven systems (o ~ U{5..30}, v = 8..12 o, naux = 4v, convergent
     integral scale) converged together: FP64, one HIP graph replay per
@@ -410,9 +461,10 @@ def main():
                     help="(T) triples timed, spread evenly over the i>=j>=k list (0: skip)")
     ap.add_argument("--lhtr-vectors", type=int, default=16,
                     help="left-transformation trial vectors timed at o=32 v=256 naux=1024 (0: skip)")
-    ap.add_argument("--workload", choices=["ccsd", "pt", "pt48", "batch", "ao", "copy", "blas"], default="ccsd",
+    ap.add_argument("--workload", choices=["ccsd", "pt", "pt48", "batch", "ao", "lambda", "copy", "blas"], default="ccsd",
                     help="ccsd: the headline line (default); pt: the (T) leg alone at --o/--v; "
-                         "ao: the AO Cholesky front end; pt48: full (T) at o=48 v=384 in both "
+                         "ao: the AO Cholesky front end; lambda: the L0 multiplier solve at "
+                         "o=32 v=256 in both precisions; pt48: full (T) at o=48 v=384 in both "
                          "precisions on a convergent system; batch: 32 uneven systems converged in "
                          "one graph replay per iteration")
     args = ap.parse_args()
@@ -425,6 +477,9 @@ def main():
         return
     if args.workload == "ao":
         print(json.dumps(ao_workload()), flush=True)
+        return
+    if args.workload == "lambda":
+        print(json.dumps(lambda_workload()), flush=True)
         return
     if args.workload == "pt48":
         print(json.dumps(pt48_workload()), flush=True)

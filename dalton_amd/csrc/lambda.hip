@@ -22,9 +22,16 @@
 // formed on the device.  The next trial vector is -r / Delta (Delta the
 // orbital-energy differences of each packed element), orthogonalised twice
 // against the space.  A full space collapses onto lambda (its transform is
-// r - eta, so the collapse costs no transformation).  With the BF16x3 bulk
-// arithmetic the solve first converges to a loose threshold, then continues
-// in FP64 from that lambda (a new space whose first vector is lambda).
+// r - eta, so the collapse costs no transformation).
+//
+// With the BF16x3 bulk arithmetic the solve is an iterative refinement: the
+// forward intermediates are formed once in FP64; each step takes the FP64
+// residual r64 = lambda A + eta (one FP64 transformation) and solves the
+// correction delta A + r64 = 0 to 1e-4 |r64| in a space of BF16x3
+// transformations, which is kept from step to step (its matrix does not
+// change, only the right-hand side), then lambda += delta.  The converged
+// lambda satisfies the FP64 equations; the BF16x3 rounding (about 1e-5
+// relative) only slows the contraction of each step.
 //
 // Density: D_pq = dL/dh_pq.  One left transformation of lambda leaves in Fb
 // the cotangent of h~ after the first half of the T1 dressing
@@ -148,23 +155,31 @@ int CCSD::lambda_solve(double thr, int maxit, int maxred, double* l1, double* l2
   CCGPU_REQUIRE(thr > 0.0 && maxit > 0 && maxred >= 2, "ccgpu_lambda_solve: bad threshold, iteration or space limit");
   const long n = dm_.ntamp, nov = dm_.nov;
   const GemmPrec user = bulk_;
+  const bool refine = user != GemmPrec::F64;
   maxred = std::min(maxred, 64);  // dots and coefficients staged in 128-entry device arrays
+  // forward intermediates in FP64 for both modes: the BF16x3 transforms of
+  // the refinement use them with BF16x3 bulk products
+  set_precision(GemmPrec::F64);
   lhtr_prepare();
   ensure_lhtr();
-  // space for eta, r, w, lambda and 2 x maxred vectors, within the free memory
+  // space for eta, r, w, lambda, the FP64 residual, the correction and 2 x
+  // maxred vectors, within the free memory
+  constexpr long kFixed = 6;
   if (!dryrun()) {
     size_t fr = 0, tot = 0;
     CCGPU_HIP(hipMemGetInfo(&fr, &tot));
-    const long fit = (long)((0.9 * (double)fr / (8.0 * (double)n) - 4.0) / 2.0);
+    const long fit = (long)((0.9 * (double)fr / (8.0 * (double)n) - kFixed) / 2.0);
     maxred = (int)std::min<long>(maxred, fit);
     CCGPU_REQUIRE(maxred >= 2, "ccgpu_lambda_solve: not enough device memory for the reduced space");
   }
-  DBuf<double> work((size_t)(4 + 2 * (long)maxred) * n), sm(256);
+  DBuf<double> work((size_t)(kFixed + 2 * (long)maxred) * n), sm(256);
   double* eta = work.p;
-  double* r = work.p + n;
-  double* w = work.p + 2 * n;
+  double* r = work.p + n;       // residual of the reduced-space solution
+  double* w = work.p + 2 * n;   // Gram-Schmidt work
   double* lam = work.p + 3 * n;
-  double* Bs = work.p + 4 * n;
+  double* r64 = work.p + 4 * n;  // FP64 residual lambda A + eta (refinement)
+  double* dl = work.p + 5 * n;   // correction of one refinement step
+  double* Bs = work.p + kFixed * n;
   double* Ss = Bs + (long)maxred * n;
   double* dd = sm.p;        // dots
   double* cd = sm.p + 128;  // coefficients
@@ -184,11 +199,21 @@ int CCSD::lambda_solve(double thr, int maxit, int maxred, double* l1, double* l2
   };
   // (a dry run reads no values: unit norms keep its control flow going)
   auto norm = [&](const double* x) { return dryrun() ? 1.0 : std::sqrt(std::max(0.0, dots(x, x, 1)[0])); };
+  int it = 0;
+  // y = x A in the given arithmetic (the forward intermediates stay FP64)
+  auto transform = [&](const double* x, double* y, GemmPrec p) {
+    set_precision(p);
+    lhtr_packed(x, y);
+    set_precision(GemmPrec::F64);
+    ++it;
+  };
 
-  std::vector<double> G((size_t)maxred * maxred), gr(maxred);
+  // Reduced space: G_kl = b_k . sigma_l; the right-hand side is rebuilt from
+  // the vector f of the equations x A + f = 0 (f = eta, or the FP64 residual).
+  std::vector<double> G((size_t)maxred * maxred);
   int K = 0;
-  // Appends x (overwritten) to the space: two Gram-Schmidt passes, normalise, transform.
-  auto add = [&](double* x) {
+  const GemmPrec sp = refine ? user : GemmPrec::F64;  // arithmetic of the space's transforms
+  auto add = [&](double* x) {  // appends x (overwritten): two Gram-Schmidt passes, normalise, transform
     for (int pass = 0; pass < 2 && K > 0; ++pass) {
       std::vector<double> d = dots(x, Bs, K);
       for (double& z : d) z = -z;
@@ -198,73 +223,82 @@ int CCSD::lambda_solve(double thr, int maxit, int maxred, double* l1, double* l2
     const double nx = norm(x);
     if (!(nx > 1e-14)) return false;
     axpby(Bs + (long)K * n, 1.0 / nx, x, 0.0, nullptr, n, st_);
-    lhtr_packed(Bs + (long)K * n, Ss + (long)K * n);
+    transform(Bs + (long)K * n, Ss + (long)K * n, sp);
     const std::vector<double> row = dots(Bs + (long)K * n, Ss, K + 1);  // b_K . sigma_l
     const std::vector<double> col = dots(Ss + (long)K * n, Bs, K);      // b_l . sigma_K
     for (int l = 0; l <= K; ++l) G[(size_t)K * maxred + l] = row[l];
     for (int l = 0; l < K; ++l) G[(size_t)l * maxred + K] = col[l];
-    gr[K] = -dots(Bs + (long)K * n, eta, 1)[0];
     ++K;
     return true;
   };
-
-  // phases: the context arithmetic, then FP64 from its lambda when that was
-  // BF16x3 (whose rounding floors the residual near 1e-5 |eta|: the first
-  // phase ends there, or once the residual stops halving every two steps)
-  std::vector<std::pair<GemmPrec, double>> phases;
-  if (user != GemmPrec::F64) phases.push_back({user, std::max(thr, 1e-5 * norm(eta))});
-  phases.push_back({GemmPrec::F64, thr});
-  int it = 0;
-  double rn = 0.0;
-  bool conv = false;
-  for (size_t ph = 0; ph < phases.size() && !conv; ++ph) {
-    set_precision(phases[ph].first);
-    lhtr_prepare();
-    K = 0;
-    if (ph == 0) {
-      CCGPU_LAUNCH(k_precond_packed, dim3(grid_of(n)), dim3(256), 0, st_, r, eta, eps_.p, dm_.o, dm_.v, nov, n);
-    } else {
-      axpby(r, 1.0, lam, 0.0, nullptr, n, st_);
-    }
-    if (!add(r)) {  // eta = 0: lambda = 0
-      fill(lam, 0.0, n, st_);
-      rn = 0.0;
-      conv = true;
-      break;
-    }
+  // Solves x A + f = 0 in the space (x into out, r its residual) until
+  // |r| < tol, the space gives no new direction or the transformation limit
+  // is reached; returns |r|.  stall: also leave once |r| stops halving every
+  // two steps.  A full space collapses onto x (FP64 solve) or ends the call
+  // (refinement step; the caller restarts the space).
+  auto solve_space = [&](const double* f, double tol, bool stall, double* out) {
     std::vector<double> hist;
-    while (it < maxit) {
-      ++it;
-      std::vector<double> Gk((size_t)K * K), gk(K), c;
-      for (int a = 0; a < K; ++a) {
-        gk[a] = gr[a];
-        for (int b = 0; b < K; ++b) Gk[(size_t)a * K + b] = G[(size_t)a * maxred + b];
+    if (K == 0) {
+      CCGPU_LAUNCH(k_precond_packed, dim3(grid_of(n)), dim3(256), 0, st_, r, f, eps_.p, dm_.o, dm_.v, nov, n);
+      if (!add(r)) {
+        fill(out, 0.0, n, st_);
+        return 0.0;
       }
+    }
+    while (true) {
+      std::vector<double> Gk((size_t)K * K), gk = dots(f, Bs, K), c;
+      for (double& z : gk) z = -z;
+      for (int a = 0; a < K; ++a)
+        for (int b = 0; b < K; ++b) Gk[(size_t)a * K + b] = G[(size_t)a * maxred + b];
       CCGPU_REQUIRE(small_solve(Gk, gk, K, c) || dryrun(), "ccgpu_lambda_solve: singular reduced matrix");
       if (dryrun()) c.assign(K, 0.0);
-      combine(lam, Bs, c);
+      combine(out, Bs, c);
       combine(r, Ss, c);
-      axpy(r, 1.0, eta, n, st_);
-      rn = norm(r);
+      axpy(r, 1.0, f, n, st_);
+      const double rn = norm(r);
       hist.push_back(rn);
-      const bool last = ph + 1 == phases.size();
-      const bool stalled = !last && hist.size() >= 3 && rn > 0.5 * hist[hist.size() - 3];
-      if (rn < phases[ph].second || stalled || dryrun()) {
-        conv = rn < thr || dryrun();
-        break;
-      }
-      if (K == maxred) {  // collapse onto lambda; its transform is r - eta
-        const double nl = norm(lam);
-        axpby(Bs, 1.0 / nl, lam, 0.0, nullptr, n, st_);
-        axpby(Ss, 1.0 / nl, r, -1.0 / nl, eta, n, st_);
-        G[0] = dots(Bs, Ss, 1)[0];
-        gr[0] = -dots(Bs, eta, 1)[0];
-        K = 1;
+      const bool stalled = stall && hist.size() >= 3 && rn > 0.5 * hist[hist.size() - 3];
+      if (rn < tol || stalled || it >= maxit || dryrun()) return rn;
+      if (K == maxred) {
+        if (refine) return rn;  // the outer step takes this correction; the space restarts
+        {  // collapse onto x; its transform is r - f
+          const double nl = norm(out);
+          axpby(Bs, 1.0 / nl, out, 0.0, nullptr, n, st_);
+          axpby(Ss, 1.0 / nl, r, -1.0 / nl, f, n, st_);
+          G[0] = dots(Bs, Ss, 1)[0];
+          K = 1;
+        }
       }
       CCGPU_LAUNCH(k_precond_packed, dim3(grid_of(n)), dim3(256), 0, st_, r, r, eps_.p, dm_.o, dm_.v, nov, n);
-      if (!add(r)) break;
+      if (!add(r)) return rn;
+    }
+  };
+
+  double rn = 0.0;
+  if (!refine) {
+    rn = solve_space(eta, thr, false, lam);
+  } else {
+    // iterative refinement: r64 = lambda A + eta in FP64, the correction
+    // delta A + r64 = 0 solved to 1e-4 |r64| in the BF16x3 space (kept across
+    // steps: its matrix does not change), lambda += delta
+    fill(lam, 0.0, n, st_);
+    axpby(r64, 1.0, eta, 0.0, nullptr, n, st_);
+    rn = norm(r64);
+    while (rn >= thr && it < maxit && !dryrun()) {
+      solve_space(r64, 1e-4 * rn, true, dl);
+      axpy(lam, 1.0, dl, n, st_);
+      if (K == maxred) K = 0;
+      transform(lam, r64, GemmPrec::F64);
+      axpy(r64, 1.0, eta, n, st_);
+      rn = norm(r64);
+    }
+    if (dryrun()) {  // one pass of each step for the span checks
+      solve_space(r64, 1.0, true, dl);
+      axpy(lam, 1.0, dl, n, st_);
+      transform(lam, r64, GemmPrec::F64);
     }
   }
+  const bool conv = rn < thr || dryrun();
   set_precision(user);
   if (l1) CCGPU_HIPX(hipMemcpyAsync(l1, lam, nov * sizeof(double), hipMemcpyDeviceToHost, st_));
   if (l2p) CCGPU_HIPX(hipMemcpyAsync(l2p, lam + nov, (n - nov) * sizeof(double), hipMemcpyDeviceToHost, st_));
@@ -272,7 +306,7 @@ int CCSD::lambda_solve(double thr, int maxit, int maxred, double* l1, double* l2
   CCGPU_HIPX(hipGetLastError());
   if (resnorm) *resnorm = rn;
   if (niter) *niter = it;
-  if (!conv) throw Error(4, "ccgpu_lambda_solve: residual norm above the threshold after maxit iterations");
+  if (!conv) throw Error(4, "ccgpu_lambda_solve: residual norm above the threshold after maxit transformations");
   return it;
 }
 

@@ -85,7 +85,9 @@ int ccgpu_eta(ccgpu_ctx* ctx, double* eta1, double* eta2);
  * the 2-norm of lambda A + eta is below thr (Dalton's THRLEQ); at most maxit
  * transformations, maxred trial vectors (capped at 64 and by free device
  * memory; a full space collapses onto the current solution).  precision 1
- * (BF16x3 bulk) converges to about 1e-5 |eta| and then continues in FP64.
+ * (BF16x3 bulk) runs an iterative refinement: the FP64 residual of lambda is
+ * formed once per step and the correction solved with BF16x3 transformations,
+ * so the result meets thr in FP64 arithmetic; niter counts both kinds.
  * l1 (NT1AMX) and l2 (NT2AMX) receive lambda; niter and resnorm (may be
  * NULL) the transformation count and final residual norm.  Returns 4 if thr
  * is not reached within maxit (the last lambda is still returned). */
