@@ -345,6 +345,59 @@ def lambda_workload(o=32, v=256, naux=1024, seed=4, thr=1e-8):
     return res
 
 
+def restart_workload(o=64, v=512, naux=2048, seed=3):
+    """Dalton 'R0' amplitude file at o=64 v=512: MP2 amplitudes of the
+    synthetic system written with ccgpu_save_amplitudes (device -> host ->
+    Fortran records; the 4.3 GB doubles record is split into gfortran
+    subrecords) and read back with ccgpu_restart_amplitudes, on local disk."""
+    import os
+    import shutil
+    import struct
+    import tempfile
+
+    import dalton_amd as d
+    from dalton_amd.synthetic import make_system
+
+    eps, B = make_system(o, v, naux, seed)
+    res = {"metric": "R0 file save / restart at o=64 v=512", "unit": "s",
+           "workload": f"synthetic o={o} v={v} naux={naux} seed {seed}, MP2 amplitudes"}
+    tmp = tempfile.mkdtemp(prefix="ccgpu_rsp_")
+    try:
+        with d.CCContext(eps, B, o) as c:
+            del B
+            e_mp2 = c.guess()
+            path = os.path.join(tmp, d.rsp_filename("R0"))
+            sync()
+            t0 = time.perf_counter()
+            c.save_amplitudes(path)
+            ts = time.perf_counter() - t0
+            size = os.path.getsize(path)
+            _log(f"save: {size / 1e9:.2f} GB in {ts:.2f} s")
+            with open(path, "rb") as fh:  # count the subrecords of the doubles record (the fourth)
+                recs = []
+                while len(recs) < 4:
+                    nsub = 0
+                    while True:
+                        (lead,) = struct.unpack("<i", fh.read(4))
+                        fh.seek(abs(lead), 1)
+                        fh.read(4)
+                        nsub += 1
+                        if lead >= 0:
+                            break
+                    recs.append(nsub)
+            t0 = time.perf_counter()
+            e = c.restart_amplitudes(path)
+            tr = time.perf_counter() - t0
+            _log(f"restart: {tr:.2f} s, E = {e:.12f} (saved {e_mp2:.12f})")
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    res.update({"file_bytes": size, "doubles_subrecords": recs[3], "save_seconds": round(ts, 3),
+                "save_GBps": round(size / ts / 1e9, 3), "restart_seconds": round(tr, 3),
+                "restart_GBps": round(size / tr / 1e9, 3), "energy_saved": e_mp2, "energy_restarted": e,
+                "abs_dE": abs(e - e_mp2)})
+    return res
+
+
 def batch_workload(nsys=32, seed=5):
     """32 uneven systems (o ~ U{5..30}, v = 8..12 o, naux = 4v, convergent
     integral scale) converged together: FP64, one HIP graph replay per
@@ -461,10 +514,10 @@ def main():
                     help="(T) triples timed, spread evenly over the i>=j>=k list (0: skip)")
     ap.add_argument("--lhtr-vectors", type=int, default=16,
                     help="left-transformation trial vectors timed at o=32 v=256 naux=1024 (0: skip)")
-    ap.add_argument("--workload", choices=["ccsd", "pt", "pt48", "batch", "ao", "lambda", "copy", "blas"], default="ccsd",
+    ap.add_argument("--workload", choices=["ccsd", "pt", "pt48", "batch", "ao", "lambda", "restart", "copy", "blas"], default="ccsd",
                     help="ccsd: the headline line (default); pt: the (T) leg alone at --o/--v; "
                          "ao: the AO Cholesky front end; lambda: the L0 multiplier solve at "
-                         "o=32 v=256 in both precisions; pt48: full (T) at o=48 v=384 in both "
+                         "o=32 v=256 in both precisions; restart: the R0 amplitude file at o=64 v=512; pt48: full (T) at o=48 v=384 in both "
                          "precisions on a convergent system; batch: 32 uneven systems converged in "
                          "one graph replay per iteration")
     args = ap.parse_args()
@@ -480,6 +533,9 @@ def main():
         return
     if args.workload == "lambda":
         print(json.dumps(lambda_workload()), flush=True)
+        return
+    if args.workload == "restart":
+        print(json.dumps(restart_workload()), flush=True)
         return
     if args.workload == "pt48":
         print(json.dumps(pt48_workload()), flush=True)

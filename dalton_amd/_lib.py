@@ -26,6 +26,14 @@ SIGNATURES = {
     "ccgpu_rhs": (C.c_int, [C.c_void_p, _dp, _dp, _dp, _dp, C.c_int]),
     "ccgpu_lhtr": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _dp, _dp, _dp, _dp]),
     "ccgpu_eta": (C.c_int, [C.c_void_p, _dp, _dp]),
+    "ccgpu_ccsd_resume": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_int, _dp, _dp,
+                                    C.POINTER(C.c_int), _dp]),
+    "ccgpu_rsp_filename": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_char_p]),
+    "ccgpu_rsp_write": (C.c_int, [C.c_char_p, C.c_char_p, C.c_int, C.c_long, C.c_long, C.c_long, _dp, _dp, _dp]),
+    "ccgpu_rsp_read": (C.c_int, [C.c_char_p, C.c_long, C.c_long, C.c_long, _dp, _dp, _dp, C.c_char_p,
+                                 C.POINTER(C.c_int)]),
+    "ccgpu_save_amplitudes": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int]),
+    "ccgpu_restart_amplitudes": (C.c_int, [C.c_void_p, C.c_char_p, _dp]),
     "ccgpu_lambda_solve": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_int, C.c_int, _dp, _dp,
                                      C.POINTER(C.c_int), _dp]),
     "ccgpu_onedens": (C.c_int, [C.c_void_p, C.c_int, _dp, _dp, _dp]),

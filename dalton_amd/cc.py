@@ -55,6 +55,43 @@ def df_factor_ao(cmo: np.ndarray, eri_pairs: np.ndarray, nbas: int, thr: float =
     return out[: naux.value * nmo * nmo].reshape(naux.value, nmo, nmo)
 
 
+def rsp_filename(lst: str = "R0", isym: int = 1, idx: int = 0) -> str:
+    """Name Dalton gives a CC response-vector file (CC_RWPRE): 'CCR0_1___0'."""
+    buf = C.create_string_buffer(16)
+    check(lib().ccgpu_rsp_filename(lst.encode(), isym, idx, buf))
+    return buf.value.decode()
+
+
+def rsp_write(path: str, vec1: np.ndarray, vec2: np.ndarray | None = None, model: str = "CCSD",
+              vec0: np.ndarray | None = None, nallai: int | None = None, intsize: int = 4):
+    """Write a Dalton CC response-vector file (CC_WRRSP records); nallai
+    defaults to len(vec1) (no frozen orbitals)."""
+    v1 = np.ascontiguousarray(vec1, dtype=np.float64).reshape(-1)
+    v2 = None if vec2 is None else np.ascontiguousarray(vec2, dtype=np.float64).reshape(-1)
+    na = v1.size if nallai is None else nallai
+    v0 = None if vec0 is None else np.ascontiguousarray(vec0, dtype=np.float64).reshape(-1)
+    if v0 is not None and v0.size != 2 * na:
+        raise ValueError("vec0 must hold 2*nallai values")
+    check(lib().ccgpu_rsp_write(path.encode(), model.encode(), intsize, na, v1.size, 0 if v2 is None else v2.size,
+                                None if v0 is None else ptr(v0), ptr(v1), None if v2 is None else ptr(v2)))
+
+
+def rsp_read(path: str, nt1: int, nt2: int, nallai: int | None = None, orbital: bool = False) -> dict:
+    """Read a Dalton CC response-vector file (CC_RDRSP): model, nwrite,
+    singles, doubles (None when the file holds none) and, if asked, the
+    orbital part."""
+    na = nt1 if nallai is None else nallai
+    hdr = C.create_string_buffer(11)
+    nw = C.c_int()
+    check(lib().ccgpu_rsp_read(path.encode(), na, nt1, nt2, None, None, None, hdr, C.byref(nw)))
+    v0 = np.empty(2 * na) if orbital else None
+    v1 = np.empty(nt1) if nw.value >= 2 else None
+    v2 = np.empty(nt2) if nw.value >= 3 else None
+    check(lib().ccgpu_rsp_read(path.encode(), na, nt1, nt2, None if v0 is None else ptr(v0),
+                               None if v1 is None else ptr(v1), None if v2 is None else ptr(v2), None, None))
+    return {"model": hdr.value.decode(), "nwrite": nw.value, "vec0": v0, "vec1": v1, "vec2": v2}
+
+
 class CCContext:
     def __init__(self, eps: np.ndarray, B: np.ndarray, nocc: int, mxdiis: int = 8):
         eps = np.ascontiguousarray(eps, dtype=np.float64)
@@ -141,6 +178,24 @@ class CCContext:
         if n:
             check(lib().ccgpu_lhtr(self._h, precision, n, ptr(c1), ptr(c2), ptr(r1), ptr(r2)))
         return (r1[0], r2[0]) if single else (r1, r2)
+
+    def save_amplitudes(self, path: str, intsize: int = 4):
+        """Write the amplitudes as Dalton's 'R0' file (CCR0_1___0)."""
+        check(lib().ccgpu_save_amplitudes(self._h, path.encode(), intsize))
+
+    def restart_amplitudes(self, path: str) -> float:
+        """Read an 'R0' file into the amplitudes (.RESTAR); returns their energy."""
+        e = C.c_double()
+        check(lib().ccgpu_restart_amplitudes(self._h, path.encode(), C.byref(e)))
+        return e.value
+
+    def ccsd_resume(self, maxit: int = 100, threnr: float = 1e-10, thrvec: float = 1e9, precision: int = FP64):
+        """CCSD iterations from the current amplitudes (a restart)."""
+        e0, e, n = C.c_double(), C.c_double(), C.c_int()
+        hist = np.zeros(maxit)
+        check(lib().ccgpu_ccsd_resume(self._h, maxit, threnr, thrvec, precision, C.byref(e0), C.byref(e),
+                                      C.byref(n), ptr(hist)))
+        return {"e_start": e0.value, "e_ccsd": e.value, "niter": n.value, "history": hist[: n.value].copy()}
 
     def eta(self):
         """CC_ETA at the current amplitudes: eta = dE/dt, packed storage."""

@@ -182,33 +182,101 @@ int ccgpu_ccsd_iterate(ccgpu_ctx* ctx, int precision, double* e_corr, double* om
   });
 }
 
+// The CCSD_ENERGY loop from the context's current amplitudes, whose energy
+// is en2 (MP2 start or restart).
+static void ccsd_loop(ccgpu_ctx* ctx, int maxit, double threnr, double thrvec, int precision, double en2, double* e_ccsd,
+                      int* niter, double* history) {
+  const ccgpu::GemmPrec bulk = prec_of(precision);
+  // Mixed precision: bulk contractions in BF16x3 until the iteration is
+  // close (|dE| < kSwitch and norm < sqrt(kSwitch)), then FP64; convergence is
+  // only accepted after at least two FP64 iterations.
+  const double kSwitch = 1e-6;
+  bool fp64 = bulk == ccgpu::GemmPrec::F64;
+  int nfp64 = 0;
+  for (int it = 1;; ++it) {
+    const double en1 = en2;
+    double nrm = 0.0;
+    ctx->cc->set_precision(fp64 ? ccgpu::GemmPrec::F64 : bulk);
+    en2 = ctx->cc->iterate(&nrm);
+    if (fp64) ++nfp64;
+    if (history) history[it - 1] = en2;
+    if (niter) *niter = it;
+    if (e_ccsd) *e_ccsd = en2;
+    const bool conv = std::fabs(en2 - en1) < threnr && nrm < thrvec;
+    if (conv && (bulk == ccgpu::GemmPrec::F64 || nfp64 >= 2)) break;
+    if (!fp64 && std::fabs(en2 - en1) < kSwitch && nrm < std::sqrt(kSwitch)) fp64 = true;
+    if (it + 1 > maxit) throw ccgpu::Error(4, "CC equations not converged.");
+  }
+}
+
 int ccgpu_ccsd_energy(ccgpu_ctx* ctx, int maxit, double threnr, double thrvec, int precision,
                       double* e_mp2, double* e_ccsd, int* niter, double* history) {
   return guarded([&] {
     CCGPU_REQUIRE(ctx && maxit > 0, "ccgpu_ccsd_energy: bad argument");
-    const ccgpu::GemmPrec bulk = prec_of(precision);
-    double en2 = ctx->cc->mp2_start();
+    const double en2 = ctx->cc->mp2_start();
     if (e_mp2) *e_mp2 = en2;
-    // Mixed precision: bulk contractions in BF16x3 until the iteration is
-    // close (|dE| < kSwitch and norm < sqrt(kSwitch)), then FP64; convergence is
-    // only accepted after at least two FP64 iterations.
-    const double kSwitch = 1e-6;
-    bool fp64 = bulk == ccgpu::GemmPrec::F64;
-    int nfp64 = 0;
-    for (int it = 1;; ++it) {
-      const double en1 = en2;
-      double nrm = 0.0;
-      ctx->cc->set_precision(fp64 ? ccgpu::GemmPrec::F64 : bulk);
-      en2 = ctx->cc->iterate(&nrm);
-      if (fp64) ++nfp64;
-      if (history) history[it - 1] = en2;
-      if (niter) *niter = it;
-      if (e_ccsd) *e_ccsd = en2;
-      const bool conv = std::fabs(en2 - en1) < threnr && nrm < thrvec;
-      if (conv && (bulk == ccgpu::GemmPrec::F64 || nfp64 >= 2)) break;
-      if (!fp64 && std::fabs(en2 - en1) < kSwitch && nrm < std::sqrt(kSwitch)) fp64 = true;
-      if (it + 1 > maxit) throw ccgpu::Error(4, "CC equations not converged.");
-    }
+    ccsd_loop(ctx, maxit, threnr, thrvec, precision, en2, e_ccsd, niter, history);
+  });
+}
+
+int ccgpu_ccsd_resume(ccgpu_ctx* ctx, int maxit, double threnr, double thrvec, int precision, double* e_start,
+                      double* e_ccsd, int* niter, double* history) {
+  return guarded([&] {
+    CCGPU_REQUIRE(ctx && maxit > 0, "ccgpu_ccsd_resume: bad argument");
+    const double en2 = ctx->cc->energy_current();
+    if (e_start) *e_start = en2;
+    ccsd_loop(ctx, maxit, threnr, thrvec, precision, en2, e_ccsd, niter, history);
+  });
+}
+
+int ccgpu_rsp_filename(const char* list, int isym, int idx, char* name) {
+  return guarded([&] {
+    CCGPU_REQUIRE(name, "ccgpu_rsp_filename: null argument");
+    const std::string n = ccgpu::rsp_filename(list, isym, idx);
+    std::memcpy(name, n.c_str(), n.size() + 1);
+  });
+}
+
+int ccgpu_rsp_write(const char* path, const char* model, int intsize, long nallai, long nt1, long nt2,
+                    const double* vec0, const double* vec1, const double* vec2) {
+  return guarded([&] {
+    CCGPU_REQUIRE(path, "ccgpu_rsp_write: null path");
+    ccgpu::rsp_write(path, model, intsize, nallai, nt1, nt2, vec0, vec1, vec2);
+  });
+}
+
+int ccgpu_rsp_read(const char* path, long nallai, long nt1, long nt2, double* vec0, double* vec1, double* vec2,
+                   char* model, int* nwrite) {
+  return guarded([&] {
+    CCGPU_REQUIRE(path, "ccgpu_rsp_read: null path");
+    const int nw = ccgpu::rsp_read(path, nallai, nt1, nt2, vec0, vec1, vec2, model);
+    if (nwrite) *nwrite = nw;
+    if ((vec1 && nw < 2) || (vec2 && nw < 3))
+      throw ccgpu::Error(4, "ccgpu_rsp_read: the file holds fewer parts than requested");
+  });
+}
+
+int ccgpu_save_amplitudes(ccgpu_ctx* ctx, const char* path, int intsize) {
+  return guarded([&] {
+    CCGPU_REQUIRE(ctx && path, "ccgpu_save_amplitudes: null argument");
+    const auto& d = ctx->cc->dims();
+    std::vector<double> t1((size_t)d.nov), t2((size_t)(d.ntamp - d.nov));
+    ctx->cc->get_amplitudes(t1.data(), t2.data());
+    ccgpu::rsp_write(path, "CCSD", intsize, d.nov, d.nov, d.ntamp - d.nov, nullptr, t1.data(), t2.data());
+  });
+}
+
+int ccgpu_restart_amplitudes(ccgpu_ctx* ctx, const char* path, double* e_corr) {
+  return guarded([&] {
+    CCGPU_REQUIRE(ctx && path, "ccgpu_restart_amplitudes: null argument");
+    const auto& d = ctx->cc->dims();
+    std::vector<double> t1((size_t)d.nov), t2((size_t)(d.ntamp - d.nov));
+    char model[11];
+    const int nw = ccgpu::rsp_read(path, d.nov, d.nov, d.ntamp - d.nov, nullptr, t1.data(), t2.data(), model);
+    if (nw < 3) throw ccgpu::Error(4, "ccgpu_restart_amplitudes: the file holds no doubles");
+    ctx->cc->set_amplitudes(t1.data(), t2.data());
+    const double e = ctx->cc->energy_current();
+    if (e_corr) *e_corr = e;
   });
 }
 

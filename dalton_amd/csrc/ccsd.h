@@ -57,6 +57,9 @@ class CCSD {
   void set_amplitudes(const double* t1_host, const double* t2_packed_host);
   void get_amplitudes(double* t1_host, double* t2_packed_host);
 
+  // Correlation energy of the current amplitudes (CCSD_ECCSD).
+  double energy_current();
+
   // residual with host arrays (Dalton layouts), used by ccgpu_rhs
   void residual_host(const double* t1, const double* t2p, double* om1, double* om2p);
 

@@ -407,6 +407,15 @@ double CCSD::mp2_start() {
   return e;
 }
 
+double CCSD::energy_current() {
+  CCGPU_REQUIRE(iter_ >= 1, "ccgpu: no amplitudes");
+  ccsd_energy(scal_.p + 1, L_.p, T_.p, t1_.p, dm_.nov, st_);
+  double e = 0.0;
+  CCGPU_HIPX(hipMemcpyAsync(&e, scal_.p + 1, sizeof(double), hipMemcpyDeviceToHost, st_));
+  CCGPU_HIPX(hipStreamSynchronize(st_));
+  return e;
+}
+
 // One pass of the CCSD_ENERGY loop (cc/ccsd_energy.F:742-982): residual,
 // CCSD_NXTAM, CCSD_DIIS (cc/ccsd_diis.F:251-365), CCSD_ECCSD, all on the
 // device: the DIIS slot, the B-matrix extension and the bordered solve take

@@ -184,6 +184,16 @@ void bf16x3_run(const BMulti& p, int batch, bool c_float, double flops, hipStrea
 std::vector<double> df_factor_ao(int nbas, int nmo, const double* cmo, const double* eri_ao, double thr, long maxvec,
                                  long* naux, bool* converged, hipStream_t st);
 
+// Dalton's CC response-vector files (rspfile.hip): the name CC_RWPRE gives
+// list/symmetry/index; writing and reading the records of CC_WVRSP for a
+// closed-shell singlet vector (rsp_read returns NWRITE, the number of leading
+// parts present; null vectors are skipped; model gets 10 characters + NUL).
+std::string rsp_filename(const char* list, int isym, int idx);
+void rsp_write(const char* path, const char* model, int intsize, long nallai, long nt1, long nt2, const double* vec0,
+               const double* vec1, const double* vec2);
+int rsp_read(const char* path, long nallai, long nt1, long nt2, double* vec0, double* vec1, double* vec2,
+             char* model);
+
 // V+/- of one particle-ladder a-batch from an FP32 Y block straight into the
 // split-bf16 operand layout hi/lo[2][Rp][Kp] (V+ then V-), zero padded.
 void ladder_vpm_pack(unsigned short* hi, unsigned short* lo, const float* Y, long ldY, int a0, int a1, int v,

@@ -68,6 +68,51 @@ int ccgpu_rhs(ccgpu_ctx* ctx, const double* t1am, const double* t2am, double* om
 int ccgpu_lhtr(ccgpu_ctx* ctx, int precision, int nsimtr, const double* ctr1, const double* ctr2, double* rho1,
                double* rho2);
 
+/* CCSD iterations from the context's current amplitudes (a restart, as
+ * CCSD_ENERGY with .RESTAR, cc/ccsd_energy.F:558, after
+ * ccgpu_restart_amplitudes or ccgpu_set_amplitudes): the loop and arguments
+ * of ccgpu_ccsd_energy, with e_start the energy of the starting amplitudes in
+ * place of the MP2 energy.  The DIIS history starts empty. */
+int ccgpu_ccsd_resume(ccgpu_ctx* ctx, int maxit, double threnr, double thrvec, int precision, double* e_start,
+                      double* e_ccsd, int* niter, double* history);
+
+/* Dalton's CC response-vector files.  ccgpu_rsp_filename writes (at least 11
+ * bytes) the name CC_RWPRE (cc/cc_rdrsp.F:354) gives list `list` ("R0",
+ * "L0", "R1", ...), symmetry isym and file index idx: "CCR0_1___0" for the
+ * ground-state amplitudes, "CCL0_1___0" for the multipliers. */
+int ccgpu_rsp_filename(const char* list, int isym, int idx, char* name);
+
+/* Writes a closed-shell singlet response vector in the records of CC_WRRSP
+ * (cc/cc_wrrsp.F:20; CC_WVRSP :811) for a model without R12 or CC3 parts:
+ * header (NWRITE, MODFIL), orbital part (2*nallai values of vec0, zeros when
+ * vec0 is NULL), singles (nt1), doubles (nt2, Dalton packed; NWRITE = 2 and
+ * no doubles when nt2 = 0), then six empty records.  Fortran unformatted
+ * sequential framing with 4-byte length markers as gfortran writes it
+ * (records over 2147483639 bytes split into subrecords).  intsize is the byte
+ * size of the Dalton build's default INTEGER (4, or 8 for 64-bit integers);
+ * model is the CC model label ("CCSD"), at most 10 characters. */
+int ccgpu_rsp_write(const char* path, const char* model, int intsize, long nallai, long nt1, long nt2,
+                    const double* vec0, const double* vec1, const double* vec2);
+
+/* Reads such a file (CC_RDRSP, cc/cc_rdrsp.F:20; CC_RVRSP :1463).  Either
+ * integer size is recognised from the header.  vec0/vec1/vec2 may be NULL to
+ * skip a part; a part that is read must have exactly 2*nallai, nt1 or nt2
+ * values.  model (11 bytes, may be NULL) receives the label, nwrite (may be
+ * NULL) the number of leading parts the file holds.  Returns 4 if vec1 or
+ * vec2 is requested and the file does not hold that part. */
+int ccgpu_rsp_read(const char* path, long nallai, long nt1, long nt2, double* vec0, double* vec1, double* vec2,
+                   char* model, int* nwrite);
+
+/* Writes the context's amplitudes as the 'R0' file of a CCSD run
+ * (CC_WRRSP('R0', ...) at the end of CCSD_ENERGY; orbital part zero). */
+int ccgpu_save_amplitudes(ccgpu_ctx* ctx, const char* path, int intsize);
+
+/* Reads an 'R0' file (any default-integer size) into the context's
+ * amplitudes, as CCSD_ENERGY does for .RESTAR (cc/ccsd_energy.F:558);
+ * e_corr (may be NULL) receives their correlation energy.  Continue with
+ * ccgpu_ccsd_resume.  Returns 4 if the file holds no doubles. */
+int ccgpu_restart_amplitudes(ccgpu_ctx* ctx, const char* path, double* e_corr);
+
 /* Right-hand side of the ground-state multiplier equations at the context's
  * current amplitudes: replaces CC_ETA (cc/cc_fop.F:2955) for MODEL = CCSD
  * with canonical Hartree-Fock orbitals.  eta = dE/dt in the packed storage

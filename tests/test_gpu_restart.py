@@ -1,0 +1,64 @@
+"""Restart through Dalton's 'R0' amplitude file (.RESTAR) and the 'L0'
+multiplier file: amplitudes written by the GPU context, read back into a new
+context, and the CCSD iterations resumed from them (FP64 and mixed)."""
+import numpy as np
+import pytest
+
+from conftest import load_molecule
+
+pytestmark = pytest.mark.gpu
+
+
+def test_restart_from_converged_amplitudes_is_exact(tmp_path, golden):
+    import dalton_amd as d
+
+    eps, B, o, escf = load_molecule("h2o_ccpvdz_test")
+    print("restart from converged R0 file h2o/cc-pVDZ", flush=True)
+    path = str(tmp_path / d.rsp_filename("R0"))
+    with d.CCContext(eps, B, o) as c:
+        e = c.ccsd_energy(maxit=100, threnr=1e-12)["e_ccsd"]
+        c.save_amplitudes(path)
+        t1, t2 = c.amplitudes()
+    f = d.rsp_read(path, t1.size, t2.size)
+    assert f["model"].strip() == "CCSD" and np.array_equal(f["vec1"], t1) and np.array_equal(f["vec2"], t2)
+    with d.CCContext(eps, B, o) as c:
+        e0 = c.restart_amplitudes(path)
+        r = c.ccsd_resume(maxit=10, threnr=1e-12)
+    assert abs(e0 - e) < 1e-13
+    assert r["niter"] <= 3 and abs(r["e_ccsd"] - e) < 1e-11
+    assert abs(escf + r["e_ccsd"] - golden["h2o_ccpvdz_test"]["e_ccsd"]) < 1e-9
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_restart_mid_iteration_converges_to_daltons_energy(tmp_path, golden, prec):
+    import dalton_amd as d
+    from dalton_amd._lib import CCGPUError
+
+    eps, B, o, escf = load_molecule("hf_631gss")
+    print(f"restart after 4 iterations hf/6-31G** precision {prec}", flush=True)
+    path = str(tmp_path / "CCR0_1___0")
+    with d.CCContext(eps, B, o) as c:
+        with pytest.raises(CCGPUError):
+            c.ccsd_energy(maxit=4, threnr=1e-12, precision=prec)  # stopped early on purpose
+        c.save_amplitudes(path, intsize=8)
+    with d.CCContext(eps, B, o) as c:
+        c.restart_amplitudes(path)
+        r = c.ccsd_resume(maxit=60, threnr=1e-11, precision=prec)
+    tol = 1e-9 if prec == 0 else 1e-6
+    assert abs(escf + r["e_ccsd"] - golden["hf_631gss"]["e_ccsd"]) < tol
+
+
+def test_multipliers_through_the_l0_file(tmp_path):
+    import dalton_amd as d
+
+    eps, B, o, _ = load_molecule("h2o_sto3g_exgr")
+    print("L0 file h2o/STO-3G", flush=True)
+    with d.CCContext(eps, B, o) as c:
+        c.ccsd_energy(maxit=100, threnr=1e-12)
+        l1, l2, _, _ = c.lambda_solve(thr=1e-10)
+        D = c.onedens(l1, l2)
+        p = str(tmp_path / d.rsp_filename("L0"))
+        d.rsp_write(p, l1, l2, model="CCSD")
+        f = d.rsp_read(p, l1.size, l2.size)
+        assert np.array_equal(f["vec1"], l1) and np.array_equal(f["vec2"], l2)
+        assert np.array_equal(c.onedens(f["vec1"], f["vec2"]), D)
