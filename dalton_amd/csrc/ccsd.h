@@ -158,11 +158,12 @@ class CCSD {
   DBuf<float> W32_, W32r_;  // W and its two re-ordered copies (energy reads)
   // (T), persistent kernel: bf16 hi/lo of every T(p,q) [x][d] and Q_rq [z][l]
   // slice and of -Tvvo_p in both (x,y) orders (mixed), Tvvo with (x,y)
-  // swapped (FP64); four W slots (in W32_ / Wt_); launch tables and counters
+  // swapped (FP64); four W slots (in W32_ / Wt_); launch tables
   PtPath pt_path_ = PtPath::Launch;
   DBuf<unsigned short> ppH_[4], ppL_[4];
   DBuf<double> TvvoS_;
-  DBuf<int> ptctr_, pttri_, ptctab_;
+  DBuf<int> pttri_, ptctab_;
+  DBuf<unsigned long long> ptclaim_;
   DBuf<unsigned char> ptdesc_;
   // left transformation: cotangents (suffix b) of the residual's intermediates
   bool lhtr_ready_ = false;

@@ -194,7 +194,8 @@ void CCSD::pt_release() {
     ppH_[q].release();
     ppL_[q].release();
   }
-  for (DBuf<int>* b : {&ptctr_, &pttri_, &ptctab_}) b->release();
+  for (DBuf<int>* b : {&pttri_, &ptctab_}) b->release();
+  ptclaim_.release();
   TvvoS_.release();
   ptdesc_.release();
   W32_.release();
@@ -394,10 +395,9 @@ void CCSD::pt_triples_mixed(long t, long, long) {
   bf16x3_multi(g3, 4, vv, v, 1, W, true, v, 1, 0, 1.0, 1.0, 2 * fv + 2 * fo, st_);
 }
 
-// (T) driver from CCGPU_PT_PATH: "launch" (default), "legacy", or
-// "persistent".  The persistent kernel is switched off unless asked for by
-// name: on an MI355X its first run (H2O, o=5 v=19, FP64) did not finish,
-// with or without the bounded waits, and the cause is not yet known.
+// (T) driver from CCGPU_PT_PATH: "launch" (default: per-triple launches of
+// the pass kernels), "persistent" (one cooperative kernel per chunk of
+// triples) or "legacy" (the first per-triple driver).
 static int pt_path_from_env() {
   const char* e = std::getenv("CCGPU_PT_PATH");
   if (e && std::string(e) == "persistent") return 1;

@@ -2,30 +2,34 @@
 // (mlcc/mlccsdpt_e_calc.F90:71, loop :326-411; W as in w_calc :966, energy
 // as in w_prod :1547-1640 and v_calc :1345, :1483-1527).
 //
-// Work items, claimed in order from one device counter by resident blocks:
-// per triple q, three GEMM passes that build W_q (G1: W[a][(b,c)] with
-// beta = 0; G2: batched over a, W[a][b][c] += ...; G3: W[(a,b)][c] += ...,
-// four of the twelve terms each, one 128 x 128 output tile per item) and an
-// energy pass (E, one item per group of canonical 8x8x8 virtual blocks).
-// The claim order is a software pipeline: step s holds
-// E(s-3) | G3(s-2) | G2(s-1) | G1(s), and triple q owns W slot q mod 4.
-// Every item waits only on items claimed before it (G2/G3/E on the whole
-// previous pass of the same triple, G1 on the end of triple q-4 that used
-// the same slot), so the items in flight always include one whose
-// dependencies are complete and the grid cannot deadlock; by the time an
-// item is claimed its dependencies have normally finished, which hides the
-// tails of the passes.  Each wait is bounded: past its time limit it raises
-// an error flag, every block leaves, and the host reports the error.
+// A cooperative launch of as many blocks as the device holds at once walks
+// the triples of the launch in order.  Per triple q three GEMM passes build
+// W_q (G1: W[a][(b,c)] with beta = 0; G2: batched over a, W[a][b][c] += ...;
+// G3: W[(a,b)][c] += ...; four of the twelve terms each, one 128 x 128
+// output tile per item) and an energy pass (E, one item per group of
+// canonical 8x8x8 virtual blocks) reads it.  Items of a phase are claimed
+// dynamically by the blocks (one counter per XCD group, never waited on);
+// phases are separated by grid-wide barriers
+// (cooperative groups), and the energy pass of triple q shares a phase with
+// G1 of triple q + 1, which writes another of the four W slots (the energy
+// items interleaved with the G1 tiles, so they share CUs with GEMM work):
+//   [E(q-1) | G1(q)]  sync  [G2(q)]  sync  [G3(q)]  sync
+// so a triple costs three barriers and the energy fills the tail of G1.  The
+// partial energies of triple q are summed in a fixed order by block 0 at the
+// start of the next phase.  Every block runs every barrier: the loop bounds
+// depend only on the launch, and the runtime refuses a cooperative grid
+// larger than what can be resident together.
 //
 // The energy pass reads W once: for canonical blocks A <= B <= C it stages
 // the six block permutations of W, the eighteen 8x8 blocks of the (ia|jb)
 // integrals and the t1 slices it needs in LDS and evaluates the
-// permutation-symmetrised functional per orbit {a <= b <= c}.  The last
-// energy item of a triple sums the partials in a fixed order.
+// permutation-symmetrised functional per orbit {a <= b <= c}.
 #include <algorithm>
 #include <cstdlib>
 #include <cstring>
 #include <type_traits>
+
+#include <hip/hip_cooperative_groups.h>
 
 #include "bf16x3_tile.h"
 #include "ccsd.h"
@@ -249,13 +253,12 @@ struct PtRun {
   long wslot;
   double* part;  // four slots of 2 nE energy partials
   double* etrip;  // (E4, E5) per triple of the launch
-  int* ctr;       // [0,1] claim counter (64 bit), [2] error flag, [4 + 5q + phase] completions
+  unsigned long long* claims;  // one item counter per block group (zero at launch)
   const int* tri;  // (i, j, k) per triple
   EArgs e;
   long nq;
   long ng[3];        // items of G1, G2, G3
   long tm[3], tn[3];  // tiles per batch of G1, G2, G3 (m, n)
-  long spin_limit;  // wall-clock ticks a dependency wait may take before the error flag is raised
 };
 
 // Tile (tm, tn) of a linear index: groups of 8 m-tiles walked n-major, so
@@ -270,41 +273,12 @@ __device__ __forceinline__ void tile_grouped(long t, long tiles_m, long tiles_n,
   tn = r / gsz;
 }
 
-// Thread 0: wait until *p >= target, polling with relaxed loads (an
-// acquire per poll would invalidate the caches on every poll, and hundreds
-// of polling blocks would keep the L2s of the producers cold); the caller
-// issues one acquire after the wait.  Returns 0 if the error flag is (or
-// gets) set: after limit ticks of the constant-rate wall clock at the latest.
-__device__ __forceinline__ int wait_ge(int* p, int target, int* err, long limit) {
-  const unsigned long long t0 = wall_clock64();
-  for (;;) {
-    if (__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= target) return 1;
-    if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return 0;
-    if ((long)(wall_clock64() - t0) > limit) {
-      __hip_atomic_store(err, 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      return 0;
-    }
-    __builtin_amdgcn_s_sleep(16);
-  }
-}
-
 // A block-uniform value read back from LDS, marked uniform so that what is
 // derived from it (item, triple, descriptor address) stays in scalar registers.
 __device__ __forceinline__ long uniform64(long x) {
   const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)(unsigned long)x);
   const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)((unsigned long)x >> 32));
   return (long)(((unsigned long)hi << 32) | lo);
-}
-
-// Count one completion of this block's item.  Every wave first waits for its
-// own global stores to complete (a barrier alone only orders LDS), then thread
-// 0 increments with release semantics (one L2 write-back per item).
-__device__ __forceinline__ int signal_done(int* p) {
-  __builtin_amdgcn_s_waitcnt(0);
-  __syncthreads();
-  int old = 0;
-  if (threadIdx.x == 0) old = __hip_atomic_fetch_add(p, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-  return old;
 }
 
 // Energy item e of triple (i, j, k): canonical blocks e, e + nE, ...; writes
@@ -459,6 +433,81 @@ __device__ __forceinline__ void reduce_partials(const double* part, long nE, dou
   }
 }
 
+// One GEMM tile (item t of pass ph of triple q) into W slot q mod 4.
+template <bool MIXED>
+__device__ __forceinline__ void pass_tile(const PtRun& r, long q, int ph, long t, unsigned char* lds) {
+  const long tmn = r.tm[ph] * r.tn[ph];
+  const long z = t / tmn;
+  long tmi, tni;
+  tile_grouped(t - z * tmn, r.tm[ph], r.tn[ph], tmi, tni);
+  if constexpr (MIXED) {
+    const BMulti& d = static_cast<const BMulti*>(r.desc)[q * 3 + ph];
+    bf3::tile<float, 2>(d, tmi * 128, tni * 128, z, *reinterpret_cast<bf3::Smem<2>*>(lds));
+  } else {
+    const DMulti& d = static_cast<const DMulti*>(r.desc)[q * 3 + ph];
+    dt::Smem& GS = *reinterpret_cast<dt::Smem*>(lds);
+    const long m0 = tmi * 128, n0 = tni * 128;
+    bool inside = m0 + 128 <= d.M && n0 + 128 <= d.N;  // interior tile: no bounds checks
+    for (int k = 0; k < 4; ++k) inside = inside && d.t[k].K % dt::TK == 0;
+    if (ph < 2) {
+      if (inside)
+        dt::tile4<true, false>(d, m0, n0, z, GS);
+      else
+        dt::tile4<true, true>(d, m0, n0, z, GS);
+    } else {
+      if (inside)
+        dt::tile4<false, false>(d, m0, n0, z, GS);
+      else
+        dt::tile4<false, true>(d, m0, n0, z, GS);
+    }
+  }
+}
+
+// Dynamic claims per block group.  Blocks run on XCD b mod 8, so the items
+// of a phase are cut into min(8, nb) contiguous ranges, one per group of
+// blocks b = x (mod groups), and the group's blocks claim the items of their
+// range from the group's counter (neighbouring tiles then share operand
+// panels in one XCD's L2, and a slow item delays nothing else).  Each block
+// of a group makes exactly one claim past the range per phase, so the
+// counter advances by (range + blocks in the group) per phase and every
+// block knows its base without reading it; claims never wait on other
+// blocks.
+struct GroupClaims {
+  unsigned long long* ctr;
+  long G, x, nbx, base = 0;
+  __device__ GroupClaims(unsigned long long* c, long b, long nb) : ctr(c) {
+    G = nb < 8 ? nb : 8;
+    x = b % G;
+    nbx = (nb - x + G - 1) / G;
+  }
+  __device__ long lo(long T) const { return T * x / G; }
+  __device__ long hi(long T) const { return T * (x + 1) / G; }
+  // Next item of the range of a phase of T items, or -1 (then the phase is done for this block).
+  __device__ long next(long T, long* s_claim) {
+    if (threadIdx.x == 0) s_claim[0] = (long)__hip_atomic_fetch_add(ctr + x, 1ull, __ATOMIC_RELAXED,
+                                                                     __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    const long c = uniform64(s_claim[0]) - base;
+    __syncthreads();
+    const long i = lo(T) + c;
+    return i < hi(T) ? i : -1;
+  }
+  __device__ void end_phase(long T) { base += (hi(T) - lo(T)) + nbx; }
+};
+
+// Phase 1 item order: the ne energy items of the previous triple spread
+// evenly among the ng G1 tiles (positions 0, R, 2R, ... with R = T / ne),
+// so that energy items share CUs with GEMM tiles.  Returns the energy item
+// (>= 0) or -(tile + 1).
+__device__ __forceinline__ long phase1_item(long idx, long ne, long ng) {
+  if (ne == 0) return -(idx + 1);
+  const long R = (ne + ng) / ne;
+  const long e = idx / R;
+  if (idx % R == 0 && e < ne) return e;
+  const long before = e + 1 < ne ? e + 1 : ne;  // energy positions <= idx
+  return -(idx - before + 1);
+}
+
 template <bool MIXED>
 // The launch block is read through a pointer: its fields are re-read per
 // item instead of being held in registers across the GEMM loops.
@@ -468,93 +517,41 @@ __global__ __launch_bounds__(256, 2) void k_pt_persistent(const PtRun* rp) {
   constexpr size_t SG = MIXED ? sizeof(bf3::Smem<2>) : sizeof(dt::Smem);
   constexpr size_t SE = sizeof(ESmem<WT>);
   __shared__ __attribute__((aligned(16))) unsigned char lds[SG > SE ? SG : SE];
-  __shared__ long s_item;
-  __shared__ int s_flag;
   ESmem<WT>& ES = *reinterpret_cast<ESmem<WT>*>(lds);
-  const long nE = r.e.nE;
-  const long per_step = nE + r.ng[2] + r.ng[1] + r.ng[0];
-  const long total = (r.nq + 3) * per_step;
-  unsigned long long* claim = reinterpret_cast<unsigned long long*>(r.ctr);
-  int* err = r.ctr + 2;
-  int* done = r.ctr + 4;
-  const int tid = threadIdx.x;
-  for (;;) {
-    if (tid == 0) {
-      long it = -1;
-      if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0)
-        it = (long)__hip_atomic_fetch_add(claim, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      s_item = (it >= 0 && it < total) ? it : -1;
-    }
-    __syncthreads();
-    const long it = uniform64(s_item);
-    __syncthreads();
-    if (it < 0) break;
-    // step s: E(s-3) | G3(s-2) | G2(s-1) | G1(s)
-    const long step = it / per_step;
-    long rr = it - step * per_step;
-    int ph;
-    long q;
-    if (rr < nE) {
-      ph = 3;
-      q = step - 3;
-    } else if ((rr -= nE) < r.ng[2]) {
-      ph = 2;
-      q = step - 2;
-    } else if ((rr -= r.ng[2]) < r.ng[1]) {
-      ph = 1;
-      q = step - 1;
-    } else {
-      rr -= r.ng[1];
-      ph = 0;
-      q = step;
-    }
-    if (q < 0 || q >= r.nq) continue;
-    if (tid == 0) {
-      int ok = 1;
-      if (ph == 0) {
-        if (q >= 4) ok = wait_ge(&done[(q - 4) * 5 + 4], 1, err, r.spin_limit);
+  __shared__ long s_claim[1];
+  cooperative_groups::grid_group grid = cooperative_groups::this_grid();
+  const long nE = r.e.nE, nq = r.nq;
+  const long b0 = blockIdx.x;
+  GroupClaims gc(r.claims, b0, gridDim.x);
+  for (long q = 0; q <= nq; ++q) {
+    // phase 1: E(q-1) items among the G1(q) tiles
+    const long ne = q > 0 ? nE : 0, ng = q < nq ? r.ng[0] : 0;
+    for (long idx; (idx = gc.next(ne + ng, s_claim)) >= 0;) {
+      const long it = phase1_item(idx, ne, ng);
+      if (it >= 0) {
+        const long p = q - 1;
+        const int* tq = r.tri + 3 * p;
+        const WT* W = static_cast<const WT*>(r.W) + (p & 3) * r.wslot;
+        energy_item<WT>(r.e, W, tq[0], tq[1], tq[2], it, r.part + (p & 3) * 2 * nE, ES);
       } else {
-        ok = wait_ge(&done[q * 5 + ph - 1], (int)r.ng[ph - 1], err, r.spin_limit);
+        pass_tile<MIXED>(r, q, 0, -it - 1, lds);
       }
-      // one acquire for the block: its waves share this CU's L1 and the XCD's L2
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-      s_flag = ok;
+      __syncthreads();  // the next item reuses the LDS
     }
-    __syncthreads();
-    if (!__builtin_amdgcn_readfirstlane(s_flag)) break;
-    const int slot = (int)(q & 3);
-    WT* W = static_cast<WT*>(r.W) + slot * r.wslot;
-    if (ph < 3) {
-      const long tmn = r.tm[ph] * r.tn[ph];
-      const long z = rr / tmn;
-      long tmi, tni;
-      tile_grouped(rr - z * tmn, r.tm[ph], r.tn[ph], tmi, tni);
-      if constexpr (MIXED) {
-        const BMulti& d = static_cast<const BMulti*>(r.desc)[q * 3 + ph];
-        bf3::tile<float, 2>(d, tmi * 128, tni * 128, z, *reinterpret_cast<bf3::Smem<2>*>(lds));
-      } else {
-        const DMulti& d = static_cast<const DMulti*>(r.desc)[q * 3 + ph];
-        dt::Smem& GS = *reinterpret_cast<dt::Smem*>(lds);
-        if (ph < 2)
-          dt::tile4<true>(d, tmi * 128, tni * 128, z, GS);
-        else
-          dt::tile4<false>(d, tmi * 128, tni * 128, z, GS);
-      }
-      signal_done(&done[q * 5 + ph]);
-    } else {
-      const int* tq = r.tri + 3 * q;
-      double* part = r.part + slot * 2 * nE;
-      energy_item<WT>(r.e, W, tq[0], tq[1], tq[2], rr, part, ES);
-      const int old = signal_done(&done[q * 5 + 3]);
-      if (tid == 0) {
-        s_flag = old == (int)nE - 1;
-        if (s_flag) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-      }
+    gc.end_phase(ne + ng);
+    grid.sync();
+    if (q > 0 && b0 == 0) {  // the energy of triple q-1, partials in a fixed order
+      reduce_partials(r.part + ((q - 1) & 3) * 2 * nE, nE, r.etrip + 2 * (q - 1), ES.red);
       __syncthreads();
-      if (__builtin_amdgcn_readfirstlane(s_flag)) {  // the last energy item of the triple
-        reduce_partials(part, nE, r.etrip + 2 * q, ES.red);
-        signal_done(&done[q * 5 + 4]);
+    }
+    if (q == nq) break;
+    for (int ph = 1; ph < 3; ++ph) {
+      for (long idx; (idx = gc.next(r.ng[ph], s_claim)) >= 0;) {
+        pass_tile<MIXED>(r, q, ph, idx, lds);
+        __syncthreads();
       }
+      gc.end_phase(r.ng[ph]);
+      grid.sync();
     }
   }
 }
@@ -597,8 +594,8 @@ __global__ __launch_bounds__(256) void k_reduce_pairs(const double* part, long n
   reduce_partials(part, n, out, red);
 }
 
-__global__ void k_zero_int(int* p, long n) {
-  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) p[e] = 0;
+__global__ void k_zero_u64(unsigned long long* p, int n) {
+  if ((int)threadIdx.x < n) p[threadIdx.x] = 0ull;
 }
 
 // FP64 descriptor with every operand span checked.
@@ -641,19 +638,6 @@ int resident_blocks() {
     r.blocks = std::max(1, per) * pr.multiProcessorCount;
   }
   return r.blocks;
-}
-
-// Ticks per millisecond of the device's constant-rate wall clock.
-long wall_clock_khz() {
-  static int khz = 0;
-  if (khz == 0) {
-    if (dryrun()) return 100000;
-    int dev = 0;
-    CCGPU_HIP(hipGetDevice(&dev));
-    CCGPU_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
-    if (khz <= 0) khz = 100000;
-  }
-  return khz;
 }
 
 // Bytes reserved for the launch block at the head of the descriptor buffer.
@@ -829,7 +813,6 @@ void CCSD::pt_energy_persistent(long first, long n, long stride, double* e4, dou
   r.part = ptpart_.p;
   r.e = energy_args(ptctab_.p, J_.p, t1_.p, eps_.p, o, v);
   const long nE = r.e.nE;
-  r.spin_limit = env_long("CCGPU_PT_WAIT_MS", 5000) * wall_clock_khz();
   check_span(r.W, 0, 4 * vvv - 1, mixed ? sizeof(float) : sizeof(double), "pt W slots");
   check_span(r.part, 0, 8 * nE - 1, sizeof(double), "pt partials");
   check_span(r.e.ctab, 0, r.e.nct - 1, sizeof(int), "pt block table");
@@ -840,17 +823,15 @@ void CCSD::pt_energy_persistent(long first, long n, long stride, double* e4, dou
     r.tn[g] = ps.tn[g];
     r.ng[g] = ps.ng[g];
   }
-  const long per_step = nE + r.ng[0] + r.ng[1] + r.ng[2];
   const double ftri = 12.0 * (double)vv * vv + 12.0 * (double)o * vvv;
   long chunk = env_long("CCGPU_PT_CHUNK", 0);
   if (chunk <= 0) chunk = std::min<long>(4096, std::max<long>(8, (long)((mixed ? 150e12 : 35e12) / ftri)));
-  CCGPU_REQUIRE((chunk + 3) * per_step < (1L << 40), "pt: launch too large");
   etrip_.ensure(2 * (size_t)std::max<long>(n, 1));
   const size_t dsz = pt_desc_bytes();
   const long cq = std::min(chunk, std::max<long>(n, 1));
   ptdesc_.ensure(kRunBytes + 3 * cq * dsz);
   pttri_.ensure(3 * (size_t)cq);
-  ptctr_.ensure(4 + 5 * (size_t)cq);
+  ptclaim_.ensure(8);
   std::vector<unsigned char> hd(kRunBytes + 3 * cq * dsz);
   std::vector<int> ht(3 * cq);
   double s4 = 0.0, s5 = 0.0;
@@ -867,31 +848,28 @@ void CCSD::pt_energy_persistent(long first, long n, long stride, double* e4, dou
     r.nq = nq;
     r.desc = ptdesc_.p + kRunBytes;
     r.tri = pttri_.p;
-    r.ctr = ptctr_.p;
+    r.claims = ptclaim_.p;
     r.etrip = etrip_.p + 2 * c0;
     check_span(r.etrip, 0, 2 * nq - 1, sizeof(double), "pt energies");
-    check_span(r.ctr, 0, 4 + 5 * nq - 1, sizeof(int), "pt counters");
     std::memcpy(hd.data(), &r, sizeof(PtRun));
     CCGPU_HIPX(hipMemcpyAsync(ptdesc_.p, hd.data(), kRunBytes + 3 * nq * dsz, hipMemcpyHostToDevice, st_));
     CCGPU_HIPX(hipMemcpyAsync(pttri_.p, ht.data(), 3 * nq * sizeof(int), hipMemcpyHostToDevice, st_));
-    CCGPU_LAUNCH(k_zero_int, dim3(cdiv(4 + 5 * nq, 256)), dim3(256), 0, st_, ptctr_.p, 4 + 5 * nq);
+    check_span(ptclaim_.p, 0, 7, sizeof(unsigned long long), "pt claim counters");
+    CCGPU_LAUNCH(k_zero_u64, dim3(1), dim3(64), 0, st_, ptclaim_.p, 8);
     const PtRun* run = reinterpret_cast<const PtRun*>(ptdesc_.p);
-    const long total = (nq + 3) * per_step;
-    // at most one pipeline step of items in flight: fewer blocks polling on small systems
-    const long grid = std::min<long>(std::min<long>(total, per_step),
-                                     mixed ? resident_blocks<true>() : resident_blocks<false>());
+    // every block of a cooperative grid must be resident at once: at most
+    // the occupancy limit, and no more blocks than the widest phase has items
+    const long widest = std::max<long>(std::max(nE + r.ng[0], r.ng[1]), r.ng[2]);
+    const long grid = std::max<long>(1, std::min<long>(widest, mixed ? resident_blocks<true>()
+                                                                       : resident_blocks<false>()));
     const int id = prof_begin(mixed ? "k_pt_persistent<true>" : "k_pt_persistent<false>", (double)nq * ftri, st_);
-    if (mixed)
-      CCGPU_LAUNCH(k_pt_persistent<true>, dim3((unsigned)grid), dim3(256), 0, st_, run);
-    else
-      CCGPU_LAUNCH(k_pt_persistent<false>, dim3((unsigned)grid), dim3(256), 0, st_, run);
-    CCGPU_HIPX(hipGetLastError());
+    if (!dryrun()) {
+      void* args[] = {(void*)&run};
+      const void* fn = mixed ? reinterpret_cast<const void*>(&k_pt_persistent<true>)
+                             : reinterpret_cast<const void*>(&k_pt_persistent<false>);
+      CCGPU_HIP(hipLaunchCooperativeKernel(fn, dim3((unsigned)grid), dim3(256), args, 0, st_));
+    }
     prof_end(id, st_);
-    int flag = 0;
-    CCGPU_HIPX(hipMemcpyAsync(&flag, ptctr_.p + 2, sizeof(int), hipMemcpyDeviceToHost, st_));
-    CCGPU_HIPX(hipStreamSynchronize(st_));
-    if (flag != 0)
-      throw Error(2, "pt: a dependency wait of the persistent (T) kernel exceeded its time limit");
   }
   if (n > 0) {
     std::vector<double> et(2 * n, 0.0);

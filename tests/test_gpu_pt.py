@@ -51,16 +51,12 @@ def test_pt_matches_oracle_random_amplitudes():
     assert fa + fb == pytest.approx(nt * (12.0 * v**4 + 12.0 * o * v**3))
 
 
-PERSISTENT_OFF = ("persistent (T) kernel switched off: its first GPU run (H2O, FP64) did not finish; "
-                  "cause not yet found, so it is not run on the GPU")
-
-
-@pytest.mark.parametrize("path", ["launch", pytest.param("persistent", marks=pytest.mark.skip(reason=PERSISTENT_OFF))])
+@pytest.mark.parametrize("path", ["launch", "persistent"])
 @pytest.mark.parametrize("precision", [0, 1])
 def test_pt_driver_matches_first_launch_sequence(path, precision, monkeypatch):
     """The (T) drivers on pre-packed operands (three four-term W passes and
     the blocked energy reads, as per-triple launches or as the persistent
-    kernel in launches of 5 triples, so its W slots wrap) against the first
+    cooperative kernel in launches of 5 triples, so its W slots wrap) against the first
     driver (twelve GEMMs, per-triple packing, transposed W copies), with
     v = 136: edge tiles in every pass and a partial last 8-block."""
     import dalton_amd as d
@@ -85,3 +81,19 @@ def test_pt_driver_matches_first_launch_sequence(path, precision, monkeypatch):
     for a, b in zip(res["legacy"], res[path]):
         for x, y in zip(a, b):
             assert abs(x - y) < 1e-11 * max(1.0, abs(x)), (x, y)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_persistent_pt_matches_dalton_water(precision, golden, monkeypatch):
+    """The cooperative persistent kernel (every triple of the list in one
+    launch) against Dalton's printed E(T) of water/cc-pVDZ."""
+    import dalton_amd as d
+
+    monkeypatch.setenv("CCGPU_PT_PATH", "persistent")
+    eps, B, o, escf = load_molecule("h2o_ccpvdz_test")
+    print(f"persistent (T) water precision {precision}", flush=True)
+    with d.CCContext(eps, B, o) as c:
+        c.ccsd_energy(maxit=100, threnr=1e-12)
+        e4, e5 = c.pt(precision=precision)
+    tol = 1e-9 if precision == 0 else 1e-6
+    assert abs(e4 + e5 - golden["h2o_ccpvdz_test"]["e_t"]) < tol
