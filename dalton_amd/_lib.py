@@ -26,6 +26,8 @@ SIGNATURES = {
     "ccgpu_rhs": (C.c_int, [C.c_void_p, _dp, _dp, _dp, _dp, C.c_int]),
     "ccgpu_lhtr": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _dp, _dp, _dp, _dp]),
     "ccgpu_eta": (C.c_int, [C.c_void_p, _dp, _dp]),
+    "ccgpu_energy": (C.c_int, [C.c_void_p, _dp, _dp, _dp]),
+    "ccgpu_diis": (C.c_int, [C.c_void_p, _dp, _dp, C.c_int]),
     "ccgpu_ccsd_resume": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_int, _dp, _dp,
                                     C.POINTER(C.c_int), _dp]),
     "ccgpu_rsp_filename": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_char_p]),

@@ -179,6 +179,26 @@ class CCContext:
             check(lib().ccgpu_lhtr(self._h, precision, n, ptr(c1), ptr(c2), ptr(r1), ptr(r2)))
         return (r1[0], r2[0]) if single else (r1, r2)
 
+    def energy_of(self, t1: np.ndarray, t2p: np.ndarray) -> float:
+        """CCSD_ECCSD of host amplitudes (Dalton layouts)."""
+        t1 = np.ascontiguousarray(t1, dtype=np.float64).reshape(-1)
+        t2p = np.ascontiguousarray(t2p, dtype=np.float64).reshape(-1)
+        if t1.size != self.nt1 or t2p.size != self.nt2:
+            raise ValueError("amplitude sizes do not match NT1AMX/NT2AMX")
+        e = C.c_double()
+        check(lib().ccgpu_energy(self._h, ptr(t1), ptr(t2p), C.byref(e)))
+        return e.value
+
+    def diis(self, s: np.ndarray, t: np.ndarray, niter: int) -> np.ndarray:
+        """CCSD_DIIS: extrapolated [t1; packed t2] from the new estimate s and the
+        previous amplitudes t (returns a new array)."""
+        s = np.ascontiguousarray(s, dtype=np.float64).reshape(-1)
+        out = np.array(t, dtype=np.float64).reshape(-1)
+        if s.size != self.nt1 + self.nt2 or out.size != s.size:
+            raise ValueError("vectors must hold NT1AMX + NT2AMX values")
+        check(lib().ccgpu_diis(self._h, ptr(s), ptr(out), niter))
+        return out
+
     def save_amplitudes(self, path: str, intsize: int = 4):
         """Write the amplitudes as Dalton's 'R0' file (CCR0_1___0)."""
         check(lib().ccgpu_save_amplitudes(self._h, path.encode(), intsize))

@@ -229,6 +229,20 @@ int ccgpu_ccsd_resume(ccgpu_ctx* ctx, int maxit, double threnr, double thrvec, i
   });
 }
 
+int ccgpu_energy(ccgpu_ctx* ctx, const double* t1, const double* t2, double* e_corr) {
+  return guarded([&] {
+    CCGPU_REQUIRE(ctx && t1 && t2 && e_corr, "ccgpu_energy: null argument");
+    *e_corr = ctx->cc->energy_host(t1, t2);
+  });
+}
+
+int ccgpu_diis(ccgpu_ctx* ctx, const double* s, double* t, int niter) {
+  return guarded([&] {
+    CCGPU_REQUIRE(ctx && s && t, "ccgpu_diis: null argument");
+    if (!ctx->cc->diis_host(s, t, niter)) throw ccgpu::Error(3, "DIIS: ZERO PIVOT - MATRIX SINGULAR");
+  });
+}
+
 int ccgpu_rsp_filename(const char* list, int isym, int idx, char* name) {
   return guarded([&] {
     CCGPU_REQUIRE(name, "ccgpu_rsp_filename: null argument");

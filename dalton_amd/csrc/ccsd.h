@@ -59,6 +59,14 @@ class CCSD {
 
   // Correlation energy of the current amplitudes (CCSD_ECCSD).
   double energy_current();
+  // CCSD_ECCSD and CCSD_DIIS on host arrays in Dalton's layouts, for a
+  // caller that keeps its own iteration loop: the energy of (t1, t2p); one
+  // DIIS step from the new estimate s to the extrapolated vector, written
+  // over t (NTAMP values each), with the history held on the device (shared
+  // with iterate()).  diis_host returns false if the DIIS matrix was
+  // singular (t then holds the newest estimate).
+  double energy_host(const double* t1, const double* t2p);
+  bool diis_host(const double* s, double* t, int niter);
 
   // residual with host arrays (Dalton layouts), used by ccgpu_rhs
   void residual_host(const double* t1, const double* t2p, double* om1, double* om2p);

@@ -416,6 +416,43 @@ double CCSD::energy_current() {
   return e;
 }
 
+// CCSD_ECCSD (cc/ccsd_energy.F:1964) on host amplitudes.
+double CCSD::energy_host(const double* t1, const double* t2p) {
+  const long nov = dm_.nov, np2 = dm_.ntamp - nov;
+  DBuf<double> d((size_t)dm_.ntamp);
+  CCGPU_HIPX(hipMemcpyAsync(d.p, t1, nov * sizeof(double), hipMemcpyHostToDevice, st_));
+  CCGPU_HIPX(hipMemcpyAsync(d.p + nov, t2p, np2 * sizeof(double), hipMemcpyHostToDevice, st_));
+  ccsd_energy_packed(scal_.p + 1, L_.p, d.p + nov, d.p, nov, st_);
+  double e = 0.0;
+  CCGPU_HIPX(hipMemcpyAsync(&e, scal_.p + 1, sizeof(double), hipMemcpyDeviceToHost, st_));
+  CCGPU_HIPX(hipStreamSynchronize(st_));
+  return e;
+}
+
+// CCSD_DIIS (cc/ccsd_diis.F:251-365) on host vectors: slot, B-matrix row and
+// bordered solve as in iterate(), with the cycle position from niter.
+bool CCSD::diis_host(const double* s, double* t, int niter) {
+  CCGPU_REQUIRE(niter >= 1, "ccgpu_diis: niter >= 1");
+  const long nt = dm_.ntamp;
+  ensure_diis();
+  const int c[2] = {niter, 0};
+  CCGPU_HIPX(hipMemcpyAsync(ctr_.p, c, sizeof c, hipMemcpyHostToDevice, st_));
+  DBuf<double> d((size_t)(2 * nt));
+  CCGPU_HIPX(hipMemcpyAsync(d.p, s, nt * sizeof(double), hipMemcpyHostToDevice, st_));
+  CCGPU_HIPX(hipMemcpyAsync(d.p + nt, t, nt * sizeof(double), hipMemcpyHostToDevice, st_));
+  const int mx = mxdiis_;
+  pack_diis_slot_flat(diisS_.p, diisE_.p, ctr_.p, mx, d.p, d.p + nt, nt, st_);
+  multidot_slot(dots_.p, diisE_.p, nt, ctr_.p, mx, nt, st_);
+  diis_solve(Amat_.p, kMaxVec + 1, dots_.p, coef_.p, ctr_.p, mx, diiswork_.p, st_);
+  lincomb_slot(d.p + nt, diisS_.p, nt, coef_.p, ctr_.p, mx, nt, st_);
+  int flag[2] = {0, 0};
+  CCGPU_HIPX(hipMemcpyAsync(t, d.p + nt, nt * sizeof(double), hipMemcpyDeviceToHost, st_));
+  CCGPU_HIPX(hipMemcpyAsync(flag, ctr_.p, sizeof flag, hipMemcpyDeviceToHost, st_));
+  CCGPU_HIPX(hipStreamSynchronize(st_));
+  lhtr_ready_ = false;
+  return flag[1] == 0;
+}
+
 // One pass of the CCSD_ENERGY loop (cc/ccsd_energy.F:742-982): residual,
 // CCSD_NXTAM, CCSD_DIIS (cc/ccsd_diis.F:251-365), CCSD_ECCSD, all on the
 // device: the DIIS slot, the B-matrix extension and the bordered solve take

@@ -36,6 +36,9 @@ void multidot(double* dots, const double* e, const double* E, long ld, int nvec,
 // out = sum_i c[i] * S_i  (c on device)
 void lincomb(double* out, const double* S, long ld, const double* c, int nvec, long n, hipStream_t st);
 
+// The same energy from packed doubles t2p[bj(bj+1)/2 + ai] (ai <= bj).
+void ccsd_energy_packed(double* out, const double* L, const double* t2p, const double* t1, long nov,
+                        hipStream_t st);
 // E = sum L[(i,a),(j,b)] * (T + t1 (x) t1)   -> out[0]
 void ccsd_energy(double* out, const double* L, const double* t2, const double* t1, long nov,
                  hipStream_t st);
@@ -93,6 +96,9 @@ namespace ccgpu {
 // svec/evec of slot ivec from S = [s1; pack(s2)] and T = [t1; pack(t2)]
 void pack_diis_slot(double* diisS, double* diisE, const int* ctr, int mx, const double* s1, const double* s2,
                     const double* t1, const double* t2, long nov, hipStream_t st);
+// slot ivec of flat vectors: S_ivec = s, E_ivec = s - t (n values each)
+void pack_diis_slot_flat(double* diisS, double* diisE, const int* ctr, int mx, const double* s, const double* t,
+                         long n, hipStream_t st);
 // dots[i] = <E_ivec, E_i> for i < nloop (deterministic two-pass)
 void multidot_slot(double* dots, const double* diisE, long ld, const int* ctr, int mx, long n, hipStream_t st);
 // Extends the B matrix (Amat, leading dimension ld) by the new dots and

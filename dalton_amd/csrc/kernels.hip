@@ -406,6 +406,35 @@ void ccsd_energy(double* out, const double* L, const double* t2, const double* t
   CCGPU_HIPX(hipGetLastError());
 }
 
+// E from packed doubles: row bj of the packed triangle (ai <= bj) per block,
+// off-diagonal elements counted twice (the full sum is symmetric).
+__global__ void k_energy_packed(const double* __restrict__ L, const double* __restrict__ t2p,
+                                const double* __restrict__ t1, long nov, double* part) {
+  __shared__ double sh[16];
+  const long bj = blockIdx.x;
+  const long base = bj * (bj + 1) / 2;
+  const double tb = t1[bj];
+  double s = 0.0;
+  for (long ai = threadIdx.x; ai <= bj; ai += blockDim.x) {
+    const double w = ai == bj ? 1.0 : 2.0;
+    s += w * L[bj * nov + ai] * (t2p[base + ai] + t1[ai] * tb);
+  }
+  const double tot = block_sum(s, sh);
+  if (threadIdx.x == 0) part[bj] = tot;
+}
+
+void ccsd_energy_packed(double* out, const double* L, const double* t2p, const double* t1, long nov,
+                        hipStream_t st) {
+  check_span(out, 0, 0, sizeof(double), "energy out");
+  check_span(L, 0, nov * nov - 1, sizeof(double), "energy L");
+  check_span(t2p, 0, nov * (nov + 1) / 2 - 1, sizeof(double), "energy t2 packed");
+  check_span(t1, 0, nov - 1, sizeof(double), "energy t1");
+  double* part = partials(nov, st);
+  CCGPU_LAUNCH(k_energy_packed, dim3((unsigned)nov), dim3(256), 0, st, L, t2p, t1, nov, part);
+  CCGPU_LAUNCH(k_sum_partials, dim3(1), dim3(64), 0, st, out, part, (int)nov, 1, 1);
+  CCGPU_HIPX(hipGetLastError());
+}
+
 // ------------------------------------------------------------------ ladder helpers
 __device__ __forceinline__ void tri_decode(long p, long& lo, long& hi) {
   // p = hi*(hi+1)/2 + lo, 0 <= lo <= hi
@@ -819,6 +848,18 @@ void pack_diis_slot(double* diisS, double* diisE, const int* ctr, int mx, const 
   CCGPU_LAUNCH(k_diff1_slot, dim3(cdiv(nov, 256)), dim3(256), 0, st, diisS, diisE, ntot, ctr, mx, s1, t1, nov);
   CCGPU_LAUNCH(k_pack_diis2_slot, dim3((unsigned)nov), dim3(256), 0, st, diisS, diisE, ntot, nov, ctr, mx, s2,
                t2, nov);
+  CCGPU_HIPX(hipGetLastError());
+}
+
+void pack_diis_slot_flat(double* diisS, double* diisE, const int* ctr, int mx, const double* s, const double* t,
+                         long n, hipStream_t st) {
+  check_span(diisS, 0, (long)mx * n - 1, sizeof(double), "diis S");
+  check_span(diisE, 0, (long)mx * n - 1, sizeof(double), "diis E");
+  check_span(ctr, 0, 1, sizeof(int), "diis ctr");
+  check_span(s, 0, n - 1, sizeof(double), "diis s");
+  check_span(t, 0, n - 1, sizeof(double), "diis t");
+  CCGPU_LAUNCH(k_diff1_slot, dim3((unsigned)std::min<long>(cdiv(n, 256), 1L << 20)), dim3(256), 0, st, diisS, diisE,
+               n, ctr, mx, s, t, n);
   CCGPU_HIPX(hipGetLastError());
 }
 

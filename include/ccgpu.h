@@ -68,6 +68,23 @@ int ccgpu_rhs(ccgpu_ctx* ctx, const double* t1am, const double* t2am, double* om
 int ccgpu_lhtr(ccgpu_ctx* ctx, int precision, int nsimtr, const double* ctr1, const double* ctr2, double* rho1,
                double* rho2);
 
+/* Correlation energy of host amplitudes t1 (NT1AMX) and packed t2 (NT2AMX):
+ * CCSD_ECCSD (cc/ccsd_energy.F:1964) for canonical Hartree-Fock orbitals,
+ * E = sum_{ai,bj} L_iajb (t_aibj + t_ai t_bj), L_iajb = 2(ia|jb) - (ib|ja).
+ * The context's own amplitudes are not touched. */
+int ccgpu_energy(ccgpu_ctx* ctx, const double* t1, const double* t2, double* e_corr);
+
+/* One DIIS step for a caller that keeps Dalton's iteration loop: CCSD_DIIS
+ * (cc/ccsd_diis.F:251), with s the new estimate and t the previous
+ * amplitudes, both [t1; packed t2] (NTAMP = NT1AMX + NT2AMX values); t
+ * receives the extrapolated vector.  niter is the loop's ITER: the cycle
+ * position niter - ((niter - 1)/MXDIIS)*MXDIIS picks the slot, and the
+ * subspace restarts every MXDIIS calls as in CCSD_DIIS.  The history lives
+ * in device memory of the context, shared with ccgpu_ccsd_iterate (use one
+ * or the other within a cycle).  Returns 3 on a singular DIIS matrix (t then
+ * holds s). */
+int ccgpu_diis(ccgpu_ctx* ctx, const double* s, double* t, int niter);
+
 /* CCSD iterations from the context's current amplitudes (a restart, as
  * CCSD_ENERGY with .RESTAR, cc/ccsd_energy.F:558, after
  * ccgpu_restart_amplitudes or ccgpu_set_amplitudes): the loop and arguments

@@ -51,6 +51,10 @@ for (o, v, naux) in {shapes!r}:
     for prec, nvec in ((0, 2), (1, 1), (0, 1)):  # left transformation, both precisions
         rc = L.ccgpu_lhtr(h, prec, nvec, _lib.ptr(d), _lib.ptr(d), _lib.ptr(d), _lib.ptr(d))
         assert rc == 0, L.ccgpu_last_error()
+    # host-loop pieces: CCSD_ECCSD and CCSD_DIIS on host vectors (slot 1 and a wrapped slot)
+    assert L.ccgpu_energy(h, _lib.ptr(d), _lib.ptr(d), C.byref(e)) == 0, L.ccgpu_last_error()
+    for it in (1, 9):
+        assert L.ccgpu_diis(h, _lib.ptr(d), _lib.ptr(d), it) == 0, L.ccgpu_last_error()
     # multiplier equations: eta, the reduced-space solve (both precisions), density
     assert L.ccgpu_eta(h, _lib.ptr(d), _lib.ptr(d)) == 0, L.ccgpu_last_error()
     it = C.c_int(); rn = C.c_double()

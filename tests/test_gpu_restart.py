@@ -62,3 +62,32 @@ def test_multipliers_through_the_l0_file(tmp_path):
         f = d.rsp_read(p, l1.size, l2.size)
         assert np.array_equal(f["vec1"], l1) and np.array_equal(f["vec2"], l2)
         assert np.array_equal(c.onedens(f["vec1"], f["vec2"]), D)
+
+
+def test_dalton_driven_loop_with_gpu_pieces_reproduces_dalton(golden):
+    """Dalton's own CCSD_ENERGY loop kept on the host, with CCRHSN, CCSD_DIIS
+    and CCSD_ECCSD replaced by ccgpu_rhs, ccgpu_diis and ccgpu_energy (the
+    Jacobi step CCSD_NXTAM in numpy): the iteration energies equal Dalton's
+    printed water trajectory."""
+    import dalton_amd as d
+    from oracle import ccsd as occ
+
+    eps, B, o, escf = load_molecule("h2o_ccpvdz_test")
+    print("host-driven loop with ccgpu_rhs/ccgpu_diis/ccgpu_energy h2o", flush=True)
+    ref = golden["h2o_ccpvdz_test"]["ccsd_iterations"]
+    s = occ.DFSystem(eps, B, o)
+    D1, D2 = occ.denominators(s)
+    d1 = occ.pack_t1(D1)
+    d2 = occ.pack_t2(D2)
+    with d.CCContext(eps, B, o) as c:
+        c.guess()
+        t1, t2 = c.amplitudes()
+        t = np.concatenate([t1, t2])
+        assert abs(c.energy_of(t1, t2) - (golden["h2o_ccpvdz_test"]["e_mp2"] - escf)) < 1e-9
+        es = []
+        for it in range(1, len(ref) + 1):
+            om1, om2 = c.rhs(t[: c.nt1], t[c.nt1:])
+            snew = t + np.concatenate([om1 / d1, om2 / d2])
+            t = c.diis(snew, t, it)
+            es.append(escf + c.energy_of(t[: c.nt1], t[c.nt1:]))
+    assert np.abs(np.array(es) - np.array(ref)).max() < 5e-11
