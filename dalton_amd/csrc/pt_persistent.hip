@@ -11,8 +11,9 @@
 // dynamically by the blocks (one counter per XCD group, never waited on);
 // phases are separated by grid-wide barriers
 // (cooperative groups), and the energy pass of triple q shares a phase with
-// G1 of triple q + 1, which writes another of the four W slots (the energy
-// items interleaved with the G1 tiles, so they share CUs with GEMM work):
+// G1 of triple q + 1, which writes another of the four W slots (each block
+// group claims its energy items first, so that the long items start early
+// and the phase ends on short GEMM tiles):
 //   [E(q-1) | G1(q)]  sync  [G2(q)]  sync  [G3(q)]  sync
 // so a triple costs three barriers and the energy fills the tail of G1.  The
 // partial energies of triple q are summed in a fixed order by block 0 at the
@@ -482,31 +483,18 @@ struct GroupClaims {
   }
   __device__ long lo(long T) const { return T * x / G; }
   __device__ long hi(long T) const { return T * (x + 1) / G; }
-  // Next item of the range of a phase of T items, or -1 (then the phase is done for this block).
-  __device__ long next(long T, long* s_claim) {
+  // Next position in this group's list of count items, or -1 (then the
+  // phase is done for this block).
+  __device__ long next(long count, long* s_claim) {
     if (threadIdx.x == 0) s_claim[0] = (long)__hip_atomic_fetch_add(ctr + x, 1ull, __ATOMIC_RELAXED,
                                                                      __HIP_MEMORY_SCOPE_AGENT);
     __syncthreads();
     const long c = uniform64(s_claim[0]) - base;
     __syncthreads();
-    const long i = lo(T) + c;
-    return i < hi(T) ? i : -1;
+    return c < count ? c : -1;
   }
-  __device__ void end_phase(long T) { base += (hi(T) - lo(T)) + nbx; }
+  __device__ void end_phase(long count) { base += count + nbx; }
 };
-
-// Phase 1 item order: the ne energy items of the previous triple spread
-// evenly among the ng G1 tiles (positions 0, R, 2R, ... with R = T / ne),
-// so that energy items share CUs with GEMM tiles.  Returns the energy item
-// (>= 0) or -(tile + 1).
-__device__ __forceinline__ long phase1_item(long idx, long ne, long ng) {
-  if (ne == 0) return -(idx + 1);
-  const long R = (ne + ng) / ne;
-  const long e = idx / R;
-  if (idx % R == 0 && e < ne) return e;
-  const long before = e + 1 < ne ? e + 1 : ne;  // energy positions <= idx
-  return -(idx - before + 1);
-}
 
 template <bool MIXED>
 // The launch block is read through a pointer: its fields are re-read per
@@ -524,21 +512,23 @@ __global__ __launch_bounds__(256, 2) void k_pt_persistent(const PtRun* rp) {
   const long b0 = blockIdx.x;
   GroupClaims gc(r.claims, b0, gridDim.x);
   for (long q = 0; q <= nq; ++q) {
-    // phase 1: E(q-1) items among the G1(q) tiles
+    // phase 1: E(q-1) items, then G1(q) tiles: group x takes energy items
+    // x, x + G, ... first (long items, claimed early) and then its range of tiles
     const long ne = q > 0 ? nE : 0, ng = q < nq ? r.ng[0] : 0;
-    for (long idx; (idx = gc.next(ne + ng, s_claim)) >= 0;) {
-      const long it = phase1_item(idx, ne, ng);
-      if (it >= 0) {
+    const long ne_x = ne > gc.x ? (ne - gc.x + gc.G - 1) / gc.G : 0;
+    const long n1 = ne_x + gc.hi(ng) - gc.lo(ng);
+    for (long c; (c = gc.next(n1, s_claim)) >= 0;) {
+      if (c < ne_x) {
         const long p = q - 1;
         const int* tq = r.tri + 3 * p;
         const WT* W = static_cast<const WT*>(r.W) + (p & 3) * r.wslot;
-        energy_item<WT>(r.e, W, tq[0], tq[1], tq[2], it, r.part + (p & 3) * 2 * nE, ES);
+        energy_item<WT>(r.e, W, tq[0], tq[1], tq[2], gc.x + c * gc.G, r.part + (p & 3) * 2 * nE, ES);
       } else {
-        pass_tile<MIXED>(r, q, 0, -it - 1, lds);
+        pass_tile<MIXED>(r, q, 0, gc.lo(ng) + c - ne_x, lds);
       }
       __syncthreads();  // the next item reuses the LDS
     }
-    gc.end_phase(ne + ng);
+    gc.end_phase(n1);
     grid.sync();
     if (q > 0 && b0 == 0) {  // the energy of triple q-1, partials in a fixed order
       reduce_partials(r.part + ((q - 1) & 3) * 2 * nE, nE, r.etrip + 2 * (q - 1), ES.red);
@@ -546,11 +536,12 @@ __global__ __launch_bounds__(256, 2) void k_pt_persistent(const PtRun* rp) {
     }
     if (q == nq) break;
     for (int ph = 1; ph < 3; ++ph) {
-      for (long idx; (idx = gc.next(r.ng[ph], s_claim)) >= 0;) {
-        pass_tile<MIXED>(r, q, ph, idx, lds);
+      const long n = gc.hi(r.ng[ph]) - gc.lo(r.ng[ph]);
+      for (long c; (c = gc.next(n, s_claim)) >= 0;) {
+        pass_tile<MIXED>(r, q, ph, gc.lo(r.ng[ph]) + c, lds);
         __syncthreads();
       }
-      gc.end_phase(r.ng[ph]);
+      gc.end_phase(n);
       grid.sync();
     }
   }
