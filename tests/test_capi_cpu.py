@@ -49,3 +49,40 @@ def test_synthetic_system_statistics():
     assert abs(B.std() - 0.05) < 0.002
     e2, B2 = make_system(4, 12, 300, seed=2)
     assert np.array_equal(B, B2) and np.array_equal(eps, e2)
+
+
+def test_null_arguments_are_rejected_before_any_device_call():
+    """Every entry point added for the multipliers, the host loop and the
+    files reports a null context or array as error 1 with a message, without
+    touching the GPU."""
+    import ctypes as C
+
+    from dalton_amd import _lib
+
+    L = _lib.lib()
+    d = np.zeros(4)
+    p = _lib.ptr(d)
+    i = C.c_int()
+    x = C.c_double()
+    calls = [
+        ("ccgpu_eta", lambda: L.ccgpu_eta(None, p, p)),
+        ("ccgpu_lambda_solve", lambda: L.ccgpu_lambda_solve(None, 0, 1e-8, 10, 8, p, p, C.byref(i), C.byref(x))),
+        ("ccgpu_onedens", lambda: L.ccgpu_onedens(None, 0, p, p, p)),
+        ("ccgpu_energy", lambda: L.ccgpu_energy(None, p, p, C.byref(x))),
+        ("ccgpu_diis", lambda: L.ccgpu_diis(None, p, p, 1)),
+        ("ccgpu_ccsd_resume", lambda: L.ccgpu_ccsd_resume(None, 5, 1e-8, 1e9, 0, p, p, C.byref(i), p)),
+        ("ccgpu_save_amplitudes", lambda: L.ccgpu_save_amplitudes(None, b"x", 4)),
+        ("ccgpu_restart_amplitudes", lambda: L.ccgpu_restart_amplitudes(None, b"x", p)),
+        ("ccgpu_rsp_write", lambda: L.ccgpu_rsp_write(None, b"CCSD", 4, 1, 1, 1, None, p, p)),
+        ("ccgpu_rsp_read", lambda: L.ccgpu_rsp_read(None, 1, 1, 1, None, p, p, None, None)),
+        ("ccgpu_rsp_filename", lambda: L.ccgpu_rsp_filename(b"R0", 1, 0, None)),
+    ]
+    for name, f in calls:
+        assert f() == 1, name
+        assert L.ccgpu_last_error(), name
+    # out-of-range arguments of the host-only file functions
+    buf = C.create_string_buffer(16)
+    assert L.ccgpu_rsp_filename(b"R0", 9, 0, buf) == 1
+    assert L.ccgpu_rsp_filename(b"R0000", 1, 0, buf) == 1
+    assert L.ccgpu_rsp_write(b"/nonexistent_dir/x", b"CCSD", 4, 1, 1, 1, None, p, p) == 1
+    assert L.ccgpu_rsp_write(b"/tmp/ccgpu_bad_intsize", b"CCSD", 2, 1, 1, 1, None, p, p) == 1
