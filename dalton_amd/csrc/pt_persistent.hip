@@ -7,19 +7,19 @@
 // W_q (G1: W[a][(b,c)] with beta = 0; G2: batched over a, W[a][b][c] += ...;
 // G3: W[(a,b)][c] += ...; four of the twelve terms each, one 128 x 128
 // output tile per item) and an energy pass (E, one item per group of
-// canonical 8x8x8 virtual blocks) reads it.  Items of a phase are claimed
-// dynamically by the blocks (one counter per XCD group, never waited on);
-// phases are separated by grid-wide barriers
-// (cooperative groups), and the energy pass of triple q shares a phase with
-// G1 of triple q + 1, which writes another of the four W slots (each block
-// group claims its energy items first, so that the long items start early
-// and the phase ends on short GEMM tiles):
-//   [E(q-1) | G1(q)]  sync  [G2(q)]  sync  [G3(q)]  sync
-// so a triple costs three barriers and the energy fills the tail of G1.  The
-// partial energies of triple q are summed in a fixed order by block 0 at the
-// start of the next phase.  Every block runs every barrier: the loop bounds
-// depend only on the launch, and the runtime refuses a cooperative grid
-// larger than what can be resident together.
+// canonical 8x8x8 virtual blocks) reads it.  The passes form a software
+// pipeline over the four W slots (triple q owns slot q mod 4): step s runs
+//   E(s-3) | G3(s-2) | G2(s-1) | G1(s)
+// and ends with one grid-wide barrier (cooperative groups), so every item
+// depends only on items of earlier steps, and a triple costs one barrier.
+// Items of a step are claimed dynamically, from one counter per XCD group
+// that is never waited on; each group takes its energy items first (long
+// items, started early) and then its ranges of the G3, G2 and G1 tiles
+// (neighbouring tiles share operand panels in the XCD's L2).  The partial
+// energies of a triple are summed in a fixed order by block 0 after the
+// barrier that completes them.  Every block runs every barrier: the loop
+// bounds depend only on the launch, and the runtime refuses a cooperative
+// grid larger than what can be resident together.
 //
 // The energy pass reads W once: for canonical blocks A <= B <= C it stages
 // the six block permutations of W, the eighteen 8x8 blocks of the (ia|jb)
@@ -511,39 +511,44 @@ __global__ __launch_bounds__(256, 2) void k_pt_persistent(const PtRun* rp) {
   const long nE = r.e.nE, nq = r.nq;
   const long b0 = blockIdx.x;
   GroupClaims gc(r.claims, b0, gridDim.x);
-  for (long q = 0; q <= nq; ++q) {
-    // phase 1: E(q-1) items, then G1(q) tiles: group x takes energy items
-    // x, x + G, ... first (long items, claimed early) and then its range of tiles
-    const long ne = q > 0 ? nE : 0, ng = q < nq ? r.ng[0] : 0;
+  // step s: E(s-3) | G3(s-2) | G2(s-1) | G1(s), one grid barrier per step
+  for (long st = 0; st < nq + 3; ++st) {
+    if (st >= 4 && b0 == 0) {  // energy of triple st-4 (finished in step st-1), partials in a fixed order
+      const long p = st - 4;
+      reduce_partials(r.part + (p & 3) * 2 * nE, nE, r.etrip + 2 * p, ES.red);
+      __syncthreads();
+    }
+    const long qe = st - 3;
+    const long ne = (qe >= 0 && qe < nq) ? nE : 0;
     const long ne_x = ne > gc.x ? (ne - gc.x + gc.G - 1) / gc.G : 0;
-    const long n1 = ne_x + gc.hi(ng) - gc.lo(ng);
+    long cnt[3];
+    for (int ph = 0; ph < 3; ++ph) {
+      const long q = st - ph;
+      const long T = (q >= 0 && q < nq) ? r.ng[ph] : 0;
+      cnt[ph] = gc.hi(T) - gc.lo(T);
+    }
+    // this group's list: its energy items (x, x + G, ...; long, claimed first), then its tiles of G3, G2, G1
+    const long n1 = ne_x + cnt[2] + cnt[1] + cnt[0];
     for (long c; (c = gc.next(n1, s_claim)) >= 0;) {
       if (c < ne_x) {
-        const long p = q - 1;
-        const int* tq = r.tri + 3 * p;
-        const WT* W = static_cast<const WT*>(r.W) + (p & 3) * r.wslot;
-        energy_item<WT>(r.e, W, tq[0], tq[1], tq[2], gc.x + c * gc.G, r.part + (p & 3) * 2 * nE, ES);
+        const int* tq = r.tri + 3 * qe;
+        const WT* W = static_cast<const WT*>(r.W) + (qe & 3) * r.wslot;
+        energy_item<WT>(r.e, W, tq[0], tq[1], tq[2], gc.x + c * gc.G, r.part + (qe & 3) * 2 * nE, ES);
       } else {
-        pass_tile<MIXED>(r, q, 0, gc.lo(ng) + c - ne_x, lds);
+        long k = c - ne_x;
+        int ph = 2;
+        while (k >= cnt[ph]) k -= cnt[ph--];
+        const long q = st - ph;
+        pass_tile<MIXED>(r, q, ph, gc.lo(r.ng[ph]) + k, lds);
       }
       __syncthreads();  // the next item reuses the LDS
     }
     gc.end_phase(n1);
     grid.sync();
-    if (q > 0 && b0 == 0) {  // the energy of triple q-1, partials in a fixed order
-      reduce_partials(r.part + ((q - 1) & 3) * 2 * nE, nE, r.etrip + 2 * (q - 1), ES.red);
-      __syncthreads();
-    }
-    if (q == nq) break;
-    for (int ph = 1; ph < 3; ++ph) {
-      const long n = gc.hi(r.ng[ph]) - gc.lo(r.ng[ph]);
-      for (long c; (c = gc.next(n, s_claim)) >= 0;) {
-        pass_tile<MIXED>(r, q, ph, gc.lo(r.ng[ph]) + c, lds);
-        __syncthreads();
-      }
-      gc.end_phase(n);
-      grid.sync();
-    }
+  }
+  if (b0 == 0 && nq >= 1) {  // energy of the last triple
+    const long p = nq - 1;
+    reduce_partials(r.part + (p & 3) * 2 * nE, nE, r.etrip + 2 * p, ES.red);
   }
 }
 
@@ -850,7 +855,7 @@ void CCSD::pt_energy_persistent(long first, long n, long stride, double* e4, dou
     const PtRun* run = reinterpret_cast<const PtRun*>(ptdesc_.p);
     // every block of a cooperative grid must be resident at once: at most
     // the occupancy limit, and no more blocks than the widest phase has items
-    const long widest = std::max<long>(std::max(nE + r.ng[0], r.ng[1]), r.ng[2]);
+    const long widest = nE + r.ng[0] + r.ng[1] + r.ng[2];  // items of one full pipeline step
     const long grid = std::max<long>(1, std::min<long>(widest, mixed ? resident_blocks<true>()
                                                                        : resident_blocks<false>()));
     const int id = prof_begin(mixed ? "k_pt_persistent<true>" : "k_pt_persistent<false>", (double)nq * ftri, st_);
