@@ -522,8 +522,8 @@ def main():
                          "one graph replay per iteration")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build runs on one GPU only; --gpus must be 1 "
-              "(how the path would shard is described in DESIGN.md)", file=sys.stderr)
+        print("bench.py: runs on one GPU only; --gpus must be 1 (the CC path is not sharded "
+              "across GPUs)", file=sys.stderr)
         sys.exit(2)
     if args.workload == "pt":
         print(json.dumps(pt_workload(args.o, args.v, args.naux, args.seed, args.pt_triples)), flush=True)
