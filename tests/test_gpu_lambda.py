@@ -59,6 +59,7 @@ def test_onedens_matches_oracle_at_random_multipliers():
         D = c.onedens(l1, l2)
     Q = ole.density(occ.DFSystem(eps, B, o), t1, t2, l1, l2)
     assert np.abs(D - Q).max() < 1e-11 * max(1.0, np.abs(Q).max()), np.abs(D - Q).max()
+    assert abs(np.trace(D) - 2 * o) < 1e-10  # the correlation part is traceless
 
 
 def test_lambda_matches_oracle_water_ccpvdz():

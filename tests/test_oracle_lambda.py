@@ -118,3 +118,15 @@ def test_water_sto3g_unrelaxed_ccsd_dipole_matches_dalton():
     assert np.abs(mu - np.array(gold["dipole_el_ccsd"])).max() < 1e-8, mu
     # a plausible mistake moves it far outside: the reference part alone is the SCF value
     assert abs(mu[2] - mu_scf[2]) > 0.05
+
+
+def test_density_trace_is_the_electron_count():
+    """h -> h + x 1 leaves Omega unchanged (the Fock shift cancels in the E
+    terms) and moves only E_ref by 2 nocc x, so Tr D = 2 nocc exactly: the
+    correlation part of the unrelaxed density is traceless."""
+    s, t1p, t2p = _synthetic(3, 6, 18, 5)
+    rng = np.random.default_rng(0)
+    for l1, l2 in [le.solve_lambda(s, t1p, t2p, thr=1e-11)[:2],
+                   (rng.standard_normal(t1p.size), rng.standard_normal(t2p.size))]:
+        D = le.density(s, t1p, t2p, l1, l2)
+        assert abs(np.trace(D) - 2 * s.o) < 1e-12
