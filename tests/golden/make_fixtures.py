@@ -24,7 +24,7 @@ Molecules (geometries from the reference's test inputs):
                     printed there), dip[d][p][q]
 
 Usage:  python tests/golden/make_fixtures.py [/path/to/reference]
-        python tests/golden/make_fixtures.py /path/to/reference --ao   (AO fixture of h2o_ccpvdz_test)
+        python tests/golden/make_fixtures.py /path/to/reference --ao [--only NAME]   (AO fixture; default h2o_ccpvdz_test)
         python tests/golden/make_fixtures.py /path/to/reference --only NAME   (one molecule)
 """
 from __future__ import annotations
@@ -105,14 +105,18 @@ def build_ao(name: str, spec: dict) -> dict:
     Vs = G[il[0], il[1]][:, il[0], il[1]]
     ip = np.tril_indices(Vs.shape[0])
     print(f"{name} (AO): nbf={n} pairs={Vs.shape[0]} E_SCF={E:.12f}")
-    return dict(eri_pairs=Vs[ip[0], ip[1]], cmo=np.ascontiguousarray(C.T).reshape(-1), eps=eps, escf=E,
-                enuc=enuc, nocc=spec["nocc"], nbas=n)
+    out = dict(eri_pairs=Vs[ip[0], ip[1]], cmo=np.ascontiguousarray(C.T).reshape(-1), eps=eps, escf=E,
+               enuc=enuc, nocc=spec["nocc"], nbas=n)
+    if spec.get("dipole"):
+        out["dip_ao"] = gto.dipole(bas)  # [d][mu][nu], origin 0
+    return out
 
 
 if __name__ == "__main__":
     if "--ao" in sys.argv:
-        d = build_ao("h2o_ccpvdz_test", MOLECULES["h2o_ccpvdz_test"])
-        np.savez_compressed(os.path.join(HERE, "h2o_ccpvdz_test_ao.npz"), **d)
+        name = sys.argv[sys.argv.index("--only") + 1] if "--only" in sys.argv else "h2o_ccpvdz_test"
+        d = build_ao(name, MOLECULES[name])
+        np.savez_compressed(os.path.join(HERE, name + "_ao.npz"), **d)
         sys.exit(0)
     only = sys.argv[sys.argv.index("--only") + 1] if "--only" in sys.argv else None
     for name, spec in MOLECULES.items():

@@ -127,3 +127,29 @@ def test_small_space_collapse_and_iteration_limit():
     assert ei.value.code == 4
     assert np.abs(a1 - b1).max() < 1e-9 and np.abs(a2 - b2).max() < 1e-9
     assert it_b >= it_a
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_dipole_from_ao_integrals_matches_dalton(golden, precision):
+    """The whole property path from Dalton's side of the boundary: AO
+    two-electron integrals and MO coefficients into ccgpu_init_ao (Cholesky on
+    the GPU), CCSD, the L0 multipliers and the unrelaxed density on the GPU,
+    then the density back-transformed to the AO basis (D_AO = C D C^T) and
+    traced with AO dipole integrals: the unrelaxed CCSD dipole of water/STO-3G
+    printed by the reference (cc_exgr_prop)."""
+    import dalton_amd as d
+
+    g = golden["h2o_sto3g_exgr"]
+    a = np.load(os.path.join(GOLDEN, "h2o_sto3g_exgr_ao.npz"))
+    nbas, o = int(a["nbas"]), int(a["nocc"])
+    print(f"AO -> CCSD -> lambda -> dipole h2o/STO-3G precision {precision}", flush=True)
+    with d.CCContext.from_ao(a["eps"], a["cmo"], a["eri_pairs"], o, thr=1e-12) as c:
+        e = c.ccsd_energy(maxit=100, threnr=1e-12, precision=precision)["e_ccsd"]
+        l1, l2, _, rn = c.lambda_solve(thr=1e-10, precision=precision)
+        D = c.onedens(l1, l2)
+    C = a["cmo"].reshape(nbas, nbas).T  # C[mu][p]
+    dao = C @ D @ C.T
+    mu = -np.einsum("mn,dmn->d", dao, a["dip_ao"])
+    tol_e, tol_mu = (1e-9, 1e-8) if precision == 0 else (1e-6, 1e-6)
+    assert abs(float(a["escf"]) + e - g["e_ccsd"]) < tol_e
+    assert np.abs(mu - np.array(g["dipole_el_ccsd"])).max() < tol_mu, mu
