@@ -32,8 +32,9 @@ the explicit matrix A on small systems; the density against central finite
 differences of the CCSD energy under h -> h + x V at fixed orbitals with the
 amplitudes re-solved at every x (a check of eta, the multipliers and the
 density together that does not depend on the packed convention); and the
-unrelaxed CCSD dipole moment of water (STO-3G) against the reference's
-printed value (DALTON/test/trueresult/cc_exgr_prop.ref:5796).
+unrelaxed CCSD dipole moment and electronic second moments of water (STO-3G) against the
+reference's printed values (DALTON/test/trueresult/cc_exgr_prop.ref:5796,
+5820-5822); and Tr D = 2 nocc.
 """
 from __future__ import annotations
 

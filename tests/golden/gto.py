@@ -362,6 +362,47 @@ def dipole(basis: Basis, origin=(0.0, 0.0, 0.0)) -> np.ndarray:
     return M
 
 
+def second_moment(basis: Basis, origin=(0.0, 0.0, 0.0)) -> np.ndarray:
+    """Second-moment integrals <mu| (r_d - O_d)(r_e - O_e) |nu>, shape (3, 3, n, n).
+
+    Per dimension (x - O) = (x - B) + (B - O), so the first moment is
+    S(j+1) + (B - O) S(j) and the second S(j+2) + 2 (B - O) S(j+1) + (B - O)^2 S(j),
+    with S(j) the one-dimensional overlap at ket power j."""
+    n = basis.nbf
+    M = np.zeros((3, 3, n, n))
+    Oc = np.asarray(origin, float)
+    for A, sa in enumerate(basis.shells):
+        for B, sb in enumerate(basis.shells):
+            a = sa.exps[:, None]
+            b = sb.exps[None, :]
+            p = a + b
+            Sx = [hermite_E(sa.l, sb.l + 2, a, b, sa.center[d], sb.center[d])[:, :, 0] for d in range(3)]
+            pref = (np.pi / p) ** 1.5
+            ca, cb = cart_powers(sa.l), cart_powers(sb.l)
+            Mp = np.zeros((3, 3, len(sa.exps), len(sb.exps), len(ca), len(cb)))
+            for ia, pa in enumerate(ca):
+                for ib, pb in enumerate(cb):
+                    s0 = [Sx[d][pa[d], pb[d]] for d in range(3)]
+                    sh = [sb.center[d] - Oc[d] for d in range(3)]
+                    m1 = [Sx[d][pa[d], pb[d] + 1] + sh[d] * s0[d] for d in range(3)]
+                    m2 = [Sx[d][pa[d], pb[d] + 2] + 2 * sh[d] * Sx[d][pa[d], pb[d] + 1] + sh[d] ** 2 * s0[d]
+                          for d in range(3)]
+                    for d in range(3):
+                        for e in range(3):
+                            f = [s0[k] for k in range(3)]
+                            if d == e:
+                                f[d] = m2[d]
+                            else:
+                                f[d], f[e] = m1[d], m1[e]
+                            Mp[d, e, :, :, ia, ib] = pref * f[0] * f[1] * f[2]
+            ia0, ib0 = basis.off[A], basis.off[B]
+            for d in range(3):
+                for e in range(3):
+                    blk = _contract_pair(sa, sb, Mp[d, e])
+                    M[d, e, ia0 : ia0 + blk.shape[0], ib0 : ib0 + blk.shape[1]] = blk
+    return M
+
+
 def two_electron(basis: Basis) -> np.ndarray:
     """Full AO (pq|rs) array, chemists' notation, shape (n,n,n,n)."""
     n = basis.nbf

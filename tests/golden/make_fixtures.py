@@ -20,8 +20,9 @@ Molecules (geometries from the reference's test inputs):
                     output exists for it)
 * h2o_sto3g_exgr    DALTON/test/cc_exgr_prop (STO-3G, bohr), whose output
                     holds the unrelaxed CCSD dipole moment; this fixture also
-                    stores the MO dipole-length integrals (origin 0, as
-                    printed there), dip[d][p][q]
+                    stores the MO dipole-length and second-moment integrals
+                    (origin 0, as printed there), dip[d][p][q] and
+                    smom[d][e][p][q]
 
 Usage:  python tests/golden/make_fixtures.py [/path/to/reference]
         python tests/golden/make_fixtures.py /path/to/reference --ao [--only NAME]   (AO fixture; default h2o_ccpvdz_test)
@@ -84,6 +85,7 @@ def build(name: str, spec: dict) -> dict:
     if spec.get("dipole"):
         M = gto.dipole(bas)
         out["dip"] = np.einsum("dab,ap,bq->dpq", M, C, C, optimize=True)
+        out["smom"] = np.einsum("deab,ap,bq->depq", gto.second_moment(bas), C, C, optimize=True)
         mu = -2.0 * np.einsum("dii->d", out["dip"][:, : spec["nocc"], : spec["nocc"]])
         print(f"{name}: electronic SCF dipole {mu}")
     return out

@@ -130,3 +130,20 @@ def test_density_trace_is_the_electron_count():
                    (rng.standard_normal(t1p.size), rng.standard_normal(t2p.size))]:
         D = le.density(s, t1p, t2p, l1, l2)
         assert abs(np.trace(D) - 2 * s.o) < 1e-12
+
+
+def test_water_sto3g_unrelaxed_ccsd_second_moments_match_dalton():
+    """The same unrelaxed CCSD density traced with the second-moment
+    operators xx, yy, zz (origin 0): the electronic second moments printed
+    by the reference for SCF (CCS) and CCSD."""
+    gold = json.load(open(os.path.join(GOLD, "dalton_golden.json")))["h2o_sto3g_exgr"]
+    f, s = _water_sto3g()
+    sm = f["smom"]
+    scf = [2.0 * np.trace(sm[d, d][: s.o, : s.o]) for d in range(3)]
+    assert np.abs(np.array(scf) - np.array(gold["second_moment_el_ccs_diag"])).max() < 1e-7
+    r = ccsd.solve(s, maxiter=200, threnr=1e-13, thrvec=1e-11)
+    t1p, t2p = ccsd.pack_t1(r["t1"]), ccsd.pack_t2(r["t2"])
+    l1, l2, _ = le.solve_lambda(s, t1p, t2p, thr=1e-11)
+    D = le.density(s, t1p, t2p, l1, l2)
+    cc = [float(np.sum(D * sm[d, d])) for d in range(3)]
+    assert np.abs(np.array(cc) - np.array(gold["second_moment_el_ccsd_diag"])).max() < 1e-7, cc
