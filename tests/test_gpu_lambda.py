@@ -99,6 +99,8 @@ def test_unrelaxed_ccsd_dipole_matches_dalton(golden):
     assert abs(escf + e - g["e_ccsd"]) < 1e-9
     mu = -np.einsum("pq,dpq->d", D, f["dip"])
     assert np.abs(mu - np.array(g["dipole_el_ccsd"])).max() < 1e-8, mu
+    sm = [float(np.sum(D * f["smom"][k, k])) for k in range(3)]  # electronic second moments xx, yy, zz
+    assert np.abs(np.array(sm) - np.array(g["second_moment_el_ccsd_diag"])).max() < 1e-7, sm
 
 
 def test_mixed_solve_ends_in_fp64_at_the_same_multipliers():
