@@ -155,3 +155,30 @@ def test_dipole_from_ao_integrals_matches_dalton(golden, precision):
     tol_e, tol_mu = (1e-9, 1e-8) if precision == 0 else (1e-6, 1e-6)
     assert abs(float(a["escf"]) + e - g["e_ccsd"]) < tol_e
     assert np.abs(mu - np.array(g["dipole_el_ccsd"])).max() < tol_mu, mu
+
+
+@pytest.mark.parametrize("o,v,naux", [(1, 1, 1), (2, 1, 3), (1, 6, 4), (3, 5, 2), (4, 2, 7)])
+def test_multipliers_at_degenerate_shapes_match_oracle(o, v, naux):
+    """eta, the L0 solve and the density at tiny and lopsided shapes (one
+    occupied or one virtual orbital, fewer DF vectors than pairs), at small
+    random amplitudes, against the oracle."""
+    import dalton_amd as d
+
+    eps, B, _ = _synthetic(o, v, naux, seed=3)
+    print(f"multipliers at o={o} v={v} naux={naux}", flush=True)
+    rng = np.random.default_rng(o * 100 + v)
+    with d.CCContext(eps, B, o) as c:
+        t1 = 0.02 * rng.standard_normal(c.nt1)
+        t2 = 0.02 * rng.standard_normal(c.nt2)
+        c.set_amplitudes(t1, t2)
+        e1, e2 = c.eta()
+        l1, l2, _, rn = c.lambda_solve(thr=1e-11)
+        D = c.onedens(l1, l2)
+    s = occ.DFSystem(eps, B, o)
+    q1, q2 = ole.eta(s, t1, t2)
+    assert np.abs(e1 - q1).max() < 1e-12 and np.abs(e2 - q2).max() < 1e-12
+    m1, m2, _ = ole.solve_lambda(s, t1, t2, thr=1e-12)
+    assert rn < 1e-11
+    assert np.abs(l1 - m1).max() < 1e-9 and np.abs(l2 - m2).max() < 1e-9
+    Q = ole.density(s, t1, t2, m1, m2)
+    assert np.abs(D - Q).max() < 1e-9 and abs(np.trace(D) - 2 * o) < 1e-10
