@@ -140,13 +140,16 @@ __device__ __forceinline__ double block_sum(double v, double* sh) {
   return r;  // valid in thread 0
 }
 
-__global__ void k_sum_partials(double* out, const double* part, int n, int stride, int nout) {
-  // out[j] = sum_b part[b*stride + j], one thread per j, fixed order
-  const int j = threadIdx.x + blockIdx.x * blockDim.x;
-  if (j >= nout) return;
+// out[j] = sum_b part[b*stride + j] for j = blockIdx.x < nout: thread t sums
+// b = t, t + 256, ... in order, then a fixed shuffle/wave tree (the same
+// order on every run).
+__global__ __launch_bounds__(256) void k_sum_partials(double* out, const double* part, int n, int stride, int nout) {
+  __shared__ double sh[16];
+  const int j = blockIdx.x;
   double s = 0.0;
-  for (int b = 0; b < n; ++b) s += part[(long)b * stride + j];
-  out[j] = s;
+  for (int b = threadIdx.x; b < n; b += blockDim.x) s += part[(long)b * stride + j];
+  const double tot = block_sum(s, sh);
+  if (threadIdx.x == 0 && j < nout) out[j] = tot;
 }
 
 namespace {
@@ -291,7 +294,7 @@ void nxtam(double* om1, double* om2, const double* t1, const double* t2, const d
   double* part = partials(nov + 1, st);
   CCGPU_LAUNCH(k_nxtam1, dim3(1), dim3(256), 0, st, om1, t1, eps, o, v, part + nov);
   CCGPU_LAUNCH(k_nxtam2, dim3((unsigned)nov), dim3(256), 0, st, om2, t2, eps, o, v, part);
-  CCGPU_LAUNCH(k_sum_partials, dim3(1), dim3(64), 0, st, norm2, part, (int)(nov + 1), 1, 1);
+  CCGPU_LAUNCH(k_sum_partials, dim3((unsigned)(1)), dim3(256), 0, st, norm2, part, (int)(nov + 1), 1, 1);
   CCGPU_HIPX(hipGetLastError());
 }
 
@@ -359,7 +362,7 @@ void multidot(double* dots, const double* e, const double* E, long ld, int nvec,
   for (int g0 = 0; g0 < nvec; g0 += kDotGroup) {
     const int ng = std::min(kDotGroup, nvec - g0);
     CCGPU_LAUNCH(k_multidot, dim3(nb), dim3(256), 0, st, e, E, ld, g0, ng, n, part);
-    CCGPU_LAUNCH(k_sum_partials, dim3(1), dim3(64), 0, st, dots + g0, part, nb, kDotGroup, ng);
+    CCGPU_LAUNCH(k_sum_partials, dim3((unsigned)(ng)), dim3(256), 0, st, dots + g0, part, nb, kDotGroup, ng);
   }
   CCGPU_HIPX(hipGetLastError());
 }
@@ -402,7 +405,7 @@ void ccsd_energy(double* out, const double* L, const double* t2, const double* t
   check_span(t1, 0, nov - 1, sizeof(double), "energy t1");
   double* part = partials(nov, st);
   CCGPU_LAUNCH(k_energy, dim3((unsigned)nov), dim3(256), 0, st, L, t2, t1, nov, part);
-  CCGPU_LAUNCH(k_sum_partials, dim3(1), dim3(64), 0, st, out, part, (int)nov, 1, 1);
+  CCGPU_LAUNCH(k_sum_partials, dim3((unsigned)(1)), dim3(256), 0, st, out, part, (int)nov, 1, 1);
   CCGPU_HIPX(hipGetLastError());
 }
 
@@ -431,7 +434,7 @@ void ccsd_energy_packed(double* out, const double* L, const double* t2p, const d
   check_span(t1, 0, nov - 1, sizeof(double), "energy t1");
   double* part = partials(nov, st);
   CCGPU_LAUNCH(k_energy_packed, dim3((unsigned)nov), dim3(256), 0, st, L, t2p, t1, nov, part);
-  CCGPU_LAUNCH(k_sum_partials, dim3(1), dim3(64), 0, st, out, part, (int)nov, 1, 1);
+  CCGPU_LAUNCH(k_sum_partials, dim3((unsigned)(1)), dim3(256), 0, st, out, part, (int)nov, 1, 1);
   CCGPU_HIPX(hipGetLastError());
 }
 
@@ -892,7 +895,7 @@ void multidot_slot(double* dots, const double* diisE, long ld, const int* ctr, i
   for (int g0 = 0; g0 < mx; g0 += kDotGroup) {
     const int ng = std::min(kDotGroup, mx - g0);
     CCGPU_LAUNCH(k_multidot_slot, dim3(nb), dim3(256), 0, st, diisE, ld, ctr, mx, g0, n, part);
-    CCGPU_LAUNCH(k_sum_partials, dim3(1), dim3(64), 0, st, dots + g0, part, nb, kDotGroup, ng);
+    CCGPU_LAUNCH(k_sum_partials, dim3((unsigned)(ng)), dim3(256), 0, st, dots + g0, part, nb, kDotGroup, ng);
   }
   CCGPU_HIPX(hipGetLastError());
 }
