@@ -329,14 +329,16 @@ def lambda_workload(o=32, v=256, naux=1024, seed=4, thr=1e-8):
             sync()
             dt = time.perf_counter() - t0
             prof = profile_stop()
+            n64, nmx = c.lambda_stats()
             D = c.onedens(l1, l2)
             gem = sum(k_["flops"] for k_ in prof["kernels"].values())
             gms = sum(k_["ms"] for k_ in prof["kernels"].values())
-            res[label] = {"seconds": round(dt, 3), "transforms": it, "s_per_transform": round(dt / it, 4),
+            res[label] = {"seconds": round(dt, 3), "transforms": it, "transforms_fp64": n64,
+                          "transforms_bf16x3": nmx, "s_per_transform": round(dt / it, 4),
                           "resnorm": rn, "gemm_tflops": round(gem / max(gms, 1e-9) / 1e9, 2),
                           "property_V": float(np.sum(D * V)), "kernels": kern_table_of(prof)}
             out[label] = (l1, l2)
-            _log(f"lambda {label}: {it} transforms in {dt:.2f} s, |r| = {rn:.2e}")
+            _log(f"lambda {label}: {it} transforms ({n64} FP64, {nmx} BF16x3) in {dt:.2f} s, |r| = {rn:.2e}")
     a1, a2 = out["fp64"]
     b1, b2 = out["mixed"]
     res["max_abs_dev_lambda_mixed_vs_fp64"] = float(max(np.abs(a1 - b1).max(), np.abs(a2 - b2).max()))

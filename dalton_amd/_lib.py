@@ -39,6 +39,7 @@ SIGNATURES = {
     "ccgpu_lambda_solve": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_int, C.c_int, _dp, _dp,
                                      C.POINTER(C.c_int), _dp]),
     "ccgpu_onedens": (C.c_int, [C.c_void_p, C.c_int, _dp, _dp, _dp]),
+    "ccgpu_lambda_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "ccgpu_ccsd_guess": (C.c_int, [C.c_void_p, _dp]),
     "ccgpu_ccsd_iterate": (C.c_int, [C.c_void_p, C.c_int, _dp, _dp]),
     "ccgpu_ccsd_energy": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_int, _dp, _dp,

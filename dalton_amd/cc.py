@@ -235,6 +235,12 @@ class CCContext:
                                        C.byref(rn)))
         return l1, l2, it.value, rn.value
 
+    def lambda_stats(self):
+        """(FP64, BF16x3) transformations of the last lambda_solve."""
+        a, b = C.c_int(), C.c_int()
+        check(lib().ccgpu_lambda_stats(self._h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
     def onedens(self, l1: np.ndarray, l2: np.ndarray, precision: int = FP64) -> np.ndarray:
         """Unrelaxed MO one-electron density D[p,q] = dL/dh_pq (CC_D1AO, MO basis)."""
         l1 = np.ascontiguousarray(l1, dtype=np.float64)

@@ -155,6 +155,14 @@ int ccgpu_lambda_solve(ccgpu_ctx* ctx, int precision, double thr, int maxit, int
   });
 }
 
+int ccgpu_lambda_stats(ccgpu_ctx* ctx, int* n_fp64, int* n_mixed) {
+  return guarded([&] {
+    CCGPU_REQUIRE(ctx && n_fp64 && n_mixed, "ccgpu_lambda_stats: null argument");
+    *n_fp64 = ctx->cc->lambda_counts()[0];
+    *n_mixed = ctx->cc->lambda_counts()[1];
+  });
+}
+
 int ccgpu_onedens(ccgpu_ctx* ctx, int precision, const double* l1, const double* l2, double* dens) {
   return guarded([&] {
     CCGPU_REQUIRE(ctx && l1 && l2 && dens, "ccgpu_onedens: null argument");

@@ -95,6 +95,8 @@ class CCSD {
   void eta_host(double* e1, double* e2p);
   int lambda_solve(double thr, int maxit, int maxred, double* l1, double* l2p, double* resnorm, int* niter);
   void onedens_host(const double* l1, const double* l2p, double* D);
+  // Transformations of the last lambda_solve: [0] FP64, [1] BF16x3.
+  const int* lambda_counts() const { return lam_counts_; }
 
   const double* t1() const { return t1_.p; }
   const double* T() const { return T_.p; }
@@ -174,6 +176,7 @@ class CCSD {
   DBuf<unsigned long long> ptclaim_;
   DBuf<unsigned char> ptdesc_;
   // left transformation: cotangents (suffix b) of the residual's intermediates
+  int lam_counts_[2] = {0, 0};
   bool lhtr_ready_ = false;
   GemmPrec lhtr_prec_ = GemmPrec::F64;
   DBuf<double> Tb_, Ub_, Txb_, Db_, Btb_, Bvvb_, Boob_, BvoTb_, Zb_, Wb_, Fb_, E1b_, E2b_, Jdb_, Gamb_, G1b_,

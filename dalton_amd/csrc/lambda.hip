@@ -201,11 +201,13 @@ int CCSD::lambda_solve(double thr, int maxit, int maxred, double* l1, double* l2
   auto norm = [&](const double* x) { return dryrun() ? 1.0 : std::sqrt(std::max(0.0, dots(x, x, 1)[0])); };
   int it = 0;
   // y = x A in the given arithmetic (the forward intermediates stay FP64)
+  lam_counts_[0] = lam_counts_[1] = 0;
   auto transform = [&](const double* x, double* y, GemmPrec p) {
     set_precision(p);
     lhtr_packed(x, y);
     set_precision(GemmPrec::F64);
     ++it;
+    ++lam_counts_[p == GemmPrec::F64 ? 0 : 1];
   };
 
   // Reduced space: G_kl = b_k . sigma_l; the right-hand side is rebuilt from

@@ -156,6 +156,11 @@ int ccgpu_eta(ccgpu_ctx* ctx, double* eta1, double* eta2);
 int ccgpu_lambda_solve(ccgpu_ctx* ctx, int precision, double thr, int maxit, int maxred, double* l1, double* l2,
                        int* niter, double* resnorm);
 
+/* Transformations made by the context's last ccgpu_lambda_solve, by
+ * arithmetic: n_fp64 FP64, n_mixed BF16x3 (the sum is its niter).  No
+ * counterpart in the reference (CCEQ_SOL prints its iteration count). */
+int ccgpu_lambda_stats(ccgpu_ctx* ctx, int* n_fp64, int* n_mixed);
+
 /* Unrelaxed one-electron density in the MO basis from the amplitudes and
  * multipliers lambda (l1, l2 as returned by ccgpu_lambda_solve): CC_D1AO
  * (cc/cc_d1ao.F:20) without its back-transformation to the AO basis.

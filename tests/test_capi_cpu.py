@@ -68,6 +68,7 @@ def test_null_arguments_are_rejected_before_any_device_call():
         ("ccgpu_eta", lambda: L.ccgpu_eta(None, p, p)),
         ("ccgpu_lambda_solve", lambda: L.ccgpu_lambda_solve(None, 0, 1e-8, 10, 8, p, p, C.byref(i), C.byref(x))),
         ("ccgpu_onedens", lambda: L.ccgpu_onedens(None, 0, p, p, p)),
+        ("ccgpu_lambda_stats", lambda: L.ccgpu_lambda_stats(None, C.byref(i), C.byref(i))),
         ("ccgpu_energy", lambda: L.ccgpu_energy(None, p, p, C.byref(x))),
         ("ccgpu_diis", lambda: L.ccgpu_diis(None, p, p, 1)),
         ("ccgpu_ccsd_resume", lambda: L.ccgpu_ccsd_resume(None, 5, 1e-8, 1e9, 0, p, p, C.byref(i), p)),

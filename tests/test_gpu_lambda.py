@@ -111,7 +111,9 @@ def test_mixed_solve_ends_in_fp64_at_the_same_multipliers():
     with _converged(d, eps, B, o) as c:
         a1, a2, _, _ = c.lambda_solve(thr=1e-10)
         m1, m2, it, rn = c.lambda_solve(thr=1e-10, precision=d.MIXED)
+        n64, nmx = c.lambda_stats()
     assert rn < 1e-10
+    assert n64 + nmx == it and n64 >= 1 and nmx >= 1  # FP64 residuals around BF16x3 corrections
     assert np.abs(a1 - m1).max() < 1e-9 and np.abs(a2 - m2).max() < 1e-9
 
 
