@@ -87,3 +87,14 @@ def test_null_arguments_are_rejected_before_any_device_call():
     assert L.ccgpu_rsp_filename(b"R0000", 1, 0, buf) == 1
     assert L.ccgpu_rsp_write(b"/nonexistent_dir/x", b"CCSD", 4, 1, 1, 1, None, p, p) == 1
     assert L.ccgpu_rsp_write(b"/tmp/ccgpu_bad_intsize", b"CCSD", 2, 1, 1, 1, None, p, p) == 1
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py --gpus N with N > 1 exits with status 2 and says why (the path
+    runs on one GPU), before touching any device."""
+    import subprocess
+    import sys
+
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2 and "one GPU" in r.stderr
