@@ -42,7 +42,7 @@ class CCSDResidual(torch.autograd.Function):
         cc.set_amplitudes(a1, a2)  # the Jacobian is taken at the forward amplitudes
         z1 = np.zeros(cc.nt1) if g1 is None else _np(g1)
         z2 = np.zeros(cc.nt2) if g2 is None else _np(g2)
-        r1, r2 = cc.lhtr(z1, z2, fctx.precision)
+        r1, r2 = cc.lhtr(z1, z2, fctx.precision, convention="natural")
         return torch.from_numpy(r1), torch.from_numpy(r2), None, None
 
 

@@ -19,6 +19,11 @@ FP64 = 0
 MIXED = 1
 
 
+def _check_convention(convention: str):
+    if convention not in ("dalton", "natural"):
+        raise ValueError("convention must be 'dalton' or 'natural'")
+
+
 def npack(o: int, v: int) -> int:
     n = o * v
     return n * (n + 1) // 2
@@ -39,6 +44,21 @@ def unpack_t2(p: np.ndarray, o: int, v: int) -> np.ndarray:
     M[r, c] = p
     M[c, r] = p
     return M.reshape(o, v, o, v).transpose(1, 0, 3, 2).copy()
+
+
+def scale_pdiag(t2p: np.ndarray, f: float) -> np.ndarray:
+    """Copy of packed doubles (one vector, or a stack along the first axis)
+    with the diagonal elements (ai, ai), at ai(ai+3)/2, times f: the step
+    between Dalton's left-vector convention and plain packed dot products
+    (CCLR_DIASCL, cc/cclr_util.F:331)."""
+    out = np.array(t2p, dtype=np.float64, copy=True)
+    n = out.shape[-1]
+    nov = int((np.sqrt(8 * n + 1) - 1) // 2)
+    if nov * (nov + 1) // 2 != n:
+        raise ValueError("not a packed doubles length")
+    ai = np.arange(nov)
+    out[..., ai * (ai + 3) // 2] *= f
+    return out
 
 
 def df_factor_ao(cmo: np.ndarray, eri_pairs: np.ndarray, nbas: int, thr: float = 1e-12, maxvec: int = 0):
@@ -100,6 +120,7 @@ class CCContext:
         if B.ndim != 3 or B.shape[1] != N or B.shape[2] != N:
             raise ValueError("B must have shape (naux, N, N) with N = len(eps)")
         self.o, self.v, self.naux = nocc, N - nocc, B.shape[0]
+        self.mxdiis = mxdiis
         h = C.c_void_p()
         check(lib().ccgpu_init(self.o, self.v, self.naux, ptr(eps), ptr(B), mxdiis, C.byref(h)))
         self._h = h
@@ -119,6 +140,7 @@ class CCContext:
             raise ValueError("cmo must hold nbas x len(eps) values and eri_pairs the packed pair supermatrix")
         self = cls.__new__(cls)
         self.o, self.v = nocc, N - nocc
+        self.mxdiis = mxdiis
         h, naux = C.c_void_p(), C.c_long()
         check(lib().ccgpu_init_ao(self.o, self.v, nbas, ptr(eps), ptr(cmo), ptr(eri_pairs), thr, mxdiis,
                                   C.byref(h), C.byref(naux)))
@@ -162,11 +184,18 @@ class CCContext:
         check(lib().ccgpu_rhs(self._h, ptr(t1), ptr(t2p), ptr(om1), ptr(om2), precision))
         return om1, om2
 
-    def lhtr(self, ctr1: np.ndarray, ctr2: np.ndarray, precision: int = FP64):
+    def lhtr(self, ctr1: np.ndarray, ctr2: np.ndarray, precision: int = FP64, convention: str = "dalton"):
         """CC_LHTR at the current amplitudes: rho = ctr A for one vector (1-D
-        inputs) or a stack of vectors (2-D, one per row), packed storage."""
+        inputs) or a stack of vectors (2-D, one per row), packed storage.
+        convention "dalton": Dalton's left vectors, rho = D J^T S ctr (the C
+        ABI); "natural": rho = J^T ctr, the gradient of ctr . Omega(t) with
+        plain packed dot products (J = dOmega/dt; S, D halve / double the
+        packed doubles diagonal)."""
+        _check_convention(convention)
         c1 = np.ascontiguousarray(ctr1, dtype=np.float64)
         c2 = np.ascontiguousarray(ctr2, dtype=np.float64)
+        if convention == "natural" and c2.size:
+            c2 = scale_pdiag(c2, 2.0)
         single = c1.ndim == 1
         c1 = c1.reshape(-1, self.nt1) if c1.size else c1.reshape(0, self.nt1)
         c2 = c2.reshape(-1, self.nt2) if c2.size else c2.reshape(0, self.nt2)
@@ -177,6 +206,8 @@ class CCContext:
         r2 = np.empty((n, self.nt2))
         if n:
             check(lib().ccgpu_lhtr(self._h, precision, n, ptr(c1), ptr(c2), ptr(r1), ptr(r2)))
+            if convention == "natural":
+                r2 = scale_pdiag(r2, 0.5)
         return (r1[0], r2[0]) if single else (r1, r2)
 
     def energy_of(self, t1: np.ndarray, t2p: np.ndarray) -> float:
@@ -217,22 +248,30 @@ class CCContext:
                                       C.byref(n), ptr(hist)))
         return {"e_start": e0.value, "e_ccsd": e.value, "niter": n.value, "history": hist[: n.value].copy()}
 
-    def eta(self):
-        """CC_ETA at the current amplitudes: eta = dE/dt, packed storage."""
+    def eta(self, convention: str = "dalton"):
+        """CC_ETA at the current amplitudes, packed storage: D dE/dt as Dalton
+        forms it ("dalton"), or dE/dt ("natural")."""
+        _check_convention(convention)
         e1 = np.empty(self.nt1)
         e2 = np.empty(self.nt2)
         check(lib().ccgpu_eta(self._h, ptr(e1), ptr(e2)))
-        return e1, e2
+        return (e1, scale_pdiag(e2, 0.5)) if convention == "natural" else (e1, e2)
 
-    def lambda_solve(self, thr: float = 1e-10, maxit: int = 100, maxred: int = 20, precision: int = FP64):
-        """Ground-state multipliers: lambda A + eta = 0 (CCEQ_SOL for 'L0').
+    def lambda_solve(self, thr: float = 1e-10, maxit: int = 100, maxred: int = 20, precision: int = FP64,
+                     convention: str = "dalton"):
+        """Ground-state multipliers: lambda A + eta = 0 (CCEQ_SOL for 'L0'),
+        in Dalton's left-vector convention ("dalton") or as the multipliers of
+        plain packed dot products ("natural", the doubles diagonal halved).
         Returns (l1, l2, niter, resnorm)."""
+        _check_convention(convention)
         l1 = np.empty(self.nt1)
         l2 = np.empty(self.nt2)
         it = C.c_int()
         rn = C.c_double()
         check(lib().ccgpu_lambda_solve(self._h, precision, thr, maxit, maxred, ptr(l1), ptr(l2), C.byref(it),
                                        C.byref(rn)))
+        if convention == "natural":
+            l2 = scale_pdiag(l2, 0.5)
         return l1, l2, it.value, rn.value
 
     def lambda_stats(self):
@@ -241,10 +280,14 @@ class CCContext:
         check(lib().ccgpu_lambda_stats(self._h, C.byref(a), C.byref(b)))
         return a.value, b.value
 
-    def onedens(self, l1: np.ndarray, l2: np.ndarray, precision: int = FP64) -> np.ndarray:
-        """Unrelaxed MO one-electron density D[p,q] = dL/dh_pq (CC_D1AO, MO basis)."""
+    def onedens(self, l1: np.ndarray, l2: np.ndarray, precision: int = FP64, convention: str = "dalton") -> np.ndarray:
+        """Unrelaxed MO one-electron density D[p,q] = dL/dh_pq (CC_D1AO, MO
+        basis) from multipliers in the given convention (as lambda_solve)."""
+        _check_convention(convention)
         l1 = np.ascontiguousarray(l1, dtype=np.float64)
         l2 = np.ascontiguousarray(l2, dtype=np.float64)
+        if convention == "natural":
+            l2 = scale_pdiag(l2, 2.0)
         if l1.size != self.nt1 or l2.size != self.nt2:
             raise ValueError("multipliers must have NT1AMX and NT2AMX elements")
         N = self.o + self.v

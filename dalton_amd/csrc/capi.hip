@@ -135,14 +135,14 @@ int ccgpu_lhtr(ccgpu_ctx* ctx, int precision, int nsimtr, const double* ctr1, co
     CCGPU_REQUIRE(ctx, "ccgpu_lhtr: null context");
     CCGPU_REQUIRE(nsimtr == 0 || (ctr1 && ctr2 && rho1 && rho2), "ccgpu_lhtr: null argument");
     ctx->cc->set_precision(prec_of(precision));
-    ctx->cc->lhtr_host(nsimtr, ctr1, ctr2, rho1, rho2);
+    ctx->cc->lhtr_host(nsimtr, ctr1, ctr2, rho1, rho2, true);
   });
 }
 
 int ccgpu_eta(ccgpu_ctx* ctx, double* eta1, double* eta2) {
   return guarded([&] {
     CCGPU_REQUIRE(ctx && eta1 && eta2, "ccgpu_eta: null argument");
-    ctx->cc->eta_host(eta1, eta2);
+    ctx->cc->eta_host(eta1, eta2, true);
   });
 }
 
@@ -151,7 +151,7 @@ int ccgpu_lambda_solve(ccgpu_ctx* ctx, int precision, double thr, int maxit, int
   return guarded([&] {
     CCGPU_REQUIRE(ctx && l1 && l2, "ccgpu_lambda_solve: null argument");
     ctx->cc->set_precision(prec_of(precision));
-    ctx->cc->lambda_solve(thr, maxit, maxred, l1, l2, resnorm, niter);
+    ctx->cc->lambda_solve(thr, maxit, maxred, l1, l2, resnorm, niter, true);
   });
 }
 
@@ -167,7 +167,7 @@ int ccgpu_onedens(ccgpu_ctx* ctx, int precision, const double* l1, const double*
   return guarded([&] {
     CCGPU_REQUIRE(ctx && l1 && l2 && dens, "ccgpu_onedens: null argument");
     ctx->cc->set_precision(prec_of(precision));
-    ctx->cc->onedens_host(l1, l2, dens);
+    ctx->cc->onedens_host(l1, l2, dens, true);
   });
 }
 

@@ -68,7 +68,8 @@ class CCSD {
   double energy_host(const double* t1, const double* t2p);
   bool diis_host(const double* s, double* t, int niter);
 
-  // residual with host arrays (Dalton layouts), used by ccgpu_rhs
+  // residual with host arrays (Dalton layouts), used by ccgpu_rhs; leaves the
+  // context's amplitudes, iteration count and DIIS state untouched
   void residual_host(const double* t1, const double* t2p, double* om1, double* om2p);
 
   // (T) energy over the restricted triple list i >= j >= k (i = j = k
@@ -83,8 +84,17 @@ class CCSD {
   // Left transformation rho = lambda A at the current amplitudes (CC_LHTR):
   // nvec trial vectors in Dalton's layouts (host arrays, vector-major).  The
   // forward intermediates are formed once per amplitude set and precision
-  // and reused for every vector; the DIIS history is released first.
-  void lhtr_host(int nvec, const double* l1, const double* l2p, double* rho1, double* rho2p);
+  // and reused for every vector; the DIIS history is released first unless
+  // the transformation's buffers fit beside it.
+  // dalton: Dalton's packed conventions for left vectors (CC_TRDRV,
+  // cc/cc_trdrv.F:484-486, and the left product of CCLR_DUMTRR on the
+  // finite-difference Jacobian of CCLR_FDJAC, cc/cclr_util.F:1228,1343):
+  // the packed doubles diagonal of the trial vector enters the squared
+  // vector unscaled and the result's diagonal is twice the packed-gradient
+  // one, rho_D(L) = D rho(S L) with S (D) halving (doubling) the doubles
+  // diagonal; otherwise rho_mu = sum_nu l_nu dOmega_nu/dt_mu with packed
+  // dot products on both sides.
+  void lhtr_host(int nvec, const double* l1, const double* l2p, double* rho1, double* rho2p, bool dalton = false);
 
   // Ground-state multipliers (lambda.hip): eta = dE/dt (CC_ETA), the
   // reduced-space solution of lambda A + eta = 0 (CCEQ_SOL from the left;
@@ -92,9 +102,12 @@ class CCSD {
   // converged, lambda and *niter still set) and the
   // unrelaxed MO one-electron density D[p][q] = dL/dh[p][q] (CC_D1AO).
   // Host arrays in the layouts of lhtr_host.
-  void eta_host(double* e1, double* e2p);
-  int lambda_solve(double thr, int maxit, int maxred, double* l1, double* l2p, double* resnorm, int* niter);
-  void onedens_host(const double* l1, const double* l2p, double* D);
+  // dalton: eta_D = D eta (CC_ETA, cc/cc_fop.F:2955), lambda_D = D lambda
+  // (the solution of rho_D(lambda_D) + eta_D = 0), onedens reads lambda_D.
+  void eta_host(double* e1, double* e2p, bool dalton = false);
+  int lambda_solve(double thr, int maxit, int maxred, double* l1, double* l2p, double* resnorm, int* niter,
+                   bool dalton = false);
+  void onedens_host(const double* l1, const double* l2p, double* D, bool dalton = false);
   // Transformations of the last lambda_solve: [0] FP64, [1] BF16x3.
   const int* lambda_counts() const { return lam_counts_; }
 
@@ -146,6 +159,7 @@ class CCSD {
   DBuf<double> Bt_, Bvv_, Boo_, BvoT_, F_, ht_, Jd_, G1_, Gam_, Z_, W_, E1_, E2_, hx_;
   // amplitudes / residual
   DBuf<double> t1_, T_, om1_, Om_;
+  DBuf<double> rt1_, rT_;  // amplitudes handed to residual_host (ccgpu_rhs)
   DBuf<double> bufR_, bufX2_, bufTx_, bufU_, bufC_, bufD_;
   // ladder
   DBuf<double> taup_, taum_, Y_, Vp_, Vm_, Sp_, Sm_;

@@ -173,7 +173,7 @@ void CCSD::release_work() {
                           &bufR_, &bufX2_, &bufTx_, &bufU_, &bufC_, &bufD_, &taup_, &taum_, &Y_, &Vp_,
                           &Vm_, &Sp_, &Sm_, &S2_, &diisS_, &diisE_, &Tb_, &Ub_, &Txb_, &Db_, &Btb_, &Bvvb_, &Boob_,
                           &BvoTb_, &Zb_, &Wb_, &Fb_, &E1b_, &E2b_, &Jdb_, &Gamb_, &G1b_, &taupb_, &taumb_, &vecb_,
-                          &lst_})
+                          &lst_, &rt1_, &rT_})
     b->release();
   for (DBuf<unsigned short>* b : {&lpBh_, &lpBl_, &lpTh_, &lpTl_, &lpVh_, &lpVl_}) b->release();
   work_ = false;
@@ -525,13 +525,21 @@ void CCSD::get_amplitudes(double* t1_host, double* t2p_host) {
 }
 
 void CCSD::residual_host(const double* t1, const double* t2p, double* om1, double* om2p) {
-  const long nov = dm_.nov;
-  set_amplitudes(t1, t2p);
-  residual(t1_.p, T_.p, om1_.p, Om_.p);
+  // The vector function of the given amplitudes only: they are staged in
+  // buffers of their own, so the context's amplitudes, iteration count and
+  // DIIS history and cycle are left as they were (CCRHSN has no side effects
+  // on the caller's iteration state either).
+  const long nov = dm_.nov, np2 = dm_.ntamp - nov;
+  ensure_work();
+  rt1_.ensure((size_t)nov);
+  rT_.ensure((size_t)dm_.nov2);
+  CCGPU_HIPX(hipMemcpyAsync(rt1_.p, t1, nov * sizeof(double), hipMemcpyHostToDevice, st_));
+  CCGPU_HIPX(hipMemcpyAsync(bufX2_.p, t2p, np2 * sizeof(double), hipMemcpyHostToDevice, st_));
+  unpack_sym(rT_.p, bufX2_.p, nov, st_);
+  residual(rt1_.p, rT_.p, om1_.p, Om_.p);
   CCGPU_HIPX(hipMemcpyAsync(om1, om1_.p, nov * sizeof(double), hipMemcpyDeviceToHost, st_));
   pack_sym(bufX2_.p, Om_.p, nov, st_);
-  CCGPU_HIPX(hipMemcpyAsync(om2p, bufX2_.p, (dm_.ntamp - nov) * sizeof(double), hipMemcpyDeviceToHost,
-                           st_));
+  CCGPU_HIPX(hipMemcpyAsync(om2p, bufX2_.p, np2 * sizeof(double), hipMemcpyDeviceToHost, st_));
   CCGPU_HIPX(hipStreamSynchronize(st_));
 }
 

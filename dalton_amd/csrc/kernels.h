@@ -71,6 +71,8 @@ namespace ccgpu {
 // ---- left transformation (adjoint) helpers
 // M[i][i] *= f for i < n (row stride ld)
 void scale_diag(double* M, long n, long ld, double f, hipStream_t st);
+// t2p[ai(ai+3)/2] *= f for ai < nov: the packed doubles diagonal.
+void scale_pdiag(double* t2p, long nov, double f, hipStream_t st);
 // Bt[P][k][k] += f * x[P] for k < o
 void add_occ_diag(double* Bt, const double* x, long naux, int N, int o, double f, hipStream_t st);
 // S+/-bar[r][q] gathered from the full cotangent L (adjoint of ladder_scatter)

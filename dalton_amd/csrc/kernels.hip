@@ -648,6 +648,21 @@ void scale_diag(double* M, long n, long ld, double f, hipStream_t st) {
   CCGPU_HIPX(hipGetLastError());
 }
 
+// Diagonal elements (ai, ai) of a packed doubles vector, at ai(ai+3)/2
+// (INDEX(NAI,NAI) - 1 of cc/ccsd_energy.F:1666), times f: the conversion
+// between packed-vector conventions (CCLR_DIASCL, cc/cclr_util.F:331).
+__global__ void k_scale_pdiag(double* t2p, long nov, double f) {
+  const long ai = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (ai < nov) t2p[ai * (ai + 3) / 2] *= f;
+}
+
+void scale_pdiag(double* t2p, long nov, double f, hipStream_t st) {
+  if (nov <= 0) return;
+  check_span(t2p, 0, nov * (nov + 1) / 2 - 1, sizeof(double), "scale_pdiag");
+  CCGPU_LAUNCH(k_scale_pdiag, dim3(cdiv(nov, 256)), dim3(256), 0, st, t2p, nov, f);
+  CCGPU_HIPX(hipGetLastError());
+}
+
 __global__ void k_add_occ_diag(double* Bt, const double* x, long naux, int N, int o, double f) {
   const long P = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (P >= naux) return;

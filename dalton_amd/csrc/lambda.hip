@@ -140,17 +140,19 @@ void CCSD::eta_device(double* e) {
   pack_sym(e + nov, bufX2_.p, nov, st_);
 }
 
-void CCSD::eta_host(double* e1, double* e2p) {
+void CCSD::eta_host(double* e1, double* e2p, bool dalton) {
   CCGPU_REQUIRE(iter_ >= 1, "ccgpu_eta: no amplitudes (run the CCSD iterations or set amplitudes first)");
   ensure_work();
   DBuf<double> e((size_t)dm_.ntamp);
   eta_device(e.p);
+  if (dalton) scale_pdiag(e.p + dm_.nov, dm_.nov, 2.0, st_);
   CCGPU_HIPX(hipMemcpyAsync(e1, e.p, dm_.nov * sizeof(double), hipMemcpyDeviceToHost, st_));
   CCGPU_HIPX(hipMemcpyAsync(e2p, e.p + dm_.nov, (dm_.ntamp - dm_.nov) * sizeof(double), hipMemcpyDeviceToHost, st_));
   CCGPU_HIPX(hipStreamSynchronize(st_));
 }
 
-int CCSD::lambda_solve(double thr, int maxit, int maxred, double* l1, double* l2p, double* resnorm, int* niter) {
+int CCSD::lambda_solve(double thr, int maxit, int maxred, double* l1, double* l2p, double* resnorm, int* niter,
+                       bool dalton) {
   CCGPU_REQUIRE(iter_ >= 1, "ccgpu_lambda_solve: no amplitudes (run the CCSD iterations or set amplitudes first)");
   CCGPU_REQUIRE(thr > 0.0 && maxit > 0 && maxred >= 2, "ccgpu_lambda_solve: bad threshold, iteration or space limit");
   const long n = dm_.ntamp, nov = dm_.nov;
@@ -302,6 +304,7 @@ int CCSD::lambda_solve(double thr, int maxit, int maxred, double* l1, double* l2
   }
   const bool conv = rn < thr || dryrun();
   set_precision(user);
+  if (dalton) scale_pdiag(lam + nov, nov, 2.0, st_);
   if (l1) CCGPU_HIPX(hipMemcpyAsync(l1, lam, nov * sizeof(double), hipMemcpyDeviceToHost, st_));
   if (l2p) CCGPU_HIPX(hipMemcpyAsync(l2p, lam + nov, (n - nov) * sizeof(double), hipMemcpyDeviceToHost, st_));
   CCGPU_HIPX(hipStreamSynchronize(st_));
@@ -312,7 +315,7 @@ int CCSD::lambda_solve(double thr, int maxit, int maxred, double* l1, double* l2
   return it;
 }
 
-void CCSD::onedens_host(const double* l1, const double* l2p, double* D) {
+void CCSD::onedens_host(const double* l1, const double* l2p, double* D, bool dalton) {
   CCGPU_REQUIRE(iter_ >= 1, "ccgpu_onedens: no amplitudes (run the CCSD iterations or set amplitudes first)");
   const long n = dm_.ntamp, nov = dm_.nov, N = dm_.N, NN = N * N;
   const int o = dm_.o, v = dm_.v;
@@ -321,6 +324,7 @@ void CCSD::onedens_host(const double* l1, const double* l2p, double* D) {
   DBuf<double> x((size_t)2 * n), Dd((size_t)NN);
   CCGPU_HIPX(hipMemcpyAsync(x.p, l1, nov * sizeof(double), hipMemcpyHostToDevice, st_));
   CCGPU_HIPX(hipMemcpyAsync(x.p + nov, l2p, (n - nov) * sizeof(double), hipMemcpyHostToDevice, st_));
+  if (dalton) scale_pdiag(x.p + nov, nov, 0.5, st_);
   lhtr_packed(x.p, x.p + n);  // leaves the cotangent of the half-dressed h~ in Fb
   CCGPU_HIPX(hipMemcpyAsync(Dd.p, Fb_.p, NN * sizeof(double), hipMemcpyDeviceToDevice, st_));
   g(o, N, v, -1.0, t1_.p, v, 1, Fb_.p + o * N, N, 1, 1.0, Dd.p, N, 1, st_);

@@ -51,8 +51,23 @@ void CCSD::ensure_lhtr() {
 // Jd, Tx, U, C, D, Gam, E1, E2, Z, W, F_kc and tau+/- on the device.
 void CCSD::lhtr_prepare() {
   if (lhtr_ready_ && lhtr_prec_ == bulk_) return;
-  diisS_.release();  // the converged ground state needs no DIIS history
-  diisE_.release();
+  if (diisS_.p && !Tb_.p) {
+    // The DIIS history stays when the cotangent buffers and the trial-vector
+    // staging fit beside it (with 4 GB to spare); otherwise it is released,
+    // as the converged ground state needs none (a later iterate restarts
+    // the DIIS cycle).
+    const int o = dm_.o, v = dm_.v, naux = dm_.naux;
+    const double N = (double)dm_.N, nov = (double)dm_.nov, oo = (double)o * o;
+    const double need = 8.0 * (4.0 * dm_.nov2 + naux * (N * N + (double)v * v + oo + 2.0 * nov + 1.0) +
+                               oo * nov + N * N + (double)v * v + oo + 2.0 * oo * oo +
+                               2.0 * dm_.npo * dm_.npv + 2.0 * nov + 4.0 * dm_.ntamp) + 4.0e9;
+    size_t fr = 0, tot = 0;
+    if (!dryrun()) CCGPU_HIP(hipMemGetInfo(&fr, &tot));
+    if (!dryrun() && (double)fr < need) {
+      diisS_.release();
+      diisE_.release();
+    }
+  }
   residual(t1_.p, T_.p, om1_.p, Om_.p);
   ensure_lhtr();
   lhtr_ready_ = true;
@@ -215,7 +230,7 @@ void CCSD::lhtr_apply(const double* l1, double* Xb, double* rho1, double* rho2s)
 // Vectors are pipelined: while vector k is transformed on the compute
 // stream, vector k+1 is copied in and the result of vector k-1 copied out
 // on a copy stream, through two sets of device staging buffers.
-void CCSD::lhtr_host(int nvec, const double* l1, const double* l2p, double* rho1, double* rho2p) {
+void CCSD::lhtr_host(int nvec, const double* l1, const double* l2p, double* rho1, double* rho2p, bool dalton) {
   CCGPU_REQUIRE(nvec >= 0, "ccgpu_lhtr: nvec < 0");
   CCGPU_REQUIRE(iter_ >= 1, "ccgpu_lhtr: no amplitudes (run the CCSD iterations or set amplitudes first)");
   if (nvec == 0) return;
@@ -238,8 +253,11 @@ void CCSD::lhtr_host(int nvec, const double* l1, const double* l2p, double* rho1
     const int q = iv & 1;
     CCGPU_HIPX(hipStreamWaitEvent(st_, ev_in_[q], 0));
     unpack_sym(Om_.p, in2(q), nov, st_);
-    scale_diag(Om_.p, nov, nov, 2.0, st_);  // Xb = Lf + diag(Lf)
+    // Xb = 2 Lt: Lf + diag(Lf) for packed dots; the squared vector itself in
+    // Dalton's convention (S halves the diagonal, then the same 2 Lt)
+    if (!dalton) scale_diag(Om_.p, nov, nov, 2.0, st_);
     lhtr_apply(in1(q), Om_.p, out1(q), bufX2_.p);
+    if (dalton) scale_diag(bufX2_.p, nov, nov, 2.0, st_);  // D on the result
     pack_sym(out2(q), bufX2_.p, nov, st_);
     CCGPU_HIPX(hipEventRecord(ev_done_[q], st_));
   };

@@ -50,21 +50,31 @@ int ccgpu_finalize(ccgpu_ctx* ctx);
  *   SUBROUTINE CCRHSN(OMEGA1,OMEGA2,T1AM,T2AM,WORK,LWORK,APROXR12)
  * cc/ccsd_rhs.F:20 (plain closed-shell CCSD, APROXR12 blank).  T1AM/T2AM
  * are read only (Dalton squares T2AM in place and restores it; no caller
- * workspace is needed here).  Also replaces the context's current amplitudes. */
+ * workspace is needed here).  Like CCRHSN it has no effect on the iteration
+ * state: the context's amplitudes, iteration count and DIIS history and
+ * cycle are unchanged, so it can be mixed with ccgpu_ccsd_iterate or called
+ * for finite-difference Jacobians (CCLR_FDJAC, cc/cclr_util.F:946). */
 int ccgpu_rhs(ccgpu_ctx* ctx, const double* t1am, const double* t2am, double* omega1,
               double* omega2, int precision);
 
 /* Left-hand transformation of nsimtr trial vectors at the context's current
  * amplitudes: replaces CC_LHTR (cc/cc_lhtr.F:20) for MODEL = CCSD
  * (closed shell, C1 symmetry, no R12/CC3/PE terms), whose CTR1/CTR2 trial
- * vectors and RHO1/RHO2 results are held NSIMTR at a time.  In packed
- * storage on both sides,
- *   rho_mu = sum_nu ctr_nu dOmega_nu/dt_mu,
- * Omega the vector function of ccgpu_rhs (CCRHSN).  ctr1/rho1 are
- * nsimtr x NT1AMX, ctr2/rho2 nsimtr x NT2AMX, vector-major.  The forward
+ * vectors and RHO1/RHO2 results are held NSIMTR at a time.  Dalton's packed
+ * convention for left vectors: the doubles part of a trial vector is squared
+ * without scaling its diagonal (CC_TRDRV scales only right vectors,
+ * cc/cc_trdrv.F:484-486), and the result is the left product with Dalton's
+ * Jacobian as CCLR_FDJAC defines it (cc/cclr_util.F:1228,1343: diagonal
+ * doubles rows halved, diagonal doubles columns doubled; applied from the
+ * left by CCLR_DUMTRR, :1624).  With J_mu,nu = dOmega_mu/dt_nu on packed
+ * arrays (Omega the vector function of ccgpu_rhs) and S, D the diagonal
+ * matrices that halve, respectively double, the packed doubles diagonal,
+ *   rho = D J^T S ctr.
+ * ctr1/rho1 are nsimtr x NT1AMX, ctr2/rho2 nsimtr x NT2AMX, vector-major.  The forward
  * intermediates at the current amplitudes are built on the first call and
- * reused until the amplitudes or the precision change; the DIIS history is
- * released (a later iterate restarts the DIIS cycle). */
+ * reused until the amplitudes or the precision change.  The DIIS history is
+ * kept when the transformation's buffers fit beside it in device memory,
+ * and released otherwise (a later iterate then restarts the DIIS cycle). */
 int ccgpu_lhtr(ccgpu_ctx* ctx, int precision, int nsimtr, const double* ctr1, const double* ctr2, double* rho1,
                double* rho2);
 
@@ -133,18 +143,23 @@ int ccgpu_restart_amplitudes(ccgpu_ctx* ctx, const char* path, double* e_corr);
 /* Right-hand side of the ground-state multiplier equations at the context's
  * current amplitudes: replaces CC_ETA (cc/cc_fop.F:2955) for MODEL = CCSD
  * with canonical Hartree-Fock orbitals.  eta = dE/dt in the packed storage
- * of ccgpu_lhtr (eta1 NT1AMX, eta2 NT2AMX values):
+ * of ccgpu_lhtr (eta1 NT1AMX, eta2 NT2AMX values), as CC_ETA forms it
+ * (2 F^_ia and 2 TCME of (ia|jb) over every packed element, :3019-3069):
  *   eta1[(a,i)] = 2 sum_bj L_iajb t_bj,
- *   eta2[ai<=bj] = 2 L_iajb (ai < bj), L_iaia (ai = bj),  L_iajb = 2(ia|jb) - (ib|ja). */
+ *   eta2[ai<=bj] = 2 L_iajb,  L_iajb = 2(ia|jb) - (ib|ja),
+ * which is D dE/dt in the notation of ccgpu_lhtr. */
 int ccgpu_eta(ccgpu_ctx* ctx, double* eta1, double* eta2);
 
 /* Solves the ground-state multiplier (L0) equations lambda A + eta = 0 at
  * the context's current amplitudes: replaces the CCEQ_SOL call for the list
  * 'L0' in CC_FOP (cc/cc_fop.F:274; cc/cceq_solv.F:20, ISIDE = -1) by a
- * reduced-space solver whose trial vectors stay in device memory.  A is the
- * Jacobian of ccgpu_lhtr, both sides packed, so lambda is the multiplier of
- * L = E + sum_p lambda_p Omega_p with packed dot products.  Converged when
- * the 2-norm of lambda A + eta is below thr (Dalton's THRLEQ); at most maxit
+ * reduced-space solver whose trial vectors stay in device memory.  lambda A
+ * is ccgpu_lhtr and eta ccgpu_eta, both in Dalton's left-vector convention,
+ * so lambda = D lambda_p with lambda_p the multiplier of
+ * L = E + sum_p lambda_p,p Omega_p in plain packed dot products.  Converged
+ * when the 2-norm of J^T lambda_p + dE/dt (the packed-gradient form of the
+ * equations, at most 2x smaller than that of lambda A + eta) is below thr
+ * (Dalton's THRLEQ); at most maxit
  * transformations, maxred trial vectors (capped at 64 and by free device
  * memory; a full space collapses onto the current solution).  precision 1
  * (BF16x3 bulk) runs an iterative refinement: the FP64 residual of lambda is
@@ -162,7 +177,8 @@ int ccgpu_lambda_solve(ccgpu_ctx* ctx, int precision, double thr, int maxit, int
 int ccgpu_lambda_stats(ccgpu_ctx* ctx, int* n_fp64, int* n_mixed);
 
 /* Unrelaxed one-electron density in the MO basis from the amplitudes and
- * multipliers lambda (l1, l2 as returned by ccgpu_lambda_solve): CC_D1AO
+ * multipliers lambda (l1, l2 as returned by ccgpu_lambda_solve, Dalton's
+ * left-vector convention): CC_D1AO
  * (cc/cc_d1ao.F:20) without its back-transformation to the AO basis.
  * dens[p*N + q] = dL/dh_pq, N = nocc + nvir, reference part 2 delta_ij
  * included, so that an unrelaxed first-order property (CC_FOP) of a
@@ -205,7 +221,8 @@ int ccgpu_ccsd_iterate(ccgpu_ctx* ctx, int precision, double* e_corr, double* om
 
 /* The whole loop: guess, then iterate until |dE| < threnr and norm < thrvec
  * (cc/ccsd_energy.F:946-949) or maxit (returns 4, "CC equations not
- * converged", :952-956).  history (may be NULL) receives maxit energies. */
+ * converged", :952-956).  history (may be NULL, else room for maxit values)
+ * receives the energy of each iteration made, at most maxit. */
 int ccgpu_ccsd_energy(ccgpu_ctx* ctx, int maxit, double threnr, double thrvec, int precision,
                       double* e_mp2, double* e_ccsd, int* niter, double* history);
 

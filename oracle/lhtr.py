@@ -16,10 +16,20 @@ tests/test_oracle_lhtr.py checks it against central finite differences of
 lambda . Omega(t) (h = 1e-5) in random directions, and checks that the
 restated residual equals oracle.ccsd.residual.
 
-Parity unpinned: the reference's own CC_LHTR output cannot be produced here
-(Dalton is not built in this environment), so the convention that CC_LHTR
-applies to the packed doubles diagonal of its trial and result vectors is
-not checked against a reference number.
+Dalton's convention for left vectors (left_transform_dalton) differs on
+the packed doubles diagonal only: CC_TRDRV squares a left trial vector
+without scaling its diagonal (cc/cc_trdrv.F:484-486 scales right vectors
+only), and the reference defines its Jacobian by the finite-difference
+protocol of CCLR_FDJAC (cc/cclr_util.F:946): column nu is
+(Omega(t + delta/2 e_nu) - Omega(t - delta/2 e_nu))/delta with the doubles
+diagonal rows halved (:1228) and, for a diagonal doubles nu, the column
+doubled (:1343); CCLR_DUMTRR (:1624) applies it from the left as
+rho_nu = sum_mu column_nu[mu] L_mu.  Hence rho_D = D J^T S L with S and D
+halving and doubling the packed doubles diagonal.  tests/test_oracle_lhtr.py
+builds that Jacobian column by column from oracle.ccsd.residual and checks
+left_transform_dalton against the DUMTRR product.  The reference's CC_LHTR
+itself is not run here (Dalton's cmake build is out of bounds), so the
+protocol above, read from the reference, is what pins the convention.
 """
 from __future__ import annotations
 
@@ -108,6 +118,23 @@ def left_transform(sys: DFSystem, t1p: np.ndarray, t2p: np.ndarray, l1p: np.ndar
     L = torch.dot(om1.T.reshape(-1), _T(l1p)) + torch.dot(_pack_t2_torch(om2), _T(l2p))
     g1, g2 = torch.autograd.grad(L, (a1, a2))
     return g1.numpy(), g2.numpy()
+
+
+def scale_pdiag(t2p: np.ndarray, f: float) -> np.ndarray:
+    """Copy of packed doubles with the diagonal (ai, ai), at ai(ai+3)/2, times f
+    (CCLR_DIASCL, cc/cclr_util.F:331)."""
+    out = np.array(t2p, dtype=np.float64, copy=True)
+    nov = int((np.sqrt(8 * out.size + 1) - 1) // 2)
+    ai = np.arange(nov)
+    out[ai * (ai + 3) // 2] *= f
+    return out
+
+
+def left_transform_dalton(sys: DFSystem, t1p, t2p, l1p, l2p):
+    """rho_D = D J^T S L: CC_LHTR's result for a left vector L in Dalton's
+    packed convention (module docstring)."""
+    r1, r2 = left_transform(sys, t1p, t2p, l1p, scale_pdiag(l2p, 0.5))
+    return r1, scale_pdiag(r2, 2.0)
 
 
 def lagrangian(sys: DFSystem, t1p, t2p, l1p, l2p) -> float:

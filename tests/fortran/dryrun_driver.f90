@@ -1,0 +1,60 @@
+! The calls a Dalton build makes through the ISO_C_BINDING module
+! dalton_amd/fortran/ccgpu_interface.f90, run against libccgpu with
+! CCGPU_DRYRUN=1 (no GPU: launches and copies are skipped, every operand span
+! is still checked).  Argument "ok": every call succeeds and the driver
+! prints "dryrun driver ok"; "bad": a negative trial-vector count reaches
+! QUIT through ccgpu_check.  QUIT below stands in for Dalton's
+! (gp/gptrygve.F:256): it prints the text and stops with status 3.
+program dryrun_driver
+  use ccgpu_interface
+  implicit none
+  integer, parameter :: o = 3, v = 5, naux = 7, n = o + v, nt1 = o * v, nt2 = nt1 * (nt1 + 1) / 2
+  real(c_double) :: eps(n), b(naux * n * n), t1(nt1), t2(nt2), om1(nt1), om2(nt2)
+  real(c_double) :: c1(2 * nt1), c2(2 * nt2), r1(2 * nt1), r2(2 * nt2)
+  real(c_double) :: s(nt1 + nt2), t(nt1 + nt2), l1(nt1), l2(nt2), dens(n * n), hist(4)
+  real(c_double) :: e4, e5, emp2, ecorr, omnorm, eccsd, rn
+  integer(c_int) :: niter
+  character(kind=c_char) :: fname(16)
+  character(len=10) :: name
+  character(len=16) :: mode
+  integer :: i
+  call get_command_argument(1, mode)
+  eps = [(-1.0d0 + 0.25d0 * i, i = 1, n)]
+  b = 0.01d0
+  t1 = 0.0d0
+  t2 = 0.0d0
+  c1 = 1.0d0
+  c2 = 1.0d0
+  s = 0.0d0
+  t = 0.0d0
+  if (ccgpu_version() /= 1) call quit('unexpected library version')
+  call ccgpu_check(ccgpu_init(o, v, naux, eps, b, 8, ccgpu_ctx), 'ccgpu_init')
+  call ccgpu_check(ccgpu_ccsd_guess(ccgpu_ctx, emp2), 'CCSD_GUESS')
+  call ccgpu_ccrhsn(om1, om2, t1, t2)
+  call ccgpu_check(ccgpu_ccsd_iterate(ccgpu_ctx, CCGPU_MIXED, ecorr, omnorm), 'CCSD_ENERGY')
+  call ccgpu_ccsd_diis(s, t, 1)
+  call ccgpu_check(ccgpu_energy(ccgpu_ctx, t1, t2, ecorr), 'CCSD_ECCSD')
+  call ccgpu_check(ccgpu_ccsd_energy(ccgpu_ctx, 4, 1.0d-8, 1.0d9, CCGPU_FP64, emp2, eccsd, niter, hist), &
+                   'CCSD_ENERGY')
+  if (trim(mode) == 'bad') call ccgpu_cc_lhtr(-1, c1, c2, r1, r2)
+  call ccgpu_cc_lhtr(2, c1, c2, r1, r2)
+  call ccgpu_check(ccgpu_eta(ccgpu_ctx, l1, l2), 'CC_ETA')
+  call ccgpu_check(ccgpu_lambda_solve(ccgpu_ctx, CCGPU_FP64, 1.0d-8, 20, 8, l1, l2, niter, rn), 'CCEQ_SOL')
+  call ccgpu_check(ccgpu_onedens(ccgpu_ctx, CCGPU_FP64, l1, l2, dens), 'CC_D1AO')
+  call ccgpu_ccsd_pt(e4, e5)
+  call ccgpu_check(ccgpu_rsp_filename('R0' // c_null_char, 1, 0, fname), 'CC_RWPRE')
+  name = ' '
+  do i = 1, 10
+    if (fname(i) == c_null_char) exit
+    name(i:i) = fname(i)
+  end do
+  if (name /= 'CCR0_1___0') call quit('file name '//name)
+  call ccgpu_check(ccgpu_finalize(ccgpu_ctx), 'ccgpu_finalize')
+  write (*, '(a)') 'dryrun driver ok'
+end program dryrun_driver
+
+subroutine quit(text)
+  character(len=*), intent(in) :: text
+  write (*, '(a)') 'QUIT: ' // text
+  error stop 3
+end subroutine quit

@@ -128,7 +128,7 @@ def test_degenerate_shapes_match_oracle(o, v, naux):
     with d.CCContext(eps, B, o) as c:
         om1, om2 = c.rhs(p1, p2)
         c.set_amplitudes(p1, p2)
-        g1, g2 = c.lhtr(l1, l2)
+        g1, g2 = c.lhtr(l1, l2, convention="natural")
         e4, e5 = c.pt()
     s = max(1.0, np.abs(occ.pack_t2(r2)).max())
     assert np.abs(om1 - occ.pack_t1(r1)).max() < 1e-12 * s and np.abs(om2 - occ.pack_t2(r2)).max() < 1e-12 * s

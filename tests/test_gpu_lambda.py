@@ -10,6 +10,7 @@ import pytest
 from conftest import GOLDEN, load_molecule
 from oracle import ccsd as occ
 from oracle import lambda_eq as ole
+from dalton_amd.cc import scale_pdiag
 
 pytestmark = pytest.mark.gpu
 
@@ -38,7 +39,7 @@ def test_eta_matches_oracle(name):
         t1 = 0.03 * rng.standard_normal(c.nt1)
         t2 = 0.03 * rng.standard_normal(c.nt2)
         c.set_amplitudes(t1, t2)
-        e1, e2 = c.eta()
+        e1, e2 = c.eta(convention="natural")
     q1, q2 = ole.eta(occ.DFSystem(eps, B, o), t1, t2)
     s = max(1.0, np.abs(q2).max())
     assert np.abs(e1 - q1).max() < 1e-12 * s and np.abs(e2 - q2).max() < 1e-12 * s
@@ -56,8 +57,10 @@ def test_onedens_matches_oracle_at_random_multipliers():
         l1 = rng.standard_normal(c.nt1)
         l2 = rng.standard_normal(c.nt2)
         c.set_amplitudes(t1, t2)
-        D = c.onedens(l1, l2)
+        D = c.onedens(l1, l2, convention="natural")
+        Dd = c.onedens(l1, scale_pdiag(l2, 2.0))  # the same multipliers in Dalton's convention
     Q = ole.density(occ.DFSystem(eps, B, o), t1, t2, l1, l2)
+    assert np.array_equal(D, Dd)
     assert np.abs(D - Q).max() < 1e-11 * max(1.0, np.abs(Q).max()), np.abs(D - Q).max()
     assert abs(np.trace(D) - 2 * o) < 1e-10  # the correlation part is traceless
 
@@ -74,8 +77,11 @@ def test_lambda_matches_oracle_water_ccpvdz():
         e1, e2 = c.eta()
         r1, r2 = c.lhtr(l1, l2)
     assert rn < 1e-10 and 0 < it < 60
-    # the returned multipliers satisfy the equations through the separate lhtr entry point
-    assert np.sqrt(np.sum((r1 + e1) ** 2) + np.sum((r2 + e2) ** 2)) < 2e-10
+    # the returned multipliers satisfy the equations through the separate
+    # lhtr and eta entry points, all in Dalton's left-vector convention
+    # (that residual is D times the packed-gradient one: at most 2x its norm)
+    assert np.sqrt(np.sum((r1 + e1) ** 2) + np.sum((r2 + e2) ** 2)) < 4e-10
+    l2 = scale_pdiag(l2, 0.5)  # the multipliers of plain packed dot products (the oracle's)
     s = occ.DFSystem(eps, B, o)
     q1, q2, _ = ole.solve_lambda(s, t1, t2, thr=1e-11)
     assert np.abs(l1 - q1).max() < 1e-8 and np.abs(l2 - q2).max() < 1e-8
@@ -173,9 +179,9 @@ def test_multipliers_at_degenerate_shapes_match_oracle(o, v, naux):
         t1 = 0.02 * rng.standard_normal(c.nt1)
         t2 = 0.02 * rng.standard_normal(c.nt2)
         c.set_amplitudes(t1, t2)
-        e1, e2 = c.eta()
-        l1, l2, _, rn = c.lambda_solve(thr=1e-11)
-        D = c.onedens(l1, l2)
+        e1, e2 = c.eta(convention="natural")
+        l1, l2, _, rn = c.lambda_solve(thr=1e-11, convention="natural")
+        D = c.onedens(l1, l2, convention="natural")
     s = occ.DFSystem(eps, B, o)
     q1, q2 = ole.eta(s, t1, t2)
     assert np.abs(e1 - q1).max() < 1e-12 and np.abs(e2 - q2).max() < 1e-12

@@ -47,7 +47,7 @@ def test_lhtr_matches_oracle(name):
     with d.CCContext(eps, B, o) as c:
         t1, t2, l1, l2 = _vectors(c.nt1, c.nt2, rng)
         c.set_amplitudes(t1, t2)
-        r1, r2 = c.lhtr(l1, l2)
+        r1, r2 = c.lhtr(l1, l2, convention="natural")
         q1, q2 = olh.left_transform(sysm, t1, t2, l1, l2)
     s = max(1.0, np.abs(q1).max(), np.abs(q2).max())
     assert np.abs(r1 - q1).max() < 1e-12 * s, np.abs(r1 - q1).max()
@@ -69,7 +69,7 @@ def test_lhtr_at_converged_ccsd_matches_oracle():
         t1, t2 = c.amplitudes()
         L1 = rng.standard_normal((3, c.nt1))
         L2 = rng.standard_normal((3, c.nt2))
-        R1, R2 = c.lhtr(L1, L2)
+        R1, R2 = c.lhtr(L1, L2, convention="natural")
     for k in range(3):
         q1, q2 = olh.left_transform(sysm, t1, t2, L1[k], L2[k])
         s = max(1.0, np.abs(q2).max())
@@ -88,7 +88,7 @@ def test_lhtr_matches_finite_differences(name):
     with d.CCContext(eps, B, o) as c:
         t1, t2, l1, l2 = _vectors(c.nt1, c.nt2, rng)
         c.set_amplitudes(t1, t2)
-        r1, r2 = c.lhtr(l1, l2)
+        r1, r2 = c.lhtr(l1, l2, convention="natural")
         h = 1e-5
         worst = 0.0
         for _ in range(32):
@@ -131,8 +131,8 @@ def _c4_check(sigma, label, strict):
         L1 /= nrm[:, None]
         L2 /= nrm[:, None]
         c.set_amplitudes(t1, t2)
-        R1, R2 = c.lhtr(L1, L2)
-        s1, s2 = c.lhtr(L1[5], L2[5])
+        R1, R2 = c.lhtr(L1, L2, convention="natural")
+        s1, s2 = c.lhtr(L1[5], L2[5], convention="natural")
         assert np.abs(s1 - R1[5]).max() == 0.0 and np.abs(s2 - R2[5]).max() == 0.0
         h = 1e-5
         worst = 0.0
@@ -191,7 +191,7 @@ def test_lhtr_and_rhs_with_batched_ladder(budget_mb, monkeypatch):
         t1, t2, l1, l2 = _vectors(c.nt1, c.nt2, rng)
         om1, om2 = c.rhs(t1, t2)
         c.set_amplitudes(t1, t2)
-        r1, r2 = c.lhtr(l1, l2)
+        r1, r2 = c.lhtr(l1, l2, convention="natural")
     v = eps.size - o
     p1, p2 = occ.residual(sysm, occ.unpack_t1(t1, o, v), occ.unpack_t2(t2, o, v))
     p1, p2 = occ.pack_t1(p1), occ.pack_t2(p2)
@@ -255,8 +255,61 @@ def test_lhtr_hf_molecule_matches_oracle():
         t1, t2 = c.amplitudes()
         L1 = rng.standard_normal((2, c.nt1))
         L2 = rng.standard_normal((2, c.nt2))
-        R1, R2 = c.lhtr(L1, L2)
+        R1, R2 = c.lhtr(L1, L2, convention="natural")
     for k in range(2):
         q1, q2 = olh.left_transform(sysm, t1, t2, L1[k], L2[k])
         s = max(1.0, np.abs(q2).max())
         assert np.abs(R1[k] - q1).max() < 1e-12 * s and np.abs(R2[k] - q2).max() < 1e-12 * s
+
+
+def test_lhtr_is_daltons_left_product_of_its_fd_jacobian():
+    """ccgpu_lhtr in Dalton's left-vector convention against the reference's
+    own definition of its Jacobian: the records of CCLR_FDJAC
+    (cc/cclr_util.F:946-1400: central differences of the vector function,
+    here ccgpu_rhs, with the doubles diagonal rows halved and the diagonal
+    doubles records doubled) applied from the left as CCLR_DUMTRR does
+    (:1624, NSIDE = -1), at the converged ground state of water/STO-3G and at
+    random amplitudes of a synthetic system."""
+    import dalton_amd as d
+    from dalton_amd.cc import scale_pdiag
+
+    for label in ("h2o_sto3g_exgr", "synthetic_o3_v5_n12"):
+        if label.startswith("synthetic"):
+            from dalton_amd.synthetic import make_system
+
+            eps, B = make_system(3, 5, 12, 4)
+            o = 3
+        else:
+            eps, B, o, _ = load_molecule(label)
+        print(f"lhtr vs FDJAC protocol {label}", flush=True)
+        rng = np.random.default_rng(17)
+        with d.CCContext(eps, B, o) as c:
+            if label.startswith("synthetic"):
+                t1 = 0.03 * rng.standard_normal(c.nt1)
+                t2 = 0.03 * rng.standard_normal(c.nt2)
+                c.set_amplitudes(t1, t2)
+            else:
+                c.ccsd_energy(maxit=60, threnr=1e-12)
+                t1, t2 = c.amplitudes()
+            nt1, nt2 = c.nt1, c.nt2
+            diag = {ai * (ai + 3) // 2 for ai in range(nt1)}
+            delta = 1e-4
+            recs = np.empty((nt1 + nt2, nt1 + nt2))
+            for nu in range(nt1 + nt2):
+                a1, a2 = t1.copy(), t2.copy()
+                x, k = (a1, nu) if nu < nt1 else (a2, nu - nt1)
+                x[k] -= 0.5 * delta
+                m1, m2 = c.rhs(a1, a2)
+                x[k] += delta
+                p1, p2 = c.rhs(a1, a2)
+                col = np.concatenate([p1 - m1, scale_pdiag(p2 - m2, 0.5)]) / delta
+                recs[nu] = 2.0 * col if (nu >= nt1 and k in diag) else col
+            L1 = rng.standard_normal((2, nt1))
+            L2 = rng.standard_normal((2, nt2))
+            R1, R2 = c.lhtr(L1, L2)
+        for q in range(2):
+            dum = recs @ np.concatenate([L1[q], L2[q]])
+            got = np.concatenate([R1[q], R2[q]])
+            err = np.abs(dum - got).max() / np.abs(got).max()
+            print(f"  vector {q}: max rel |DUMTRR - lhtr| = {err:.2e}", flush=True)
+            assert err < 1e-7

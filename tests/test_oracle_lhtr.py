@@ -55,3 +55,70 @@ def test_left_transform_single_packed_elements():
         e2[idx] = 1.0
         fd = (lhtr.lagrangian(s, t1, t2 + h * e2, l1, l2) - lhtr.lagrangian(s, t1, t2 - h * e2, l1, l2)) / (2 * h)
         assert abs(fd - rho2[idx]) < 1e-7 * max(1.0, abs(fd))
+
+
+def _fdjac_protocol(s, t1, t2, delta=1e-4):
+    """Dalton's finite-difference Jacobian as CCLR_FDJAC builds it
+    (cc/cclr_util.F:946-1400): one record per packed amplitude nu, the
+    central difference of the vector function with step delta/2 either side,
+    the doubles diagonal of every record halved (CCLR_DIASCL(.., XHALF),
+    :1228/1323), and the record of a diagonal doubles nu doubled (:1343)."""
+    from oracle.ccsd import pack_t1, pack_t2, residual, unpack_t1, unpack_t2
+
+    o, v = s.o, s.v
+    nt1, nt2 = t1.size, t2.size
+    nov = nt1
+    diag = {ai * (ai + 3) // 2 for ai in range(nov)}
+
+    def omega(a1, a2):
+        r1, r2 = residual(s, unpack_t1(a1, o, v), unpack_t2(a2, o, v))
+        return np.concatenate([pack_t1(r1), pack_t2(r2)])
+
+    recs = []
+    for nu in range(nt1 + nt2):
+        a1, a2 = t1.copy(), t2.copy()
+        x = a1 if nu < nt1 else a2
+        k = nu if nu < nt1 else nu - nt1
+        x[k] -= 0.5 * delta
+        om_m = omega(a1, a2)
+        x[k] += delta
+        om_p = omega(a1, a2)
+        col = (om_p - om_m) / delta
+        col[nt1:] = lhtr.scale_pdiag(col[nt1:], 0.5)
+        if nu >= nt1 and k in diag:
+            col *= 2.0
+        recs.append(col)
+    return np.array(recs)  # recs[nu][mu]
+
+
+def test_dalton_left_convention_is_the_fdjac_left_product():
+    """CCLR_DUMTRR from the left (NSIDE = -1): rho_nu = sum_mu record_nu[mu]
+    L_mu with the records of CCLR_FDJAC, equal to left_transform_dalton; and
+    the plain packed-gradient transform differs from it on the doubles
+    diagonal (a convention mix-up fails here)."""
+    s, t1, t2, l1, l2, _ = _setup(o=2, v=4, naux=9, seed=4)
+    recs = _fdjac_protocol(s, t1, t2)
+    L = np.concatenate([l1, l2])
+    dum = recs @ L
+    d1, d2 = lhtr.left_transform_dalton(s, t1, t2, l1, l2)
+    ref = np.concatenate([d1, d2])
+    scale = np.abs(ref).max()
+    assert np.abs(dum - ref).max() < 1e-7 * scale, np.abs(dum - ref).max()
+    n1, n2 = lhtr.left_transform(s, t1, t2, l1, l2)
+    assert np.abs(np.concatenate([n1, n2]) - dum).max() > 1e-3 * scale
+
+
+def test_dalton_eta_convention():
+    """CC_ETA (cc/cc_fop.F:2955): the doubles part is 2 x TCME of (ia|jb) over
+    every packed element, diagonal included (CCSD_TCMEPK with SCAL = 1,
+    cc/cc3_omeg.F:2398, then DSCAL by 2), i.e. D times the packed gradient
+    dE/dt of the oracle."""
+    from oracle import lambda_eq
+    from oracle.ccsd import pack_t2
+
+    s, t1, t2, _, _, _ = _setup(o=2, v=4, naux=9, seed=4)
+    o = s.o
+    g = np.einsum("Pai,Pbj->aibj", s.B[:, o:, :o], s.B[:, o:, :o])  # (ai|bj)
+    Lg = 2.0 * g - g.transpose(0, 3, 2, 1)  # L_iajb = 2(ia|jb) - (ib|ja)
+    _, e2 = lambda_eq.eta(s, t1, t2)
+    assert np.abs(lhtr.scale_pdiag(e2, 2.0) - 2.0 * pack_t2(Lg)).max() < 1e-13
