@@ -59,10 +59,11 @@ def sync():
         pass
 
 
-def cpu_baseline(sample=(16, 128, 512), seed=3):
+def cpu_baseline(o=32, v=256, naux=1024, seed=3):
     """The BLAS restatement of the residual in oracle/ (numpy + multithreaded
-    OpenBLAS) on a smaller instance, scaled to o=64/v=512/naux=2048 by the
-    algorithmic flop ratio."""
+    OpenBLAS on the host cores): one whole residual of the o=32 v=256
+    naux=1024 synthetic system at its MP2 amplitudes, and that time scaled to
+    o=64 v=512 naux=2048 by the ratio of the algorithmic flop counts."""
     from dalton_amd.synthetic import make_system
     from oracle import ccsd as occ
     from oracle import ccsd_blas
@@ -70,29 +71,173 @@ def cpu_baseline(sample=(16, 128, 512), seed=3):
     e0, B0 = make_system(4, 12, 20, seed)  # warm the BLAS threads
     s0 = occ.DFSystem(e0, B0, 4)
     ccsd_blas.residual(s0, *occ.mp2_guess(s0))
-    o, v, naux = sample
     eps, B = make_system(o, v, naux, seed)
     sysm = occ.DFSystem(eps, B, o)
     t1, t2 = occ.mp2_guess(sysm)
     t0 = time.perf_counter()
     ccsd_blas.residual(sysm, t1, t2)
     dt = time.perf_counter() - t0
+    cores = len(os.sched_getaffinity(0))
     try:
         from threadpoolctl import threadpool_info
 
-        info = threadpool_info()
-        cores = max(int(i.get("num_threads", 1)) for i in info) if info else 1
+        blas = max((int(i.get("num_threads", 1)) for i in threadpool_info() if i.get("user_api") == "blas"),
+                   default=cores)
     except Exception:
-        cores = os.cpu_count() or 1
+        blas = cores
     scale = f_ccsd(64, 512, 2048) / f_ccsd(o, v, naux)
     return {
         "value": round(dt * scale, 2),
         "unit": "s/iter",
-        "cores": cores,
+        "cores": min(cores, blas),
         "kind": "port",
-        "sample": f"one residual of the BLAS restatement (oracle/ccsd_blas.py) at o={o} v={v} naux={naux} "
-                  f"({dt:.2f} s), scaled x{scale:.1f} by algorithmic flops to o=64 v=512 naux=2048",
+        "sample": f"one whole residual of the BLAS restatement (oracle/ccsd_blas.py, numpy + OpenBLAS with "
+                  f"{min(cores, blas)} threads) at o={o} v={v} naux={naux}: {dt:.2f} s, "
+                  f"{f_ccsd(o, v, naux) / dt / 1e9:.1f} GFLOP/s; scaled x{scale:.1f} by algorithmic flops "
+                  f"to o=64 v=512 naux=2048",
+        "sample_seconds": round(dt, 3),
+        "sample_gflops": round(f_ccsd(o, v, naux) / dt / 1e9, 2),
     }
+
+
+def cpu_baseline_child():
+    """Child-process side of the CPU baseline: the system is built and the
+    BLAS threads warmed at once, the timed residual starts when the parent
+    writes a line to stdin, and the result is printed as one JSON line."""
+    from dalton_amd.synthetic import make_system
+
+    make_system(4, 12, 20, 3)  # imports and first numpy calls
+    sys.stdout.write("ready\n")
+    sys.stdout.flush()
+    if sys.stdin.readline().strip() != "go":  # the parent ended (or gave up) without starting it
+        return
+    print(json.dumps(cpu_baseline()), flush=True)
+
+
+def start_cpu_baseline():
+    """Starts the CPU baseline in a child process before this process touches
+    the GPU; the parent lets it run (go()) once the headline timing is over,
+    so it shares the host with the later GPU legs only.  Two host cores are
+    left to the process that drives the GPU."""
+    import subprocess
+
+    threads = max(1, len(os.sched_getaffinity(0)) - 2)
+    env = dict(os.environ)
+    for k in ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS", "MKL_NUM_THREADS"):
+        env[k] = str(threads)
+    p = subprocess.Popen([sys.executable, os.path.abspath(__file__), "--workload", "cpu-baseline"],
+                         stdin=subprocess.PIPE, stdout=subprocess.PIPE, text=True, env=env)
+    import atexit
+
+    atexit.register(lambda: p.poll() is None and p.kill())  # never outlives a failed run
+    return p
+
+
+def finish_cpu_baseline(p, log):
+    out, _ = p.communicate(timeout=900)
+    if p.returncode != 0:
+        raise RuntimeError(f"CPU baseline process failed with status {p.returncode}")
+    res = json.loads([ln for ln in out.splitlines() if ln.startswith("{")][-1])
+    log(f"CPU baseline: {res['sample']}")
+    return res
+
+
+def convergent_leg(o, v, naux, seed, ntrip, log, threnr=1e-9, thrvec=1e-7, maxit=60):
+    """The metric's accuracy gate at the headline shape.  The same synthetic
+    distribution with the integral scale of dalton_amd.synthetic.batch_sigma,
+    at which CCSD converges: solved in FP64 (every iteration timed, as the
+    CCSD_ENERGY loop: |dE| < threnr and |Omega| < thrvec) and in mixed
+    precision (the library's loop, ccgpu_ccsd_energy, whose last two
+    iterations are FP64), each followed by (T) over ntrip triples spread
+    evenly over the i>=j>=k list.  Reports the errors of E_CCSD, E_(T) and
+    the amplitudes of the mixed solve against the FP64 one, the FP64 residual
+    norm of the mixed amplitudes (one FP64 ccgpu_rhs) and both solve times."""
+    import dalton_amd as d
+    from dalton_amd.synthetic import batch_sigma, make_system
+
+    sigma = batch_sigma(v, naux)
+    t0 = time.perf_counter()
+    eps, B = make_system(o, v, naux, seed, sigma=sigma)
+    log(f"convergent system o={o} v={v} naux={naux} sigma={sigma:.6f} generated in {time.perf_counter() - t0:.1f} s")
+    res = {"workload": f"synthetic o={o} v={v} naux={naux} seed {seed}, B ~ N(0, sigma^2) with "
+                       f"sigma = {sigma:.6f} (batch_sigma), MP2 start",
+           "threnr": threnr, "thrvec": thrvec}
+    F = f_ccsd(o, v, naux)
+
+    def pt_subset(c, prec):
+        _, _, _, ntr = c.pt_partial(0, 0, 1, precision=prec)  # (T) integral layouts (untimed)
+        stride = max(1, ntr // ntrip)
+        sync()
+        t0 = time.perf_counter()
+        e4, e5, fl, _ = c.pt_partial(0, ntrip, stride, precision=prec)
+        sync()
+        dt = time.perf_counter() - t0
+        return {"e4": e4, "e5": e5, "seconds": round(dt, 4), "tflops": round(fl / max(dt, 1e-9) / 1e12, 2),
+                "triples": ntrip, "stride": stride, "triples_total": ntr}
+
+    # FP64: the CCSD_ENERGY loop from the host, one timed iteration at a time
+    with d.CCContext(eps, B, o) as c:
+        e_prev = c.guess()
+        times, hist, norms = [], [], []
+        conv = False
+        for it in range(1, maxit + 1):
+            sync()
+            t0 = time.perf_counter()
+            e, n = c.iterate(d.FP64)
+            times.append(time.perf_counter() - t0)
+            hist.append(e)
+            norms.append(n)
+            log(f"convergent fp64 iteration {it}: E = {e:.12f} dE = {e - e_prev:.2e} |Omega| = {n:.3e} "
+                f"({times[-1]:.2f} s)")
+            conv = abs(e - e_prev) < threnr and n < thrvec
+            e_prev = e
+            if conv:
+                break
+        if not conv:
+            raise RuntimeError("convergent leg: the FP64 solve did not converge")
+        t1_d, t2_d = c.amplitudes()
+        pt_d = pt_subset(c, d.FP64)
+        log(f"convergent fp64 (T) subset: E4 = {pt_d['e4']:.14f} E5 = {pt_d['e5']:.14f}")
+    res["fp64"] = {"iterations": len(times), "solve_seconds": round(sum(times), 3),
+                   "s_per_iter": round(sum(times) / len(times), 4),
+                   "s_per_iter_min": round(min(times), 4), "ccsd_tflops": round(F * len(times) / sum(times) / 1e12, 2),
+                   "e_ccsd": hist[-1], "history": hist, "omega_norms": norms, "pt_subset": pt_d}
+    # mixed: the library's loop (BF16x3 until |dE| < 1e-6 and |Omega| < 1e-3, then FP64)
+    with d.CCContext(eps, B, o) as c:
+        del B
+        sync()
+        t0 = time.perf_counter()
+        r = c.ccsd_energy(maxit=maxit, threnr=threnr, thrvec=thrvec, precision=d.MIXED)
+        sync()
+        tm = time.perf_counter() - t0
+        log(f"convergent mixed: {r['niter']} iterations in {tm:.2f} s, E = {r['e_ccsd']:.12f}")
+        t1_m, t2_m = c.amplitudes()
+        om1, om2 = c.rhs(t1_m, t2_m, d.FP64)  # FP64 residual of the mixed amplitudes
+        rnorm = float(np.sqrt(np.dot(om1, om1) + np.dot(om2, om2)))
+        pt_m = pt_subset(c, d.MIXED)
+        log(f"convergent mixed (T) subset: E4 = {pt_m['e4']:.14f} E5 = {pt_m['e5']:.14f}")
+    res["mixed"] = {"iterations": r["niter"], "solve_seconds": round(tm, 3), "e_ccsd": r["e_ccsd"],
+                    "history": [float(x) for x in r["history"]], "pt_subset": pt_m,
+                    "fp64_residual_norm": rnorm}
+    dccsd = abs(r["e_ccsd"] - hist[-1])
+    dpt = abs(pt_m["e4"] + pt_m["e5"] - pt_d["e4"] - pt_d["e5"])
+    ntr, nsub = pt_d["triples_total"], pt_d["triples"]
+    res.update({
+        "abs_dE_ccsd": dccsd,
+        "abs_dE_pt_subset": dpt,
+        "abs_dE_pt_full_estimate": dpt * ntr / nsub,
+        "abs_dE_ccsd_t_estimate": dccsd + dpt * ntr / nsub,
+        "pt_subset_note": f"(T) over {nsub} of {ntr} triples (stride {pt_d['stride']}); the full-list error is "
+                          f"estimated by scaling with {ntr}/{nsub}",
+        "max_abs_dT1": float(np.abs(t1_m - t1_d).max()),
+        "max_abs_dT2": float(np.abs(t2_m - t2_d).max()),
+        "residual_norm_mixed_amplitudes_fp64": rnorm,
+        "solve_speedup_mixed_vs_fp64": round(res["fp64"]["solve_seconds"] / max(tm, 1e-9), 3),
+        "meets_1uEh": bool(dccsd + dpt * ntr / nsub <= 1e-6),
+    })
+    log(f"convergent: |dE_CCSD| = {dccsd:.2e}, |dE_(T)| subset {dpt:.2e} (full estimate "
+        f"{dpt * ntr / nsub:.2e}), max|dT2| = {res['max_abs_dT2']:.2e}, |R| = {rnorm:.2e}")
+    return res
 
 
 def lhtr_leg(nvec, log, kern_table, o=32, v=256, naux=1024, seed=4):
@@ -125,12 +270,12 @@ def lhtr_leg(nvec, log, kern_table, o=32, v=256, naux=1024, seed=4):
         sync()
         dt = time.perf_counter() - t0
         prof = profile_stop()
-        gem = sum(k_["flops"] for k_ in prof["kernels"].values())
-        gms = sum(k_["ms"] for k_ in prof["kernels"].values())
+        gem = sum(k_["flops"] for k_ in prof.get("kernels", {}).values())
+        gms = sum(k_["ms"] for k_ in prof.get("kernels", {}).values())
         res[label] = {"seconds": round(dt, 4), "s_per_vector": round(dt / nvec, 4),
                       "gemm_tflops": round(gem / max(gms, 1e-9) / 1e9, 2),
                       "gemm_flops_per_vector": gem / nvec,
-                      "phases_ms_per_vector": {k_: round(v_ / nvec, 2) for k_, v_ in prof["phases"].items()},
+                      "phases_ms_per_vector": {k_: round(v_ / nvec, 2) for k_, v_ in prof.get("phases", {}).items()},
                       "kernels": kern_table(prof)}
         out[label] = (R1, R2)
         log(f"lhtr {label}: {nvec} vectors in {dt:.3f} s")
@@ -145,7 +290,7 @@ def lhtr_leg(nvec, log, kern_table, o=32, v=256, naux=1024, seed=4):
 
 def kern_table_of(prof):
     return {k: {"count": v_["count"], "ms": round(v_["ms"], 3),
-                "tflops": round(v_["flops"] / max(v_["ms"], 1e-9) / 1e9, 2)} for k, v_ in prof["kernels"].items()}
+                "tflops": round(v_["flops"] / max(v_["ms"], 1e-9) / 1e9, 2)} for k, v_ in prof.get("kernels", {}).items()}
 
 
 def _log(msg):
@@ -181,7 +326,7 @@ def pt_leg(ctx, ntrip, o, v, log):
     for e in ("e4", "e5"):  # relative, or absolute where the FP64 value is zero (E5 at t1 = 0)
         dev = abs(pt["mixed"][e] - pt["fp64"][e])
         pt[f"rel_err_{e}_mixed_vs_fp64"] = dev / abs(pt["fp64"][e]) if pt["fp64"][e] != 0.0 else dev
-    pt["speedup_mixed_vs_fp64"] = round(pt["fp64"]["seconds"] / pt["mixed"]["seconds"], 3)
+    pt["speedup_mixed_vs_fp64"] = round(pt["fp64"]["seconds"] / max(pt["mixed"]["seconds"], 1e-9), 3)
     return pt
 
 
@@ -331,8 +476,8 @@ def lambda_workload(o=32, v=256, naux=1024, seed=4, thr=1e-8):
             prof = profile_stop()
             n64, nmx = c.lambda_stats()
             D = c.onedens(l1, l2)
-            gem = sum(k_["flops"] for k_ in prof["kernels"].values())
-            gms = sum(k_["ms"] for k_ in prof["kernels"].values())
+            gem = sum(k_["flops"] for k_ in prof.get("kernels", {}).values())
+            gms = sum(k_["ms"] for k_ in prof.get("kernels", {}).values())
             res[label] = {"seconds": round(dt, 3), "transforms": it, "transforms_fp64": n64,
                           "transforms_bf16x3": nmx, "s_per_transform": round(dt / it, 4),
                           "resnorm": rn, "gemm_tflops": round(gem / max(gms, 1e-9) / 1e9, 2),
@@ -505,7 +650,11 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3, help="timed mixed-precision iterations")
-    ap.add_argument("--fp64-steps", type=int, default=2, help="timed FP64 iterations (same starting point)")
+    ap.add_argument("--fp64-steps", type=int, default=2,
+                    help="timed FP64 iterations of the headline system (the convergent leg times every FP64 "
+                         "iteration of a whole solve at the same shape)")
+    ap.add_argument("--no-accuracy", dest="accuracy", action="store_false",
+                    help="skip the convergent-scale leg (FP64 and mixed solves at the same shape)")
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--o", type=int, default=64)
     ap.add_argument("--v", type=int, default=512)
@@ -516,7 +665,8 @@ def main():
                     help="(T) triples timed, spread evenly over the i>=j>=k list (0: skip)")
     ap.add_argument("--lhtr-vectors", type=int, default=16,
                     help="left-transformation trial vectors timed at o=32 v=256 naux=1024 (0: skip)")
-    ap.add_argument("--workload", choices=["ccsd", "pt", "pt48", "batch", "ao", "lambda", "restart", "copy", "blas"], default="ccsd",
+    ap.add_argument("--workload", choices=["ccsd", "pt", "pt48", "batch", "ao", "lambda", "restart", "copy", "blas",
+                                           "cpu-baseline"], default="ccsd",
                     help="ccsd: the headline line (default); pt: the (T) leg alone at --o/--v; "
                          "ao: the AO Cholesky front end; lambda: the L0 multiplier solve at "
                          "o=32 v=256 in both precisions; restart: the R0 amplitude file at o=64 v=512; pt48: full (T) at o=48 v=384 in both "
@@ -527,6 +677,9 @@ def main():
         print("bench.py: runs on one GPU only; --gpus must be 1 (the CC path is not sharded "
               "across GPUs)", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "cpu-baseline":
+        cpu_baseline_child()
+        return
     if args.workload == "pt":
         print(json.dumps(pt_workload(args.o, args.v, args.naux, args.seed, args.pt_triples)), flush=True)
         return
@@ -552,13 +705,14 @@ def main():
         print(json.dumps(blas_workload()), flush=True)
         return
 
+    # the CPU baseline's process is started before this one touches the GPU
+    cpu = start_cpu_baseline() if not args.no_cpu_baseline else None
+
     import dalton_amd as d
     from dalton_amd.cc import profile_start, profile_stop
     from dalton_amd.synthetic import make_system
 
-    def log(msg):
-        print(f"[bench] {msg}", file=sys.stderr, flush=True)
-
+    log = _log
     o, v, naux = args.o, args.v, args.naux
     t0 = time.perf_counter()
     eps, B = make_system(o, v, naux, args.seed)
@@ -567,38 +721,51 @@ def main():
     del B
     e_mp2 = ctx.guess()
     log(f"E_MP2 = {e_mp2:.10f}")
+    period = ctx.mxdiis
+
+    def steps(prec, nsteps, label, timed):
+        """nsteps iterations from the MP2 start.  The integrals of this
+        distribution are far outside the convergent regime, and the iteration
+        overflows after its first DIIS restart, so the MP2 start is re-seeded
+        (outside the timer) at the start of every DIIS cycle of MXDIIS steps;
+        every step is checked to stay finite.  Returns s/step, energies,
+        residual norms and the library's event profile."""
+        hist, norms, dt = [], [], 0.0
+        if timed:
+            profile_start()
+        for s_ in range(nsteps):
+            if s_ % period == 0:
+                ctx.guess()  # MP2 amplitudes, empty DIIS history (untimed)
+            sync()
+            t0 = time.perf_counter()
+            e, n = ctx.iterate(prec)  # returns after the step (its energy is read back)
+            dt += time.perf_counter() - t0
+            if not (np.isfinite(e) and np.isfinite(n)):
+                raise RuntimeError(f"{label} step {s_ + 1}: non-finite energy or residual norm ({e}, {n})")
+            hist.append(e)
+            norms.append(n)
+            log(f"{label} step {s_ + 1}: E = {e:.10f} |Omega| = {n:.3e}")
+        return dt / max(nsteps, 1), hist, norms, (profile_stop() if timed else None)
+
     for w in range(args.warmup):  # allocates the iteration workspace
         e, n = ctx.iterate(d.MIXED)
         log(f"warmup {w + 1} (mixed): E = {e:.10f} |Omega| = {n:.3e}")
-
-    def timed_steps(prec, nsteps, label):
-        ctx.guess()  # both precisions start from the MP2 amplitudes with an empty DIIS history
-        sync()
-        profile_start()
-        t0 = time.perf_counter()
-        hist = []
-        for s_ in range(nsteps):
-            e, n = ctx.iterate(prec)
-            hist.append(e)
-            log(f"{label} step {s_ + 1}: E = {e:.10f} |Omega| = {n:.3e}")
-        sync()
-        dt = time.perf_counter() - t0
-        return dt / nsteps, hist, profile_stop()
-
     if args.steps < 1:  # profiling runs of the FP64 leg alone
-        spi_d, hist_d, prof_d = timed_steps(d.FP64, max(1, args.fp64_steps), "fp64")
+        spi_d, hist_d, _, prof_d = steps(d.FP64, max(1, args.fp64_steps), "fp64", True)
         print(json.dumps({"fp64_s_per_iter": spi_d, "kernels": kern_table_of(prof_d)}), flush=True)
         ctx.close()
+        if cpu is not None:
+            cpu.kill()
         return
-    spi_m, hist_m, prof_m = timed_steps(d.MIXED, args.steps, "mixed")
+    spi_m, hist_m, norm_m, prof_m = steps(d.MIXED, args.steps, "mixed", True)
     fp64_leg = args.fp64_steps > 0
     if fp64_leg:
-        spi_d, hist_d, prof_d = timed_steps(d.FP64, args.fp64_steps, "fp64")
-
-    kern_table = kern_table_of
+        spi_d, hist_d, norm_d, prof_d = steps(d.FP64, args.fp64_steps, "fp64", True)
 
     def roofline(prof, dt_total, peak, peak_note, attainable=None):
-        kern = prof["kernels"]
+        kern = prof.get("kernels", {})
+        if not kern:  # dry run: nothing was launched
+            return None
         dom = max(kern, key=lambda k: kern[k]["ms"])
         kd = kern[dom]
         ach = kd["flops"] / (kd["ms"] * 1e-3) / 1e12
@@ -612,14 +779,24 @@ def main():
                 "frac_attainable": round(ach / attainable, 4) if attainable else None}
 
     F = f_ccsd(o, v, naux)
+    if cpu is not None:  # headline timing done: the CPU baseline may run beside the remaining GPU legs
+        cpu.stdin.write("go\n")
+        cpu.stdin.flush()
     pt = pt_leg(ctx, args.pt_triples, o, v, log) if args.pt_triples > 0 else {}
     ctx.close()
     del ctx
 
-    lh = None
-    if args.lhtr_vectors > 0:
-        lh = lhtr_leg(args.lhtr_vectors, log, kern_table)
+    conv = convergent_leg(o, v, naux, args.seed, args.pt_triples, log) if args.accuracy else None
+    lh = lhtr_leg(args.lhtr_vectors, log, kern_table_of) if args.lhtr_vectors > 0 else None
 
+    fp64 = None
+    if fp64_leg:
+        fp64 = {"s_per_iter": round(spi_d, 4), "steps": args.fp64_steps, "ccsd_tflops": round(F / spi_d / 1e12, 3),
+                "energies": hist_d, "omega_norms": norm_d,
+                "roofline": roofline(prof_d, spi_d * args.fp64_steps, PEAK["fp64"], "FP64 matrix",
+                                     ATTAINABLE["fp64"]),
+                "phases_ms_per_iter": {k: round(v_ / args.fp64_steps, 2) for k, v_ in prof_d.get("phases", {}).items()},
+                "kernels": kern_table_of(prof_d)}
     out = {
         "metric": "CCSD s/iter + (T) TFLOP/s at o64/v512: mixed vs FP64 MFMA, |dE|<=1uEh",
         "value": round(spi_m, 4),
@@ -633,29 +810,30 @@ def main():
         "vs_baseline": None,
         "dtype": "bf16x3/f32-accumulate (bulk contractions); f64 update, DIIS, energy",
         "data": "synthetic",
-        "config": {"workload": f"synthetic DF-CCSD iterations from the MP2 start, o={o} v={v} naux={naux}, "
-                               f"seed {args.seed}",
+        "config": {"workload": f"synthetic DF-CCSD iterations from the MP2 start (re-seeded every "
+                               f"{period} steps), o={o} v={v} naux={naux}, B ~ N(0, 0.05^2), seed {args.seed}",
                    "o": o, "v": v, "naux": naux, "precision": "mixed"},
         "ccsd_tflops": round(F / spi_m / 1e12, 3),
         "ccsd_flops_per_iter": F,
-        "fp64": {"s_per_iter": round(spi_d, 4), "steps": args.fp64_steps, "ccsd_tflops": round(F / spi_d / 1e12, 3),
-                 "energies": hist_d, "roofline": roofline(prof_d, spi_d * args.fp64_steps, PEAK["fp64"], "FP64 matrix",
-                                                     ATTAINABLE["fp64"]),
-                 "phases_ms_per_iter": {k: round(v_ / args.fp64_steps, 2) for k, v_ in prof_d["phases"].items()},
-                 "kernels": kern_table(prof_d)} if fp64_leg else None,
+        "fp64": fp64,
         "speedup_mixed_vs_fp64": round(spi_d / spi_m, 3) if fp64_leg else None,
         "energies": hist_m,
-        "rel_dE_per_step_mixed_vs_fp64": [abs(a - b) / abs(b) for a, b in zip(hist_m, hist_d)] if fp64_leg else None,
+        "omega_norms": norm_m,
+        "rel_dE_per_step_mixed_vs_fp64": [abs(a - b) / max(abs(b), 1e-300) for a, b in zip(hist_m, hist_d)]
+        if fp64_leg else None,
         "roofline": roofline(prof_m, spi_m * args.steps, round(PEAK["bf16"] / 3, 1),
                              "BF16 dense matrix peak / 3 (three bf16 products per FP32-equivalent product)",
                              round(ATTAINABLE["bf16x3"], 1)),
-        "phases_ms_per_iter": {k: round(v_ / args.steps, 2) for k, v_ in prof_m["phases"].items()},
-        "kernels": kern_table(prof_m),
+        "phases_ms_per_iter": {k: round(v_ / args.steps, 2) for k, v_ in prof_m.get("phases", {}).items()},
+        "kernels": kern_table_of(prof_m),
         "pt": pt or None,
+        "convergent": conv,
         "lhtr": lh,
     }
-    if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline()
+    if cpu is not None:
+        out["cpu_baseline"] = finish_cpu_baseline(cpu, log)
+        out["cpu_baseline"]["note"] = ("timed in a child process on the host cores minus two, while this "
+                                       "process drove the later GPU legs")
     print(json.dumps(out), flush=True)
 
 

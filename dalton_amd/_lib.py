@@ -21,6 +21,8 @@ SIGNATURES = {
     "ccgpu_init": (C.c_int, [C.c_int, C.c_int, C.c_int, _dp, _dp, C.c_int, C.POINTER(C.c_void_p)]),
     "ccgpu_finalize": (C.c_int, [C.c_void_p]),
     "ccgpu_df_factor_ao": (C.c_int, [C.c_int, C.c_int, _dp, _dp, C.c_double, C.c_long, _dp, C.POINTER(C.c_long)]),
+    "ccgpu_df_factor_ao_cols": (C.c_int, [C.c_int, C.c_int, _dp, _dp, C.c_void_p, C.c_void_p, C.c_double,
+                                          C.c_long, _dp, C.POINTER(C.c_long)]),
     "ccgpu_init_ao": (C.c_int, [C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, C.c_double, C.c_int,
                                 C.POINTER(C.c_void_p), C.POINTER(C.c_long)]),
     "ccgpu_rhs": (C.c_int, [C.c_void_p, _dp, _dp, _dp, _dp, C.c_int]),
@@ -35,6 +37,7 @@ SIGNATURES = {
     "ccgpu_rsp_read": (C.c_int, [C.c_char_p, C.c_long, C.c_long, C.c_long, _dp, _dp, _dp, C.c_char_p,
                                  C.POINTER(C.c_int)]),
     "ccgpu_save_amplitudes": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int]),
+    "ccgpu_write_e_intermediates": (C.c_int, [C.c_void_p, C.c_int, C.c_char_p, C.c_char_p, _dp, _dp]),
     "ccgpu_restart_amplitudes": (C.c_int, [C.c_void_p, C.c_char_p, _dp]),
     "ccgpu_lambda_solve": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_int, C.c_int, _dp, _dp,
                                      C.POINTER(C.c_int), _dp]),

@@ -16,7 +16,8 @@ from ._lib import CCGPUError, check, lib, ptr  # noqa: F401
 from ._lib import _dp as _lib_dp
 
 FP64 = 0
-MIXED = 1
+MIXED = 1  # BF16x3 bulk contractions
+FP32 = 2  # FP32-MFMA bulk contractions (not for (T))
 
 
 def _check_convention(convention: str):
@@ -72,6 +73,44 @@ def df_factor_ao(cmo: np.ndarray, eri_pairs: np.ndarray, nbas: int, thr: float =
     out = np.empty(cap * nmo * nmo)
     naux = C.c_long()
     check(lib().ccgpu_df_factor_ao(nbas, nmo, ptr(cmo), ptr(eri_pairs), thr, cap, ptr(out), C.byref(naux)))
+    return out[: naux.value * nmo * nmo].reshape(naux.value, nmo, nmo)
+
+
+AO_COLUMNS_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int, C.POINTER(C.c_long), C.POINTER(C.c_double))
+
+
+def df_factor_ao_cols(cmo: np.ndarray, diag: np.ndarray, columns, nbas: int, thr: float = 1e-12, maxvec: int = 0):
+    """MO Cholesky factor B[P,p,q] with the AO integrals supplied column by
+    column (ccgpu_df_factor_ao_cols): diag holds the pair diagonals (ab|ab),
+    columns(pairs) returns the integral columns (cd|ab) for the requested
+    pairs (ab) as an array of shape (len(pairs), M), M = nbas(nbas+1)/2."""
+    cmo = np.ascontiguousarray(cmo, dtype=np.float64).reshape(-1)
+    diag = np.ascontiguousarray(diag, dtype=np.float64).reshape(-1)
+    nmo = cmo.size // nbas
+    M = nbas * (nbas + 1) // 2
+    if diag.size != M:
+        raise ValueError("diag must hold nbas(nbas+1)/2 values")
+    cap = M if maxvec <= 0 else min(maxvec, M)
+    out = np.empty(cap * nmo * nmo)
+    naux = C.c_long()
+    failure = []
+
+    def cb(_user, ncol, pairs, cols):
+        try:
+            p = np.ctypeslib.as_array(pairs, shape=(ncol,)).copy()
+            dst = np.ctypeslib.as_array(cols, shape=(ncol * M,))
+            dst[:] = np.ascontiguousarray(columns(p), dtype=np.float64).reshape(-1)
+            return 0
+        except Exception as exc:  # reported after the call
+            failure.append(exc)
+            return 1
+
+    fn = AO_COLUMNS_FN(cb)
+    rc = lib().ccgpu_df_factor_ao_cols(nbas, nmo, ptr(cmo), ptr(diag), C.cast(fn, C.c_void_p), None, thr, cap,
+                                       ptr(out), C.byref(naux))
+    if failure:
+        raise failure[0]
+    check(rc)
     return out[: naux.value * nmo * nmo].reshape(naux.value, nmo, nmo)
 
 
@@ -229,6 +268,17 @@ class CCContext:
             raise ValueError("vectors must hold NT1AMX + NT2AMX values")
         check(lib().ccgpu_diis(self._h, ptr(s), ptr(out), niter))
         return out
+
+    def write_e_intermediates(self, e1_path: str | None = None, e2_path: str | None = None, precision: int = FP64):
+        """CCRHSN's RSPIM E intermediates at the current amplitudes: writes the
+        CC_E1IM / CC_E2IM files (one Fortran record each) for the paths given
+        and returns (E1[b,c], E2[k,j])."""
+        e1 = np.empty(self.v * self.v)
+        e2 = np.empty(self.o * self.o)
+        check(lib().ccgpu_write_e_intermediates(self._h, precision, e1_path.encode() if e1_path else None,
+                                                e2_path.encode() if e2_path else None, ptr(e1), ptr(e2)))
+        # Dalton's order is column-major (the first index fastest)
+        return e1.reshape(self.v, self.v).T.copy(), e2.reshape(self.o, self.o).T.copy()
 
     def save_amplitudes(self, path: str, intsize: int = 4):
         """Write the amplitudes as Dalton's 'R0' file (CCR0_1___0)."""

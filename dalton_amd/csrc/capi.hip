@@ -4,6 +4,7 @@
 #include <cstring>
 #include <memory>
 #include <string>
+#include <vector>
 
 #include "../../include/ccgpu.h"
 #include "batch.h"
@@ -35,8 +36,17 @@ int guarded(F&& f) {
 }
 
 ccgpu::GemmPrec prec_of(int precision) {
-  CCGPU_REQUIRE(precision == CCGPU_FP64 || precision == CCGPU_MIXED, "unknown precision selector");
-  return precision == CCGPU_MIXED ? ccgpu::GemmPrec::BF16X3 : ccgpu::GemmPrec::F64;
+  CCGPU_REQUIRE(precision == CCGPU_FP64 || precision == CCGPU_MIXED || precision == CCGPU_FP32,
+                "unknown precision selector");
+  return precision == CCGPU_MIXED  ? ccgpu::GemmPrec::BF16X3
+         : precision == CCGPU_FP32 ? ccgpu::GemmPrec::F32
+                                   : ccgpu::GemmPrec::F64;
+}
+// (T) runs its W passes in FP64 or as BF16x3 products only.
+ccgpu::GemmPrec pt_prec_of(int precision) {
+  CCGPU_REQUIRE(precision == CCGPU_FP64 || precision == CCGPU_MIXED,
+                "(T): precision must be CCGPU_FP64 or CCGPU_MIXED");
+  return prec_of(precision);
 }
 // A non-blocking stream for work outside a context, with its scratch released at the end.
 struct SideStream {
@@ -92,6 +102,21 @@ int ccgpu_df_factor_ao(int nbas, int nmo, const double* cmo, const double* eri_a
     const std::vector<double> f = ccgpu::df_factor_ao(nbas, nmo, cmo, eri_ao, thr, maxvec, naux, &conv, ss.s);
     if (B != nullptr && !dry) std::memcpy(B, f.data(), f.size() * sizeof(double));
     if (!conv) throw ccgpu::Error(4, "ccgpu_df_factor_ao: threshold not reached within maxvec vectors");
+  });
+}
+
+int ccgpu_df_factor_ao_cols(int nbas, int nmo, const double* cmo, const double* diag, ccgpu_ao_columns_fn columns,
+                            void* user, double thr, long maxvec, double* B, long* naux) {
+  return guarded([&] {
+    const bool dry = dryrun_env();
+    CCGPU_REQUIRE(naux != nullptr && maxvec > 0 && diag != nullptr && columns != nullptr && (dry || cmo != nullptr),
+                  "ccgpu_df_factor_ao_cols: null argument or no vector room");
+    SideStream ss;
+    bool conv = false;
+    const std::vector<double> f =
+        ccgpu::df_factor_ao_cols(nbas, nmo, cmo, diag, columns, user, thr, maxvec, naux, &conv, ss.s);
+    if (B != nullptr && !dry) std::memcpy(B, f.data(), f.size() * sizeof(double));
+    if (!conv && !dry) throw ccgpu::Error(4, "ccgpu_df_factor_ao_cols: threshold not reached within maxvec vectors");
   });
 }
 
@@ -278,6 +303,21 @@ int ccgpu_rsp_read(const char* path, long nallai, long nt1, long nt2, double* ve
   });
 }
 
+int ccgpu_write_e_intermediates(ccgpu_ctx* ctx, int precision, const char* e1_path, const char* e2_path,
+                                double* e1, double* e2) {
+  return guarded([&] {
+    CCGPU_REQUIRE(ctx, "ccgpu_write_e_intermediates: null context");
+    const auto& d = ctx->cc->dims();
+    std::vector<double> b1((size_t)d.v * d.v), b2((size_t)d.o * d.o);
+    ctx->cc->set_precision(prec_of(precision));
+    ctx->cc->e_intermediates_host(b1.data(), b2.data());
+    if (e1_path) ccgpu::write_one_record(e1_path, b1.data(), (long)b1.size());
+    if (e2_path) ccgpu::write_one_record(e2_path, b2.data(), (long)b2.size());
+    if (e1) std::memcpy(e1, b1.data(), b1.size() * sizeof(double));
+    if (e2) std::memcpy(e2, b2.data(), b2.size() * sizeof(double));
+  });
+}
+
 int ccgpu_save_amplitudes(ccgpu_ctx* ctx, const char* path, int intsize) {
   return guarded([&] {
     CCGPU_REQUIRE(ctx && path, "ccgpu_save_amplitudes: null argument");
@@ -368,7 +408,7 @@ int ccgpu_set_amplitudes(ccgpu_ctx* ctx, const double* t1am, const double* t2am)
 int ccgpu_pt(ccgpu_ctx* ctx, int precision, double* e4, double* e5) {
   return guarded([&] {
     CCGPU_REQUIRE(ctx, "ccgpu_pt: null context");
-    ctx->cc->set_precision(prec_of(precision));
+    ctx->cc->set_precision(pt_prec_of(precision));
     ctx->cc->pt_energy(0, ctx->cc->pt_ntriples(), 1, e4, e5, nullptr);
   });
 }
@@ -377,7 +417,7 @@ int ccgpu_pt_partial(ccgpu_ctx* ctx, int precision, long long first, long long c
                      double* e4, double* e5, double* flops, long long* ntriples) {
   return guarded([&] {
     CCGPU_REQUIRE(ctx, "ccgpu_pt_partial: null context");
-    ctx->cc->set_precision(prec_of(precision));
+    ctx->cc->set_precision(pt_prec_of(precision));
     if (ntriples) *ntriples = ctx->cc->pt_ntriples();
     ctx->cc->pt_energy(first, count, stride, e4, e5, flops);
   });

@@ -72,6 +72,12 @@ class CCSD {
   // context's amplitudes, iteration count and DIIS state untouched
   void residual_host(const double* t1, const double* t2p, double* om1, double* om2p);
 
+  // The E intermediates of the residual at the current amplitudes
+  // (CCRHS_EFCK/CCRHS_E, cc/ccsd_rhs.F:8604,2280) in Dalton's column-major
+  // order: e1[b + v*c] = E1(b,c) (NMATAB values), e2[k + o*j] = E2(k,j)
+  // (NMATIJ values).  Runs one residual; the amplitudes are not changed.
+  void e_intermediates_host(double* e1, double* e2);
+
   // (T) energy over the restricted triple list i >= j >= k (i = j = k
   // excluded), enumerated i-major; triples first, first+stride, ... (count
   // of them).  Adds E4/E5 of those triples; *flops gets their algorithmic

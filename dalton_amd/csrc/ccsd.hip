@@ -524,6 +524,19 @@ void CCSD::get_amplitudes(double* t1_host, double* t2p_host) {
   CCGPU_HIPX(hipStreamSynchronize(st_));
 }
 
+void CCSD::e_intermediates_host(double* e1, double* e2) {
+  CCGPU_REQUIRE(iter_ >= 1, "ccgpu: no amplitudes (run the CCSD iterations or set amplitudes first)");
+  const int o = dm_.o, v = dm_.v;
+  residual(t1_.p, T_.p, om1_.p, Om_.p);  // E1_ [b][c], E2_ [k][j] at the current amplitudes
+  // transposed into Dalton's column-major order (the leading index fastest)
+  perm(bufR_.p, E1_.p, 1, 1, v, v, 0, 0, 1, v, 1.0, 0.0, st_);
+  perm(bufR_.p + (long)v * v, E2_.p, 1, 1, o, o, 0, 0, 1, o, 1.0, 0.0, st_);
+  if (e1) CCGPU_HIPX(hipMemcpyAsync(e1, bufR_.p, (size_t)v * v * sizeof(double), hipMemcpyDeviceToHost, st_));
+  if (e2)
+    CCGPU_HIPX(hipMemcpyAsync(e2, bufR_.p + (long)v * v, (size_t)o * o * sizeof(double), hipMemcpyDeviceToHost, st_));
+  CCGPU_HIPX(hipStreamSynchronize(st_));
+}
+
 void CCSD::residual_host(const double* t1, const double* t2p, double* om1, double* om2p) {
   // The vector function of the given amplitudes only: they are staged in
   // buffers of their own, so the context's amplitudes, iteration count and

@@ -183,6 +183,11 @@ void bf16x3_run(const BMulti& p, int batch, bool c_float, double flops, hipStrea
 // returns naux x nmo x nmo values; maxvec <= 0 allows one vector per AO pair.
 std::vector<double> df_factor_ao(int nbas, int nmo, const double* cmo, const double* eri_ao, double thr, long maxvec,
                                  long* naux, bool* converged, hipStream_t st);
+// The same factor from the pair diagonal and integral columns supplied by a
+// callback (qualified panels of at most 64 columns; no M x M array).
+std::vector<double> df_factor_ao_cols(int nbas, int nmo, const double* cmo, const double* diag,
+                                      int (*columns)(void*, int, const long*, double*), void* user, double thr,
+                                      long maxvec, long* naux, bool* converged, hipStream_t st);
 
 // Dalton's CC response-vector files (rspfile.hip): the name CC_RWPRE gives
 // list/symmetry/index; writing and reading the records of CC_WVRSP for a
@@ -193,6 +198,8 @@ void rsp_write(const char* path, const char* model, int intsize, long nallai, lo
                const double* vec1, const double* vec2);
 int rsp_read(const char* path, long nallai, long nt1, long nt2, double* vec0, double* vec1, double* vec2,
              char* model);
+// One record of n doubles (the single-record RSPIM files of CCRHSN).
+void write_one_record(const char* path, const double* x, long n);
 
 // V+/- of one particle-ladder a-batch from an FP32 Y block straight into the
 // split-bf16 operand layout hi/lo[2][Rp][Kp] (V+ then V-), zero padded.

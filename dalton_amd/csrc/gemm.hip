@@ -441,11 +441,16 @@ void gemm_f64(const GemmDesc& d, hipStream_t s) {
 }
 
 void gemm_bf16x3(const GemmDesc& d, hipStream_t s);
+void gemm_f32(const GemmDesc& d, hipStream_t s);
 
 void gemm(const GemmDesc& d, GemmPrec prec, hipStream_t s) {
   if (prec == GemmPrec::BF16X3) {
     CCGPU_REQUIRE(d.batch <= 65535, "gemm: batch too large for the mixed-precision kernel");
     gemm_bf16x3(d, s);
+    return;
+  }
+  if (prec == GemmPrec::F32) {
+    gemm_f32(d, s);
     return;
   }
   CCGPU_REQUIRE(prec == GemmPrec::F64, "gemm: precision not available");

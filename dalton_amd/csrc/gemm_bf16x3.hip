@@ -109,130 +109,22 @@ __global__ __launch_bounds__(128 * RW, 4 / RW) void k_gemm_bf16x3(BMulti p) {
 }
 
 
-// Large-tile variant for launches with many tiles and long K: block tile
-// 256x256 (4 waves of 128x128, 8x8 MFMA tiles each, accumulators held in
-// the accumulation registers), K step 32.  Every staged operand element
-// feeds twice as many products as in the 128x128 kernel, halving the
-// operand traffic from L2 and beyond per flop; one block per CU.
-constexpr int BT = 256, BTK = 32, BLDK = BTK + 8;
-constexpr int BNCH = BT * BTK / 8 / 256;  // 16-byte chunks per thread per array per K step
-
-template <typename OutT>
-__global__ __launch_bounds__(256, 1) void k_gemm_bf16x3_256(BMulti p) {
-  __shared__ __attribute__((aligned(16))) unsigned short sA[2][BT][BLDK];
-  __shared__ __attribute__((aligned(16))) unsigned short sB[2][BT][BLDK];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wm = wave >> 1, wn = wave & 1;
-  const long tiles_m = p.Mp / BT, tiles_n = p.Np / BT;
-  long tm, tn;
-  tile_of3(blockIdx.x, tiles_m, tiles_n, tm, tn, p.gm);
-  const long m0 = tm * BT, n0 = tn * BT, z = blockIdx.z;
-  constexpr int CPR = BTK / 8;
-  f4 acc[8][8];
-#pragma unroll
-  for (int i = 0; i < 8; ++i)
-#pragma unroll
-    for (int j = 0; j < 8; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
-  const int lr = lane & 15, lk = (lane >> 4) * 8;
-  int loff[BNCH];
-#pragma unroll
-  for (int q = 0; q < BNCH; ++q) {
-    const int c = tid + q * 256;
-    loff[q] = (c / CPR) * BLDK + (c % CPR) * 8;
-  }
-  for (int t = 0; t < p.nterm; ++t) {
-    const BTerm& T = p.t[t];
-    const long Kp = T.Kp;
-    const unsigned short* pAh = T.Ah + z * T.a_bs + m0 * Kp;
-    const unsigned short* pAl = T.Al + z * T.a_bs + m0 * Kp;
-    const unsigned short* pBh = T.Bh + z * T.b_bs + n0 * Kp;
-    const unsigned short* pBl = T.Bl + z * T.b_bs + n0 * Kp;
-    u32x4 rah[BNCH], ral[BNCH], rbh[BNCH], rbl[BNCH];
-    long goff[BNCH];
-#pragma unroll
-    for (int q = 0; q < BNCH; ++q) {
-      const int c = tid + q * 256;
-      goff[q] = (long)(c / CPR) * Kp + (c % CPR) * 8;
-    }
-#define CCGPU_BF3B_LOAD(k0)                                                      \
-  _Pragma("unroll") for (int q = 0; q < BNCH; ++q) {                             \
-    rah[q] = *reinterpret_cast<const u32x4*>(pAh + goff[q] + (k0));              \
-    ral[q] = *reinterpret_cast<const u32x4*>(pAl + goff[q] + (k0));              \
-    rbh[q] = *reinterpret_cast<const u32x4*>(pBh + goff[q] + (k0));              \
-    rbl[q] = *reinterpret_cast<const u32x4*>(pBl + goff[q] + (k0));              \
-  }
-#define CCGPU_BF3B_STORE()                                                       \
-  _Pragma("unroll") for (int q = 0; q < BNCH; ++q) {                             \
-    *reinterpret_cast<u32x4*>(&sA[0][0][0] + loff[q]) = rah[q];                  \
-    *reinterpret_cast<u32x4*>(&sA[1][0][0] + loff[q]) = ral[q];                  \
-    *reinterpret_cast<u32x4*>(&sB[0][0][0] + loff[q]) = rbh[q];                  \
-    *reinterpret_cast<u32x4*>(&sB[1][0][0] + loff[q]) = rbl[q];                  \
-  }
-    CCGPU_BF3B_LOAD(0)
-    for (long k0 = 0; k0 < Kp; k0 += BTK) {
-      __syncthreads();
-      CCGPU_BF3B_STORE()
-      __syncthreads();
-      if (k0 + BTK < Kp) {
-        CCGPU_BF3B_LOAD(k0 + BTK)
-      }
-      bf16x8 bh[8], bl[8];
-#pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        bh[u] = *reinterpret_cast<const bf16x8*>(&sB[0][wn * 128 + u * 16 + lr][lk]);
-        bl[u] = *reinterpret_cast<const bf16x8*>(&sB[1][wn * 128 + u * 16 + lr][lk]);
-      }
-#pragma unroll
-      for (int i = 0; i < 8; ++i) {
-        const bf16x8 ah = *reinterpret_cast<const bf16x8*>(&sA[0][wm * 128 + i * 16 + lr][lk]);
-        const bf16x8 al = *reinterpret_cast<const bf16x8*>(&sA[1][wm * 128 + i * 16 + lr][lk]);
-        // the three split products of one accumulator are 8 MFMAs apart
-#pragma unroll
-        for (int j = 0; j < 8; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[j], acc[i][j], 0, 0, 0);
-#pragma unroll
-        for (int j = 0; j < 8; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[j], acc[i][j], 0, 0, 0);
-#pragma unroll
-        for (int j = 0; j < 8; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[j], acc[i][j], 0, 0, 0);
-      }
-    }
-#undef CCGPU_BF3B_LOAD
-#undef CCGPU_BF3B_STORE
-    __syncthreads();
-  }
-  // D[i][j] of a 16x16 f32 MFMA tile: lane = j + 16*(i/4), element i%4.
-  OutT* C = static_cast<OutT*>(p.C) + z * p.c_b;
-#pragma unroll
-  for (int i = 0; i < 8; ++i)
-#pragma unroll
-    for (int j = 0; j < 8; ++j)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const long m = m0 + wm * 128 + i * 16 + (lane >> 4) * 4 + r;
-        const long n = n0 + wn * 128 + j * 16 + lr;
-        if (m < p.M && n < p.N) {
-          OutT* q = C + m * p.c_m + n * p.c_n;
-          double v = p.alpha * (double)acc[i][j][r];
-          if (p.beta != 0.0) v += p.beta * (double)*q;
-          *q = static_cast<OutT>(v);
-        }
-      }
-}
-
-// Block tile for one launch.  128 unless CCGPU_BF16X3_TILE=256: measured at
-// o=64 v=512 the 256-tile kernel (one block per CU) runs the residual GEMMs
-// at 269 TFLOP/s against 319 for the 128-tile kernel (two blocks per CU),
-// its halved operand traffic not making up for the exposed load latency.
-// Direct-to-LDS large-tile variant: block 256x256, 4 waves of 128x128 (8x8
-// MFMA tiles, accumulators in the accumulation registers), K step 32.
-// Operands go global -> LDS with global_load_lds_dwordx4 (no staging
-// registers, no LDS store instructions) into two LDS stages, so the next
-// step's loads are in flight during the current step's 192 MFMAs per wave,
-// with one barrier per step.  LDS rows are 64 B (four 16-byte chunks); the
-// chunk of row r sits in slot c ^ ((r >> 1) & 3), which keeps the 16-byte
-// fragment reads of eight consecutive lanes on distinct bank groups.  Each
-// wave instruction fills 1 KiB = 16 rows; per stage and array a wave issues
-// four of them.  The two stages are separate LDS objects and the K loop is
-// unrolled by two, so every LDS read names its stage statically.
+// Large-tile kernel (CCGPU_BF16X3_VARIANT=1): block tile 256x256 run by
+// eight waves (two per SIMD), each a 128x64 sub-tile of 8x4 MFMA tiles, K
+// step 32.  Per K step a wave reads 24 KiB of operand fragments from LDS for
+// 96 MFMAs, a quarter less than the 64x64-per-wave 128x128 kernel, and the
+// operands reach LDS with global_load_lds_dwordx4 (no staging registers, no
+// LDS store instructions) into two stages: the next step's loads are in
+// flight while the current step's MFMAs run, with one barrier per step.
+// LDS rows are 64 B (four 16-byte chunks); the chunk c of row r sits in slot
+// c ^ ((r >> 1) & 3), so the 16-byte fragment reads of consecutive lanes fall
+// on distinct bank groups.  One wave instruction fills 1 KiB = 16 rows; per
+// stage each wave issues two per array (hi and lo of A and B).  The two
+// stages are separate LDS objects (64 KiB each) and the K loop is unrolled by
+// two, so every LDS access names its stage statically.  The accumulation
+// order of every output element (k ascending; lo*hi, hi*lo, hi*hi per
+// k-slice of 32) is that of the 128x128 kernel, so the results are
+// bit-identical.
 constexpr int DT = 256, DTK = 32;
 __device__ __forceinline__ int swz(int r) { return (r >> 1) & 3; }
 
@@ -240,86 +132,90 @@ typedef __attribute__((address_space(3))) void lds_void;
 typedef const __attribute__((address_space(1))) void glb_void;
 
 template <typename OutT>
-__global__ __launch_bounds__(256, 1) void k_gemm_bf16x3_dl(BMulti p) {
+__global__ __launch_bounds__(512, 1) void k_gemm_bf16x3_w8(BMulti p) {
   __shared__ __attribute__((aligned(16))) unsigned short s0[4][DT * DTK];
   __shared__ __attribute__((aligned(16))) unsigned short s1[4][DT * DTK];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wm = wave >> 1, wn = wave & 1;
+  const int wm = wave >> 2, wn = wave & 3;  // 2 x 4 waves of 128 x 64
   const long tiles_m = p.Mp / DT, tiles_n = p.Np / DT;
   long tm, tn;
   tile_of3(blockIdx.x, tiles_m, tiles_n, tm, tn, p.gm);
   const long m0 = tm * DT, n0 = tn * DT, z = blockIdx.z;
-  f4 acc[8][8];
+  f4 acc[8][4];
 #pragma unroll
   for (int i = 0; i < 8; ++i)
 #pragma unroll
-    for (int j = 0; j < 8; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+    for (int j = 0; j < 4; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
   const int lr = lane & 15, kc = lane >> 4;
   const int lrow = lane >> 2, lslot = lane & 3;
   // fragment offsets (elements) of this lane within one 256 x 32 array
-  int offB[8], offA[8];
+  int offA[8], offB[4];
 #pragma unroll
   for (int u = 0; u < 8; ++u) {
-    const int rb = wn * 128 + u * 16 + lr, ra = wm * 128 + u * 16 + lr;
-    offB[u] = rb * DTK + 8 * (kc ^ swz(rb));
+    const int ra = wm * 128 + u * 16 + lr;
     offA[u] = ra * DTK + 8 * (kc ^ swz(ra));
+  }
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int rb = wn * 64 + u * 16 + lr;
+    offB[u] = rb * DTK + 8 * (kc ^ swz(rb));
   }
   for (int t = 0; t < p.nterm; ++t) {
     const BTerm& T = p.t[t];
     const long Kp = T.Kp;
     const unsigned short* src[4] = {T.Ah + z * T.a_bs + m0 * Kp, T.Al + z * T.a_bs + m0 * Kp,
                                     T.Bh + z * T.b_bs + n0 * Kp, T.Bl + z * T.b_bs + n0 * Kp};
-    long goff[4];
+    long goff[2];
 #pragma unroll
-    for (int ins = 0; ins < 4; ++ins) {
-      const int row = (wave * 4 + ins) * 16 + lrow;
+    for (int ins = 0; ins < 2; ++ins) {
+      const int row = (wave * 2 + ins) * 16 + lrow;
       goff[ins] = (long)row * Kp + 8 * (lslot ^ swz(row));
     }
     const long nsteps = Kp / DTK;
-#define CCGPU_DL_ISSUE(S, k0)                                                                       \
+#define CCGPU_W8_ISSUE(S, k0)                                                                       \
   _Pragma("unroll") for (int a = 0; a < 4; ++a)                                                    \
-    _Pragma("unroll") for (int ins = 0; ins < 4; ++ins)                                            \
+    _Pragma("unroll") for (int ins = 0; ins < 2; ++ins)                                            \
       __builtin_amdgcn_global_load_lds((glb_void*)(src[a] + goff[ins] + (k0)),                     \
-                                       (lds_void*)(&S[a][(wave * 4 + ins) * 16 * DTK]), 16, 0, 0);
-#define CCGPU_DL_COMPUTE(S)                                                                        \
+                                       (lds_void*)(&S[a][(wave * 2 + ins) * 16 * DTK]), 16, 0, 0);
+#define CCGPU_W8_COMPUTE(S)                                                                        \
   {                                                                                                \
-    bf16x8 bh[8], bl[8];                                                                           \
-    _Pragma("unroll") for (int u = 0; u < 8; ++u) {                                                \
+    bf16x8 bh[4], bl[4];                                                                           \
+    _Pragma("unroll") for (int u = 0; u < 4; ++u) {                                                \
       bh[u] = *reinterpret_cast<const bf16x8*>(&S[2][offB[u]]);                                    \
       bl[u] = *reinterpret_cast<const bf16x8*>(&S[3][offB[u]]);                                    \
     }                                                                                              \
     _Pragma("unroll") for (int i = 0; i < 8; ++i) {                                                \
       const bf16x8 ah = *reinterpret_cast<const bf16x8*>(&S[0][offA[i]]);                          \
       const bf16x8 al = *reinterpret_cast<const bf16x8*>(&S[1][offA[i]]);                          \
-      _Pragma("unroll") for (int j = 0; j < 8; ++j)                                                \
+      _Pragma("unroll") for (int j = 0; j < 4; ++j)                                                \
         acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[j], acc[i][j], 0, 0, 0);        \
-      _Pragma("unroll") for (int j = 0; j < 8; ++j)                                                \
+      _Pragma("unroll") for (int j = 0; j < 4; ++j)                                                \
         acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[j], acc[i][j], 0, 0, 0);        \
-      _Pragma("unroll") for (int j = 0; j < 8; ++j)                                                \
+      _Pragma("unroll") for (int j = 0; j < 4; ++j)                                                \
         acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[j], acc[i][j], 0, 0, 0);        \
     }                                                                                              \
   }
-    CCGPU_DL_ISSUE(s0, 0)
+    CCGPU_W8_ISSUE(s0, 0)
     for (long st = 0; st < nsteps; st += 2) {
       // even step: stage 0 (its loads are the only ones outstanding)
       __builtin_amdgcn_s_waitcnt(0);
       __syncthreads();
       if (st + 1 < nsteps) {
-        CCGPU_DL_ISSUE(s1, (st + 1) * DTK)
+        CCGPU_W8_ISSUE(s1, (st + 1) * DTK)
       }
-      CCGPU_DL_COMPUTE(s0)
+      CCGPU_W8_COMPUTE(s0)
       if (st + 1 < nsteps) {
         // odd step: stage 1
         __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
         if (st + 2 < nsteps) {
-          CCGPU_DL_ISSUE(s0, (st + 2) * DTK)
+          CCGPU_W8_ISSUE(s0, (st + 2) * DTK)
         }
-        CCGPU_DL_COMPUTE(s1)
+        CCGPU_W8_COMPUTE(s1)
       }
     }
-#undef CCGPU_DL_ISSUE
-#undef CCGPU_DL_COMPUTE
+#undef CCGPU_W8_ISSUE
+#undef CCGPU_W8_COMPUTE
     __syncthreads();  // the next term's first loads overwrite stage 0
   }
   // D[i][j] of a 16x16 f32 MFMA tile: lane = j + 16*(i/4), element i%4.
@@ -327,11 +223,11 @@ __global__ __launch_bounds__(256, 1) void k_gemm_bf16x3_dl(BMulti p) {
 #pragma unroll
   for (int i = 0; i < 8; ++i)
 #pragma unroll
-    for (int j = 0; j < 8; ++j)
+    for (int j = 0; j < 4; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const long m = m0 + wm * 128 + i * 16 + (lane >> 4) * 4 + r;
-        const long n = n0 + wn * 128 + j * 16 + lr;
+        const long n = n0 + wn * 64 + j * 16 + lr;
         if (m < p.M && n < p.N) {
           OutT* q = C + m * p.c_m + n * p.c_n;
           double v = p.alpha * (double)acc[i][j][r];
@@ -360,25 +256,15 @@ int group_m() {
 
 template <typename OutT>
 void launch_multi(BMulti p, int batch, hipStream_t s) {
-  // Block tile: CCGPU_BF16X3_VARIANT = 0 (default) 128x128, 1 256x128,
-  // 2 256x256, 3 256x256 with direct-to-LDS double-buffered loads.  Measured at o=64 v=512, the 256x256 kernel (one block of
-  // four waves per CU) runs the residual GEMMs at 269 TFLOP/s against 319
-  // for 128x128 (two blocks per CU).
+  // Block tile: 128x128, four waves of 64x64, two blocks per CU (default);
+  // CCGPU_BF16X3_VARIANT=1 runs products whose padded sizes are multiples
+  // of 256 on the eight-wave 256x256 kernel.
   p.gm = group_m();
-  const int var = variant();
   CCGPU_REQUIRE(batch < 65536, "bf16x3 gemm: grid too large");
-  if (var == 3 && p.Mp % DT == 0 && p.Np % DT == 0) {
+  if (variant() == 1 && p.Mp % DT == 0 && p.Np % DT == 0) {
     const long ntiles = (p.Mp / DT) * (p.Np / DT);
     CCGPU_REQUIRE(ntiles < (1L << 23), "bf16x3 gemm: grid too large");
-    CCGPU_LAUNCH(k_gemm_bf16x3_dl<OutT>, dim3((unsigned)ntiles, 1, (unsigned)batch), dim3(256), 0, s, p);
-  } else if (var == 2 && p.Mp % BT == 0 && p.Np % BT == 0) {
-    const long ntiles = (p.Mp / BT) * (p.Np / BT);
-    CCGPU_REQUIRE(ntiles < (1L << 23), "bf16x3 gemm: grid too large");
-    CCGPU_LAUNCH(k_gemm_bf16x3_256<OutT>, dim3((unsigned)ntiles, 1, (unsigned)batch), dim3(256), 0, s, p);
-  } else if (var == 1 && p.Mp % 256 == 0) {
-    const long ntiles = (p.Mp / 256) * (p.Np / TN);
-    CCGPU_REQUIRE(ntiles < (1L << 23), "bf16x3 gemm: grid too large");
-    CCGPU_LAUNCH((k_gemm_bf16x3<OutT, 4>), dim3((unsigned)ntiles, 1, (unsigned)batch), dim3(512), 0, s, p);
+    CCGPU_LAUNCH(k_gemm_bf16x3_w8<OutT>, dim3((unsigned)ntiles, 1, (unsigned)batch), dim3(512), 0, s, p);
   } else {
     const long ntiles = (p.Mp / TM) * (p.Np / TN);
     CCGPU_REQUIRE(ntiles < (1L << 23), "bf16x3 gemm: grid too large");
@@ -391,7 +277,7 @@ void launch_multi(BMulti p, int batch, hipStream_t s) {
 
 long bf16x3_kpad(long K) { return std::max<long>(TK, (K + TK - 1) / TK * TK); }
 long bf16x3_rpad(long R) {
-  const long t = variant() != 0 ? BT : TM;  // rows padded to the block tile in use
+  const long t = variant() != 0 ? DT : TM;  // rows padded to the block tile in use
   return (R + t - 1) / t * t;
 }
 

@@ -199,4 +199,13 @@ int rsp_read(const char* path, long nallai, long nt1, long nt2, double* vec0, do
   return (int)nwrite;
 }
 
+// A file of one record of n doubles, as CCRHSN writes its RSPIM
+// intermediates (WRITE(LU) (X(I), I = 1, N), e.g. CC_E1IM and CC_E2IM,
+// cc/ccsd_rhs.F:1898-1915), with the framing of write_record.
+void write_one_record(const char* path, const double* x, long n) {
+  CCGPU_REQUIRE(path != nullptr && n >= 0 && (n == 0 || x != nullptr), "write_one_record: bad argument");
+  File F(path, "wb");
+  write_record(F, {{x, (size_t)n * sizeof(double)}});
+}
+
 }  // namespace ccgpu

@@ -150,6 +150,14 @@ module ccgpu_interface
       integer(c_int), value :: isym, idx
       character(kind=c_char), intent(out) :: name(*)  ! at least 11 characters
     end function
+    integer(c_int) function ccgpu_write_e_intermediates(ctx, prec, e1path, e2path, e1, e2) &
+        bind(c, name="ccgpu_write_e_intermediates")
+      import :: c_int, c_char, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: prec
+      character(kind=c_char), intent(in) :: e1path(*), e2path(*)   ! NUL-terminated
+      real(c_double), intent(out) :: e1(*), e2(*)
+    end function
     integer(c_int) function ccgpu_save_amplitudes(ctx, path, intsize) bind(c, name="ccgpu_save_amplitudes")
       import :: c_int, c_char, c_ptr
       type(c_ptr), value :: ctx

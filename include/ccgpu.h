@@ -32,7 +32,9 @@ typedef struct ccgpu_ctx ccgpu_ctx;
 
 /* precision selectors */
 #define CCGPU_FP64 0  /* all contractions in FP64 on MFMA */
-#define CCGPU_MIXED 1 /* bulk contractions FP32 / BF16x3 on MFMA, FP64 update/DIIS/energy */
+#define CCGPU_MIXED 1 /* bulk contractions BF16x3 on MFMA (FP32 accumulation), FP64 update/DIIS/energy */
+#define CCGPU_FP32 2  /* bulk contractions FP32 on MFMA (v_mfma_f32_16x16x4f32), FP64 update/DIIS/energy;
+                         the CCSD iterations, the left transformation and the multipliers, not (T) */
 
 int ccgpu_version(void);
 const char* ccgpu_last_error(void);
@@ -130,6 +132,19 @@ int ccgpu_rsp_write(const char* path, const char* model, int intsize, long nalla
 int ccgpu_rsp_read(const char* path, long nallai, long nt1, long nt2, double* vec0, double* vec1, double* vec2,
                    char* model, int* nwrite);
 
+/* The E intermediates that CCRHSN writes for the response code when RSPIM is
+ * set (cc/ccsd_rhs.F:1898-1915), at the context's current amplitudes (one
+ * vector-function evaluation in the given precision; the amplitudes and the
+ * iteration state are not changed): E1(b,c) (CCRHS_EFCK, Fock included,
+ * NMATAB = v*v values, b fastest) and E2(k,j) (NMATIJ = o*o values, k
+ * fastest), as CCRHS_E contracts them (cc/ccsd_rhs.F:2280).  Each non-NULL
+ * path receives a Fortran unformatted sequential file of one record, the
+ * CC_E1IM / CC_E2IM files that CC_LHTR, CC_RHTR and CC_BMAT read
+ * (cc/cc_lhtr.F:1678, cc/cc_rhtr.F:1553, cc/cc_bmat.F:4803); e1 and e2 (may
+ * be NULL) receive the same values. */
+int ccgpu_write_e_intermediates(ccgpu_ctx* ctx, int precision, const char* e1_path, const char* e2_path,
+                                double* e1, double* e2);
+
 /* Writes the context's amplitudes as the 'R0' file of a CCSD run
  * (CC_WRRSP('R0', ...) at the end of CCSD_ENERGY; orbital part zero). */
 int ccgpu_save_amplitudes(ccgpu_ctx* ctx, const char* path, int intsize);
@@ -201,6 +216,24 @@ int ccgpu_onedens(ccgpu_ctx* ctx, int precision, const double* l1, const double*
  * are made (returns 4 if the threshold is not reached within them). */
 int ccgpu_df_factor_ao(int nbas, int nmo, const double* cmo, const double* eri_ao, double thr, long maxvec,
                        double* B, long* naux);
+
+/* The factor of ccgpu_df_factor_ao with the AO integrals supplied column by
+ * column instead of as the whole supermatrix, as the reference's
+ * CC_CHOLESKY computes the integral columns of a set of qualified diagonals
+ * at a time (choles/cc_cholesky.F:2) from its AO integral routines.  diag
+ * holds the nbas(nbas+1)/2 pair diagonals (ab|ab) (pairs ordered as for
+ * ccgpu_df_factor_ao).  Per panel the library picks the (at most 64) largest
+ * remaining diagonals that are at least max(thr, 0.01 x the largest) and
+ * calls columns(user, ncol, pairs, cols), which writes the integral columns
+ * cols[j*M + (cd)] = (cd|ab) for every pair (cd) < M = nbas(nbas+1)/2 and
+ * (ab) = pairs[j], and returns 0 (nonzero aborts the call with code 1).  The
+ * panel's pivots are chosen among those columns; the decomposition stops
+ * when every remaining diagonal is below thr (returns 4 if maxvec vectors do
+ * not reach it).  Device memory is O(maxvec x M): no M x M array is formed.
+ * B and naux as for ccgpu_df_factor_ao. */
+typedef int (*ccgpu_ao_columns_fn)(void* user, int ncol, const long* pairs, double* cols);
+int ccgpu_df_factor_ao_cols(int nbas, int nmo, const double* cmo, const double* diag, ccgpu_ao_columns_fn columns,
+                            void* user, double thr, long maxvec, double* B, long* naux);
 
 /* ccgpu_init from AO integrals: the factor of ccgpu_df_factor_ao (no vector
  * limit) with nmo = nocc + nvir, then the context as ccgpu_init; *naux

@@ -169,9 +169,7 @@ def residual(sys: DFSystem, t1: np.ndarray, t2: np.ndarray, ladder_chunk: int = 
     D -= np.einsum("Pac,Pki->aikc", Bvv, Boo, optimize=True)
     D += 0.5 * np.einsum("aidl,ldkc->aikc", u, Lov, optimize=True)
     X2 += 0.5 * np.einsum("bjck,aikc->aibj", u, D, optimize=True)
-    # E2 (CCRHS_EI/EFCK/E): E1[b,c] = F~_bc - sum u_kl^bd (ld|kc), E2[k,j] = F~_kj + sum u_lj^cd (kd|lc)
-    E1 = Fvv - np.einsum("bkdl,ldkc->bc", u, gov, optimize=True)
-    E2 = Foo + np.einsum("cldj,kdlc->kj", u, gov, optimize=True)
+    E1, E2 = _e_intermediates(Fvv, Foo, u, gov)
     X2 += np.einsum("aicj,bc->aibj", t, E1, optimize=True)
     X2 -= np.einsum("aibk,kj->aibj", t, E2, optimize=True)
     om2 += X2 + X2.transpose(2, 3, 0, 1)  # P_ij^ab
@@ -179,6 +177,27 @@ def residual(sys: DFSystem, t1: np.ndarray, t2: np.ndarray, ladder_chunk: int = 
 
 
 # ------------------------------------------------------------- iteration pieces
+
+
+def _e_intermediates(Fvv, Foo, u, gov):
+    """E1[b,c] = F~_bc - sum u_kl^bd (ld|kc), E2[k,j] = F~_kj + sum u_lj^cd (kd|lc):
+    the two-electron parts of CCRHS_EI1 (cc/ccsd_rhs.F:3215, the delta index
+    back-transformed with XLAMDH, negated, in CCRHS_EFCK1 :8686-8699) plus the
+    T1-dressed Fock blocks added there (:8705-8720)."""
+    E1 = Fvv - np.einsum("bkdl,ldkc->bc", u, gov, optimize=True)
+    E2 = Foo + np.einsum("cldj,kdlc->kj", u, gov, optimize=True)
+    return E1, E2
+
+
+def e_intermediates(sys: DFSystem, t1: np.ndarray, t2: np.ndarray):
+    """The E intermediates CCRHSN writes with RSPIM (CC_E1IM, CC_E2IM,
+    cc/ccsd_rhs.F:1898-1915) at amplitudes t1 [a,i], t2 [a,i,b,j]: E1 [b,c]
+    (v x v) and E2 [k,j] (o x o), as the residual uses them."""
+    o = sys.o
+    _, F = dress(sys, t1)
+    u = 2.0 * t2 - t2.transpose(0, 3, 2, 1)
+    gov = np.einsum("Pkc,Pld->kcld", sys.Bov, sys.Bov, optimize=True)
+    return _e_intermediates(F[o:, o:], F[:o, :o], u, gov)
 
 
 def mp2_guess(sys: DFSystem):

@@ -12,7 +12,7 @@ program dryrun_driver
   real(c_double) :: eps(n), b(naux * n * n), t1(nt1), t2(nt2), om1(nt1), om2(nt2)
   real(c_double) :: c1(2 * nt1), c2(2 * nt2), r1(2 * nt1), r2(2 * nt2)
   real(c_double) :: s(nt1 + nt2), t(nt1 + nt2), l1(nt1), l2(nt2), dens(n * n), hist(4)
-  real(c_double) :: e4, e5, emp2, ecorr, omnorm, eccsd, rn
+  real(c_double) :: e4, e5, emp2, ecorr, omnorm, eccsd, rn, e1(v * v), e2(o * o), x(v * v + 1)
   integer(c_int) :: niter
   character(kind=c_char) :: fname(16)
   character(len=10) :: name
@@ -41,6 +41,11 @@ program dryrun_driver
   call ccgpu_check(ccgpu_eta(ccgpu_ctx, l1, l2), 'CC_ETA')
   call ccgpu_check(ccgpu_lambda_solve(ccgpu_ctx, CCGPU_FP64, 1.0d-8, 20, 8, l1, l2, niter, rn), 'CCEQ_SOL')
   call ccgpu_check(ccgpu_onedens(ccgpu_ctx, CCGPU_FP64, l1, l2, dens), 'CC_D1AO')
+  ! RSPIM E intermediates: one record each, read back with plain Fortran I/O
+  call ccgpu_check(ccgpu_write_e_intermediates(ccgpu_ctx, CCGPU_FP64, 'CC_E1IM' // c_null_char, &
+                                               'CC_E2IM' // c_null_char, e1, e2), 'RSPIM')
+  call read_one('CC_E1IM', v * v)
+  call read_one('CC_E2IM', o * o)
   call ccgpu_ccsd_pt(e4, e5)
   call ccgpu_check(ccgpu_rsp_filename('R0' // c_null_char, 1, 0, fname), 'CC_RWPRE')
   name = ' '
@@ -51,6 +56,21 @@ program dryrun_driver
   if (name /= 'CCR0_1___0') call quit('file name '//name)
   call ccgpu_check(ccgpu_finalize(ccgpu_ctx), 'ccgpu_finalize')
   write (*, '(a)') 'dryrun driver ok'
+contains
+  ! The file holds exactly one record of n doubles (as CC_LHTR reads it,
+  ! cc/cc_lhtr.F:1678): reading n succeeds, reading n + 1 from it fails.
+  subroutine read_one(path, n)
+    character(len=*), intent(in) :: path
+    integer, intent(in) :: n
+    integer :: ios
+    open (unit=31, file=path, form='unformatted', access='sequential', status='old')
+    read (31, iostat=ios) (x(i), i = 1, n)
+    if (ios /= 0) call quit('cannot read '//path)
+    rewind (31)
+    read (31, iostat=ios) (x(i), i = 1, n + 1)
+    if (ios == 0) call quit('record of '//path//' is longer than expected')
+    close (31, status='delete')
+  end subroutine read_one
 end program dryrun_driver
 
 subroutine quit(text)

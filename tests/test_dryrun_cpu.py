@@ -45,10 +45,12 @@ for (o, v, naux) in {shapes!r}:
     assert rc == 0, L.ccgpu_last_error()
     rc = L.ccgpu_ccsd_iterate(h, 1, C.byref(e), C.byref(n))  # mixed-precision iteration
     assert rc == 0, L.ccgpu_last_error()
+    rc = L.ccgpu_ccsd_iterate(h, 2, C.byref(e), C.byref(n))  # FP32-MFMA iteration
+    assert rc == 0, L.ccgpu_last_error()
     rc = L.ccgpu_pt_partial(h, 1, 0, 2, 1, C.byref(e), C.byref(n), C.byref(fl), C.byref(nt))
     assert rc == 0, L.ccgpu_last_error()
     d = np.zeros(1)  # host arrays are never touched in a dry run
-    for prec, nvec in ((0, 2), (1, 1), (0, 1)):  # left transformation, both precisions
+    for prec, nvec in ((0, 2), (1, 1), (2, 1), (0, 1)):  # left transformation, every precision
         rc = L.ccgpu_lhtr(h, prec, nvec, _lib.ptr(d), _lib.ptr(d), _lib.ptr(d), _lib.ptr(d))
         assert rc == 0, L.ccgpu_last_error()
     # host-loop pieces: CCSD_ECCSD and CCSD_DIIS on host vectors (slot 1 and a wrapped slot)
@@ -58,7 +60,8 @@ for (o, v, naux) in {shapes!r}:
     # multiplier equations: eta, the reduced-space solve (both precisions), density
     assert L.ccgpu_eta(h, _lib.ptr(d), _lib.ptr(d)) == 0, L.ccgpu_last_error()
     it = C.c_int(); rn = C.c_double()
-    for prec in (0, 1):
+    assert L.ccgpu_pt_partial(h, 2, 0, 1, 1, C.byref(e), C.byref(n), C.byref(fl), C.byref(nt)) == 1  # no FP32 (T)
+    for prec in (0, 1, 2):
         rc = L.ccgpu_lambda_solve(h, prec, 1e-8, 5, 4, _lib.ptr(d), _lib.ptr(d), C.byref(it), C.byref(rn))
         assert rc == 0, L.ccgpu_last_error()
     assert L.ccgpu_onedens(h, 0, _lib.ptr(d), _lib.ptr(d), _lib.ptr(d)) == 0, L.ccgpu_last_error()
@@ -143,5 +146,36 @@ def test_dryrun_ao_front_end():
     env = dict(os.environ, CCGPU_DRYRUN="1")
     r = subprocess.run([sys.executable, "-c", AO_SCRIPT.format(root=ROOT)], env=env, capture_output=True, text=True,
                        timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert r.stdout.count("ok") == 3
+
+
+AO_COLS_SCRIPT = r"""
+import sys
+import numpy as np
+sys.path.insert(0, {root!r})
+import dalton_amd as d
+for nbas, nmo in ((24, 24), (7, 5), (60, 50)):
+    M = nbas * (nbas + 1) // 2
+    rng = np.random.default_rng(nbas)
+    X = rng.standard_normal((3 * nbas, M))
+    diag = np.einsum("km,km->m", X, X)
+    asked = []
+    def columns(pairs):
+        asked.append(pairs.copy())
+        assert pairs.min() >= 0 and pairs.max() < M
+        return (X[:, pairs].T @ X)
+    B = d.df_factor_ao_cols(np.eye(nbas)[:, :nmo].T.copy(), diag, columns, nbas, 1e-10, maxvec=200)
+    assert len(asked) >= 1 and all(len(a) <= 64 for a in asked)
+    print("ok", nbas, B.shape, flush=True)
+"""
+
+
+def test_dryrun_ao_column_front_end():
+    """Spans of the column-streamed factor (one qualified panel in a dry run)
+    and the callback's arguments (pair indices in range, at most 64)."""
+    env = dict(os.environ, CCGPU_DRYRUN="1")
+    r = subprocess.run([sys.executable, "-c", AO_COLS_SCRIPT.format(root=ROOT)], env=env, capture_output=True,
+                       text=True, timeout=300)
     assert r.returncode == 0, r.stdout + r.stderr
     assert r.stdout.count("ok") == 3

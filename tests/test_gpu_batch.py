@@ -72,3 +72,58 @@ def test_single_context_unchanged_after_autotuning():
         b.energy(maxit=80, threnr=1e-10)
     again = _individual(systems[:1])
     assert abs(ref[0][0]["e_ccsd"] - again[0][0]["e_ccsd"]) < 1e-11
+
+
+def test_small_batch_matches_oracle_solves():
+    """Six uneven systems (o 3..5, v = 8..10 o, naux = 4v) converged together
+    in one graph replay per round, against the oracle's CCSD_ENERGY loop
+    (oracle.ccsd.solve: same iteration count, energies to 1e-10 Eh), so the
+    batch path is pinned to the oracle and not only to single HIP solves."""
+    import dalton_amd as d
+    from dalton_amd.synthetic import batch_shapes, make_batch
+    from oracle import ccsd as occ
+
+    shapes = batch_shapes(6, 77, omin=3, omax=5, kmin=8, kmax=10)
+    systems = make_batch(shapes, 77)
+    print(f"small batch vs oracle: shapes {shapes}", flush=True)
+    with d.CCBatch(systems) as b:
+        res = b.energy(maxit=80, threnr=1e-10, graph=True)
+    for k, (eps, B, o) in enumerate(systems):
+        ref = occ.solve(occ.DFSystem(eps, B, o), maxiter=80, threnr=1e-10)
+        print(f"  system {k}: oracle {len(ref['history'])} it E = {ref['e_ccsd']:.12f}, "
+              f"batch {res['niter'][k]} it E = {res['e_ccsd'][k]:.12f}", flush=True)
+        assert res["niter"][k] == len(ref["history"])
+        assert abs(res["e_mp2"][k] - ref["e_mp2"]) < 1e-12
+        assert abs(res["e_ccsd"][k] - ref["e_ccsd"]) < 1e-10
+
+
+def test_c5b_batch_as_configured_matches_individual_fp64_solves():
+    """C5b at its configured shape: 32 systems with o ~ U{5..30},
+    v = 8..12 o and naux = 4v (batch_shapes(32, seed 5), the convergent
+    integral scale batch_sigma), converged together in FP64 with one HIP
+    graph replay per round, against each system solved on its own:
+    the same iteration count and the energy to 1e-10 Eh, per system."""
+    import time
+
+    import dalton_amd as d
+    from dalton_amd.synthetic import batch_shapes, make_batch
+
+    shapes = batch_shapes(32, 5)
+    t0 = time.perf_counter()
+    systems = make_batch(shapes, 5)
+    print(f"C5b: 32 systems generated in {time.perf_counter() - t0:.1f} s: {shapes}", flush=True)
+    ind = []
+    for k, (eps, B, o) in enumerate(systems):
+        with d.CCContext(eps, B, o) as c:
+            r = c.ccsd_energy(maxit=100, threnr=1e-8)
+        ind.append((r["e_ccsd"], r["niter"]))
+        print(f"  individual {k} o={o} v={eps.size - o}: {r['niter']} it, E = {r['e_ccsd']:.12f}", flush=True)
+    t0 = time.perf_counter()
+    with d.CCBatch(systems) as b:
+        res = b.energy(maxit=100, threnr=1e-8, graph=True, tune=False)
+        st = b.stats()
+    print(f"  batch: {time.perf_counter() - t0:.1f} s, {st}", flush=True)
+    for k, (e, n) in enumerate(ind):
+        assert res["niter"][k] == n, (k, res["niter"][k], n)
+        assert abs(res["e_ccsd"][k] - e) < 1e-10, (k, res["e_ccsd"][k], e)
+    assert st["replays"] >= 1

@@ -31,3 +31,51 @@ def test_dgemm_matches_numpy(ta, tb, m, n, k):
     got = d.dgemm(ta, tb, 1.5, A, B, -0.5, C0)
     scale = max(1.0, np.sqrt(k))
     assert np.abs(got - ref).max() < 1e-13 * scale * 10
+
+
+MIXED_CASES = [("N", "T", 512, 768, 320), ("T", "N", 300, 520, 1000), ("N", "N", 1024, 256, 64)]
+
+_VARIANT_SCRIPT = r"""
+import sys
+import numpy as np
+sys.path.insert(0, sys.argv[1])
+import dalton_amd as d
+for q, (ta, tb, m, n, k) in enumerate(%r):
+    rng = np.random.default_rng(100 + q)
+    A = rng.standard_normal((k, m) if ta == "T" else (m, k))
+    B = rng.standard_normal((n, k) if tb == "T" else (k, n))
+    C0 = rng.standard_normal((m, n))
+    np.save(sys.argv[2] + "_%%d.npy" %% q, d.dgemm(ta, tb, 0.75, A, B, -1.5, C0, precision=d.MIXED))
+print("variant script ok", flush=True)
+"""
+
+
+def test_bf16x3_large_tile_kernel_is_bit_identical(tmp_path):
+    """The eight-wave 256x256 BF16x3 kernel (CCGPU_BF16X3_VARIANT=1, read
+    once per process) against the default 128x128 kernel, each in a process
+    of its own, on products whose padded sizes are and are not multiples of
+    256: identical bits (same accumulation order per element), and within
+    the BF16x3 error of numpy FP64."""
+    import os
+    import subprocess
+    import sys
+
+    from conftest import ROOT
+
+    script = _VARIANT_SCRIPT % (MIXED_CASES,)
+    outs = {}
+    for var in ("0", "1"):
+        print(f"bf16x3 kernel variant {var}", flush=True)
+        env = dict(os.environ, CCGPU_BF16X3_VARIANT=var)
+        r = subprocess.run([sys.executable, "-c", script, ROOT, str(tmp_path / f"v{var}")], env=env,
+                           capture_output=True, text=True, timeout=300)
+        assert r.returncode == 0 and "variant script ok" in r.stdout, r.stdout + r.stderr
+        outs[var] = [np.load(str(tmp_path / f"v{var}_{q}.npy")) for q in range(len(MIXED_CASES))]
+    for q, (ta, tb, m, n, k) in enumerate(MIXED_CASES):
+        rng = np.random.default_rng(100 + q)
+        A = rng.standard_normal((k, m) if ta == "T" else (m, k))
+        B = rng.standard_normal((n, k) if tb == "T" else (k, n))
+        C0 = rng.standard_normal((m, n))
+        ref = 0.75 * (A.T if ta == "T" else A) @ (B.T if tb == "T" else B) - 1.5 * C0
+        assert np.array_equal(outs["0"][q], outs["1"][q]), (q, np.abs(outs["0"][q] - outs["1"][q]).max())
+        assert np.abs(outs["1"][q] - ref).max() < 1e-5 * max(1.0, np.abs(ref).max())

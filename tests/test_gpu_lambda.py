@@ -190,3 +190,37 @@ def test_multipliers_at_degenerate_shapes_match_oracle(o, v, naux):
     assert np.abs(l1 - m1).max() < 1e-9 and np.abs(l2 - m2).max() < 1e-9
     Q = ole.density(s, t1, t2, m1, m2)
     assert np.abs(D - Q).max() < 1e-9 and abs(np.trace(D) - 2 * o) < 1e-10
+
+
+def test_dalton_side_l0_solve_through_lhtr_and_eta(golden, tmp_path):
+    """The route on which Dalton's own CCEQ_SOL replaces CC_LHTR and CC_ETA by
+    the library (so no RSPIM side file is read): the left Jacobian is accessed
+    only through ccgpu_lhtr on unit trial vectors (NSIMTR of them in one call,
+    Dalton's left-vector convention) and eta through ccgpu_eta; the L0
+    equations lambda_D A_D + eta_D = 0 solved densely on the host give
+    ccgpu_lambda_solve's multipliers, which go through Dalton's 'L0' file
+    (CC_WRRSP records) and the density to the reference's unrelaxed CCSD
+    dipole of water/STO-3G."""
+    import dalton_amd as d
+
+    g = golden["h2o_sto3g_exgr"]
+    f = np.load(os.path.join(GOLDEN, "h2o_sto3g_exgr.npz"))
+    eps, B, o, _ = load_molecule("h2o_sto3g_exgr")
+    print("Dalton-side L0 solve through ccgpu_lhtr/ccgpu_eta h2o/STO-3G", flush=True)
+    with _converged(d, eps, B, o) as c:
+        n1, n2 = c.nt1, c.nt2
+        n = n1 + n2
+        I = np.eye(n)
+        R1, R2 = c.lhtr(I[:, :n1], I[:, n1:])  # row k: e_k A_D
+        A = np.hstack([R1, R2])
+        e1, e2 = c.eta()
+        lam = np.linalg.solve(A.T, -np.concatenate([e1, e2]))
+        l1, l2, _, rn = c.lambda_solve(thr=1e-11)
+        path = str(tmp_path / d.rsp_filename("L0"))
+        d.rsp_write(path, lam[:n1], lam[n1:])
+        r = d.rsp_read(path, n1, n2)
+        D = c.onedens(r["vec1"], r["vec2"])
+    assert rn < 1e-11
+    assert np.abs(lam[:n1] - l1).max() < 1e-9 and np.abs(lam[n1:] - l2).max() < 1e-9
+    mu = -np.einsum("pq,dpq->d", D, f["dip"])
+    assert np.abs(mu - np.array(g["dipole_el_ccsd"])).max() < 1e-8, mu

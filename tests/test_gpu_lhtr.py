@@ -105,14 +105,15 @@ def test_lhtr_matches_finite_differences(name):
     assert worst < 1e-7
 
 
-def _c4_check(sigma, label, strict):
+def _c4_check(sigma, label, strict, ndir=16):
     """o=32, v=256, naux=1024 at the MP2 amplitudes (plus small singles): 16
-    unit trial vectors in one call, each checked along its own random unit
-    direction by central differences of lambda . Omega with h = 1e-5; the
-    batched call equals one-vector calls.  The particle ladder runs in several
-    a-batches at this size.  The FD round-off floor is estimated from the
-    spread of the differences at h, 2h and h/2: an adjoint error would show in
-    all of them alike."""
+    unit trial vectors in one call, checked along ndir random unit directions
+    (direction d with vector d mod 16) by central differences of
+    lambda . Omega with h = 1e-5; the batched call equals one-vector calls.
+    The particle ladder runs in several a-batches at this size.  The FD
+    round-off floor is estimated from the spread of the differences at h, 2h
+    and h/2: an adjoint error would show in all of them alike.  The FP32-MFMA
+    transformation of the same vectors is compared with the FP64 one."""
     import dalton_amd as d
     from dalton_amd.synthetic import make_system
 
@@ -134,9 +135,14 @@ def _c4_check(sigma, label, strict):
         R1, R2 = c.lhtr(L1, L2, convention="natural")
         s1, s2 = c.lhtr(L1[5], L2[5], convention="natural")
         assert np.abs(s1 - R1[5]).max() == 0.0 and np.abs(s2 - R2[5]).max() == 0.0
+        F1, F2 = c.lhtr(L1, L2, d.FP32, convention="natural")
+        dev32 = max(np.abs(F1 - R1).max(), np.abs(F2 - R2).max()) / max(np.abs(R1).max(), np.abs(R2).max())
+        print(f"  FP32-MFMA vs FP64 left transformation: max rel deviation {dev32:.2e}", flush=True)
+        assert dev32 < 1e-4
         h = 1e-5
         worst = 0.0
-        for k in range(nv):
+        for dd in range(ndir):
+            k = dd % nv
             e1 = rng.standard_normal(c.nt1)
             e2 = rng.standard_normal(c.nt2)
             en = np.sqrt(np.dot(e1, e1) + np.dot(e2, e2))
@@ -156,7 +162,8 @@ def _c4_check(sigma, label, strict):
             else:  # FD noise floor from the spread of three step sizes
                 f2, f3 = fd(2 * h), fd(0.5 * h)
                 bound = max(1e-7, 4.0 * max(abs(f1 - f2), abs(f1 - f3)) / scale)
-            print(f"  vector {k}: rho.d = {an:.12e} FD = {f1:.12e} rel = {rel:.2e} bound = {bound:.1e}", flush=True)
+            print(f"  direction {dd} (vector {k}): rho.d = {an:.12e} FD = {f1:.12e} rel = {rel:.2e} "
+                  f"bound = {bound:.1e}", flush=True)
             assert rel < bound, (k, rel, bound)
             worst = max(worst, rel)
         print(f"  max rel |FD - rho.d| = {worst:.2e}", flush=True)
@@ -173,7 +180,7 @@ def test_lhtr_c4_shape_convergent_scale_strict():
     round-off floor is low there, so 1e-7 relative is required outright."""
     from dalton_amd.synthetic import batch_sigma
 
-    _c4_check(batch_sigma(256, 1024), "convergent scale", strict=True)
+    _c4_check(batch_sigma(256, 1024), "convergent scale, 32 directions", strict=True, ndir=32)
 
 
 @pytest.mark.parametrize("budget_mb", ["2", "0.05"])

@@ -143,3 +143,35 @@ def test_blas_residual_matches_einsum_oracle():
         a1, a2 = ccsd.residual(s, t1, t2)
         b1, b2 = ccsd_blas.residual(s, t1, t2, ladder_rows=rows)
         assert np.abs(a1 - b1).max() < 1e-13 and np.abs(a2 - b2).max() < 1e-13
+
+
+def test_e_intermediates_are_what_the_residual_contracts():
+    """oracle.ccsd.e_intermediates (the CC_E1IM / CC_E2IM content): at t2 = 0
+    the T1-dressed Fock blocks (CCRHS_EFCK1 adds them, cc/ccsd_rhs.F:8705-8720;
+    the two-electron parts vanish with u = 0), and in general the matrices
+    whose E term the residual contains: Omega2 changes by
+    P(t E1^T - t E2) when E1 and E2 are switched off (a transposed E1 or a
+    sign slip in E2 fails here)."""
+    from dalton_amd.synthetic import make_system
+
+    o, v = 3, 6
+    eps, B = make_system(o, v, 11, 5)
+    s = ccsd.DFSystem(eps, B, o)
+    rng = np.random.default_rng(5)
+    t1 = 0.05 * rng.standard_normal((v, o))
+    t2 = 0.05 * rng.standard_normal((v, o, v, o))
+    t2 = t2 + t2.transpose(2, 3, 0, 1)
+    _, F = ccsd.dress(s, t1)
+    E1, E2 = ccsd.e_intermediates(s, t1, np.zeros_like(t2))
+    assert np.abs(E1 - F[o:, o:]).max() < 1e-14 and np.abs(E2 - F[:o, :o]).max() < 1e-14
+    assert np.abs(F[o:, o:] - F[o:, o:].T).max() > 1e-4  # the dressed block is not symmetric
+    E1, E2 = ccsd.e_intermediates(s, t1, t2)
+    _, full = ccsd.residual(s, t1, t2)
+    orig = ccsd._e_intermediates
+    try:
+        ccsd._e_intermediates = lambda Fvv, Foo, u, gov: (np.zeros_like(Fvv), np.zeros_like(Foo))
+        _, noe = ccsd.residual(s, t1, t2)
+    finally:
+        ccsd._e_intermediates = orig
+    X = np.einsum("aicj,bc->aibj", t2, E1) - np.einsum("aibk,kj->aibj", t2, E2)
+    assert np.abs(full - noe - (X + X.transpose(2, 3, 0, 1))).max() < 1e-13
