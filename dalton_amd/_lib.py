@@ -38,6 +38,7 @@ SIGNATURES = {
                                  C.POINTER(C.c_int)]),
     "ccgpu_save_amplitudes": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int]),
     "ccgpu_write_e_intermediates": (C.c_int, [C.c_void_p, C.c_int, C.c_char_p, C.c_char_p, _dp, _dp]),
+    "ccgpu_write_gamma_intermediate": (C.c_int, [C.c_void_p, C.c_int, C.c_char_p, _dp]),
     "ccgpu_restart_amplitudes": (C.c_int, [C.c_void_p, C.c_char_p, _dp]),
     "ccgpu_lambda_solve": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_int, C.c_int, _dp, _dp,
                                      C.POINTER(C.c_int), _dp]),

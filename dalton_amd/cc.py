@@ -280,6 +280,16 @@ class CCContext:
         # Dalton's order is column-major (the first index fastest)
         return e1.reshape(self.v, self.v).T.copy(), e2.reshape(self.o, self.o).T.copy()
 
+    def write_gamma_intermediate(self, path: str | None = None, precision: int = FP64):
+        """CCRHSN's RSPIM Gamma intermediate at the current amplitudes: writes
+        the CC_GAMIM file (one Fortran record) if a path is given and returns
+        the packed NGAMMA vector (element (ki, lj), ki <= lj, at
+        lj (lj + 1) / 2 + ki with ki = k + o i)."""
+        n = self.o * self.o
+        g = np.empty(n * (n + 1) // 2)
+        check(lib().ccgpu_write_gamma_intermediate(self._h, precision, path.encode() if path else None, ptr(g)))
+        return g
+
     def save_amplitudes(self, path: str, intsize: int = 4):
         """Write the amplitudes as Dalton's 'R0' file (CCR0_1___0)."""
         check(lib().ccgpu_save_amplitudes(self._h, path.encode(), intsize))

@@ -318,6 +318,18 @@ int ccgpu_write_e_intermediates(ccgpu_ctx* ctx, int precision, const char* e1_pa
   });
 }
 
+int ccgpu_write_gamma_intermediate(ccgpu_ctx* ctx, int precision, const char* path, double* gamma) {
+  return guarded([&] {
+    CCGPU_REQUIRE(ctx, "ccgpu_write_gamma_intermediate: null context");
+    const long n = (long)ctx->cc->dims().o * ctx->cc->dims().o;
+    std::vector<double> g((size_t)(n * (n + 1) / 2));
+    ctx->cc->set_precision(prec_of(precision));
+    ctx->cc->gamma_host(g.data());
+    if (path) ccgpu::write_one_record(path, g.data(), (long)g.size());
+    if (gamma) std::memcpy(gamma, g.data(), g.size() * sizeof(double));
+  });
+}
+
 int ccgpu_save_amplitudes(ccgpu_ctx* ctx, const char* path, int intsize) {
   return guarded([&] {
     CCGPU_REQUIRE(ctx && path, "ccgpu_save_amplitudes: null argument");
@@ -440,27 +452,39 @@ int ccgpu_profile_stop(char* buf, int len) {
 }
 
 namespace {
+// Column-major BLAS operands (DGEMM's argument list) as a GemmDesc; the
+// element counts of the three arrays are returned in na, nb, nc.
+ccgpu::GemmDesc blas_desc(char transa, char transb, int m, int n, int k, double alpha, const double* a, int lda,
+                          const double* b, int ldb, double beta, double* c, int ldc, size_t& na, size_t& nb,
+                          size_t& nc) {
+  const bool ta = transa == 'T' || transa == 't' || transa == 'C' || transa == 'c';
+  const bool tb = transb == 'T' || transb == 't' || transb == 'C' || transb == 'c';
+  CCGPU_REQUIRE(m >= 0 && n >= 0 && k >= 0, "ccgpu_dgemm: negative dimension");
+  CCGPU_REQUIRE(lda >= std::max(1, ta ? k : m) && ldb >= std::max(1, tb ? n : k) && ldc >= std::max(1, m),
+                "ccgpu_dgemm: leading dimension too small");
+  na = (size_t)lda * (ta ? m : k);
+  nb = (size_t)ldb * (tb ? k : n);
+  nc = (size_t)ldc * n;
+  ccgpu::GemmDesc d;
+  d.M = m; d.N = n; d.K = k; d.alpha = alpha; d.beta = beta;
+  d.A = a; d.B = b; d.C = c;
+  if (!ta) { d.a_m = 1; d.a_k = lda; } else { d.a_m = lda; d.a_k = 1; }
+  if (!tb) { d.b_k = 1; d.b_n = ldb; } else { d.b_k = ldb; d.b_n = 1; }
+  d.c_m = 1; d.c_n = ldc;
+  return d;
+}
+
 int dgemm_impl(ccgpu::GemmPrec prec, char transa, char transb, int m, int n, int k, double alpha,
                const double* a, int lda, const double* b, int ldb, double beta, double* c, int ldc) {
   return guarded([&] {
-    const bool ta = transa == 'T' || transa == 't' || transa == 'C' || transa == 'c';
-    const bool tb = transb == 'T' || transb == 't' || transb == 'C' || transb == 'c';
-    CCGPU_REQUIRE(m >= 0 && n >= 0 && k >= 0, "ccgpu_dgemm: negative dimension");
-    CCGPU_REQUIRE(lda >= std::max(1, ta ? k : m) && ldb >= std::max(1, tb ? n : k) && ldc >= std::max(1, m),
-                  "ccgpu_dgemm: leading dimension too small");
+    size_t na = 0, nb = 0, nc = 0;
+    ccgpu::GemmDesc d = blas_desc(transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, na, nb, nc);
     if (m == 0 || n == 0) return;
-    const size_t na = (size_t)lda * (ta ? m : k), nb = (size_t)ldb * (tb ? k : n), nc = (size_t)ldc * n;
     ccgpu::DBuf<double> A(na), B(nb), C(nc);
     CCGPU_HIPX(hipMemcpy(A.p, a, na * sizeof(double), hipMemcpyHostToDevice));
     CCGPU_HIPX(hipMemcpy(B.p, b, nb * sizeof(double), hipMemcpyHostToDevice));
     CCGPU_HIPX(hipMemcpy(C.p, c, nc * sizeof(double), hipMemcpyHostToDevice));
-    ccgpu::GemmDesc d;
-    d.M = m; d.N = n; d.K = k; d.alpha = alpha; d.beta = beta;
     d.A = A.p; d.B = B.p; d.C = C.p;
-    if (!ta) { d.a_m = 1; d.a_k = lda; } else { d.a_m = lda; d.a_k = 1; }
-    if (!tb) { d.b_k = 1; d.b_n = ldb; } else { d.b_k = ldb; d.b_n = 1; }
-    d.c_m = 1; d.c_n = ldc;
-    if (k == 0) { d.K = 0; }
     ccgpu::gemm(d, prec, nullptr);
     CCGPU_HIPX(hipDeviceSynchronize());
     CCGPU_HIPX(hipMemcpy(c, C.p, nc * sizeof(double), hipMemcpyDeviceToHost));

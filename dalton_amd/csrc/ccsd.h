@@ -77,6 +77,9 @@ class CCSD {
   // order: e1[b + v*c] = E1(b,c) (NMATAB values), e2[k + o*j] = E2(k,j)
   // (NMATIJ values).  Runs one residual; the amplitudes are not changed.
   void e_intermediates_host(double* e1, double* e2);
+  // The hole-ladder matrix Gamma of CCRHS_A at the current amplitudes, in
+  // Dalton's packed NGAMMA order (CC_GAMIM); runs one residual.
+  void gamma_host(double* gamma);
 
   // (T) energy over the restricted triple list i >= j >= k (i = j = k
   // excluded), enumerated i-major; triples first, first+stride, ... (count

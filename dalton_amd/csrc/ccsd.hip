@@ -22,6 +22,17 @@ namespace {
 
 constexpr int kMaxVec = 80;  // Dalton MAXVEC, cc/ccsd_diis.F
 
+// Gamma [(k,l),(i,j)] (the residual's hole-ladder matrix) into Dalton's
+// packed NGAMMA order: pair ki = k + o i, element (ki, lj) with ki <= lj at
+// lj (lj + 1) / 2 + ki (cc/ccsd_rhs.F:5158-5167).  Thread x = ki, y = lj.
+__global__ void k_pack_gamma(double* __restrict__ out, const double* __restrict__ Gam, int o) {
+  const long n = (long)o * o;
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x, q = blockIdx.y;
+  if (p > q || q >= n) return;
+  const long k = p % o, i = p / o, l = q % o, j = q / o;
+  out[q * (q + 1) / 2 + p] = Gam[((k * o + l) * o + i) * o + j];
+}
+
 }  // namespace
 
 CCSD::CCSD(int o, int v, int naux, const double* eps_host, const double* B_host, int mxdiis)
@@ -534,6 +545,20 @@ void CCSD::e_intermediates_host(double* e1, double* e2) {
   if (e1) CCGPU_HIPX(hipMemcpyAsync(e1, bufR_.p, (size_t)v * v * sizeof(double), hipMemcpyDeviceToHost, st_));
   if (e2)
     CCGPU_HIPX(hipMemcpyAsync(e2, bufR_.p + (long)v * v, (size_t)o * o * sizeof(double), hipMemcpyDeviceToHost, st_));
+  CCGPU_HIPX(hipStreamSynchronize(st_));
+}
+
+void CCSD::gamma_host(double* gamma) {
+  CCGPU_REQUIRE(iter_ >= 1, "ccgpu: no amplitudes (run the CCSD iterations or set amplitudes first)");
+  const int o = dm_.o;
+  const long n = (long)o * o, ng = n * (n + 1) / 2;
+  residual(t1_.p, T_.p, om1_.p, Om_.p);  // Gam_ at the current amplitudes
+  DBuf<double> packed((size_t)ng);
+  check_span(Gam_.p, 0, n * n - 1, sizeof(double), "gamma source");
+  CCGPU_REQUIRE(n <= 65535, "gamma: occupied space too large");
+  CCGPU_LAUNCH(k_pack_gamma, dim3(cdiv(n, 256), (unsigned)n), dim3(256), 0, st_, packed.p, Gam_.p, o);
+  CCGPU_HIPX(hipGetLastError());
+  CCGPU_HIPX(hipMemcpyAsync(gamma, packed.p, (size_t)ng * sizeof(double), hipMemcpyDeviceToHost, st_));
   CCGPU_HIPX(hipStreamSynchronize(st_));
 }
 

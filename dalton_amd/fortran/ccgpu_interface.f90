@@ -158,6 +158,14 @@ module ccgpu_interface
       character(kind=c_char), intent(in) :: e1path(*), e2path(*)   ! NUL-terminated
       real(c_double), intent(out) :: e1(*), e2(*)
     end function
+    integer(c_int) function ccgpu_write_gamma_intermediate(ctx, prec, path, gamma) &
+        bind(c, name="ccgpu_write_gamma_intermediate")
+      import :: c_int, c_char, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: prec
+      character(kind=c_char), intent(in) :: path(*)   ! NUL-terminated
+      real(c_double), intent(out) :: gamma(*)         ! NGAMMA values
+    end function
     integer(c_int) function ccgpu_save_amplitudes(ctx, path, intsize) bind(c, name="ccgpu_save_amplitudes")
       import :: c_int, c_char, c_ptr
       type(c_ptr), value :: ctx

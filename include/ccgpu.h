@@ -145,6 +145,17 @@ int ccgpu_rsp_read(const char* path, long nallai, long nt1, long nt2, double* ve
 int ccgpu_write_e_intermediates(ccgpu_ctx* ctx, int precision, const char* e1_path, const char* e2_path,
                                 double* e1, double* e2);
 
+/* CCRHSN's RSPIM Gamma intermediate (CC_GAMIM, cc/ccsd_rhs.F:1603-1612),
+ * the hole-ladder matrix CCRHS_A contracts and CC_LHTR reads back for its
+ * LT22A term (cc/cc_lhtr.F:1732-1747): Gamma(ki,lj) = (ki~|lj~) +
+ * sum_cd t_ij^cd (kc|ld) at the context's current amplitudes (one residual
+ * evaluation; the amplitudes are not changed).  Packed as Dalton's NGAMMA =
+ * NMATIJ (NMATIJ + 1) / 2 vector: pair ki = k + o i (NMATIJ order), element
+ * (ki, lj) with ki <= lj at lj (lj + 1) / 2 + ki, 0-based
+ * (cc/ccsd_rhs.F:5158-5167).  Written as one Fortran record to path (if not
+ * NULL) and copied to gamma (if not NULL). */
+int ccgpu_write_gamma_intermediate(ccgpu_ctx* ctx, int precision, const char* path, double* gamma);
+
 /* Writes the context's amplitudes as the 'R0' file of a CCSD run
  * (CC_WRRSP('R0', ...) at the end of CCSD_ENERGY; orbital part zero). */
 int ccgpu_save_amplitudes(ccgpu_ctx* ctx, const char* path, int intsize);
@@ -329,6 +340,7 @@ int ccgpu_dgemm(char transa, char transb, int m, int n, int k, double alpha, con
  * variant of the DGEMM drop-in above). */
 int ccgpu_dgemm_mixed(char transa, char transb, int m, int n, int k, double alpha, const double* a,
                       int lda, const double* b, int ldb, double beta, double* c, int ldc);
+
 
 #ifdef __cplusplus
 }

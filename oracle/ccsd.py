@@ -155,8 +155,7 @@ def residual(sys: DFSystem, t1: np.ndarray, t2: np.ndarray, ladder_chunk: int = 
         Vl = np.einsum("Pac,Pbd->acbd", Bvv[:, a0:a1], Bvv, optimize=True)
         om2[a0:a1] += np.einsum("acbd,cidj->aibj", Vl, t, optimize=True)
     # B2: sum_kl t_kl^ab [(ki~|lj~) + sum_cd t_ij^cd (kc|ld)]  (CCRHS_A)
-    Gam = np.einsum("Pki,Plj->kilj", Boo, Boo, optimize=True)
-    Gam += np.einsum("cidj,kcld->kilj", t, gov, optimize=True)
+    Gam = _gamma(Boo, t, gov)
     om2 += np.einsum("akbl,kilj->aibj", t, Gam, optimize=True)
 
     # C2 (exchange ring, CCRHS_C/CIO): C[a,i,k,c] = (ki~|ac~) - 1/2 sum_dl t_li^ad (kd|lc)
@@ -187,6 +186,39 @@ def _e_intermediates(Fvv, Foo, u, gov):
     E1 = Fvv - np.einsum("bkdl,ldkc->bc", u, gov, optimize=True)
     E2 = Foo + np.einsum("cldj,kdlc->kj", u, gov, optimize=True)
     return E1, E2
+
+
+def _gamma(Boo, t, gov):
+    """Gamma[k,i,l,j] = (ki~|lj~) + sum_cd t_ij^cd (kc|ld): the integral part
+    accumulated per delta by CCRHS_GAM (cc/ccsd_rhs.F:4990-5175) plus the
+    ladder part CC_T2MO adds from the AO BF intermediate (:1583), the matrix
+    CCRHS_A contracts (:1807)."""
+    Gam = np.einsum("Pki,Plj->kilj", Boo, Boo, optimize=True)
+    Gam += np.einsum("cidj,kcld->kilj", t, gov, optimize=True)
+    return Gam
+
+
+def gamma_intermediate(sys: DFSystem, t1: np.ndarray, t2: np.ndarray):
+    """The Gamma intermediate CCRHSN writes with RSPIM (CC_GAMIM,
+    cc/ccsd_rhs.F:1603-1612) at amplitudes t1 [a,i], t2 [a,i,b,j]:
+    Gamma[k,i,l,j] as the residual uses it."""
+    o = sys.o
+    Bt, _ = dress(sys, t1)
+    gov = np.einsum("Pkc,Pld->kcld", sys.Bov, sys.Bov, optimize=True)
+    return _gamma(Bt[:, :o, :o], t2, gov)
+
+
+def pack_gamma(G: np.ndarray) -> np.ndarray:
+    """Gamma[k,i,l,j] -> Dalton's NGAMMA vector: pair ki = k + o*i (NMATIJ
+    order), element (ki, lj) with ki <= lj at lj*(lj+1)/2 + ki, 0-based
+    (GAMMA(NLJ*(NLJ-1)/2 + NKI), cc/ccsd_rhs.F:5158-5167)."""
+    o = G.shape[0]
+    n = o * o
+    M = G.transpose(1, 0, 3, 2).reshape(n, n)  # M[ki, lj] with ki = k + o*i
+    out = np.empty(n * (n + 1) // 2)
+    for q in range(n):
+        out[q * (q + 1) // 2: q * (q + 1) // 2 + q + 1] = M[: q + 1, q]
+    return out
 
 
 def e_intermediates(sys: DFSystem, t1: np.ndarray, t2: np.ndarray):

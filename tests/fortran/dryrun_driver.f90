@@ -12,7 +12,8 @@ program dryrun_driver
   real(c_double) :: eps(n), b(naux * n * n), t1(nt1), t2(nt2), om1(nt1), om2(nt2)
   real(c_double) :: c1(2 * nt1), c2(2 * nt2), r1(2 * nt1), r2(2 * nt2)
   real(c_double) :: s(nt1 + nt2), t(nt1 + nt2), l1(nt1), l2(nt2), dens(n * n), hist(4)
-  real(c_double) :: e4, e5, emp2, ecorr, omnorm, eccsd, rn, e1(v * v), e2(o * o), x(v * v + 1)
+  real(c_double) :: e4, e5, emp2, ecorr, omnorm, eccsd, rn, e1(v * v), e2(o * o), x(o**4 + v * v + 1)
+  real(c_double) :: gam((o * o) * (o * o + 1) / 2)
   integer(c_int) :: niter
   character(kind=c_char) :: fname(16)
   character(len=10) :: name
@@ -46,6 +47,9 @@ program dryrun_driver
                                                'CC_E2IM' // c_null_char, e1, e2), 'RSPIM')
   call read_one('CC_E1IM', v * v)
   call read_one('CC_E2IM', o * o)
+  ! RSPIM Gamma intermediate: NGAMMA = NMATIJ (NMATIJ + 1) / 2 values in one record
+  call ccgpu_check(ccgpu_write_gamma_intermediate(ccgpu_ctx, CCGPU_FP64, 'CC_GAMIM' // c_null_char, gam), 'RSPIM')
+  call read_one('CC_GAMIM', (o * o) * (o * o + 1) / 2)
   call ccgpu_ccsd_pt(e4, e5)
   call ccgpu_check(ccgpu_rsp_filename('R0' // c_null_char, 1, 0, fname), 'CC_RWPRE')
   name = ' '

@@ -175,3 +175,49 @@ def test_e_intermediates_are_what_the_residual_contracts():
         ccsd._e_intermediates = orig
     X = np.einsum("aicj,bc->aibj", t2, E1) - np.einsum("aibk,kj->aibj", t2, E2)
     assert np.abs(full - noe - (X + X.transpose(2, 3, 0, 1))).max() < 1e-13
+
+
+def test_gamma_intermediate_is_what_the_residual_contracts():
+    """oracle.ccsd.gamma_intermediate (the CC_GAMIM content) and its packing:
+    at t2 = 0 it is (ki~|lj~) with the hole index dressed by t1,
+    B~[P,k,i] = B[P,k,i] + sum_a B[P,k,a] t_ai (the occupied columns of
+    XLAMDH, cc/ccsd_rhs.F:5887); in general the matrix whose hole ladder the
+    residual contains (Omega2 changes by P sum_kl t_kl^ab Gamma_kilj when
+    Gamma is switched off); symmetric under (ki) <-> (lj); packed as
+    GAMMA(NLJ*(NLJ-1)/2 + NKI) with NKI = k + o(i-1) (cc/ccsd_rhs.F:5158-5167),
+    element by element on o = 2."""
+    from dalton_amd.synthetic import make_system
+
+    o, v = 3, 6
+    eps, B = make_system(o, v, 11, 5)
+    s = ccsd.DFSystem(eps, B, o)
+    rng = np.random.default_rng(6)
+    t1 = 0.05 * rng.standard_normal((v, o))
+    t2 = 0.05 * rng.standard_normal((v, o, v, o))
+    t2 = t2 + t2.transpose(2, 3, 0, 1)
+    Bd = B[:, :o, :o] + np.einsum("Pka,ai->Pki", B[:, :o, o:], t1)
+    G0 = ccsd.gamma_intermediate(s, t1, np.zeros_like(t2))
+    assert np.abs(G0 - np.einsum("Pki,Plj->kilj", Bd, Bd)).max() < 1e-14
+    G = ccsd.gamma_intermediate(s, t1, t2)
+    assert np.abs(G - G.transpose(2, 3, 0, 1)).max() < 1e-14
+    _, full = ccsd.residual(s, t1, t2)
+    orig = ccsd._gamma
+    try:
+        ccsd._gamma = lambda Boo, t, gov: np.zeros((o, o, o, o))
+        _, nog = ccsd.residual(s, t1, t2)
+    finally:
+        ccsd._gamma = orig
+    X = np.einsum("akbl,kilj->aibj", t2, G)
+    assert np.abs(full - nog - X).max() < 1e-13
+    # packing on o = 2: pairs ki = k + 2i are (0,0) (1,0) (0,1) (1,1)
+    g2 = np.arange(16.0).reshape(2, 2, 2, 2)
+    g2 = g2 + g2.transpose(2, 3, 0, 1)
+    p = ccsd.pack_gamma(g2)
+    pairs = [(0, 0), (1, 0), (0, 1), (1, 1)]
+    want = []
+    for nlj in range(4):  # Fortran NLJ, then NKI = 1..NLJ
+        for nki in range(nlj + 1):
+            (k, i), (l, j) = pairs[nki], pairs[nlj]
+            want.append(g2[k, i, l, j])
+    assert p.size == 10 and np.array_equal(p, np.array(want))
+    assert ccsd.pack_gamma(G).size == (o * o) * (o * o + 1) // 2
