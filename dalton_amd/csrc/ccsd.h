@@ -135,9 +135,8 @@ class CCSD {
   // (T) drivers: per-triple launches of the pre-packed three-pass W build and
   // blocked energy (default); the same as one persistent kernel; the first
   // per-triple driver (twelve GEMMs, per-triple packing, transposed W copies)
-  enum class PtPath { Launch = 0, Persistent = 1, Legacy = 2 };
+  enum class PtPath { Launch = 0, Persistent = 1 };
   void pt_prepare(GemmPrec mode, PtPath path);
-  void pt_triples_mixed(long first, long n, long stride);
   void pt_energy_persistent(long first, long n, long stride, double* e4, double* e5);
   void pt_energy_launches(long first, long n, long stride, double* e4, double* e5);
   void pt_descs(int i, int j, int k, int slot, unsigned char* out);
@@ -184,11 +183,11 @@ class CCSD {
   double ladder_budget_ = 10.0e9;
   // (T)
   GemmPrec pt_mode_ = GemmPrec::F64;
-  DBuf<double> G_, Gp_, Tvvo_, Q_, Wt_, Wr_, ptpart_, etrip_;
+  DBuf<double> G_, Gp_, Tvvo_, Q_, Wt_, ptpart_, etrip_;
   // (T), mixed path: (yd|zk) packed as bf16 hi/lo [k][(y,z)][d] in both (y,z)
   // orders, per-triple packed slices, FP32 W
-  DBuf<unsigned short> GTh_, GTl_, GpTh_, GpTl_, Psm_;
-  DBuf<float> W32_, W32r_;  // W and its two re-ordered copies (energy reads)
+  DBuf<unsigned short> GTh_, GTl_, GpTh_, GpTl_;
+  DBuf<float> W32_;  // four W slots (FP32)
   // (T), persistent kernel: bf16 hi/lo of every T(p,q) [x][d] and Q_rq [z][l]
   // slice and of -Tvvo_p in both (x,y) orders (mixed), Tvvo with (x,y)
   // swapped (FP64); four W slots (in W32_ / Wt_); launch tables

@@ -109,7 +109,8 @@ __global__ __launch_bounds__(128 * RW, 4 / RW) void k_gemm_bf16x3(BMulti p) {
 }
 
 
-// Large-tile kernel (CCGPU_BF16X3_VARIANT=1): block tile 256x256 run by
+// Large-tile kernel (the default; CCGPU_BF16X3_VARIANT=0 selects the
+// 128x128 kernel above): block tile 256x256 run by
 // eight waves (two per SIMD), each a 128x64 sub-tile of 8x4 MFMA tiles, K
 // step 32.  Per K step a wave reads 24 KiB of operand fragments from LDS for
 // 96 MFMAs, a quarter less than the 64x64-per-wave 128x128 kernel, and the
@@ -237,10 +238,14 @@ __global__ __launch_bounds__(512, 1) void k_gemm_bf16x3_w8(BMulti p) {
       }
 }
 
+// Block-tile kernel: 1 (default) the eight-wave 256x256 direct-to-LDS
+// kernel, 0 the four-wave 128x128 one (CCGPU_BF16X3_VARIANT=0).  At o=64
+// v=512 the 256x256 kernel runs the mixed CCSD iteration in 2.05 s against
+// 2.30 s (361 against 307 TFLOP/s per launch), bit for bit the same.
 int variant() {
   static const int v = [] {
     const char* e = std::getenv("CCGPU_BF16X3_VARIANT");
-    return e ? std::atoi(e) : 0;
+    return (e && *e) ? std::atoi(e) : 1;
   }();
   return v;
 }
@@ -256,9 +261,10 @@ int group_m() {
 
 template <typename OutT>
 void launch_multi(BMulti p, int batch, hipStream_t s) {
-  // Block tile: 128x128, four waves of 64x64, two blocks per CU (default);
-  // CCGPU_BF16X3_VARIANT=1 runs products whose padded sizes are multiples
-  // of 256 on the eight-wave 256x256 kernel.
+  // Block tile: 256x256, eight waves of 128x64, one block per CU (default;
+  // products whose padded sizes are multiples of 256, which bf16x3_rpad
+  // gives); CCGPU_BF16X3_VARIANT=0: 128x128, four waves of 64x64, two blocks
+  // per CU.
   p.gm = group_m();
   CCGPU_REQUIRE(batch < 65536, "bf16x3 gemm: grid too large");
   if (variant() == 1 && p.Mp % DT == 0 && p.Np % DT == 0) {

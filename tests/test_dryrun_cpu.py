@@ -72,7 +72,7 @@ for (o, v, naux) in {shapes!r}:
 """
 
 
-@pytest.mark.parametrize("path", ["launch", "legacy", "persistent"])
+@pytest.mark.parametrize("path", ["launch", "persistent"])
 @pytest.mark.parametrize("shapes", [
     [(5, 19, 280), (5, 14, 172), (1, 1, 1), (2, 1, 3), (6, 40, 90), (3, 200, 10), (1, 6, 4), (4, 2, 7),
      (3, 5, 2)],
