@@ -67,6 +67,10 @@ SIGNATURES = {
                               _dp, C.c_int, C.c_double, _dp, C.c_int]),
 }
 SIGNATURES["ccgpu_dgemm_mixed"] = SIGNATURES["ccgpu_dgemm"]
+# device arrays and a stream handle (raw addresses)
+SIGNATURES["ccgpu_dgemm_mixed_noalloc"] = (C.c_int, [C.c_char, C.c_char, C.c_int, C.c_int, C.c_int, C.c_double,
+                                                     C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_double,
+                                                     C.c_void_p, C.c_int, C.c_void_p])
 
 _lib = None
 

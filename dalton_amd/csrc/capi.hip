@@ -503,4 +503,19 @@ int ccgpu_dgemm_mixed(char transa, char transb, int m, int n, int k, double alph
   return dgemm_impl(ccgpu::GemmPrec::BF16X3, transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc);
 }
 
+int ccgpu_dgemm_mixed_noalloc(char transa, char transb, int m, int n, int k, double alpha, const double* a,
+                              int lda, const double* b, int ldb, double beta, double* c, int ldc, void* stream) {
+  return guarded([&] {
+    size_t na = 0, nb = 0, nc = 0;
+    const ccgpu::GemmDesc d = blas_desc(transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, na, nb, nc);
+    if (m == 0 || n == 0) return;
+    CCGPU_REQUIRE(c != nullptr && (k == 0 || (a != nullptr && b != nullptr)),
+                  "ccgpu_dgemm_mixed_noalloc: null device array");
+    const void* spans[3] = {a, b, c};
+    const size_t bytes[3] = {na * sizeof(double), nb * sizeof(double), nc * sizeof(double)};
+    ccgpu::ExternalSpans ext(spans, bytes, 3);
+    ccgpu::gemm_bf16x3_noalloc(d, static_cast<hipStream_t>(stream));
+  });
+}
+
 }  // extern "C"

@@ -38,8 +38,19 @@ bool& dryrun();
 void* dry_alloc(size_t bytes);
 void register_alloc(const void* p, size_t bytes);
 void unregister_alloc(const void* p);
-// Throws unless [p + lo*esz, p + (hi+1)*esz) lies inside one registered allocation.
+// Throws unless [p + lo*esz, p + (hi+1)*esz) lies inside one registered allocation
+// or inside one span of caller-owned memory declared by an ExternalSpans scope.
 void check_span(const void* p, long lo, long hi, size_t esz, const char* what);
+
+// Caller-owned device memory (not allocated by the library) that the span
+// checks of one call accept: at most four spans, held in a fixed per-thread
+// table for the lifetime of the scope (nothing is allocated).
+struct ExternalSpans {
+  ExternalSpans(const void* const* p, const size_t* bytes, int n);
+  ~ExternalSpans();
+  ExternalSpans(const ExternalSpans&) = delete;
+  ExternalSpans& operator=(const ExternalSpans&) = delete;
+};
 
 #define CCGPU_LAUNCH(kernel, grid, block, smem, stream, ...)                          \
   do {                                                                               \
@@ -178,6 +189,13 @@ BMulti bf16x3_desc(const BTerm* terms, int nterm, long M, long N, int batch, voi
                    long c_n, long c_b, double alpha, double beta);
 // Launch a product described by bf16x3_desc (batch as given there).
 void bf16x3_run(const BMulti& p, int batch, bool c_float, double flops, hipStream_t s);
+
+// The BF16x3 GEMM of gemm(d, GemmPrec::BF16X3, s) with its packed operands in
+// a fixed per-device scratch array of kNoAllocScratchBytes (no allocation, no
+// synchronisation: stream-ordered on s).  The operands must lie in
+// registered allocations or in an ExternalSpans scope.
+constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
+void gemm_bf16x3_noalloc(const GemmDesc& d, hipStream_t s);
 
 // MO density-fitting factor of AO integrals by pivoted Cholesky (cholesky.hip):
 // returns naux x nmo x nmo values; maxvec <= 0 allows one vector per AO pair.

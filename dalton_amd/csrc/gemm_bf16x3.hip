@@ -11,6 +11,7 @@
 // alpha/beta in FP64 to the FP64 output with arbitrary strides.
 #include <cmath>
 #include <cstdlib>
+#include <mutex>
 
 #include "bf16x3_tile.h"
 #include "common.h"
@@ -363,6 +364,60 @@ void gemm_bf16x3(const GemmDesc& d, hipStream_t s) {
               pk[0], pk[1], s, d.a_ko, d.Kin);
   bf16x3_pack(static_cast<const double*>(d.B), d.N, d.K, d.b_n, d.b_k, d.b_b, d.batch, Np, Kp, 1.0f,
               pk[2], pk[3], s, d.b_ko, d.Kin);
+  BTerm t{pk[0], pk[1], pk[2], pk[3], Kp, Mp * Kp, Np * Kp};
+  bf16x3_multi(&t, 1, d.M, d.N, d.batch, d.C, false, d.c_m, d.c_n, d.c_b, d.alpha, d.beta,
+               2.0 * d.M * d.N * d.K * d.batch, s);
+}
+
+// ------------------------------------------------------ no-allocation variant
+// The packed hi/lo operands of one product live in this array: one copy per
+// GPU, held until the process ends, shared by every call on that GPU (the
+// last call's contents stay; every element a product reads is rewritten by
+// its pack kernels first).
+static __device__ __attribute__((aligned(256))) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device, looked up once per
+// device (with that device current) and registered for the span checks.
+unsigned short* noalloc_scratch() {
+  int dev = 0;
+  if (!dryrun()) CCGPU_HIP(hipGetDevice(&dev));
+  CCGPU_REQUIRE(dev >= 0 && dev < kMaxDevices, "no-allocation scratch: device index out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    if (dryrun())
+      ptr = dry_alloc(kNoAllocScratchBytes);
+    else
+      CCGPU_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    CCGPU_REQUIRE(ptr != nullptr, "no-allocation scratch: symbol address is null");
+    register_alloc(ptr, kNoAllocScratchBytes);
+    g_scratch_ptr[dev] = ptr;
+  }
+  return static_cast<unsigned short*>(g_scratch_ptr[dev]);
+}
+}  // namespace
+
+void gemm_bf16x3_noalloc(const GemmDesc& d, hipStream_t s) {
+  if (d.M <= 0 || d.N <= 0 || d.batch <= 0) return;
+  check_gemm(d, sizeof(double));
+  const long Mp = bf16x3_rpad(d.M), Np = bf16x3_rpad(d.N), Kp = bf16x3_kpad(d.K);
+  const long na = (long)d.batch * Mp * Kp, nb = (long)d.batch * Np * Kp;
+  const size_t need = 2 * sizeof(unsigned short) * (size_t)(na + nb);
+  if (need > kNoAllocScratchBytes)
+    throw Error(1, "no-allocation BF16x3 GEMM: the packed operands need " + std::to_string(need) +
+                       " bytes, the scratch holds " + std::to_string(kNoAllocScratchBytes));
+  unsigned short* base = noalloc_scratch();
+  // na and nb are multiples of 64 elements: every array starts 128-byte aligned
+  unsigned short* pk[4] = {base, base + na, base + 2 * na, base + 2 * na + nb};
+  bf16x3_pack(static_cast<const double*>(d.A), d.M, d.K, d.a_m, d.a_k, d.a_b, d.batch, Mp, Kp, 1.0f, pk[0], pk[1],
+              s, d.a_ko, d.Kin);
+  bf16x3_pack(static_cast<const double*>(d.B), d.N, d.K, d.b_n, d.b_k, d.b_b, d.batch, Np, Kp, 1.0f, pk[2], pk[3],
+              s, d.b_ko, d.Kin);
   BTerm t{pk[0], pk[1], pk[2], pk[3], Kp, Mp * Kp, Np * Kp};
   bf16x3_multi(&t, 1, d.M, d.N, d.batch, d.C, false, d.c_m, d.c_n, d.c_b, d.alpha, d.beta,
                2.0 * d.M * d.N * d.K * d.batch, s);

@@ -76,10 +76,37 @@ void unregister_alloc(const void* p) {
   allocs().erase(reinterpret_cast<uintptr_t>(p));
 }
 
+namespace {
+constexpr int kMaxExternal = 4;
+thread_local uintptr_t t_ext_p[kMaxExternal];
+thread_local size_t t_ext_n[kMaxExternal];
+thread_local int t_ext_cnt = 0;
+}  // namespace
+
+ExternalSpans::ExternalSpans(const void* const* p, const size_t* bytes, int n) {
+  CCGPU_REQUIRE(t_ext_cnt == 0 && n >= 0 && n <= kMaxExternal, "external spans: nested scope or too many spans");
+  for (int i = 0; i < n; ++i) {
+    t_ext_p[i] = reinterpret_cast<uintptr_t>(p[i]);
+    t_ext_n[i] = bytes[i];
+  }
+  t_ext_cnt = n;
+}
+
+ExternalSpans::~ExternalSpans() { t_ext_cnt = 0; }
+
 void check_span(const void* p, long lo, long hi, size_t esz, const char* what) {
   if (hi < lo) return;  // empty
-  std::lock_guard<std::mutex> lk(g_mu);
   const uintptr_t a = reinterpret_cast<uintptr_t>(p);
+  for (int i = 0; i < t_ext_cnt; ++i) {
+    if (a < t_ext_p[i] || a >= t_ext_p[i] + t_ext_n[i]) continue;
+    const long long first = (long long)a + (long long)lo * (long long)esz;
+    const long long last = (long long)a + ((long long)hi + 1) * (long long)esz;
+    if (first < (long long)t_ext_p[i] || last > (long long)(t_ext_p[i] + t_ext_n[i]))
+      throw Error(1, std::string("bounds: ") + what + " exceeds the caller's array (span " + std::to_string(lo) +
+                         ".." + std::to_string(hi) + " elements)");
+    return;
+  }
+  std::lock_guard<std::mutex> lk(g_mu);
   auto& m = allocs();
   auto it = m.upper_bound(a);
   if (it == m.begin()) throw Error(1, std::string("bounds: ") + what + " not in a registered allocation");

@@ -341,6 +341,21 @@ int ccgpu_dgemm(char transa, char transb, int m, int n, int k, double alpha, con
 int ccgpu_dgemm_mixed(char transa, char transb, int m, int n, int k, double alpha, const double* a,
                       int lda, const double* b, int ldb, double beta, double* c, int ldc);
 
+/* ccgpu_dgemm_mixed for callers that cannot allocate device memory (code
+ * under HIP graph capture, serving loops, harnesses that forbid hipMalloc).
+ * No counterpart in the reference.  a, b and c are DEVICE arrays (same BLAS
+ * column-major semantics); the call is stream-ordered on `stream` (a
+ * hipStream_t; NULL = the null stream) and returns once the work is
+ * enqueued, without synchronising.  It calls no hipMalloc/hipFree: the
+ * packed bf16 hi/lo operands live in a fixed per-GPU scratch array of
+ * 64 MiB, so 4 * (Mp + Np) * Kp bytes must fit, with Mp, Np = m, n rounded
+ * up to 256 (128 under CCGPU_BF16X3_VARIANT=0) and Kp = k rounded up to 64
+ * (e.g. m = n = k = 2048 fits); larger products return 1 with a message.
+ * Every call on one GPU uses that GPU's array: calls that overlap in time on
+ * one GPU (different streams, or threads) race, calls on one stream do not.
+ * Results are bit-identical to ccgpu_dgemm_mixed. */
+int ccgpu_dgemm_mixed_noalloc(char transa, char transb, int m, int n, int k, double alpha, const double* a,
+                              int lda, const double* b, int ldb, double beta, double* c, int ldc, void* stream);
 
 #ifdef __cplusplus
 }
