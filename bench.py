@@ -668,12 +668,13 @@ def main():
     ap.add_argument("--lhtr-vectors", type=int, default=16,
                     help="left-transformation trial vectors timed at o=32 v=256 naux=1024 (0: skip)")
     ap.add_argument("--workload", choices=["ccsd", "pt", "pt48", "batch", "ao", "lambda", "restart", "copy", "blas",
-                                           "cpu-baseline"], default="ccsd",
+                                           "accuracy", "cpu-baseline"], default="ccsd",
                     help="ccsd: the headline line (default); pt: the (T) leg alone at --o/--v; "
                          "ao: the AO Cholesky front end; lambda: the L0 multiplier solve at "
                          "o=32 v=256 in both precisions; restart: the R0 amplitude file at o=64 v=512; pt48: full (T) at o=48 v=384 in both "
                          "precisions on a convergent system; batch: 32 uneven systems converged in "
-                         "one graph replay per iteration")
+                         "one graph replay per iteration; accuracy: the convergent-scale FP64 and mixed "
+                         "solves at --o/--v/--naux alone")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: runs on one GPU only; --gpus must be 1 (the CC path is not sharded "
@@ -699,6 +700,9 @@ def main():
         return
     if args.workload == "batch":
         print(json.dumps(batch_workload()), flush=True)
+        return
+    if args.workload == "accuracy":
+        print(json.dumps(convergent_leg(args.o, args.v, args.naux, args.seed, args.pt_triples, _log)), flush=True)
         return
     if args.workload == "copy":
         print(json.dumps(copy_workload()), flush=True)

@@ -111,8 +111,9 @@ void Batch::solve(int maxit, double threnr, double thrvec, GemmPrec prec, bool u
   replays_ = 0;
   if (tune) this->tune();
   std::vector<double> en(n), emp2(n);
-  std::vector<int> it(n, 0), nfp64(n, 0);
-  std::vector<char> fp64(n, prec == GemmPrec::F64), done(n, 0);
+  std::vector<int> it(n, 0);
+  std::vector<PrecisionSwitch> ps(n, PrecisionSwitch(prec == GemmPrec::F64));
+  std::vector<char> done(n, 0);
   for (int k = 0; k < n; ++k) {
     emp2[k] = sys_[k]->mp2_start();
     en[k] = emp2[k];
@@ -120,9 +121,7 @@ void Batch::solve(int maxit, double threnr, double thrvec, GemmPrec prec, bool u
     if (e_mp2) e_mp2[k] = emp2[k];
   }
   CCGPU_HIPX(hipDeviceSynchronize());
-  // Mixed precision: as ccgpu_ccsd_energy, BF16x3 bulk contractions until
-  // |dE| < 1e-6 and norm < 1e-3, then FP64 with at least two FP64 iterations.
-  const double kSwitch = 1e-6;
+  // Reduced precision moves to FP64 per system as in ccgpu_ccsd_energy.
   std::vector<int> active;
   for (int k = 0; k < n; ++k) active.push_back(k);
   bool dirty = true;
@@ -130,7 +129,7 @@ void Batch::solve(int maxit, double threnr, double thrvec, GemmPrec prec, bool u
   // it sizes the per-stream scratch, which must not grow inside a capture.
   bool fresh = true;
   for (int round = 1; !active.empty(); ++round) {
-    for (int k : active) sys_[k]->set_precision(fp64[k] ? GemmPrec::F64 : prec);
+    for (int k : active) sys_[k]->set_precision(ps[k].fp64 ? GemmPrec::F64 : prec);
     if (use_graph && !dryrun() && !fresh) {
       if (dirty || !exec_) capture(active);
       dirty = false;
@@ -153,15 +152,14 @@ void Batch::solve(int maxit, double threnr, double thrvec, GemmPrec prec, bool u
       en[k] = dryrun() ? 0.0 : r[1];
       const double nrm = dryrun() ? 0.0 : std::sqrt(r[0]);
       it[k] = round;
-      if (fp64[k]) ++nfp64[k];
-      const bool conv = std::fabs(en[k] - e_old) < threnr && nrm < thrvec;
-      if (conv && (prec == GemmPrec::F64 || nfp64[k] >= 2)) {
+      const bool was_fp64 = ps[k].fp64;
+      const double de = std::fabs(en[k] - e_old);
+      if (ps[k].after(de < threnr && nrm < thrvec, de, nrm)) {
         done[k] = 1;
         dirty = true;
         continue;
       }
-      if (!fp64[k] && std::fabs(en[k] - e_old) < kSwitch && nrm < std::sqrt(kSwitch)) {
-        fp64[k] = 1;
+      if (ps[k].fp64 != was_fp64) {  // new precision: re-capture after an eager round
         dirty = true;
         fresh = true;
       }

@@ -220,24 +220,17 @@ int ccgpu_ccsd_iterate(ccgpu_ctx* ctx, int precision, double* e_corr, double* om
 static void ccsd_loop(ccgpu_ctx* ctx, int maxit, double threnr, double thrvec, int precision, double en2, double* e_ccsd,
                       int* niter, double* history) {
   const ccgpu::GemmPrec bulk = prec_of(precision);
-  // Mixed precision: bulk contractions in BF16x3 until the iteration is
-  // close (|dE| < kSwitch and norm < sqrt(kSwitch)), then FP64; convergence is
-  // only accepted after at least two FP64 iterations.
-  const double kSwitch = 1e-6;
-  bool fp64 = bulk == ccgpu::GemmPrec::F64;
-  int nfp64 = 0;
+  ccgpu::PrecisionSwitch ps(bulk == ccgpu::GemmPrec::F64);
   for (int it = 1;; ++it) {
     const double en1 = en2;
     double nrm = 0.0;
-    ctx->cc->set_precision(fp64 ? ccgpu::GemmPrec::F64 : bulk);
+    ctx->cc->set_precision(ps.fp64 ? ccgpu::GemmPrec::F64 : bulk);
     en2 = ctx->cc->iterate(&nrm);
-    if (fp64) ++nfp64;
     if (history) history[it - 1] = en2;
     if (niter) *niter = it;
     if (e_ccsd) *e_ccsd = en2;
-    const bool conv = std::fabs(en2 - en1) < threnr && nrm < thrvec;
-    if (conv && (bulk == ccgpu::GemmPrec::F64 || nfp64 >= 2)) break;
-    if (!fp64 && std::fabs(en2 - en1) < kSwitch && nrm < std::sqrt(kSwitch)) fp64 = true;
+    const double de = std::fabs(en2 - en1);
+    if (ps.after(de < threnr && nrm < thrvec, de, nrm)) break;
     if (it + 1 > maxit) throw ccgpu::Error(4, "CC equations not converged.");
   }
 }

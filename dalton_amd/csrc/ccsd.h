@@ -8,6 +8,32 @@
 
 namespace ccgpu {
 
+// When a CCSD loop in reduced precision (BF16x3 or FP32 bulk contractions)
+// moves to FP64: once |dE| < kSwitch and the residual norm < sqrt(kSwitch).
+// Convergence is accepted only after at least two FP64 iterations, which
+// take the amplitudes to the FP64 fixed point; an FP64 loop accepts it at
+// once.  (The reduced-precision iteration converges to the fixed point of
+// a Hamiltonian perturbed by the rounding of its operands, about 3e-7 Eh
+// from the FP64 one at o=64 v=512: iterating it further, until its own test
+// is met, only costs time, and the FP64 iterations needed afterwards do not
+// get fewer.)
+struct PrecisionSwitch {
+  static constexpr double kSwitch = 1e-6;
+  bool fp64;
+  bool reduced;
+  int nfp64 = 0;
+  explicit PrecisionSwitch(bool start_fp64) : fp64(start_fp64), reduced(!start_fp64) {}
+  // After an iteration (run in FP64 iff fp64 was set) with energy change dE,
+  // residual norm nrm and convergence test conv: true when the loop is
+  // converged.  May set fp64 for the next iteration.
+  bool after(bool conv, double dE, double nrm) {
+    if (fp64) ++nfp64;
+    if (conv && (!reduced || nfp64 >= 2)) return true;
+    if (!fp64 && dE < kSwitch && nrm * nrm < kSwitch) fp64 = true;
+    return false;
+  }
+};
+
 struct Dims {
   int o = 0, v = 0, naux = 0;
   long N = 0, nov = 0, nov2 = 0;

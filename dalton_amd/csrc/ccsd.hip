@@ -512,9 +512,9 @@ double CCSD::iterate(double* omnm) {
 
 void CCSD::set_amplitudes(const double* t1_host, const double* t2p_host) {
   const long nov = dm_.nov;
-  CCGPU_HIPX(hipMemcpyAsync(t1_.p, t1_host, nov * sizeof(double), hipMemcpyHostToDevice, st_));
+  CCGPU_HIPX(hipMemcpyAsync(t1_.p, t1_host, nov * sizeof(double), hipMemcpyDefault, st_));
   CCGPU_HIPX(hipMemcpyAsync(Om_.p, t2p_host, (dm_.ntamp - nov) * sizeof(double),
-                           hipMemcpyHostToDevice, st_));
+                           hipMemcpyDefault, st_));
   unpack_sym(T_.p, Om_.p, nov, st_);
   CCGPU_HIPX(hipStreamSynchronize(st_));
   iter_ = 1;
@@ -526,11 +526,11 @@ void CCSD::set_amplitudes(const double* t1_host, const double* t2p_host) {
 void CCSD::get_amplitudes(double* t1_host, double* t2p_host) {
   const long nov = dm_.nov;
   if (t1_host)
-    CCGPU_HIPX(hipMemcpyAsync(t1_host, t1_.p, nov * sizeof(double), hipMemcpyDeviceToHost, st_));
+    CCGPU_HIPX(hipMemcpyAsync(t1_host, t1_.p, nov * sizeof(double), hipMemcpyDefault, st_));
   if (t2p_host) {
     pack_sym(Om_.p, T_.p, nov, st_);
     CCGPU_HIPX(hipMemcpyAsync(t2p_host, Om_.p, (dm_.ntamp - nov) * sizeof(double),
-                             hipMemcpyDeviceToHost, st_));
+                             hipMemcpyDefault, st_));
   }
   CCGPU_HIPX(hipStreamSynchronize(st_));
 }
@@ -571,13 +571,13 @@ void CCSD::residual_host(const double* t1, const double* t2p, double* om1, doubl
   ensure_work();
   rt1_.ensure((size_t)nov);
   rT_.ensure((size_t)dm_.nov2);
-  CCGPU_HIPX(hipMemcpyAsync(rt1_.p, t1, nov * sizeof(double), hipMemcpyHostToDevice, st_));
-  CCGPU_HIPX(hipMemcpyAsync(bufX2_.p, t2p, np2 * sizeof(double), hipMemcpyHostToDevice, st_));
+  CCGPU_HIPX(hipMemcpyAsync(rt1_.p, t1, nov * sizeof(double), hipMemcpyDefault, st_));
+  CCGPU_HIPX(hipMemcpyAsync(bufX2_.p, t2p, np2 * sizeof(double), hipMemcpyDefault, st_));
   unpack_sym(rT_.p, bufX2_.p, nov, st_);
   residual(rt1_.p, rT_.p, om1_.p, Om_.p);
-  CCGPU_HIPX(hipMemcpyAsync(om1, om1_.p, nov * sizeof(double), hipMemcpyDeviceToHost, st_));
+  CCGPU_HIPX(hipMemcpyAsync(om1, om1_.p, nov * sizeof(double), hipMemcpyDefault, st_));
   pack_sym(bufX2_.p, Om_.p, nov, st_);
-  CCGPU_HIPX(hipMemcpyAsync(om2p, bufX2_.p, np2 * sizeof(double), hipMemcpyDeviceToHost, st_));
+  CCGPU_HIPX(hipMemcpyAsync(om2p, bufX2_.p, np2 * sizeof(double), hipMemcpyDefault, st_));
   CCGPU_HIPX(hipStreamSynchronize(st_));
 }
 

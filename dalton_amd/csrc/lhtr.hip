@@ -245,8 +245,8 @@ void CCSD::lhtr_host(int nvec, const double* l1, const double* l2p, double* rho1
   auto out2 = [&](int q) { return lst_.p + (2 + q) * slot + nov; };
   auto copy_in = [&](int iv) {
     const int q = iv & 1;
-    CCGPU_HIPX(hipMemcpyAsync(in1(q), l1 + iv * nov, nov * sizeof(double), hipMemcpyHostToDevice, cst_));
-    CCGPU_HIPX(hipMemcpyAsync(in2(q), l2p + iv * np2, np2 * sizeof(double), hipMemcpyHostToDevice, cst_));
+    CCGPU_HIPX(hipMemcpyAsync(in1(q), l1 + iv * nov, nov * sizeof(double), hipMemcpyDefault, cst_));
+    CCGPU_HIPX(hipMemcpyAsync(in2(q), l2p + iv * np2, np2 * sizeof(double), hipMemcpyDefault, cst_));
     CCGPU_HIPX(hipEventRecord(ev_in_[q], cst_));
   };
   auto compute = [&](int iv) {
@@ -264,8 +264,8 @@ void CCSD::lhtr_host(int nvec, const double* l1, const double* l2p, double* rho1
   auto copy_out = [&](int iv) {
     const int q = iv & 1;
     CCGPU_HIPX(hipStreamWaitEvent(cst_, ev_done_[q], 0));
-    CCGPU_HIPX(hipMemcpyAsync(rho1 + iv * nov, out1(q), nov * sizeof(double), hipMemcpyDeviceToHost, cst_));
-    CCGPU_HIPX(hipMemcpyAsync(rho2p + iv * np2, out2(q), np2 * sizeof(double), hipMemcpyDeviceToHost, cst_));
+    CCGPU_HIPX(hipMemcpyAsync(rho1 + iv * nov, out1(q), nov * sizeof(double), hipMemcpyDefault, cst_));
+    CCGPU_HIPX(hipMemcpyAsync(rho2p + iv * np2, out2(q), np2 * sizeof(double), hipMemcpyDefault, cst_));
     CCGPU_HIPX(hipStreamSynchronize(cst_));  // slot q (in and out) is free again
   };
   copy_in(0);

@@ -20,6 +20,11 @@
  * message.  A Fortran caller maps nonzero codes to QUIT (gp/gptrygve.F:256),
  * which is Dalton's only error convention.
  * Calls on one context are synchronous and must not overlap.
+ * ccgpu_rhs, ccgpu_lhtr, ccgpu_set_amplitudes and ccgpu_get_amplitudes also
+ * accept device arrays (allocations on the context's GPU, e.g. torch
+ * tensors): the copy direction is taken from the addresses, and the call
+ * returns once the results are in place.  Work the caller queued on its own
+ * streams that writes their inputs must be complete before the call.
  */
 #ifndef CCGPU_H
 #define CCGPU_H

@@ -248,6 +248,42 @@ def test_autograd_function_backward_is_lhtr():
     assert np.abs(x2.grad.numpy() - q2).max() < 1e-12 * s
 
 
+def test_autograd_function_on_device_tensors():
+    """The autograd Function with amplitudes on the GPU (torch device
+    tensors, whose addresses go through the C ABI): forward and backward stay
+    on the device and give the same bits as the host-array path, and the
+    backward equals the oracle's left transformation."""
+    import torch
+
+    import dalton_amd as d
+    from dalton_amd.autograd import ccsd_residual
+
+    eps, B, o = _system("synthetic_o4_v23_n60")
+    print("autograd backward on device tensors", flush=True)
+    sysm = occ.DFSystem(eps, B, o)
+    rng = np.random.default_rng(12)
+    with d.CCContext(eps, B, o) as c:
+        t1, t2, l1, l2 = _vectors(c.nt1, c.nt2, rng)
+        h1 = torch.tensor(t1, requires_grad=True)
+        h2 = torch.tensor(t2, requires_grad=True)
+        hom1, hom2 = ccsd_residual(c, h1, h2)
+        (hom1 @ torch.tensor(l1) + hom2 @ torch.tensor(l2)).backward()
+        x1 = torch.tensor(t1, device="cuda", requires_grad=True)
+        x2 = torch.tensor(t2, device="cuda", requires_grad=True)
+        om1, om2 = ccsd_residual(c, x1, x2)
+        assert om1.is_cuda and om2.is_cuda
+        (om1 @ torch.tensor(l1, device="cuda") + om2 @ torch.tensor(l2, device="cuda")).backward()
+        assert x1.grad.is_cuda and x2.grad.is_cuda
+        g1, g2 = x1.grad.cpu().numpy(), x2.grad.cpu().numpy()
+        assert np.array_equal(om1.detach().cpu().numpy(), hom1.detach().numpy())
+        assert np.array_equal(om2.detach().cpu().numpy(), hom2.detach().numpy())
+        assert np.array_equal(g1, h1.grad.numpy()) and np.array_equal(g2, h2.grad.numpy())
+    q1, q2 = olh.left_transform(sysm, t1, t2, l1, l2)
+    s = max(1.0, np.abs(q2).max())
+    assert np.abs(g1 - q1).max() < 1e-12 * s
+    assert np.abs(g2 - q2).max() < 1e-12 * s
+
+
 def test_lhtr_hf_molecule_matches_oracle():
     """HF/6-31G** (the reference's cc_energy_triples molecule) at its converged
     ground state: the left transformation of two vectors against the oracle."""
