@@ -167,7 +167,7 @@ def convergent_leg(o, v, naux, seed, ntrip, log, threnr=1e-9, thrvec=1e-7, maxit
     F = f_ccsd(o, v, naux)
 
     def pt_subset(c, prec):
-        _, _, _, ntr = c.pt_partial(0, 0, 1, precision=prec)  # (T) integral layouts (untimed)
+        _, _, _, ntr = c.pt_partial(0, 1, 1, precision=prec)  # (T) integral layouts, one triple (untimed)
         stride = max(1, ntr // ntrip)
         sync()
         t0 = time.perf_counter()
@@ -308,7 +308,9 @@ def pt_leg(ctx, ntrip, o, v, log):
 
     pt = {}
     for label, prec in (("mixed", d.MIXED), ("fp64", d.FP64)):
-        _, _, _, ntr = ctx.pt_partial(0, 0, 1, precision=prec)  # integral layouts of this precision (untimed)
+        # integral layouts of this precision and one triple, whose launches
+        # settle the block tiles of the mixed products (untimed)
+        _, _, _, ntr = ctx.pt_partial(0, 1, 1, precision=prec)
         stride = max(1, ntr // ntrip)
         sync()
         profile_start()
@@ -416,7 +418,7 @@ def pt48_workload(o=48, v=384, naux=1536, seed=5):
             sync()
             tc = time.perf_counter() - t0
             _log(f"{label}: CCSD {r['niter']} it in {tc:.1f} s, E = {r['e_ccsd']:.12f}")
-            _, _, _, ntr = c.pt_partial(0, 0, 1, precision=prec)  # (T) integral layouts (untimed)
+            _, _, _, ntr = c.pt_partial(0, 1, 1, precision=prec)  # (T) integral layouts, one triple (untimed)
             sync()
             t0 = time.perf_counter()
             e4 = e5 = 0.0
@@ -663,7 +665,7 @@ def main():
     ap.add_argument("--naux", type=int, default=2048)
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--pt-triples", type=int, default=24,
+    ap.add_argument("--pt-triples", type=int, default=256,
                     help="(T) triples timed, spread evenly over the i>=j>=k list (0: skip)")
     ap.add_argument("--lhtr-vectors", type=int, default=16,
                     help="left-transformation trial vectors timed at o=32 v=256 naux=1024 (0: skip)")

@@ -28,6 +28,13 @@ from .cc import FP64, CCContext
 
 _dp = C.POINTER(C.c_double)
 
+# torch's HIP runtime (bundled with the torch wheel) and the library's (the
+# system ROCm) are two runtimes in one process: torch's is initialised here,
+# at import, before the library launches anything (a first torch
+# initialisation that came after many library launches, cooperative ones
+# among them, has found no device).
+torch.cuda.is_available()
+
 
 def _np(x: torch.Tensor) -> np.ndarray:
     return np.ascontiguousarray(x.detach().to("cpu", torch.float64).numpy())

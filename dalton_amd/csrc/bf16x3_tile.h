@@ -2,6 +2,8 @@
 // FP32 accumulation), shared by the GEMM launches and the persistent (T)
 // kernel.  Operands are the packed K-contiguous hi/lo arrays of BTerm.
 #pragma once
+#include <type_traits>
+
 #include "common.h"
 
 namespace ccgpu {
@@ -179,4 +181,151 @@ __device__ __forceinline__ void tile(const BMulti& p, long m0, long n0, long z, 
 }
 
 }  // namespace bf3
+
+// One 256 x 256 output tile of the BF16x3 multi-term product run by eight
+// waves (512 threads, two per SIMD), each a 128 x 64 sub-tile of 8 x 4 MFMA
+// tiles, K step 32, shared by the GEMM launches and the persistent (T)
+// kernel.  Per K step a wave reads 24 KiB of operand fragments from LDS for
+// 96 MFMAs, and the operands reach LDS with global_load_lds_dwordx4 (no
+// staging registers, no LDS store instructions) into two stages: the next
+// step's loads are in flight while the current step's MFMAs run, with one
+// barrier per step.  LDS rows are 64 B (four 16-byte chunks); the chunk c of
+// row r sits in slot c ^ ((r >> 1) & 3), so the 16-byte fragment reads of
+// consecutive lanes fall on distinct bank groups.  One wave instruction fills
+// 1 KiB = 16 rows; per stage each wave issues two per array (hi and lo of A
+// and B).  The two stages are separate arrays (64 KiB each) and the K loop is
+// unrolled by two, so every LDS access names its stage statically.  The
+// accumulation order of every output element (k ascending; lo*hi, hi*lo,
+// hi*hi per k-slice of 32) is that of the 128 x 128 tile above, so the
+// results are bit-identical.  Operand rows m0 .. m0 + 255 and n0 .. n0 + 255
+// must lie inside the packed arrays (rows padded to 256).  The tile ends with
+// a barrier after its last LDS read; the caller separates it from a later
+// item's LDS writes only by the work between them.
+namespace bf3w8 {
+
+constexpr int DT = 256, DTK = 32;
+__device__ __forceinline__ int swz(int r) { return (r >> 1) & 3; }
+
+typedef __attribute__((address_space(3))) void lds_void;
+typedef const __attribute__((address_space(1))) void glb_void;
+
+struct __attribute__((aligned(16))) Smem {
+  unsigned short s0[4][DT * DTK];
+  unsigned short s1[4][DT * DTK];
+};
+
+// PIN: keep the thread-index arithmetic inside the tile (a persistent
+// kernel's claim loop would otherwise hold it in registers across items).
+template <typename OutT, bool PIN = false>
+__device__ __forceinline__ void tile(const BMulti& p, long m0, long n0, long z, Smem& S) {
+  auto& s0 = S.s0;
+  auto& s1 = S.s1;
+  int tid = threadIdx.x;
+  if (PIN) asm volatile("" : "+v"(tid));
+  const int lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 2, wn = wave & 3;  // 2 x 4 waves of 128 x 64
+  f4 acc[8][4];
+#pragma unroll
+  for (int i = 0; i < 8; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+  const int lr = lane & 15, kc = lane >> 4;
+  const int lrow = lane >> 2, lslot = lane & 3;
+  // fragment offsets (elements) of this lane within one 256 x 32 array
+  int offA[8], offB[4];
+#pragma unroll
+  for (int u = 0; u < 8; ++u) {
+    const int ra = wm * 128 + u * 16 + lr;
+    offA[u] = ra * DTK + 8 * (kc ^ swz(ra));
+  }
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int rb = wn * 64 + u * 16 + lr;
+    offB[u] = rb * DTK + 8 * (kc ^ swz(rb));
+  }
+  for (int t = 0; t < p.nterm; ++t) {
+    const BTerm& T = p.t[t];
+    const long Kp = T.Kp;
+    const unsigned short* src[4] = {T.Ah + z * T.a_bs + m0 * Kp, T.Al + z * T.a_bs + m0 * Kp,
+                                    T.Bh + z * T.b_bs + n0 * Kp, T.Bl + z * T.b_bs + n0 * Kp};
+    long goff[2];
+#pragma unroll
+    for (int ins = 0; ins < 2; ++ins) {
+      const int row = (wave * 2 + ins) * 16 + lrow;
+      goff[ins] = (long)row * Kp + 8 * (lslot ^ swz(row));
+    }
+    const long nsteps = Kp / DTK;
+#define CCGPU_W8_ISSUE(ST, k0)                                                                      \
+  _Pragma("unroll") for (int a = 0; a < 4; ++a)                                                    \
+    _Pragma("unroll") for (int ins = 0; ins < 2; ++ins)                                            \
+      __builtin_amdgcn_global_load_lds((glb_void*)(src[a] + goff[ins] + (k0)),                     \
+                                       (lds_void*)(&ST[a][(wave * 2 + ins) * 16 * DTK]), 16, 0, 0);
+#define CCGPU_W8_COMPUTE(ST)                                                                       \
+  {                                                                                                \
+    bf16x8 bh[4], bl[4];                                                                           \
+    _Pragma("unroll") for (int u = 0; u < 4; ++u) {                                                \
+      bh[u] = *reinterpret_cast<const bf16x8*>(&ST[2][offB[u]]);                                   \
+      bl[u] = *reinterpret_cast<const bf16x8*>(&ST[3][offB[u]]);                                   \
+    }                                                                                              \
+    _Pragma("unroll") for (int i = 0; i < 8; ++i) {                                                \
+      const bf16x8 ah = *reinterpret_cast<const bf16x8*>(&ST[0][offA[i]]);                         \
+      const bf16x8 al = *reinterpret_cast<const bf16x8*>(&ST[1][offA[i]]);                         \
+      _Pragma("unroll") for (int j = 0; j < 4; ++j)                                                \
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[j], acc[i][j], 0, 0, 0);        \
+      _Pragma("unroll") for (int j = 0; j < 4; ++j)                                                \
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[j], acc[i][j], 0, 0, 0);        \
+      _Pragma("unroll") for (int j = 0; j < 4; ++j)                                                \
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[j], acc[i][j], 0, 0, 0);        \
+    }                                                                                              \
+  }
+    CCGPU_W8_ISSUE(s0, 0)
+    for (long st = 0; st < nsteps; st += 2) {
+      // even step: stage 0 (its loads are the only ones outstanding)
+      __builtin_amdgcn_s_waitcnt(0);
+      __syncthreads();
+      if (st + 1 < nsteps) {
+        CCGPU_W8_ISSUE(s1, (st + 1) * DTK)
+      }
+      CCGPU_W8_COMPUTE(s0)
+      if (st + 1 < nsteps) {
+        // odd step: stage 1
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        if (st + 2 < nsteps) {
+          CCGPU_W8_ISSUE(s0, (st + 2) * DTK)
+        }
+        CCGPU_W8_COMPUTE(s1)
+      }
+    }
+#undef CCGPU_W8_ISSUE
+#undef CCGPU_W8_COMPUTE
+    __syncthreads();  // the next term's (or item's) first loads overwrite stage 0
+  }
+  // D[i][j] of a 16x16 f32 MFMA tile: lane = j + 16*(i/4), element i%4.
+  // A descriptor read from memory (PIN) names its output through a global
+  // pointer; a kernel argument's pointer is used as it is.
+  // The descriptor's fields are read once (stores through C would otherwise
+  // make the compiler reload them from the descriptor after every store).
+  using OutP = typename std::conditional<PIN, CCGPU_GLOBAL OutT*, OutT*>::type;
+  const OutP C = (OutP)(static_cast<OutT*>(p.C) + z * p.c_b);
+  const long M = p.M, N = p.N, c_m = p.c_m, c_n = p.c_n;
+  const double alpha = p.alpha, beta = p.beta;
+#pragma unroll
+  for (int i = 0; i < 8; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const long m = m0 + wm * 128 + i * 16 + (lane >> 4) * 4 + r;
+        const long n = n0 + wn * 64 + j * 16 + lr;
+        if (m < M && n < N) {
+          OutP q = C + m * c_m + n * c_n;
+          double v = alpha * (double)acc[i][j][r];
+          if (beta != 0.0) v += beta * (double)*q;
+          *q = static_cast<OutT>(v);
+        }
+      }
+}
+
+}  // namespace bf3w8
 }  // namespace ccgpu

@@ -44,25 +44,71 @@ __global__ void k_diag(double* __restrict__ d, const double* __restrict__ V, lon
     d[q] = V[q * M + q];
 }
 
-// One block: out[0] = max_q d[q], out[1] = its index (smallest on ties).
-__global__ __launch_bounds__(1024) void k_argmax(const double* __restrict__ d, long M, double* out) {
-  __shared__ double sv[1024];
-  __shared__ long si[1024];
+// Largest remaining diagonal and its index (smallest index on ties), in two
+// stages so that the search spreads over the whole chip: every block of a
+// grid-stride pass writes the best (value, index) of its elements to
+// part[2 b], part[2 b + 1], and one block reduces the nb partials in a fixed
+// order (the pivot is the same as that of a sequential scan).
+__device__ __forceinline__ void better(double& bv, long& bi, double ov, long oi) {
+  if (ov > bv || (ov == bv && oi < bi)) {
+    bv = ov;
+    bi = oi;
+  }
+}
+
+// Block reduction of (bv, bi) over 256 threads into part[2 blockIdx.x ...].
+__device__ __forceinline__ void block_best(double bv, long bi, double* __restrict__ part) {
+  __shared__ double sv[4];
+  __shared__ long si[4];
+  for (int off = 32; off > 0; off >>= 1) {
+    const double ov = __shfl_down(bv, off, 64);
+    const long oi = __shfl_down(bi, off, 64);
+    better(bv, bi, ov, oi);
+  }
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) {
+    sv[w] = bv;
+    si[w] = bi;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int q = 1; q < (int)(blockDim.x >> 6); ++q) better(bv, bi, sv[q], si[q]);
+    part[2 * (long)blockIdx.x] = bv;
+    part[2 * (long)blockIdx.x + 1] = (double)bi;
+  }
+}
+
+// Stage 1 on its own (the first pivot, and the check at the vector limit).
+__global__ __launch_bounds__(256) void k_argmax_part(const double* __restrict__ d, long M, double* __restrict__ part) {
   double bv = -1.0;
   long bi = M;
-  for (long q = threadIdx.x; q < M; q += blockDim.x)
-    if (d[q] > bv) {
-      bv = d[q];
-      bi = q;
+  for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < M; q += (long)gridDim.x * blockDim.x)
+    better(bv, bi, d[q], q);
+  block_best(bv, bi, part);
+}
+
+// Stage 2: out[0] = the largest value, out[1] = its index.
+__global__ __launch_bounds__(256) void k_argmax_final(const double* __restrict__ part, int nb, double* out) {
+  __shared__ double sv[256];
+  __shared__ long si[256];
+  double bv = -1.0;
+  long bi = -1;
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+    const double v = part[2 * b];
+    const long ix = (long)part[2 * b + 1];
+    if (bi < 0 || v > bv || (v == bv && ix < bi)) {
+      bv = v;
+      bi = ix;
     }
+  }
   sv[threadIdx.x] = bv;
   si[threadIdx.x] = bi;
   __syncthreads();
-  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s) {
-      const double ov = sv[threadIdx.x + s];
-      const long oi = si[threadIdx.x + s];
-      if (ov > sv[threadIdx.x] || (ov == sv[threadIdx.x] && oi < si[threadIdx.x])) {
+  for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) {
+      const double ov = sv[threadIdx.x + st];
+      const long oi = si[threadIdx.x + st];
+      if (oi >= 0 && (si[threadIdx.x] < 0 || ov > sv[threadIdx.x] || (ov == sv[threadIdx.x] && oi < si[threadIdx.x]))) {
         sv[threadIdx.x] = ov;
         si[threadIdx.x] = oi;
       }
@@ -81,9 +127,12 @@ __global__ __launch_bounds__(1024) void k_argmax(const double* __restrict__ d, l
 //   d[q] -= L[K0 + j][q]^2, d[p] = 0,
 // where R = V minus the vectors of earlier panels.  Stops (st[0] = 1,
 // st[1] = vectors made) once the largest diagonal is below thr.
-__global__ void k_chol_panel_col(double* __restrict__ L, double* __restrict__ d, const double* __restrict__ R,
-                                 const double* __restrict__ piv, long* __restrict__ st, long M, long K0, int j,
-                                 double thr) {
+// The same pass writes the stage-1 partials of the next pivot search (the
+// updated diagonal) to part; a step after the stop leaves them as they are.
+__global__ __launch_bounds__(256) void k_chol_panel_col(double* __restrict__ L, double* __restrict__ d,
+                                                        const double* __restrict__ R, const double* __restrict__ piv,
+                                                        long* __restrict__ st, long M, long K0, int j, double thr,
+                                                        double* __restrict__ part) {
   if (st[0] != 0) return;
   const double dp = piv[0];
   if (dp < thr) {
@@ -95,13 +144,18 @@ __global__ void k_chol_panel_col(double* __restrict__ L, double* __restrict__ d,
   }
   const long p = (long)piv[1];
   const double rs = 1.0 / sqrt(dp);
+  double bv = -1.0;
+  long bi = M;
   for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < M; q += (long)gridDim.x * blockDim.x) {
     double s = R[p * M + q];
     for (int i = 0; i < j; ++i) s -= L[(K0 + i) * M + p] * L[(K0 + i) * M + q];
     const double l = s * rs;
     L[(K0 + j) * M + q] = l;
-    d[q] = (q == p) ? 0.0 : d[q] - l * l;
+    const double dq = (q == p) ? 0.0 : d[q] - l * l;
+    d[q] = dq;
+    better(bv, bi, dq, q);
   }
+  block_best(bv, bi, part);
   if (blockIdx.x == 0 && threadIdx.x == 0) st[1] = K0 + j + 1;
 }
 
@@ -230,15 +284,20 @@ std::vector<double> df_factor_ao(int nbas, int nmo, const double* cmo, const dou
   constexpr int kNb = 64;
   DBuf<long> state(2);
   CCGPU_HIPX(hipMemsetAsync(state.p, 0, state.bytes(), st));
+  // pivot search: stage-1 partials of every block of the column pass, then
+  // one block over them
+  const unsigned gcol = grid_for(M);
+  DBuf<double> part(2 * (size_t)gcol);
+  CCGPU_LAUNCH(k_argmax_part, dim3(gcol), dim3(256), 0, st, d.p, M, part.p);
   long K = 0;
   *converged = false;
   while (true) {
     const int nb = (int)std::min<long>(kNb, Kmax - K);
     if (nb > 0) check_span(L.p, 0, (K + nb) * M - 1, sizeof(double), "cholesky L");
     for (int j = 0; j < nb; ++j) {
-      CCGPU_LAUNCH(k_argmax, dim3(1), dim3(1024), 0, st, d.p, M, piv.p);
-      CCGPU_LAUNCH(k_chol_panel_col, dim3(grid_for(M)), dim3(256), 0, st, L.p, d.p, V.p, piv.p, state.p, M, K, j,
-                   thr);
+      CCGPU_LAUNCH(k_argmax_final, dim3(1), dim3(256), 0, st, part.p, (int)gcol, piv.p);
+      CCGPU_LAUNCH(k_chol_panel_col, dim3(gcol), dim3(256), 0, st, L.p, d.p, V.p, piv.p, state.p, M, K, j, thr,
+                   part.p);
     }
     long h[2] = {0, K + nb};
     CCGPU_HIPX(hipMemcpyAsync(h, state.p, sizeof(h), hipMemcpyDeviceToHost, st));
@@ -253,7 +312,8 @@ std::vector<double> df_factor_ao(int nbas, int nmo, const double* cmo, const dou
     }
     K = K1;
     if (K == Kmax) {  // no room for another vector: converged only if nothing above the threshold remains
-      CCGPU_LAUNCH(k_argmax, dim3(1), dim3(1024), 0, st, d.p, M, piv.p);
+      CCGPU_LAUNCH(k_argmax_part, dim3(gcol), dim3(256), 0, st, d.p, M, part.p);
+      CCGPU_LAUNCH(k_argmax_final, dim3(1), dim3(256), 0, st, part.p, (int)gcol, piv.p);
       double hv[2] = {0.0, 0.0};
       CCGPU_HIPX(hipMemcpyAsync(hv, piv.p, sizeof(hv), hipMemcpyDeviceToHost, st));
       CCGPU_HIPX(hipStreamSynchronize(st));

@@ -11,7 +11,9 @@
 // alpha/beta in FP64 to the FP64 output with arbitrary strides.
 #include <cmath>
 #include <cstdlib>
+#include <map>
 #include <mutex>
+#include <type_traits>
 
 #include "bf16x3_tile.h"
 #include "common.h"
@@ -99,144 +101,30 @@ __device__ __forceinline__ void tile_of3(long b, long tiles_m, long tiles_n, lon
 
 constexpr int TM = 128, TN = bf3::TN, TK = bf3::TK;
 
-// One block per output tile, rasterised by tile_of3.
+// One block per output tile over ceil(M/TMr) x ceil(N/TN) tiles (rows of a
+// tile past M or N are the packed padding: read, never stored), rasterised
+// by tile_of3.
 template <typename OutT, int RW>
 __global__ __launch_bounds__(128 * RW, 4 / RW) void k_gemm_bf16x3(BMulti p) {
   __shared__ bf3::Smem<RW> S;
   constexpr int TMr = 64 * RW;
   long tm, tn;
-  tile_of3(blockIdx.x, p.Mp / TMr, p.Np / TN, tm, tn, p.gm);
+  tile_of3(blockIdx.x, (p.M + TMr - 1) / TMr, (p.N + TN - 1) / TN, tm, tn, p.gm);
   bf3::tile<OutT, RW>(p, tm * TMr, tn * TN, blockIdx.z, S);
 }
 
 
 // Large-tile kernel (the default; CCGPU_BF16X3_VARIANT=0 selects the
-// 128x128 kernel above): block tile 256x256 run by
-// eight waves (two per SIMD), each a 128x64 sub-tile of 8x4 MFMA tiles, K
-// step 32.  Per K step a wave reads 24 KiB of operand fragments from LDS for
-// 96 MFMAs, a quarter less than the 64x64-per-wave 128x128 kernel, and the
-// operands reach LDS with global_load_lds_dwordx4 (no staging registers, no
-// LDS store instructions) into two stages: the next step's loads are in
-// flight while the current step's MFMAs run, with one barrier per step.
-// LDS rows are 64 B (four 16-byte chunks); the chunk c of row r sits in slot
-// c ^ ((r >> 1) & 3), so the 16-byte fragment reads of consecutive lanes fall
-// on distinct bank groups.  One wave instruction fills 1 KiB = 16 rows; per
-// stage each wave issues two per array (hi and lo of A and B).  The two
-// stages are separate LDS objects (64 KiB each) and the K loop is unrolled by
-// two, so every LDS access names its stage statically.  The accumulation
-// order of every output element (k ascending; lo*hi, hi*lo, hi*hi per
-// k-slice of 32) is that of the 128x128 kernel, so the results are
-// bit-identical.
-constexpr int DT = 256, DTK = 32;
-__device__ __forceinline__ int swz(int r) { return (r >> 1) & 3; }
-
-typedef __attribute__((address_space(3))) void lds_void;
-typedef const __attribute__((address_space(1))) void glb_void;
+// 128x128 kernel above): one 256x256 tile of bf3w8::tile per block of eight
+// waves, one block per CU, rasterised by tile_of3.
+constexpr int DT = bf3w8::DT;
 
 template <typename OutT>
 __global__ __launch_bounds__(512, 1) void k_gemm_bf16x3_w8(BMulti p) {
-  __shared__ __attribute__((aligned(16))) unsigned short s0[4][DT * DTK];
-  __shared__ __attribute__((aligned(16))) unsigned short s1[4][DT * DTK];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wm = wave >> 2, wn = wave & 3;  // 2 x 4 waves of 128 x 64
-  const long tiles_m = p.Mp / DT, tiles_n = p.Np / DT;
+  __shared__ bf3w8::Smem S;
   long tm, tn;
-  tile_of3(blockIdx.x, tiles_m, tiles_n, tm, tn, p.gm);
-  const long m0 = tm * DT, n0 = tn * DT, z = blockIdx.z;
-  f4 acc[8][4];
-#pragma unroll
-  for (int i = 0; i < 8; ++i)
-#pragma unroll
-    for (int j = 0; j < 4; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
-  const int lr = lane & 15, kc = lane >> 4;
-  const int lrow = lane >> 2, lslot = lane & 3;
-  // fragment offsets (elements) of this lane within one 256 x 32 array
-  int offA[8], offB[4];
-#pragma unroll
-  for (int u = 0; u < 8; ++u) {
-    const int ra = wm * 128 + u * 16 + lr;
-    offA[u] = ra * DTK + 8 * (kc ^ swz(ra));
-  }
-#pragma unroll
-  for (int u = 0; u < 4; ++u) {
-    const int rb = wn * 64 + u * 16 + lr;
-    offB[u] = rb * DTK + 8 * (kc ^ swz(rb));
-  }
-  for (int t = 0; t < p.nterm; ++t) {
-    const BTerm& T = p.t[t];
-    const long Kp = T.Kp;
-    const unsigned short* src[4] = {T.Ah + z * T.a_bs + m0 * Kp, T.Al + z * T.a_bs + m0 * Kp,
-                                    T.Bh + z * T.b_bs + n0 * Kp, T.Bl + z * T.b_bs + n0 * Kp};
-    long goff[2];
-#pragma unroll
-    for (int ins = 0; ins < 2; ++ins) {
-      const int row = (wave * 2 + ins) * 16 + lrow;
-      goff[ins] = (long)row * Kp + 8 * (lslot ^ swz(row));
-    }
-    const long nsteps = Kp / DTK;
-#define CCGPU_W8_ISSUE(S, k0)                                                                       \
-  _Pragma("unroll") for (int a = 0; a < 4; ++a)                                                    \
-    _Pragma("unroll") for (int ins = 0; ins < 2; ++ins)                                            \
-      __builtin_amdgcn_global_load_lds((glb_void*)(src[a] + goff[ins] + (k0)),                     \
-                                       (lds_void*)(&S[a][(wave * 2 + ins) * 16 * DTK]), 16, 0, 0);
-#define CCGPU_W8_COMPUTE(S)                                                                        \
-  {                                                                                                \
-    bf16x8 bh[4], bl[4];                                                                           \
-    _Pragma("unroll") for (int u = 0; u < 4; ++u) {                                                \
-      bh[u] = *reinterpret_cast<const bf16x8*>(&S[2][offB[u]]);                                    \
-      bl[u] = *reinterpret_cast<const bf16x8*>(&S[3][offB[u]]);                                    \
-    }                                                                                              \
-    _Pragma("unroll") for (int i = 0; i < 8; ++i) {                                                \
-      const bf16x8 ah = *reinterpret_cast<const bf16x8*>(&S[0][offA[i]]);                          \
-      const bf16x8 al = *reinterpret_cast<const bf16x8*>(&S[1][offA[i]]);                          \
-      _Pragma("unroll") for (int j = 0; j < 4; ++j)                                                \
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[j], acc[i][j], 0, 0, 0);        \
-      _Pragma("unroll") for (int j = 0; j < 4; ++j)                                                \
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[j], acc[i][j], 0, 0, 0);        \
-      _Pragma("unroll") for (int j = 0; j < 4; ++j)                                                \
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[j], acc[i][j], 0, 0, 0);        \
-    }                                                                                              \
-  }
-    CCGPU_W8_ISSUE(s0, 0)
-    for (long st = 0; st < nsteps; st += 2) {
-      // even step: stage 0 (its loads are the only ones outstanding)
-      __builtin_amdgcn_s_waitcnt(0);
-      __syncthreads();
-      if (st + 1 < nsteps) {
-        CCGPU_W8_ISSUE(s1, (st + 1) * DTK)
-      }
-      CCGPU_W8_COMPUTE(s0)
-      if (st + 1 < nsteps) {
-        // odd step: stage 1
-        __builtin_amdgcn_s_waitcnt(0);
-        __syncthreads();
-        if (st + 2 < nsteps) {
-          CCGPU_W8_ISSUE(s0, (st + 2) * DTK)
-        }
-        CCGPU_W8_COMPUTE(s1)
-      }
-    }
-#undef CCGPU_W8_ISSUE
-#undef CCGPU_W8_COMPUTE
-    __syncthreads();  // the next term's first loads overwrite stage 0
-  }
-  // D[i][j] of a 16x16 f32 MFMA tile: lane = j + 16*(i/4), element i%4.
-  OutT* C = static_cast<OutT*>(p.C) + z * p.c_b;
-#pragma unroll
-  for (int i = 0; i < 8; ++i)
-#pragma unroll
-    for (int j = 0; j < 4; ++j)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const long m = m0 + wm * 128 + i * 16 + (lane >> 4) * 4 + r;
-        const long n = n0 + wn * 64 + j * 16 + lr;
-        if (m < p.M && n < p.N) {
-          OutT* q = C + m * p.c_m + n * p.c_n;
-          double v = p.alpha * (double)acc[i][j][r];
-          if (p.beta != 0.0) v += p.beta * (double)*q;
-          *q = static_cast<OutT>(v);
-        }
-      }
+  tile_of3(blockIdx.x, p.Mp / DT, p.Np / DT, tm, tn, p.gm);
+  bf3w8::tile<OutT>(p, tm * DT, tn * DT, blockIdx.z, S);
 }
 
 // Block-tile kernel: 1 (default) the eight-wave 256x256 direct-to-LDS
@@ -260,24 +148,137 @@ int group_m() {
   return g;
 }
 
+// Block-tile kernels of a product: the four-wave 128x128 tile (two blocks
+// per CU) or the eight-wave 256x256 one (one block per CU; operands padded to
+// 256 rows, which bf16x3_rpad gives).  Both accumulate every output element
+// in the same order, so the choice never changes a result bit.
+enum BKern { kT128 = 0, kT256 = 1 };
+
 template <typename OutT>
-void launch_multi(BMulti p, int batch, hipStream_t s) {
-  // Block tile: 256x256, eight waves of 128x64, one block per CU (default;
-  // products whose padded sizes are multiples of 256, which bf16x3_rpad
-  // gives); CCGPU_BF16X3_VARIANT=0: 128x128, four waves of 64x64, two blocks
-  // per CU.
-  p.gm = group_m();
-  CCGPU_REQUIRE(batch < 65536, "bf16x3 gemm: grid too large");
-  if (variant() == 1 && p.Mp % DT == 0 && p.Np % DT == 0) {
+void launch_kernel(int kern, const BMulti& p, int batch, hipStream_t s) {
+  if (kern == kT256) {
     const long ntiles = (p.Mp / DT) * (p.Np / DT);
     CCGPU_REQUIRE(ntiles < (1L << 23), "bf16x3 gemm: grid too large");
     CCGPU_LAUNCH(k_gemm_bf16x3_w8<OutT>, dim3((unsigned)ntiles, 1, (unsigned)batch), dim3(512), 0, s, p);
   } else {
-    const long ntiles = (p.Mp / TM) * (p.Np / TN);
+    const long ntiles = (long)cdiv(p.M, TM) * cdiv(p.N, TN);
     CCGPU_REQUIRE(ntiles < (1L << 23), "bf16x3 gemm: grid too large");
     CCGPU_LAUNCH((k_gemm_bf16x3<OutT, 2>), dim3((unsigned)ntiles, 1, (unsigned)batch), dim3(256), 0, s, p);
   }
   CCGPU_HIPX(hipGetLastError());
+}
+
+// ---- autotuner: per product shape class, the faster block tile measured
+struct BKey {
+  long M, N, Mp, Np, K[4];
+  int nterm, batch, flt;
+  bool operator<(const BKey& o) const {
+    if (M != o.M) return M < o.M;
+    if (N != o.N) return N < o.N;
+    if (Mp != o.Mp) return Mp < o.Mp;
+    if (Np != o.Np) return Np < o.Np;
+    for (int t = 0; t < 4; ++t)
+      if (K[t] != o.K[t]) return K[t] < o.K[t];
+    if (nterm != o.nterm) return nterm < o.nterm;
+    if (batch != o.batch) return batch < o.batch;
+    return flt < o.flt;
+  }
+};
+struct BTuner {
+  std::mutex mu;
+  std::map<BKey, int> best;
+  long timed = 0;
+};
+BTuner& btuner() {
+  static auto* t = new BTuner;  // never destroyed (used until process exit)
+  return *t;
+}
+
+// CCGPU_BF16X3_TUNE=0 disables the timing runs (the cost model decides);
+// so does a NoTune scope on the calling thread (callers that must not create
+// events or run extra launches).
+thread_local int t_no_tune = 0;
+bool tune_on() {
+  static const bool on = [] {
+    const char* e = std::getenv("CCGPU_BF16X3_TUNE");
+    return !(e && *e && std::atoi(e) == 0);
+  }();
+  return on && t_no_tune == 0;
+}
+struct NoTune {
+  NoTune() { ++t_no_tune; }
+  ~NoTune() { --t_no_tune; }
+};
+
+// Without a measurement: the 256x256 tile computes Mp x Np (padded to 256),
+// the 128x128 one ceil(M/128) x ceil(N/128) tiles, at 361 against 307
+// TFLOP/s measured on square products; the 256 tile also needs a wave of
+// tiles over the CUs to be worth its larger grain.
+int model_choice(const BMulti& p, int batch) {
+  const double a256 = (double)p.Mp * (double)p.Np;
+  const double a128 = (double)cdiv(p.M, TM) * TM * (double)cdiv(p.N, TN) * TN;
+  const long t256 = (p.Mp / DT) * (p.Np / DT) * (long)batch;
+  return (a256 * 307.0 <= a128 * 361.0 && t256 >= 256) ? kT256 : kT128;
+}
+
+// Times both tiles on the product itself: an overwriting product (beta = 0)
+// runs as given, since the real launch that follows rewrites the same
+// values; an accumulating one runs with alpha = 0 and beta = 1, which leaves
+// C as it was (up to the sign of exact zeros).
+template <typename OutT>
+int tune(const BMulti& p, int batch, hipStream_t s) {
+  BMulti q = p;
+  if (p.beta != 0.0) {
+    q.alpha = 0.0;
+    q.beta = 1.0;
+  }
+  hipEvent_t e0, e1;
+  CCGPU_HIP(hipEventCreate(&e0));
+  CCGPU_HIP(hipEventCreate(&e1));
+  float ms[2] = {0.f, 0.f};
+  for (int k = kT128; k <= kT256; ++k) {
+    launch_kernel<OutT>(k, q, batch, s);  // warm-up
+    CCGPU_HIP(hipEventRecord(e0, s));
+    launch_kernel<OutT>(k, q, batch, s);
+    CCGPU_HIP(hipEventRecord(e1, s));
+    CCGPU_HIP(hipEventSynchronize(e1));
+    CCGPU_HIP(hipEventElapsedTime(&ms[k], e0, e1));
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return ms[kT256] <= ms[kT128] ? kT256 : kT128;
+}
+
+template <typename OutT>
+void launch_multi(BMulti p, int batch, hipStream_t s) {
+  p.gm = group_m();
+  CCGPU_REQUIRE(batch < 65536, "bf16x3 gemm: grid too large");
+  int kern = kT128;
+  if (variant() == 1 && p.Mp % DT == 0 && p.Np % DT == 0) {
+    BKey key{p.M, p.N, p.Mp, p.Np, {0, 0, 0, 0}, p.nterm, batch, std::is_same<OutT, float>::value ? 1 : 0};
+    for (int t = 0; t < p.nterm; ++t) key.K[t] = p.t[t].Kp;
+    BTuner& tu = btuner();
+    bool found = false;
+    {
+      std::lock_guard<std::mutex> lk(tu.mu);
+      auto it = tu.best.find(key);
+      if (it != tu.best.end()) {
+        kern = it->second;
+        found = true;
+      }
+    }
+    if (!found) {
+      if (tune_on() && !dryrun() && !capturing(s)) {
+        kern = tune<OutT>(p, batch, s);
+        std::lock_guard<std::mutex> lk(tu.mu);
+        tu.best[key] = kern;
+        ++tu.timed;
+      } else {
+        kern = model_choice(p, batch);
+      }
+    }
+  }
+  launch_kernel<OutT>(kern, p, batch, s);
 }
 
 }  // namespace
@@ -411,6 +412,7 @@ void gemm_bf16x3_noalloc(const GemmDesc& d, hipStream_t s) {
   if (need > kNoAllocScratchBytes)
     throw Error(1, "no-allocation BF16x3 GEMM: the packed operands need " + std::to_string(need) +
                        " bytes, the scratch holds " + std::to_string(kNoAllocScratchBytes));
+  NoTune no_tune;  // no timing runs: the block tile comes from the tuned cache or the cost model
   unsigned short* base = noalloc_scratch();
   // na and nb are multiples of 64 elements: every array starts 128-byte aligned
   unsigned short* pk[4] = {base, base + na, base + 2 * na, base + 2 * na + nb};

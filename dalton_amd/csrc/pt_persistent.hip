@@ -233,12 +233,14 @@ struct EArgs {
   long nE, nct;  // energy items (blocks) and canonical blocks
 };
 
-template <typename WT>
+// LDS of an energy item of NT threads that stages NB canonical blocks at a
+// time (block u of a round in w[u], jb[u], t1s[u], code[u]).
+template <typename WT, int NB, int NT>
 struct __attribute__((aligned(16))) ESmem {
-  WT w[6][8 * 72];     // block permutation p of W: [x][y][z] at x*72 + y*9 + z
-  double jb[18][64];   // (ia|jb) blocks: [m*6 + pair][r*8 + c]
-  double t1s[3][3][8];  // t1 of i, j, k over the blocks A, B, C
-  double red[2][4];
+  WT w[NB][6][8 * 72];     // block permutation p of W: [x][y][z] at x*72 + y*9 + z
+  double jb[NB][18][64];   // (ia|jb) blocks: [m*6 + pair][r*8 + c]
+  double t1s[NB][3][3][8];  // t1 of i, j, k over the blocks A, B, C
+  double red[2][NT / 64];
 };
 
 // Positions (0 = A, 1 = B, 2 = C) of x, y, z in slot s: abc, acb, bac, bca, cab, cba.
@@ -282,11 +284,14 @@ __device__ __forceinline__ long uniform64(long x) {
   return (long)(((unsigned long)hi << 32) | lo);
 }
 
-// Energy item e of triple (i, j, k): canonical blocks e, e + nE, ...; writes
-// its scaled (E4, E5) partial to part[2e], part[2e + 1].
-template <typename WT>
+// Energy item e of triple (i, j, k): canonical blocks e, e + nE, ..., NB
+// of them staged per round (more loads in flight per barrier); NT threads.
+// Writes its scaled (E4, E5) partial to part[2e], part[2e + 1].  With NB = 1
+// the per-thread order of the sums is that of one block per round.
+template <typename WT, int NB, int NT>
 __device__ __forceinline__ void energy_item(const EArgs& r, const WT* W_, int i, int j, int k, long e, double* part,
-                                            ESmem<WT>& S) {
+                                            ESmem<WT, NB, NT>& S) {
+  static_assert((NB * 512) % NT == 0 && NT % 64 == 0, "energy item: whole elements per thread");
   const int o = r.o, v = r.v;
   const CCGPU_GLOBAL WT* W = gptr(W_);
   const CCGPU_GLOBAL double* J = gptr(r.J);
@@ -298,20 +303,35 @@ __device__ __forceinline__ void energy_item(const EArgs& r, const WT* W_, int i,
   asm volatile("" : "+v"(tid));  // thread-index arithmetic stays inside the item
   const double eijk = eps[i] + eps[j] + eps[k];
   const bool vec = (v % 4) == 0;
+  const long nE = r.nE, nct = r.nct;
   double s4 = 0.0, s5 = 0.0;
-  for (long ct = e; ct < r.nct; ct += r.nE) {
-    const int code = ctab[ct];
-    const int b0 = code & 1023, b1 = (code >> 10) & 1023, b2 = (code >> 20) & 1023;
-    auto blk = [&](int u) { return u == 0 ? b0 : (u == 1 ? b1 : b2); };  // selects, no indexed local array
-    __syncthreads();  // the previous blocks' LDS reads are done
-    // six W block permutations: 384 rows of 8
-    for (int row = tid; row < 384; row += 256) {
-      const int p = row >> 6, x8 = (row >> 3) & 7, y8 = row & 7;
+  for (long ct0 = e; ct0 < nct; ct0 += (long)NB * nE) {
+    // the round's canonical blocks (-1 past the list), block-uniform
+    int codes[NB];
+#pragma unroll
+    for (int u = 0; u < NB; ++u) {
+      const long ct = ct0 + (long)u * nE;
+      codes[u] = ct < nct ? ctab[ct] : -1;
+    }
+    auto code_of = [&](int u) {  // selects, no indexed local array
+      int c = codes[0];
+#pragma unroll
+      for (int q = 1; q < NB; ++q) c = (u == q) ? codes[q] : c;
+      return c;
+    };
+    __syncthreads();  // the previous round's LDS reads are done
+    // six W block permutations per block: 384 rows of 8
+    for (int row = tid; row < NB * 384; row += NT) {
+      const int u = row / 384, rr = row - u * 384;
+      const int code = code_of(u);
+      const int b0 = code & 1023, b1 = (code >> 10) & 1023, b2 = (code >> 20) & 1023;
+      auto blk = [&](int q) { return q == 0 ? b0 : (q == 1 ? b1 : b2); };  // selects, no indexed local array
+      const int p = rr >> 6, x8 = (rr >> 3) & 7, y8 = rr & 7;
       const int px = p >> 1, lo = px == 0 ? 1 : 0, hi = px == 2 ? 1 : 2;
       const int py = (p & 1) ? hi : lo, pz = (p & 1) ? lo : hi;
       const long gx = blk(px) * 8 + x8, gy = blk(py) * 8 + y8, z0 = blk(pz) * 8;
-      WT* dst = &S.w[p][x8 * 72 + y8 * 9];
-      if (gx < v && gy < v) {
+      WT* dst = &S.w[u][p][x8 * 72 + y8 * 9];
+      if (code >= 0 && gx < v && gy < v) {
         const CCGPU_GLOBAL WT* src = W + gx * vv + gy * v + z0;
         if (vec && z0 + 8 <= v) {
           if (sizeof(WT) == 4) {
@@ -340,36 +360,47 @@ __device__ __forceinline__ void energy_item(const EArgs& r, const WT* W_, int i,
       }
     }
     // (ia|jb) blocks: m = 0 J_jk[y][z], 1 J_ik[x][z], 2 J_ij[x][y]; pair (u, w) of positions
-    for (int q = tid; q < 18 * 64; q += 256) {
-      const int b = q >> 6, m = b / 6, pi = b - m * 6;
-      const int u = pi >> 1, wi = pi & 1, w = wi + (wi >= u ? 1 : 0);
-      const int rr = (q >> 3) & 7, cc = q & 7;
+    for (int q = tid; q < NB * 18 * 64; q += NT) {
+      const int u = q / (18 * 64), qq = q - u * (18 * 64);
+      const int code = code_of(u);
+      const int b0 = code & 1023, b1 = (code >> 10) & 1023, b2 = (code >> 20) & 1023;
+      auto blk = [&](int x) { return x == 0 ? b0 : (x == 1 ? b1 : b2); };
+      const int b = qq >> 6, m = b / 6, pi = b - m * 6;
+      const int pu = pi >> 1, wi = pi & 1, pw = wi + (wi >= pu ? 1 : 0);
+      const int rr = (qq >> 3) & 7, cc = qq & 7;
       const int p1 = m == 0 ? j : i, p2 = m == 2 ? j : k;
-      const long row = blk(u) * 8 + rr, col = blk(w) * 8 + cc;
-      S.jb[b][rr * 8 + cc] = (row < v && col < v) ? J[((long)p1 * v + row) * nov + (long)p2 * v + col] : 0.0;
+      const long row = blk(pu) * 8 + rr, col = blk(pw) * 8 + cc;
+      S.jb[u][b][rr * 8 + cc] =
+          (code >= 0 && row < v && col < v) ? J[((long)p1 * v + row) * nov + (long)p2 * v + col] : 0.0;
     }
-    if (tid < 72) {
-      const int m = tid / 24, u = (tid / 8) % 3, x8 = tid & 7;
-      const long x = blk(u) * 8 + x8;
+    for (int q = tid; q < NB * 72; q += NT) {
+      const int u = q / 72, qq = q - u * 72;
+      const int code = code_of(u);
+      const int b0 = code & 1023, b1 = (code >> 10) & 1023, b2 = (code >> 20) & 1023;
+      const int m = qq / 24, pu = (qq / 8) % 3, x8 = qq & 7;
+      const long x = (pu == 0 ? b0 : (pu == 1 ? b1 : b2)) * 8 + x8;
       const int om = m == 0 ? i : (m == 1 ? j : k);
-      S.t1s[m][u][x8] = x < v ? t1[(long)om * v + x] : 0.0;
+      S.t1s[u][m][pu][x8] = (code >= 0 && x < v) ? t1[(long)om * v + x] : 0.0;
     }
     __syncthreads();
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const int el = tid + 256 * h;
-      const int l0 = el >> 6, l1 = (el >> 3) & 7, l2 = el & 7;
-      auto loc = [&](int u) { return u == 0 ? l0 : (u == 1 ? l1 : l2); };
+    for (int h = 0; h < NB * 512 / NT; ++h) {
+      const int el = tid + NT * h;
+      const int u = el >> 9, l0 = (el >> 6) & 7, l1 = (el >> 3) & 7, l2 = el & 7;
+      const int code = code_of(u);
+      if (code < 0) continue;
+      const int b0 = code & 1023, b1 = (code >> 10) & 1023, b2 = (code >> 20) & 1023;
+      auto loc = [&](int x) { return x == 0 ? l0 : (x == 1 ? l1 : l2); };
       const int a = b0 * 8 + l0, b = b1 * 8 + l1, c = b2 * 8 + l2;
       if (!(a <= b && b <= c && c < v)) continue;
       double w[6], y[6];
 #pragma unroll
       for (int s = 0; s < 6; ++s) {
         const int xl = loc(PX[s]), yl = loc(PY[s]), zl = loc(PZ[s]);
-        w[s] = (double)S.w[s][xl * 72 + yl * 9 + zl];
-        y[s] = S.t1s[0][PX[s]][xl] * S.jb[pair_ix(PY[s], PZ[s])][yl * 8 + zl] +
-               S.t1s[1][PY[s]][yl] * S.jb[6 + pair_ix(PX[s], PZ[s])][xl * 8 + zl] +
-               S.t1s[2][PZ[s]][zl] * S.jb[12 + pair_ix(PX[s], PY[s])][xl * 8 + yl];
+        w[s] = (double)S.w[u][s][xl * 72 + yl * 9 + zl];
+        y[s] = S.t1s[u][0][PX[s]][xl] * S.jb[u][pair_ix(PY[s], PZ[s])][yl * 8 + zl] +
+               S.t1s[u][1][PY[s]][yl] * S.jb[u][6 + pair_ix(PX[s], PZ[s])][xl * 8 + zl] +
+               S.t1s[u][2][PZ[s]][zl] * S.jb[u][12 + pair_ix(PX[s], PY[s])][xl * 8 + yl];
       }
       // slots 0 abc, 1 acb, 2 bac, 3 bca, 4 cab, 5 cba; even class {0,3,4}, odd {1,2,5}
       const double wE = w[0] + w[3] + w[4], wO = w[1] + w[2] + w[5];
@@ -404,16 +435,23 @@ __device__ __forceinline__ void energy_item(const EArgs& r, const WT* W_, int i,
   if (tid == 0) {
     // (i == j or j == k) counts the triple 3 times, otherwise 6: scale mult / 3
     const double scale = (i == j || j == k) ? 1.0 : 2.0;
-    part[2 * e] = scale * (((S.red[0][0] + S.red[0][1]) + S.red[0][2]) + S.red[0][3]);
-    part[2 * e + 1] = scale * (((S.red[1][0] + S.red[1][1]) + S.red[1][2]) + S.red[1][3]);
+    double r4 = S.red[0][0], r5 = S.red[1][0];
+#pragma unroll
+    for (int q = 1; q < NT / 64; ++q) {
+      r4 += S.red[0][q];
+      r5 += S.red[1][q];
+    }
+    part[2 * e] = scale * r4;
+    part[2 * e + 1] = scale * r5;
   }
 }
 
-// (E4, E5) of one triple = sum of its nE partials in a fixed order.
-__device__ __forceinline__ void reduce_partials(const double* part, long nE, double* out, double (&red)[2][4]) {
+// (E4, E5) of one triple = sum of its nE partials in a fixed order (NT threads).
+template <int NT>
+__device__ __forceinline__ void reduce_partials(const double* part, long nE, double* out, double (&red)[2][NT / 64]) {
   const int tid = threadIdx.x;
   double s4 = 0.0, s5 = 0.0;
-  for (long q = tid; q < nE; q += 256) {
+  for (long q = tid; q < nE; q += NT) {
     s4 += part[2 * q];
     s5 += part[2 * q + 1];
   }
@@ -429,8 +467,14 @@ __device__ __forceinline__ void reduce_partials(const double* part, long nE, dou
   }
   __syncthreads();
   if (tid == 0) {
-    out[0] = ((red[0][0] + red[0][1]) + red[0][2]) + red[0][3];
-    out[1] = ((red[1][0] + red[1][1]) + red[1][2]) + red[1][3];
+    double r4 = red[0][0], r5 = red[1][0];
+#pragma unroll
+    for (int q = 1; q < NT / 64; ++q) {
+      r4 += red[0][q];
+      r5 += red[1][q];
+    }
+    out[0] = r4;
+    out[1] = r5;
   }
 }
 
@@ -443,7 +487,7 @@ __device__ __forceinline__ void pass_tile(const PtRun& r, long q, int ph, long t
   tile_grouped(t - z * tmn, r.tm[ph], r.tn[ph], tmi, tni);
   if constexpr (MIXED) {
     const BMulti& d = static_cast<const BMulti*>(r.desc)[q * 3 + ph];
-    bf3::tile<float, 2>(d, tmi * 128, tni * 128, z, *reinterpret_cast<bf3::Smem<2>*>(lds));
+    bf3w8::tile<float, true>(d, tmi * bf3w8::DT, tni * bf3w8::DT, z, *reinterpret_cast<bf3w8::Smem*>(lds));
   } else {
     const DMulti& d = static_cast<const DMulti*>(r.desc)[q * 3 + ph];
     dt::Smem& GS = *reinterpret_cast<dt::Smem*>(lds);
@@ -496,16 +540,29 @@ struct GroupClaims {
   __device__ void end_phase(long count) { base += count + nbx; }
 };
 
+// Block shape of the persistent kernel: the mixed path runs 256 x 256 tiles
+// with eight waves (one block per CU) and stages four canonical blocks per
+// energy round; the FP64 path 128 x 128 tiles with four waves (two blocks
+// per CU), one canonical block per round.
+template <bool MIXED>
+struct PtShape {
+  static constexpr int NT = MIXED ? 512 : 256;
+  static constexpr int NB = MIXED ? 4 : 1;
+  static constexpr int TILE = MIXED ? bf3w8::DT : 128;
+};
+
 template <bool MIXED>
 // The launch block is read through a pointer: its fields are re-read per
 // item instead of being held in registers across the GEMM loops.
-__global__ __launch_bounds__(256, 2) void k_pt_persistent(const PtRun* rp) {
+__global__ __launch_bounds__(PtShape<MIXED>::NT, MIXED ? 1 : 2) void k_pt_persistent(const PtRun* rp) {
+  constexpr int NT = PtShape<MIXED>::NT, NB = PtShape<MIXED>::NB;
   const PtRun& r = *rp;
   using WT = typename std::conditional<MIXED, float, double>::type;
-  constexpr size_t SG = MIXED ? sizeof(bf3::Smem<2>) : sizeof(dt::Smem);
-  constexpr size_t SE = sizeof(ESmem<WT>);
+  using ES_t = ESmem<WT, NB, NT>;
+  constexpr size_t SG = MIXED ? sizeof(bf3w8::Smem) : sizeof(dt::Smem);
+  constexpr size_t SE = sizeof(ES_t);
   __shared__ __attribute__((aligned(16))) unsigned char lds[SG > SE ? SG : SE];
-  ESmem<WT>& ES = *reinterpret_cast<ESmem<WT>*>(lds);
+  ES_t& ES = *reinterpret_cast<ES_t*>(lds);
   __shared__ long s_claim[1];
   cooperative_groups::grid_group grid = cooperative_groups::this_grid();
   const long nE = r.e.nE, nq = r.nq;
@@ -515,7 +572,7 @@ __global__ __launch_bounds__(256, 2) void k_pt_persistent(const PtRun* rp) {
   for (long st = 0; st < nq + 3; ++st) {
     if (st >= 4 && b0 == 0) {  // energy of triple st-4 (finished in step st-1), partials in a fixed order
       const long p = st - 4;
-      reduce_partials(r.part + (p & 3) * 2 * nE, nE, r.etrip + 2 * p, ES.red);
+      reduce_partials<NT>(r.part + (p & 3) * 2 * nE, nE, r.etrip + 2 * p, ES.red);
       __syncthreads();
     }
     const long qe = st - 3;
@@ -533,7 +590,7 @@ __global__ __launch_bounds__(256, 2) void k_pt_persistent(const PtRun* rp) {
       if (c < ne_x) {
         const int* tq = r.tri + 3 * qe;
         const WT* W = static_cast<const WT*>(r.W) + (qe & 3) * r.wslot;
-        energy_item<WT>(r.e, W, tq[0], tq[1], tq[2], gc.x + c * gc.G, r.part + (qe & 3) * 2 * nE, ES);
+        energy_item<WT, NB, NT>(r.e, W, tq[0], tq[1], tq[2], gc.x + c * gc.G, r.part + (qe & 3) * 2 * nE, ES);
       } else {
         long k = c - ne_x;
         int ph = 2;
@@ -548,7 +605,7 @@ __global__ __launch_bounds__(256, 2) void k_pt_persistent(const PtRun* rp) {
   }
   if (b0 == 0 && nq >= 1) {  // energy of the last triple
     const long p = nq - 1;
-    reduce_partials(r.part + (p & 3) * 2 * nE, nE, r.etrip + 2 * p, ES.red);
+    reduce_partials<NT>(r.part + (p & 3) * 2 * nE, nE, r.etrip + 2 * p, ES.red);
   }
 }
 
@@ -581,13 +638,13 @@ __global__ __launch_bounds__(256, 2) void k_dtile4(DMulti p, long tiles_m, long 
 template <typename WT>
 __global__ __launch_bounds__(256) void k_pt_energy_blocked(EArgs a, const WT* W, int i, int j, int k,
                                                            double* part) {
-  __shared__ ESmem<WT> S;
-  energy_item<WT>(a, W, i, j, k, blockIdx.x, part, S);
+  __shared__ ESmem<WT, 1, 256> S;
+  energy_item<WT, 1, 256>(a, W, i, j, k, blockIdx.x, part, S);
 }
 
 __global__ __launch_bounds__(256) void k_reduce_pairs(const double* part, long n, double* out) {
   __shared__ double red[2][4];
-  reduce_partials(part, n, out, red);
+  reduce_partials<256>(part, n, out, red);
 }
 
 __global__ void k_zero_u64(unsigned long long* p, int n) {
@@ -630,7 +687,7 @@ int resident_blocks() {
     CCGPU_HIP(hipGetDevice(&dev));
     hipDeviceProp_t pr;
     CCGPU_HIP(hipGetDeviceProperties(&pr, dev));
-    CCGPU_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_pt_persistent<MIXED>, 256, 0));
+    CCGPU_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_pt_persistent<MIXED>, PtShape<MIXED>::NT, 0));
     r.blocks = std::max(1, per) * pr.multiProcessorCount;
   }
   return r.blocks;
@@ -653,8 +710,9 @@ PassShape pass_shape(bool mixed, int v) {
   const long vv = (long)v * v;
   const long Ms[3] = {v, v, vv}, Ns[3] = {vv, v, v}, Bs[3] = {1, v, 1};
   for (int g = 0; g < 3; ++g) {
-    ps.tm[g] = mixed ? bf16x3_rpad(Ms[g]) / 128 : cdiv(Ms[g], 128);
-    ps.tn[g] = mixed ? bf16x3_rpad(Ns[g]) / 128 : cdiv(Ns[g], 128);
+    // mixed: 256 x 256 tiles over operands padded to 256 rows; FP64: 128 x 128
+    ps.tm[g] = mixed ? cdiv(bf16x3_rpad(Ms[g]), bf3w8::DT) : cdiv(Ms[g], 128);
+    ps.tn[g] = mixed ? cdiv(bf16x3_rpad(Ns[g]), bf3w8::DT) : cdiv(Ns[g], 128);
     ps.batch[g] = Bs[g];
     ps.ng[g] = ps.tm[g] * ps.tn[g] * Bs[g];
   }
@@ -797,12 +855,16 @@ void CCSD::pt_energy_launches(long first, long n, long stride, double* e4, doubl
   if (e5) *e5 = s5;
 }
 
-// Host side of the persistent (T): descriptors per triple, launches of at
-// most CCGPU_PT_CHUNK triples (default: about one second of work each).
+// Host side of the persistent (T): descriptors per triple, one cooperative
+// launch over the whole range (CCGPU_PT_CHUNK=n: launches of n triples).
 void CCSD::pt_energy_persistent(long first, long n, long stride, double* e4, double* e5) {
   const bool mixed = pt_mode_ == GemmPrec::BF16X3;
   const int o = dm_.o, v = dm_.v;
   const long vv = (long)v * v, vvv = vv * v;
+  // the 256 x 256 tiles read whole 256-row panels: the packed operands must
+  // be padded to 256 rows (the default BF16x3 block tile)
+  CCGPU_REQUIRE(!mixed || (bf16x3_rpad(v) % bf3w8::DT == 0 && bf16x3_rpad(vv) % bf3w8::DT == 0),
+                "persistent (T): the mixed path needs operands padded to 256 rows (CCGPU_BF16X3_VARIANT=1)");
   PtRun r{};
   r.W = mixed ? (void*)W32_.p : (void*)Wt_.p;
   r.wslot = vvv;
@@ -821,7 +883,7 @@ void CCSD::pt_energy_persistent(long first, long n, long stride, double* e4, dou
   }
   const double ftri = 12.0 * (double)vv * vv + 12.0 * (double)o * vvv;
   long chunk = env_long("CCGPU_PT_CHUNK", 0);
-  if (chunk <= 0) chunk = std::min<long>(4096, std::max<long>(8, (long)((mixed ? 150e12 : 35e12) / ftri)));
+  if (chunk <= 0) chunk = std::max<long>(n, 1);  // the whole range in one cooperative launch
   etrip_.ensure(2 * (size_t)std::max<long>(n, 1));
   const size_t dsz = pt_desc_bytes();
   const long cq = std::min(chunk, std::max<long>(n, 1));
@@ -863,7 +925,8 @@ void CCSD::pt_energy_persistent(long first, long n, long stride, double* e4, dou
       void* args[] = {(void*)&run};
       const void* fn = mixed ? reinterpret_cast<const void*>(&k_pt_persistent<true>)
                              : reinterpret_cast<const void*>(&k_pt_persistent<false>);
-      CCGPU_HIP(hipLaunchCooperativeKernel(fn, dim3((unsigned)grid), dim3(256), args, 0, st_));
+      CCGPU_HIP(hipLaunchCooperativeKernel(fn, dim3((unsigned)grid), dim3(mixed ? PtShape<true>::NT : PtShape<false>::NT),
+                                           args, 0, st_));
     }
     prof_end(id, st_);
   }

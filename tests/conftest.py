@@ -14,6 +14,22 @@ def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP path)")
 
 
+def pytest_collection_finish(session):
+    """Sessions that run GPU tests: torch's HIP runtime (the one bundled with
+    the torch wheel) and the library's (the system ROCm) are two runtimes in
+    one process.  torch's is initialised first, before the library has
+    launched anything: a first torch initialisation that came after the
+    library's GEMM, mixed and (T) tests (cooperative launches among them) has
+    found no device."""
+    if any(item.get_closest_marker("gpu") for item in session.items):
+        try:
+            import torch
+
+            torch.cuda.is_available()
+        except ImportError:
+            pass
+
+
 def load_molecule(name: str):
     """Fixture molecule -> (eps, B[P,p,q], nocc, e_scf)."""
     d = np.load(os.path.join(GOLDEN, name + ".npz"))

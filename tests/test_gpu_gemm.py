@@ -51,9 +51,10 @@ print("variant script ok", flush=True)
 
 
 def test_bf16x3_large_tile_kernel_is_bit_identical(tmp_path):
-    """The eight-wave 256x256 BF16x3 kernel (CCGPU_BF16X3_VARIANT=1, read
-    once per process) against the default 128x128 kernel, each in a process
-    of its own, on products whose padded sizes are and are not multiples of
+    """The BF16x3 products with the block tile the autotuner picks per shape
+    (256x256 or 128x128 over operands padded to 256 rows), with the cost
+    model's pick (no timing runs) and with the 128x128 tile only
+    (CCGPU_BF16X3_VARIANT=0), each in a process of its own, on products whose padded sizes are and are not multiples of
     256: identical bits (same accumulation order per element), and within
     the BF16x3 error of numpy FP64."""
     import os
@@ -64,18 +65,21 @@ def test_bf16x3_large_tile_kernel_is_bit_identical(tmp_path):
 
     script = _VARIANT_SCRIPT % (MIXED_CASES,)
     outs = {}
-    for var in ("0", "1"):
-        print(f"bf16x3 kernel variant {var}", flush=True)
-        env = dict(os.environ, CCGPU_BF16X3_VARIANT=var)
-        r = subprocess.run([sys.executable, "-c", script, ROOT, str(tmp_path / f"v{var}")], env=env,
+    # "0": the 128x128 tile only; "1": the autotuner's choice per shape (operands
+    # padded to 256); "1n": the cost model's choice, no timing runs
+    for tag, var, tune in (("0", "0", "1"), ("1", "1", "1"), ("1n", "1", "0")):
+        print(f"bf16x3 kernel variant {var} tuning {tune}", flush=True)
+        env = dict(os.environ, CCGPU_BF16X3_VARIANT=var, CCGPU_BF16X3_TUNE=tune)
+        r = subprocess.run([sys.executable, "-c", script, ROOT, str(tmp_path / f"v{tag}")], env=env,
                            capture_output=True, text=True, timeout=300)
         assert r.returncode == 0 and "variant script ok" in r.stdout, r.stdout + r.stderr
-        outs[var] = [np.load(str(tmp_path / f"v{var}_{q}.npy")) for q in range(len(MIXED_CASES))]
+        outs[tag] = [np.load(str(tmp_path / f"v{tag}_{q}.npy")) for q in range(len(MIXED_CASES))]
     for q, (ta, tb, m, n, k) in enumerate(MIXED_CASES):
         rng = np.random.default_rng(100 + q)
         A = rng.standard_normal((k, m) if ta == "T" else (m, k))
         B = rng.standard_normal((n, k) if tb == "T" else (k, n))
         C0 = rng.standard_normal((m, n))
         ref = 0.75 * (A.T if ta == "T" else A) @ (B.T if tb == "T" else B) - 1.5 * C0
-        assert np.array_equal(outs["0"][q], outs["1"][q]), (q, np.abs(outs["0"][q] - outs["1"][q]).max())
+        for tag in ("1", "1n"):
+            assert np.array_equal(outs["0"][q], outs[tag][q]), (q, tag, np.abs(outs["0"][q] - outs[tag][q]).max())
         assert np.abs(outs["1"][q] - ref).max() < 1e-5 * max(1.0, np.abs(ref).max())
