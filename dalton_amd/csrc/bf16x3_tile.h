@@ -231,18 +231,12 @@ __device__ __forceinline__ void tile(const BMulti& p, long m0, long n0, long z, 
     for (int j = 0; j < 4; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
   const int lr = lane & 15, kc = lane >> 4;
   const int lrow = lane >> 2, lslot = lane & 3;
-  // fragment offsets (elements) of this lane within one 256 x 32 array
-  int offA[8], offB[4];
-#pragma unroll
-  for (int u = 0; u < 8; ++u) {
-    const int ra = wm * 128 + u * 16 + lr;
-    offA[u] = ra * DTK + 8 * (kc ^ swz(ra));
-  }
-#pragma unroll
-  for (int u = 0; u < 4; ++u) {
-    const int rb = wn * 64 + u * 16 + lr;
-    offB[u] = rb * DTK + 8 * (kc ^ swz(rb));
-  }
+  // fragment offsets (elements) of this lane within one 256 x 32 array:
+  // fragment u of A sits at row wm*128 + 16u + lr, of B at wn*64 + 16u + lr,
+  // and swz(16q + lr) = swz(lr), so every fragment is one base offset plus
+  // a compile-time 16u rows (an immediate of the LDS read)
+  const int offA = (wm * 128 + lr) * DTK + 8 * (kc ^ swz(lr));
+  const int offB = (wn * 64 + lr) * DTK + 8 * (kc ^ swz(lr));
   for (int t = 0; t < p.nterm; ++t) {
     const BTerm& T = p.t[t];
     const long Kp = T.Kp;
@@ -260,45 +254,62 @@ __device__ __forceinline__ void tile(const BMulti& p, long m0, long n0, long z, 
     _Pragma("unroll") for (int ins = 0; ins < 2; ++ins)                                            \
       __builtin_amdgcn_global_load_lds((glb_void*)(src[a] + goff[ins] + (k0)),                     \
                                        (lds_void*)(&ST[a][(wave * 2 + ins) * 16 * DTK]), 16, 0, 0);
-#define CCGPU_W8_COMPUTE(ST)                                                                       \
+#define CCGPU_W8_FRAG(ST, a, off) (*reinterpret_cast<const bf16x8*>(&ST[a][(off)]))
+  // One K step: MFMAs on stage ST while the loads of K offset kn go to stage
+  // NX.  Right after the barrier the wave reads its B fragments and the first
+  // A pair (first the five the first four MFMAs need), then runs the eight
+  // 16-row blocks of A; the A pair of block i + 1 is read and one of the
+  // eight global_load_lds of the next stage is issued before the 12 MFMAs of
+  // block i, so the MFMA stream starts at once and the load issue hides
+  // behind it.  The scheduling-group hints keep that order.
+#define CCGPU_W8_STEP(ST, NX, kn)                                                                  \
   {                                                                                                \
-    bf16x8 bh[4], bl[4];                                                                           \
-    _Pragma("unroll") for (int u = 0; u < 4; ++u) {                                                \
-      bh[u] = *reinterpret_cast<const bf16x8*>(&ST[2][offB[u]]);                                   \
-      bl[u] = *reinterpret_cast<const bf16x8*>(&ST[3][offB[u]]);                                   \
-    }                                                                                              \
+    bf16x8 bh[4], bl[4], ah[2], al[2];                                                             \
+    al[0] = CCGPU_W8_FRAG(ST, 1, offA);                                                            \
+    _Pragma("unroll") for (int u = 0; u < 4; ++u) bh[u] = CCGPU_W8_FRAG(ST, 2, offB + u * 16 * DTK); \
+    __builtin_amdgcn_sched_group_barrier(0x100, 5, 0);                                             \
+    ah[0] = CCGPU_W8_FRAG(ST, 0, offA);                                                            \
+    _Pragma("unroll") for (int u = 0; u < 4; ++u) bl[u] = CCGPU_W8_FRAG(ST, 3, offB + u * 16 * DTK); \
+    __builtin_amdgcn_sched_group_barrier(0x100, 5, 0);                                             \
     _Pragma("unroll") for (int i = 0; i < 8; ++i) {                                                \
-      const bf16x8 ah = *reinterpret_cast<const bf16x8*>(&ST[0][offA[i]]);                         \
-      const bf16x8 al = *reinterpret_cast<const bf16x8*>(&ST[1][offA[i]]);                         \
+      const int c = i & 1;                                                                         \
+      if (i + 1 < 8) {                                                                             \
+        ah[c ^ 1] = CCGPU_W8_FRAG(ST, 0, offA + (i + 1) * 16 * DTK);                               \
+        al[c ^ 1] = CCGPU_W8_FRAG(ST, 1, offA + (i + 1) * 16 * DTK);                               \
+        __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);                                         \
+      }                                                                                            \
+      __builtin_amdgcn_global_load_lds((glb_void*)(src[i >> 1] + goff[i & 1] + (kn)),              \
+                                       (lds_void*)(&NX[i >> 1][(wave * 2 + (i & 1)) * 16 * DTK]), 16, 0, 0); \
+      __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);                                           \
       _Pragma("unroll") for (int j = 0; j < 4; ++j)                                                \
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[j], acc[i][j], 0, 0, 0);        \
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[c], bh[j], acc[i][j], 0, 0, 0);     \
       _Pragma("unroll") for (int j = 0; j < 4; ++j)                                                \
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[j], acc[i][j], 0, 0, 0);        \
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[c], bl[j], acc[i][j], 0, 0, 0);     \
       _Pragma("unroll") for (int j = 0; j < 4; ++j)                                                \
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[j], acc[i][j], 0, 0, 0);        \
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[c], bh[j], acc[i][j], 0, 0, 0);     \
+      __builtin_amdgcn_sched_group_barrier(0x008, 12, 0);                                          \
     }                                                                                              \
   }
+    // Every step issues the next stage's loads; past the last step they
+    // re-read the last K slice into the free stage (in bounds, never read),
+    // which keeps each step one basic block.  The loop ends by draining them.
     CCGPU_W8_ISSUE(s0, 0)
     for (long st = 0; st < nsteps; st += 2) {
       // even step: stage 0 (its loads are the only ones outstanding)
       __builtin_amdgcn_s_waitcnt(0);
       __syncthreads();
-      if (st + 1 < nsteps) {
-        CCGPU_W8_ISSUE(s1, (st + 1) * DTK)
-      }
-      CCGPU_W8_COMPUTE(s0)
+      CCGPU_W8_STEP(s0, s1, (st + 1 < nsteps ? st + 1 : st) * DTK)
       if (st + 1 < nsteps) {
         // odd step: stage 1
         __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
-        if (st + 2 < nsteps) {
-          CCGPU_W8_ISSUE(s0, (st + 2) * DTK)
-        }
-        CCGPU_W8_COMPUTE(s1)
+        CCGPU_W8_STEP(s1, s0, (st + 2 < nsteps ? st + 2 : st + 1) * DTK)
       }
     }
+    __builtin_amdgcn_s_waitcnt(0);
 #undef CCGPU_W8_ISSUE
-#undef CCGPU_W8_COMPUTE
+#undef CCGPU_W8_STEP
+#undef CCGPU_W8_FRAG
     __syncthreads();  // the next term's (or item's) first loads overwrite stage 0
   }
   // D[i][j] of a 16x16 f32 MFMA tile: lane = j + 16*(i/4), element i%4.
