@@ -770,7 +770,12 @@ def main():
     if fp64_leg:
         spi_d, hist_d, norm_d, prof_d = steps(d.FP64, args.fp64_steps, "fp64", True)
 
-    def roofline(prof, dt_total, peak, peak_note, attainable=None):
+    # the matrix cores' own sustained rate on random operands (no memory
+    # traffic), measured now on this card: the ceiling beside the data-sheet peak
+    mfma_only = {"bf16": d.mfma_ceiling(d.MIXED), "fp64": d.mfma_ceiling(d.FP64)}
+    log(f"MFMA-only rates on random operands: bf16 {mfma_only['bf16']:.1f}, fp64 {mfma_only['fp64']:.1f} TFLOP/s")
+
+    def roofline(prof, dt_total, peak, peak_note, attainable=None, ceiling=None):
         kern = prof.get("kernels", {})
         if not kern:  # dry run: nothing was launched
             return None
@@ -784,7 +789,9 @@ def main():
                 "avg_launch_ms": round(kd["ms"] / kd["count"], 4),
                 "share_of_step": round(kd["ms"] * 1e-3 / dt_total, 4), "peak_note": peak_note,
                 "attainable": attainable,
-                "frac_attainable": round(ach / attainable, 4) if attainable else None}
+                "frac_attainable": round(ach / attainable, 4) if attainable else None,
+                "mfma_only": round(ceiling, 1) if ceiling else None,
+                "frac_mfma_only": round(ach / ceiling, 4) if ceiling else None}
 
     F = f_ccsd(o, v, naux)
     if cpu is not None:  # headline timing done: the CPU baseline may run beside the remaining GPU legs
@@ -802,7 +809,7 @@ def main():
         fp64 = {"s_per_iter": round(spi_d, 4), "steps": args.fp64_steps, "ccsd_tflops": round(F / spi_d / 1e12, 3),
                 "energies": hist_d, "omega_norms": norm_d,
                 "roofline": roofline(prof_d, spi_d * args.fp64_steps, PEAK["fp64"], "FP64 matrix",
-                                     ATTAINABLE["fp64"]),
+                                     ATTAINABLE["fp64"], mfma_only["fp64"]),
                 "phases_ms_per_iter": {k: round(v_ / args.fp64_steps, 2) for k, v_ in prof_d.get("phases", {}).items()},
                 "kernels": kern_table_of(prof_d)}
     out = {
@@ -831,7 +838,7 @@ def main():
         if fp64_leg else None,
         "roofline": roofline(prof_m, spi_m * args.steps, round(PEAK["bf16"] / 3, 1),
                              "BF16 dense matrix peak / 3 (three bf16 products per FP32-equivalent product)",
-                             round(ATTAINABLE["bf16x3"], 1)),
+                             round(ATTAINABLE["bf16x3"], 1), mfma_only["bf16"] / 3),
         "phases_ms_per_iter": {k: round(v_ / args.steps, 2) for k, v_ in prof_m.get("phases", {}).items()},
         "kernels": kern_table_of(prof_m),
         "pt": pt or None,

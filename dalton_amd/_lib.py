@@ -63,6 +63,7 @@ SIGNATURES = {
                                    C.POINTER(C.c_longlong)]),
     "ccgpu_profile_start": (C.c_int, []),
     "ccgpu_profile_stop": (C.c_int, [C.c_char_p, C.c_int]),
+    "ccgpu_mfma_ceiling": (C.c_int, [C.c_int, _dp]),
     "ccgpu_dgemm": (C.c_int, [C.c_char, C.c_char, C.c_int, C.c_int, C.c_int, C.c_double, _dp, C.c_int,
                               _dp, C.c_int, C.c_double, _dp, C.c_int]),
 }

@@ -416,6 +416,14 @@ def profile_stop() -> dict:
     return json.loads(buf.value.decode())
 
 
+def mfma_ceiling(precision: int = MIXED) -> float:
+    """Sustained matrix-core rate of this GPU in TFLOP/s: an MFMA-only loop on
+    random operands (MIXED: the bf16 16x16x32 instruction, FP64: 16x16x4)."""
+    t = C.c_double()
+    check(lib().ccgpu_mfma_ceiling(precision, C.byref(t)))
+    return t.value
+
+
 def dgemm(transa: str, transb: str, alpha: float, A: np.ndarray, B: np.ndarray, beta: float = 0.0,
           Cm: np.ndarray | None = None, precision: int = FP64) -> np.ndarray:
     """BLAS DGEMM on the MFMA kernel; arrays are Fortran-ordered matrices."""

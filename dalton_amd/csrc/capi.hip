@@ -486,6 +486,16 @@ int dgemm_impl(ccgpu::GemmPrec prec, char transa, char transb, int m, int n, int
 
 }  // namespace
 
+int ccgpu_mfma_ceiling(int precision, double* tflops) {
+  return guarded([&] {
+    CCGPU_REQUIRE(tflops, "ccgpu_mfma_ceiling: null argument");
+    CCGPU_REQUIRE(precision == CCGPU_FP64 || precision == CCGPU_MIXED,
+                  "ccgpu_mfma_ceiling: precision must be CCGPU_FP64 or CCGPU_MIXED");
+    CCGPU_REQUIRE(!dryrun_env(), "ccgpu_mfma_ceiling: nothing to measure in a dry run");
+    *tflops = ccgpu::mfma_only_tflops(precision == CCGPU_FP64);
+  });
+}
+
 int ccgpu_dgemm(char transa, char transb, int m, int n, int k, double alpha, const double* a, int lda,
                 const double* b, int ldb, double beta, double* c, int ldc) {
   return dgemm_impl(ccgpu::GemmPrec::F64, transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc);

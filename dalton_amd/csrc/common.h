@@ -197,6 +197,11 @@ void bf16x3_run(const BMulti& p, int batch, bool c_float, double flops, hipStrea
 constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
 void gemm_bf16x3_noalloc(const GemmDesc& d, hipStream_t s);
 
+// Sustained matrix-core rate (TFLOP/s) of an MFMA-only loop with random
+// operands, four waves per SIMD (mfma_ceiling.hip): bf16 16x16x32 or, with
+// f64, FP64 16x16x4.
+double mfma_only_tflops(bool f64);
+
 // MO density-fitting factor of AO integrals by pivoted Cholesky (cholesky.hip):
 // returns naux x nmo x nmo values; maxvec <= 0 allows one vector per AO pair.
 std::vector<double> df_factor_ao(int nbas, int nmo, const double* cmo, const double* eri_ao, double thr, long maxvec,

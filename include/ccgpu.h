@@ -332,6 +332,14 @@ int ccgpu_pt_partial(ccgpu_ctx* ctx, int precision, long long first, long long c
 int ccgpu_profile_start(void);
 int ccgpu_profile_stop(char* buf, int len);
 
+/* Sustained matrix-core rate of this GPU, the ceiling the MFMA kernels are
+ * read against (no counterpart in the reference: a measurement helper).
+ * Runs an MFMA-only loop (no memory traffic) for about 20 ms, four waves per
+ * SIMD, on operands with random bits, and returns its rate in TFLOP/s:
+ * CCGPU_MIXED the bf16 16x16x32 instruction (each BF16x3 product needs three),
+ * CCGPU_FP64 the FP64 16x16x4 one.  Not available in a dry run. */
+int ccgpu_mfma_ceiling(int precision, double* tflops);
+
 /* DGEMM with BLAS semantics (column-major, host arrays), the operation
  * Dalton's hot path spends its time in (pdpack/gp_blas.F DGEMM, e.g. the
  * ladder DGEMMs at cc/ccsd_rhs.F:9589-9597), run on the FP64 MFMA kernel. */

@@ -179,3 +179,16 @@ def test_dryrun_ao_column_front_end():
                        text=True, timeout=300)
     assert r.returncode == 0, r.stdout + r.stderr
     assert r.stdout.count("ok") == 3
+
+
+def test_dryrun_mfma_ceiling_refuses():
+    """The matrix-core rate probe launches kernels: a dry run reports an error
+    instead of a rate."""
+    script = (f"import ctypes as C, sys; sys.path.insert(0, {ROOT!r}); from dalton_amd import _lib; "
+              "L = _lib.lib(); t = C.c_double(-1.0); rc = L.ccgpu_mfma_ceiling(1, C.byref(t)); "
+              "print(rc, t.value, L.ccgpu_last_error().decode())")
+    env = dict(os.environ, CCGPU_DRYRUN="1")
+    r = subprocess.run([sys.executable, "-c", script], env=env, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    rc, val, msg = r.stdout.split(" ", 2)
+    assert int(rc) != 0 and float(val) == -1.0 and "dry run" in msg

@@ -213,3 +213,17 @@ def test_mixed_ccsd_t_within_1uEh_convergent_synthetic_o32_v256():
     dtot = abs(out["mixed"][0] + out["mixed"][1] - out["fp64"][0] - out["fp64"][1])
     print(f"  |dE_CCSD| = {dccsd:.2e}  |dE_(T)| = {dt:.2e}  |dE_CCSD(T)| = {dtot:.2e}", flush=True)
     assert dtot < 1e-6
+
+
+@pytest.mark.gpu
+def test_mfma_ceiling_is_a_rate_below_the_data_sheet_peak():
+    """The MFMA-only probe returns a positive sustained rate below MI355X's
+    dense peaks (bf16 2500, FP64 78.6 TFLOP/s), and above the rate the
+    library's own GEMMs reach on this card (a floor that catches a probe
+    issuing far fewer MFMAs than it counts)."""
+    import dalton_amd as d
+
+    bf16 = d.mfma_ceiling(d.MIXED)
+    f64 = d.mfma_ceiling(d.FP64)
+    assert 600.0 < bf16 < 2500.0
+    assert 30.0 < f64 < 78.6
