@@ -33,7 +33,7 @@ PEAK = {"fp64": 78.6, "fp32": 157.3, "bf16": 2500.0}  # MI355X dense matrix TFLO
 # from two rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE) scaled by the same
 # counters read on a copy of known size; the summaries and commands are in
 # profiles/gemm_bf16x3_o64_v512_traffic.md and profiles/dgemm_o64_v512_traffic.md.
-TRAFFIC_BYTES = {"k_gemm_bf16x3": 101.6e9, "k_dgemm<0,1,false,128>": 1.4979e12}
+TRAFFIC_BYTES = {"k_gemm_bf16x3": 59.0e9, "k_dgemm<0,1,false,128>": 1.4979e12}
 
 # Attainable dense rates of the vendor BLAS on this card (torch.matmul 8192^3,
 # profiles/vendor_blas_attainable_rates.md): bf16 1086, fp64 71.5 TFLOP/s.
