@@ -193,8 +193,8 @@ __device__ __forceinline__ void tile(const BMulti& p, long m0, long n0, long z, 
 // row r sits in slot c ^ ((r >> 1) & 3), so the 16-byte fragment reads of
 // consecutive lanes fall on distinct bank groups.  One wave instruction fills
 // 1 KiB = 16 rows; per stage each wave issues two per array (hi and lo of A
-// and B).  The two stages are separate arrays (64 KiB each) and the K loop is
-// unrolled by two, so every LDS access names its stage statically.  The
+// and B).  The two stages (64 KiB each) alternate through one base address
+// per step, and the K steps of all terms form one pipeline.  The
 // accumulation order of every output element (k ascending; lo*hi, hi*lo,
 // hi*hi per k-slice of 32) is that of the 128 x 128 tile above, so the
 // results are bit-identical.  Operand rows m0 .. m0 + 255 and n0 .. n0 + 255
@@ -237,81 +237,107 @@ __device__ __forceinline__ void tile(const BMulti& p, long m0, long n0, long z, 
   // a compile-time 16u rows (an immediate of the LDS read)
   const int offA = (wm * 128 + lr) * DTK + 8 * (kc ^ swz(lr));
   const int offB = (wn * 64 + lr) * DTK + 8 * (kc ^ swz(lr));
-  for (int t = 0; t < p.nterm; ++t) {
-    const BTerm& T = p.t[t];
+  unsigned short* const sbase = &S.s0[0][0];  // stage 1 follows stage 0
+  constexpr int SARR = DT * DTK, SSTG = 4 * SARR;  // elements per array, per stage
+  // One pipeline over the K steps of all terms in order: the step before a
+  // term's first issues that term's loads, so a term boundary costs neither
+  // a barrier nor an exposed load.  The load cursor (term lt, step ls) holds
+  // this lane's sources of the four arrays for the next loads; within a term
+  // it advances by one K slice, at a term boundary it reads the next term.
+  const unsigned short* src[4];
+  long goff[2];
+  auto point = [&](int tl) {
+    const BTerm& T = p.t[tl];
     const long Kp = T.Kp;
-    const unsigned short* src[4] = {T.Ah + z * T.a_bs + m0 * Kp, T.Al + z * T.a_bs + m0 * Kp,
-                                    T.Bh + z * T.b_bs + n0 * Kp, T.Bl + z * T.b_bs + n0 * Kp};
-    long goff[2];
+    src[0] = T.Ah + z * T.a_bs + m0 * Kp;
+    src[1] = T.Al + z * T.a_bs + m0 * Kp;
+    src[2] = T.Bh + z * T.b_bs + n0 * Kp;
+    src[3] = T.Bl + z * T.b_bs + n0 * Kp;
 #pragma unroll
     for (int ins = 0; ins < 2; ++ins) {
       const int row = (wave * 2 + ins) * 16 + lrow;
       goff[ins] = (long)row * Kp + 8 * (lslot ^ swz(row));
     }
-    const long nsteps = Kp / DTK;
-#define CCGPU_W8_ISSUE(ST, k0)                                                                      \
-  _Pragma("unroll") for (int a = 0; a < 4; ++a)                                                    \
-    _Pragma("unroll") for (int ins = 0; ins < 2; ++ins)                                            \
-      __builtin_amdgcn_global_load_lds((glb_void*)(src[a] + goff[ins] + (k0)),                     \
-                                       (lds_void*)(&ST[a][(wave * 2 + ins) * 16 * DTK]), 16, 0, 0);
-#define CCGPU_W8_FRAG(ST, a, off) (*reinterpret_cast<const bf16x8*>(&ST[a][(off)]))
-  // One K step: MFMAs on stage ST while the loads of K offset kn go to stage
-  // NX.  Right after the barrier the wave reads its B fragments and the first
-  // A pair (first the five the first four MFMAs need), then runs the eight
-  // 16-row blocks of A; the A pair of block i + 1 is read and one of the
-  // eight global_load_lds of the next stage is issued before the 12 MFMAs of
-  // block i, so the MFMA stream starts at once and the load issue hides
-  // behind it.  The scheduling-group hints keep that order.
-#define CCGPU_W8_STEP(ST, NX, kn)                                                                  \
-  {                                                                                                \
-    bf16x8 bh[4], bl[4], ah[2], al[2];                                                             \
-    al[0] = CCGPU_W8_FRAG(ST, 1, offA);                                                            \
-    _Pragma("unroll") for (int u = 0; u < 4; ++u) bh[u] = CCGPU_W8_FRAG(ST, 2, offB + u * 16 * DTK); \
-    __builtin_amdgcn_sched_group_barrier(0x100, 5, 0);                                             \
-    ah[0] = CCGPU_W8_FRAG(ST, 0, offA);                                                            \
-    _Pragma("unroll") for (int u = 0; u < 4; ++u) bl[u] = CCGPU_W8_FRAG(ST, 3, offB + u * 16 * DTK); \
-    __builtin_amdgcn_sched_group_barrier(0x100, 5, 0);                                             \
-    _Pragma("unroll") for (int i = 0; i < 8; ++i) {                                                \
-      const int c = i & 1;                                                                         \
-      if (i + 1 < 8) {                                                                             \
-        ah[c ^ 1] = CCGPU_W8_FRAG(ST, 0, offA + (i + 1) * 16 * DTK);                               \
-        al[c ^ 1] = CCGPU_W8_FRAG(ST, 1, offA + (i + 1) * 16 * DTK);                               \
-        __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);                                         \
-      }                                                                                            \
-      __builtin_amdgcn_global_load_lds((glb_void*)(src[i >> 1] + goff[i & 1] + (kn)),              \
-                                       (lds_void*)(&NX[i >> 1][(wave * 2 + (i & 1)) * 16 * DTK]), 16, 0, 0); \
-      __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);                                           \
-      _Pragma("unroll") for (int j = 0; j < 4; ++j)                                                \
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[c], bh[j], acc[i][j], 0, 0, 0);     \
-      _Pragma("unroll") for (int j = 0; j < 4; ++j)                                                \
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[c], bl[j], acc[i][j], 0, 0, 0);     \
-      _Pragma("unroll") for (int j = 0; j < 4; ++j)                                                \
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[c], bh[j], acc[i][j], 0, 0, 0);     \
-      __builtin_amdgcn_sched_group_barrier(0x008, 12, 0);                                          \
-    }                                                                                              \
-  }
-    // Every step issues the next stage's loads; past the last step they
-    // re-read the last K slice into the free stage (in bounds, never read),
-    // which keeps each step one basic block.  The loop ends by draining them.
-    CCGPU_W8_ISSUE(s0, 0)
-    for (long st = 0; st < nsteps; st += 2) {
-      // even step: stage 0 (its loads are the only ones outstanding)
-      __builtin_amdgcn_s_waitcnt(0);
-      __syncthreads();
-      CCGPU_W8_STEP(s0, s1, (st + 1 < nsteps ? st + 1 : st) * DTK)
-      if (st + 1 < nsteps) {
-        // odd step: stage 1
-        __builtin_amdgcn_s_waitcnt(0);
-        __syncthreads();
-        CCGPU_W8_STEP(s1, s0, (st + 2 < nsteps ? st + 2 : st + 1) * DTK)
-      }
+  };
+  long total = 0;  // K steps of all terms
+  for (int tl = 0; tl < p.nterm; ++tl) total += p.t[tl].Kp / DTK;
+  int lt = 0;
+  while (lt < p.nterm && p.t[lt].Kp < DTK) ++lt;
+  long ls = 0, lsteps = lt < p.nterm ? p.t[lt].Kp / DTK : 0;
+  // past the last step the cursor stays put: the loads re-read the last K
+  // slice into the free stage (in bounds, never read)
+  auto advance = [&]() {
+    if (ls + 1 < lsteps) {
+      ++ls;
+#pragma unroll
+      for (int a = 0; a < 4; ++a) src[a] += DTK;
+      return;
     }
-    __builtin_amdgcn_s_waitcnt(0);
-#undef CCGPU_W8_ISSUE
-#undef CCGPU_W8_STEP
-#undef CCGPU_W8_FRAG
-    __syncthreads();  // the next term's (or item's) first loads overwrite stage 0
+    int nt = lt + 1;
+    while (nt < p.nterm && p.t[nt].Kp < DTK) ++nt;
+    if (nt < p.nterm) {
+      lt = nt;
+      ls = 0;
+      lsteps = p.t[lt].Kp / DTK;
+      point(lt);
+    }
+  };
+#define CCGPU_W8_LD(stg, a, ins)                                                                   \
+  __builtin_amdgcn_global_load_lds((glb_void*)(src[a] + goff[ins]),                                \
+                                   (lds_void*)((stg) + (a) * SARR + (wave * 2 + (ins)) * 16 * DTK), 16, 0, 0);
+#define CCGPU_W8_FRAG(stg, a, off) (*reinterpret_cast<const bf16x8*>((stg) + (a) * SARR + (off)))
+  if (total > 0) {
+    point(lt);
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int ins = 0; ins < 2; ++ins) CCGPU_W8_LD(sbase, a, ins)
+    advance();
+    for (long g = 0; g < total; ++g) {
+      const unsigned short* cur = sbase + (g & 1) * SSTG;
+      unsigned short* nxt = sbase + ((g + 1) & 1) * SSTG;
+      __builtin_amdgcn_s_waitcnt(0);  // this step's loads (the only ones outstanding)
+      __syncthreads();
+      // Right after the barrier the wave reads its B fragments and the first
+      // A pair (first the five the first four MFMAs need), then runs the
+      // eight 16-row blocks of A; the A pair of block i + 1 is read and one
+      // of the eight global_load_lds of the next step is issued before the
+      // 12 MFMAs of block i, so the MFMA stream starts at once and the load
+      // issue hides behind it.  The scheduling-group hints keep that order.
+      bf16x8 bh[4], bl[4], ah[2], al[2];
+      al[0] = CCGPU_W8_FRAG(cur, 1, offA);
+#pragma unroll
+      for (int u = 0; u < 4; ++u) bh[u] = CCGPU_W8_FRAG(cur, 2, offB + u * 16 * DTK);
+      __builtin_amdgcn_sched_group_barrier(0x100, 5, 0);
+      ah[0] = CCGPU_W8_FRAG(cur, 0, offA);
+#pragma unroll
+      for (int u = 0; u < 4; ++u) bl[u] = CCGPU_W8_FRAG(cur, 3, offB + u * 16 * DTK);
+      __builtin_amdgcn_sched_group_barrier(0x100, 5, 0);
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        const int c = i & 1;
+        if (i + 1 < 8) {
+          ah[c ^ 1] = CCGPU_W8_FRAG(cur, 0, offA + (i + 1) * 16 * DTK);
+          al[c ^ 1] = CCGPU_W8_FRAG(cur, 1, offA + (i + 1) * 16 * DTK);
+          __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);
+        }
+        CCGPU_W8_LD(nxt, i >> 1, i & 1)
+        __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[c], bh[j], acc[i][j], 0, 0, 0);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[c], bl[j], acc[i][j], 0, 0, 0);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[c], bh[j], acc[i][j], 0, 0, 0);
+        __builtin_amdgcn_sched_group_barrier(0x008, 12, 0);
+      }
+      advance();
+    }
+    __builtin_amdgcn_s_waitcnt(0);  // the loads past the last step
+    __syncthreads();                // the caller's next tile reuses the LDS
   }
+#undef CCGPU_W8_LD
+#undef CCGPU_W8_FRAG
   // D[i][j] of a 16x16 f32 MFMA tile: lane = j + 16*(i/4), element i%4.
   // A descriptor read from memory (PIN) names its output through a global
   // pointer; a kernel argument's pointer is used as it is.
