@@ -13,9 +13,10 @@
 // and ends with one grid-wide barrier (cooperative groups), so every item
 // depends only on items of earlier steps, and a triple costs one barrier.
 // Items of a step are claimed dynamically, from one counter per XCD group
-// that is never waited on; each group takes its energy items first (long
-// items, started early) and then its ranges of the G3, G2 and G1 tiles
-// (neighbouring tiles share operand panels in the XCD's L2).  The partial
+// that is never waited on; each group takes its range of energy items first
+// (long items, started early) and then its ranges of the G3, G2 and G1 tiles
+// (neighbouring tiles share operand panels, neighbouring canonical blocks W
+// cache lines, in the XCD's L2).  The partial
 // energies of a triple are summed in a fixed order by block 0 after the
 // barrier that completes them.  Every block runs every barrier: the loop
 // bounds depend only on the launch, and the runtime refuses a cooperative
@@ -577,20 +578,21 @@ __global__ __launch_bounds__(PtShape<MIXED>::NT, MIXED ? 1 : 2) void k_pt_persis
     }
     const long qe = st - 3;
     const long ne = (qe >= 0 && qe < nq) ? nE : 0;
-    const long ne_x = ne > gc.x ? (ne - gc.x + gc.G - 1) / gc.G : 0;
+    const long ne_x = gc.hi(ne) - gc.lo(ne);
     long cnt[3];
     for (int ph = 0; ph < 3; ++ph) {
       const long q = st - ph;
       const long T = (q >= 0 && q < nq) ? r.ng[ph] : 0;
       cnt[ph] = gc.hi(T) - gc.lo(T);
     }
-    // this group's list: its energy items (x, x + G, ...; long, claimed first), then its tiles of G3, G2, G1
+    // this group's list: its range of energy items (long, claimed first; neighbouring canonical blocks
+    // share W cache lines in the XCD's L2), then its ranges of the G3, G2 and G1 tiles
     const long n1 = ne_x + cnt[2] + cnt[1] + cnt[0];
     for (long c; (c = gc.next(n1, s_claim)) >= 0;) {
       if (c < ne_x) {
         const int* tq = r.tri + 3 * qe;
         const WT* W = static_cast<const WT*>(r.W) + (qe & 3) * r.wslot;
-        energy_item<WT, NB, NT>(r.e, W, tq[0], tq[1], tq[2], gc.x + c * gc.G, r.part + (qe & 3) * 2 * nE, ES);
+        energy_item<WT, NB, NT>(r.e, W, tq[0], tq[1], tq[2], gc.lo(ne) + c, r.part + (qe & 3) * 2 * nE, ES);
       } else {
         long k = c - ne_x;
         int ph = 2;
@@ -635,11 +637,16 @@ __global__ __launch_bounds__(256, 2) void k_dtile4(DMulti p, long tiles_m, long 
     dt::tile4<X, true>(p, m0, n0, blockIdx.z, S);
 }
 
+// Item of block b: each XCD (block b runs on XCD b % 8) gets a contiguous
+// range of items, whose canonical blocks are neighbours along C and share W
+// cache lines in that XCD's L2 (the item's blocks, and so its partial, do not
+// depend on the mapping).
 template <typename WT>
 __global__ __launch_bounds__(256) void k_pt_energy_blocked(EArgs a, const WT* W, int i, int j, int k,
                                                            double* part) {
   __shared__ ESmem<WT, 1, 256> S;
-  energy_item<WT, 1, 256>(a, W, i, j, k, blockIdx.x, part, S);
+  const long b = blockIdx.x, per = (long)gridDim.x / 8, full = per * 8;
+  energy_item<WT, 1, 256>(a, W, i, j, k, (b < full) ? (b % 8) * per + b / 8 : b, part, S);
 }
 
 __global__ __launch_bounds__(256) void k_reduce_pairs(const double* part, long n, double* out) {
