@@ -162,9 +162,23 @@ __device__ __forceinline__ void tile(const BMulti& p, long m0, long n0, long z, 
 #undef CCGPU_BF3_STORE
   __syncthreads();  // the caller's next tile reuses the LDS
   // D[i][j] of a 16x16 f32 MFMA tile: lane = j + 16*(i/4), element i%4.
+  // Per 16-row block: the block's C values are read (beta != 0) before any
+  // of its stores, so their loads overlap instead of each waiting behind the
+  // previous element's store.
   CCGPU_GLOBAL OutT* C = gptr(static_cast<OutT*>(p.C) + z * p.c_b);
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < 4; ++i) {
+    OutT cv[4][4] = {};
+    if (p.beta != 0.0) {  // uniform; indices clamped into C, so the loads need no per-element branch
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const long m = min(m0 + wm * 64 + i * 16 + (lane >> 4) * 4 + r, p.M - 1);
+          const long n = min(n0 + wn * 64 + j * 16 + lr, p.N - 1);
+          cv[j][r] = C[m * p.c_m + n * p.c_n];
+        }
+    }
 #pragma unroll
     for (int j = 0; j < 4; ++j)
 #pragma unroll
@@ -172,12 +186,12 @@ __device__ __forceinline__ void tile(const BMulti& p, long m0, long n0, long z, 
         const long m = m0 + wm * 64 + i * 16 + (lane >> 4) * 4 + r;
         const long n = n0 + wn * 64 + j * 16 + lr;
         if (m < p.M && n < p.N) {
-          CCGPU_GLOBAL OutT* q = C + m * p.c_m + n * p.c_n;
           double v = p.alpha * (double)acc[i][j][r];
-          if (p.beta != 0.0) v += p.beta * (double)*q;
-          *q = static_cast<OutT>(v);
+          if (p.beta != 0.0) v += p.beta * (double)cv[j][r];
+          C[m * p.c_m + n * p.c_n] = static_cast<OutT>(v);
         }
       }
+  }
 }
 
 }  // namespace bf3
@@ -347,8 +361,22 @@ __device__ __forceinline__ void tile(const BMulti& p, long m0, long n0, long z, 
   const OutP C = (OutP)(static_cast<OutT*>(p.C) + z * p.c_b);
   const long M = p.M, N = p.N, c_m = p.c_m, c_n = p.c_n;
   const double alpha = p.alpha, beta = p.beta;
+  // Per 16-row block: the block's C values are read (beta != 0) before any
+  // of its stores, so their loads overlap instead of each waiting behind the
+  // previous element's store.
 #pragma unroll
-  for (int i = 0; i < 8; ++i)
+  for (int i = 0; i < 8; ++i) {
+    OutT cv[4][4] = {};
+    if (beta != 0.0) {  // uniform; indices clamped into C, so the loads need no per-element branch
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const long m = min(m0 + wm * 128 + i * 16 + (lane >> 4) * 4 + r, M - 1);
+          const long n = min(n0 + wn * 64 + j * 16 + lr, N - 1);
+          cv[j][r] = C[m * c_m + n * c_n];
+        }
+    }
 #pragma unroll
     for (int j = 0; j < 4; ++j)
 #pragma unroll
@@ -356,12 +384,12 @@ __device__ __forceinline__ void tile(const BMulti& p, long m0, long n0, long z, 
         const long m = m0 + wm * 128 + i * 16 + (lane >> 4) * 4 + r;
         const long n = n0 + wn * 64 + j * 16 + lr;
         if (m < M && n < N) {
-          OutP q = C + m * c_m + n * c_n;
           double v = alpha * (double)acc[i][j][r];
-          if (beta != 0.0) v += beta * (double)*q;
-          *q = static_cast<OutT>(v);
+          if (beta != 0.0) v += beta * (double)cv[j][r];
+          C[m * c_m + n * c_n] = static_cast<OutT>(v);
         }
       }
+  }
 }
 
 }  // namespace bf3w8

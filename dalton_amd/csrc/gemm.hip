@@ -205,9 +205,22 @@ __global__ __launch_bounds__(256, 2) void k_dgemm(GemmDesc d, long kchunk, doubl
           }
       return;
     }
+    // per 16-row block: C read (beta != 0) before any of the block's stores,
+    // so the loads overlap instead of each waiting behind a store
     double* C = static_cast<double*>(d.C) + (long)blockIdx.z * d.c_b;
   #pragma unroll
-    for (int i = 0; i < IW; ++i)
+    for (int i = 0; i < IW; ++i) {
+      double cv[IW][4] = {};
+      if (d.beta != 0.0) {  // uniform; indices clamped into C: no per-element branch
+  #pragma unroll
+        for (int j = 0; j < IW; ++j)
+  #pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const long m = min(m0 + wm * (T / 2) + i * 16 + r * 4 + lk, d.M - 1);
+            const long n = min(n0 + wn * (T / 2) + j * 16 + lr, d.N - 1);
+            cv[j][r] = C[m * d.c_m + n * d.c_n];
+          }
+      }
   #pragma unroll
       for (int j = 0; j < IW; ++j)
   #pragma unroll
@@ -215,12 +228,12 @@ __global__ __launch_bounds__(256, 2) void k_dgemm(GemmDesc d, long kchunk, doubl
           const long m = m0 + wm * (T / 2) + i * 16 + r * 4 + lk;
           const long n = n0 + wn * (T / 2) + j * 16 + lr;
           if (m < d.M && n < d.N) {
-            double* p = C + m * d.c_m + n * d.c_n;
             double v = d.alpha * acc[i][j][r];
-            if (d.beta != 0.0) v += d.beta * *p;
-            *p = v;
+            if (d.beta != 0.0) v += d.beta * cv[j][r];
+            C[m * d.c_m + n * d.c_n] = v;
           }
         }
+    }
   };
   if (inside)
     run(std::false_type{});

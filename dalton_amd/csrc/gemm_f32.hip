@@ -161,9 +161,22 @@ __global__ __launch_bounds__(256, 2) void k_gemm_f32(FGemm p) {
         }
   }
   // D[i][j] of a 16x16 f32 MFMA tile: lane = j + 16*(i/4), element i%4.
+  // per 16-row block: C read (beta != 0) before any of the block's stores,
+  // so the loads overlap instead of each waiting behind a store
   double* C = p.C + z * p.c_b;
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < 4; ++i) {
+    double cv[4][4] = {};
+    if (p.beta != 0.0) {  // uniform; indices clamped into C: no per-element branch
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const long m = min(m0 + wm * 64 + i * 16 + (lane >> 4) * 4 + r, p.M - 1);
+          const long n = min(n0 + wn * 64 + j * 16 + lr, p.N - 1);
+          cv[j][r] = C[m * p.c_m + n * p.c_n];
+        }
+    }
 #pragma unroll
     for (int j = 0; j < 4; ++j)
 #pragma unroll
@@ -171,12 +184,12 @@ __global__ __launch_bounds__(256, 2) void k_gemm_f32(FGemm p) {
         const long m = m0 + wm * 64 + i * 16 + (lane >> 4) * 4 + r;
         const long n = n0 + wn * 64 + j * 16 + lr;
         if (m < p.M && n < p.N) {
-          double* q = C + m * p.c_m + n * p.c_n;
           double v = p.alpha * (double)acc[i][j][r];
-          if (p.beta != 0.0) v += p.beta * *q;
-          *q = v;
+          if (p.beta != 0.0) v += p.beta * cv[j][r];
+          C[m * p.c_m + n * p.c_n] = v;
         }
       }
+  }
 }
 
 long f32_rpad(long R) { return (R + FT - 1) / FT * FT; }

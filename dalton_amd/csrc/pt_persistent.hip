@@ -204,9 +204,22 @@ __device__ __forceinline__ void tile4(const DMulti& p, long m0, long n0, long z,
   // D[i][j] of a 16x16 f64 MFMA tile: lane = j + 16*(i%4), element i/4.
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wm = wave >> 1, wn = wave & 1, lr = lane & 15, lk = lane >> 4;
+  // per 16-row block: C read (beta != 0) before any of the block's stores,
+  // so the loads overlap instead of each waiting behind a store
   CCGPU_GLOBAL double* C = gptr(p.C + z * p.c_b);
 #pragma unroll
-  for (int i = 0; i < IW; ++i)
+  for (int i = 0; i < IW; ++i) {
+    double cv[IW][4] = {};
+    if (p.beta != 0.0) {  // uniform; indices clamped into C: no per-element branch
+#pragma unroll
+      for (int j = 0; j < IW; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const long m = min(m0 + wm * 64 + i * 16 + r * 4 + lk, p.M - 1);
+          const long n = min(n0 + wn * 64 + j * 16 + lr, p.N - 1);
+          cv[j][r] = C[m * p.c_m + n * p.c_n];
+        }
+    }
 #pragma unroll
     for (int j = 0; j < IW; ++j)
 #pragma unroll
@@ -214,12 +227,12 @@ __device__ __forceinline__ void tile4(const DMulti& p, long m0, long n0, long z,
         const long m = m0 + wm * 64 + i * 16 + r * 4 + lk;
         const long n = n0 + wn * 64 + j * 16 + lr;
         if (m < p.M && n < p.N) {
-          CCGPU_GLOBAL double* q = C + m * p.c_m + n * p.c_n;
           double v = acc[i][j][r];
-          if (p.beta != 0.0) v += p.beta * *q;
-          *q = v;
+          if (p.beta != 0.0) v += p.beta * cv[j][r];
+          C[m * p.c_m + n * p.c_n] = v;
         }
       }
+  }
 }
 }  // namespace dt
 
