@@ -333,68 +333,116 @@ __device__ __forceinline__ void energy_item(const EArgs& r, const WT* W_, int i,
       for (int q = 1; q < NB; ++q) c = (u == q) ? codes[q] : c;
       return c;
     };
-    __syncthreads();  // the previous round's LDS reads are done
+    // Every global load of the round is issued into registers first (fixed
+    // trip counts, fully unrolled), then written to LDS after the barrier
+    // that ends the previous round's reads: the loads overlap each other and
+    // that round's tail, instead of each waiting before its LDS store.
     // six W block permutations per block: 384 rows of 8
-    for (int row = tid; row < NB * 384; row += NT) {
-      const int u = row / 384, rr = row - u * 384;
-      const int code = code_of(u);
-      const int b0 = code & 1023, b1 = (code >> 10) & 1023, b2 = (code >> 20) & 1023;
-      auto blk = [&](int q) { return q == 0 ? b0 : (q == 1 ? b1 : b2); };  // selects, no indexed local array
-      const int p = rr >> 6, x8 = (rr >> 3) & 7, y8 = rr & 7;
-      const int px = p >> 1, lo = px == 0 ? 1 : 0, hi = px == 2 ? 1 : 2;
-      const int py = (p & 1) ? hi : lo, pz = (p & 1) ? lo : hi;
-      const long gx = blk(px) * 8 + x8, gy = blk(py) * 8 + y8, z0 = blk(pz) * 8;
-      WT* dst = &S.w[u][p][x8 * 72 + y8 * 9];
-      if (code >= 0 && gx < v && gy < v) {
-        const CCGPU_GLOBAL WT* src = W + gx * vv + gy * v + z0;
-        if (vec && z0 + 8 <= v) {
-          if (sizeof(WT) == 4) {
-            const f4 a = *reinterpret_cast<const CCGPU_GLOBAL f4*>(src);
-            const f4 b = *reinterpret_cast<const CCGPU_GLOBAL f4*>(src + 4);
+    constexpr int WIT = (NB * 384 + NT - 1) / NT, JIT = (NB * 18 * 64 + NT - 1) / NT, TIT = (NB * 72 + NT - 1) / NT;
+    WT wv[WIT][8];
 #pragma unroll
-            for (int h = 0; h < 4; ++h) {
-              dst[h] = (WT)a[h];
-              dst[4 + h] = (WT)b[h];
+    for (int it = 0; it < WIT; ++it) {
+      const int row = tid + it * NT;
+#pragma unroll
+      for (int z = 0; z < 8; ++z) wv[it][z] = (WT)0;
+      if (row < NB * 384) {
+        const int u = row / 384, rr = row - u * 384;
+        const int code = code_of(u);
+        const int b0 = code & 1023, b1 = (code >> 10) & 1023, b2 = (code >> 20) & 1023;
+        auto blk = [&](int q) { return q == 0 ? b0 : (q == 1 ? b1 : b2); };  // selects, no indexed local array
+        const int p = rr >> 6, x8 = (rr >> 3) & 7, y8 = rr & 7;
+        const int px = p >> 1, lo = px == 0 ? 1 : 0, hi = px == 2 ? 1 : 2;
+        const int py = (p & 1) ? hi : lo, pz = (p & 1) ? lo : hi;
+        const long gx = blk(px) * 8 + x8, gy = blk(py) * 8 + y8, z0 = blk(pz) * 8;
+        if (code >= 0 && gx < v && gy < v) {
+          const CCGPU_GLOBAL WT* src = W + gx * vv + gy * v + z0;
+          if (vec && z0 + 8 <= v) {
+            if (sizeof(WT) == 4) {
+              const f4 x0 = *reinterpret_cast<const CCGPU_GLOBAL f4*>(src);
+              const f4 x1 = *reinterpret_cast<const CCGPU_GLOBAL f4*>(src + 4);
+#pragma unroll
+              for (int h = 0; h < 4; ++h) {
+                wv[it][h] = (WT)x0[h];
+                wv[it][4 + h] = (WT)x1[h];
+              }
+            } else {
+#pragma unroll
+              for (int h = 0; h < 4; ++h) {
+                const d2 x = *reinterpret_cast<const CCGPU_GLOBAL d2*>(src + 2 * h);
+                wv[it][2 * h] = (WT)x[0];
+                wv[it][2 * h + 1] = (WT)x[1];
+              }
             }
           } else {
 #pragma unroll
-            for (int h = 0; h < 4; ++h) {
-              const d2 a = *reinterpret_cast<const CCGPU_GLOBAL d2*>(src + 2 * h);
-              dst[2 * h] = (WT)a[0];
-              dst[2 * h + 1] = (WT)a[1];
-            }
+            for (int z = 0; z < 8; ++z) wv[it][z] = (z0 + z < v) ? src[z] : (WT)0;
           }
-        } else {
-#pragma unroll
-          for (int z = 0; z < 8; ++z) dst[z] = (z0 + z < v) ? src[z] : (WT)0;
         }
-      } else {
-#pragma unroll
-        for (int z = 0; z < 8; ++z) dst[z] = (WT)0;
       }
     }
     // (ia|jb) blocks: m = 0 J_jk[y][z], 1 J_ik[x][z], 2 J_ij[x][y]; pair (u, w) of positions
-    for (int q = tid; q < NB * 18 * 64; q += NT) {
-      const int u = q / (18 * 64), qq = q - u * (18 * 64);
-      const int code = code_of(u);
-      const int b0 = code & 1023, b1 = (code >> 10) & 1023, b2 = (code >> 20) & 1023;
-      auto blk = [&](int x) { return x == 0 ? b0 : (x == 1 ? b1 : b2); };
-      const int b = qq >> 6, m = b / 6, pi = b - m * 6;
-      const int pu = pi >> 1, wi = pi & 1, pw = wi + (wi >= pu ? 1 : 0);
-      const int rr = (qq >> 3) & 7, cc = qq & 7;
-      const int p1 = m == 0 ? j : i, p2 = m == 2 ? j : k;
-      const long row = blk(pu) * 8 + rr, col = blk(pw) * 8 + cc;
-      S.jb[u][b][rr * 8 + cc] =
-          (code >= 0 && row < v && col < v) ? J[((long)p1 * v + row) * nov + (long)p2 * v + col] : 0.0;
+    double jv[JIT];
+#pragma unroll
+    for (int it = 0; it < JIT; ++it) {
+      const int q = tid + it * NT;
+      jv[it] = 0.0;
+      if (q < NB * 18 * 64) {
+        const int u = q / (18 * 64), qq = q - u * (18 * 64);
+        const int code = code_of(u);
+        const int b0 = code & 1023, b1 = (code >> 10) & 1023, b2 = (code >> 20) & 1023;
+        auto blk = [&](int x) { return x == 0 ? b0 : (x == 1 ? b1 : b2); };
+        const int b = qq >> 6, m = b / 6, pi = b - m * 6;
+        const int pu = pi >> 1, wi = pi & 1, pw = wi + (wi >= pu ? 1 : 0);
+        const int rr = (qq >> 3) & 7, cc = qq & 7;
+        const int p1 = m == 0 ? j : i, p2 = m == 2 ? j : k;
+        const long row = blk(pu) * 8 + rr, col = blk(pw) * 8 + cc;
+        if (code >= 0 && row < v && col < v) jv[it] = J[((long)p1 * v + row) * nov + (long)p2 * v + col];
+      }
     }
-    for (int q = tid; q < NB * 72; q += NT) {
-      const int u = q / 72, qq = q - u * 72;
-      const int code = code_of(u);
-      const int b0 = code & 1023, b1 = (code >> 10) & 1023, b2 = (code >> 20) & 1023;
-      const int m = qq / 24, pu = (qq / 8) % 3, x8 = qq & 7;
-      const long x = (pu == 0 ? b0 : (pu == 1 ? b1 : b2)) * 8 + x8;
-      const int om = m == 0 ? i : (m == 1 ? j : k);
-      S.t1s[u][m][pu][x8] = (code >= 0 && x < v) ? t1[(long)om * v + x] : 0.0;
+    double tv[TIT];
+#pragma unroll
+    for (int it = 0; it < TIT; ++it) {
+      const int q = tid + it * NT;
+      tv[it] = 0.0;
+      if (q < NB * 72) {
+        const int u = q / 72, qq = q - u * 72;
+        const int code = code_of(u);
+        const int b0 = code & 1023, b1 = (code >> 10) & 1023, b2 = (code >> 20) & 1023;
+        const int m = qq / 24, pu = (qq / 8) % 3, x8 = qq & 7;
+        const long x = (pu == 0 ? b0 : (pu == 1 ? b1 : b2)) * 8 + x8;
+        const int om = m == 0 ? i : (m == 1 ? j : k);
+        if (code >= 0 && x < v) tv[it] = t1[(long)om * v + x];
+      }
+    }
+    __syncthreads();  // the previous round's LDS reads are done
+#pragma unroll
+    for (int it = 0; it < WIT; ++it) {
+      const int row = tid + it * NT;
+      if (row < NB * 384) {
+        const int u = row / 384, rr = row - u * 384;
+        const int p = rr >> 6, x8 = (rr >> 3) & 7, y8 = rr & 7;
+        WT* dst = &S.w[u][p][x8 * 72 + y8 * 9];
+#pragma unroll
+        for (int z = 0; z < 8; ++z) dst[z] = wv[it][z];
+      }
+    }
+#pragma unroll
+    for (int it = 0; it < JIT; ++it) {
+      const int q = tid + it * NT;
+      if (q < NB * 18 * 64) {
+        const int u = q / (18 * 64), qq = q - u * (18 * 64);
+        const int b = qq >> 6, rr = (qq >> 3) & 7, cc = qq & 7;
+        S.jb[u][b][rr * 8 + cc] = jv[it];
+      }
+    }
+#pragma unroll
+    for (int it = 0; it < TIT; ++it) {
+      const int q = tid + it * NT;
+      if (q < NB * 72) {
+        const int u = q / 72, qq = q - u * 72;
+        const int m = qq / 24, pu = (qq / 8) % 3, x8 = qq & 7;
+        S.t1s[u][m][pu][x8] = tv[it];
+      }
     }
     __syncthreads();
 #pragma unroll
