@@ -192,3 +192,19 @@ def test_dryrun_mfma_ceiling_refuses():
     assert r.returncode == 0, r.stderr
     rc, val, msg = r.stdout.split(" ", 2)
     assert int(rc) != 0 and float(val) == -1.0 and "dry run" in msg
+
+
+def test_mfma_ceiling_argument_checks():
+    """The probe accepts CCGPU_MIXED and CCGPU_FP64 only, and a non-null
+    output; both checks come before anything is launched."""
+    script = (f"import ctypes as C, sys; sys.path.insert(0, {ROOT!r}); from dalton_amd import _lib; "
+              "L = _lib.lib(); t = C.c_double(); "
+              "r1 = L.ccgpu_mfma_ceiling(2, C.byref(t)); m1 = L.ccgpu_last_error().decode(); "
+              "r2 = L.ccgpu_mfma_ceiling(0, None); m2 = L.ccgpu_last_error().decode(); "
+              "print(r1, r2); print(m1); print(m2)")
+    env = dict(os.environ, CCGPU_DRYRUN="1")
+    r = subprocess.run([sys.executable, "-c", script], env=env, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    codes, m1, m2 = r.stdout.strip().split("\n")
+    assert all(int(c) != 0 for c in codes.split())
+    assert "precision" in m1 and "null" in m2
