@@ -133,6 +133,19 @@ void CCSD::ensure_work() {
   // bytes per V+/- / Y buffer (10 GB unless set; CCGPU_LADDER_BUDGET_MB overrides)
   const double budget = ladder_budget_;
   long maxrows = std::max<long>(1, std::min<long>(65535, (long)(budget / (8.0 * dm_.npv))));
+  // Rows per batch in whole waves of the S product's 256 x 256 tiles over
+  // the CUs (V+ and V- each ceil(npo/256) tiles wide), so that no batch ends
+  // in a nearly empty wave; the grouping of rows changes no result bit.
+  {
+    int dev = 0, ncu = 0;
+    if (!dryrun() && hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && ncu > 0) {
+      const long wide = 2 * ((dm_.npo + 255) / 256);
+      const long blocks_per_wave = ncu / wide;
+      const long step = blocks_per_wave * 256;
+      if (blocks_per_wave > 0 && maxrows >= step) maxrows = maxrows / step * step;
+    }
+  }
   lad_batches_.clear();
   lad_batches_.push_back(0);
   long rows = 0, maxr = 0, maxy = 0;
